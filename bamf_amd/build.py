@@ -1,0 +1,77 @@
+"""Build recipe for libbamf_gpu.so (gfx950) — run in-tree, no JIT cache.
+
+    python -m bamf_amd.build          # compiles bamf_amd/libbamf_gpu.so
+
+Every HIP translation unit is compiled with packed-FP32 instruction
+selection disabled (-packed-fp32-ops): ordinary float code is issued as
+scalar VALU ops.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libbamf_gpu.so"
+HIP_SOURCES = ["runtime.cpp", "mle.hip", "findroi.hip", "rjmcmc.hip", "rjmcmc_host.cpp", "net.hip"]
+ARCH = os.environ.get("BAMF_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found: the HIP toolchain is required to build libbamf_gpu.so")
+
+
+def compile_cmds(out: Path = LIB) -> list[list[str]]:
+    objs = []
+    cmds = []
+    build_dir = PKG / "_build"
+    build_dir.mkdir(exist_ok=True)
+    common = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-Wno-unused-command-line-argument", f"-I{ROOT / 'include'}"]
+    for src in HIP_SOURCES:
+        p = CSRC / src
+        if not p.exists():
+            continue
+        obj = build_dir / (p.stem + ".o")
+        cmds.append(common + ["-c", str(p), "-o", str(obj)])
+        objs.append(str(obj))
+    cmds.append([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(out)] + objs)
+    return cmds
+
+
+def build(verbose: bool = False, jobs: int = 4) -> Path:
+    from concurrent.futures import ThreadPoolExecutor
+
+    cmds = compile_cmds()
+
+    def run(cmd):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        return cmd, subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for cmd, r in ex.map(run, cmds[:-1]):
+            if r.returncode != 0:
+                sys.stderr.write(r.stdout.decode(errors="replace"))
+                raise RuntimeError(f"compile failed: {' '.join(cmd)}")
+    tmp = LIB.with_suffix(".so.tmp")
+    link = [str(tmp) if a == str(LIB) else a for a in cmds[-1]]
+    _, r = run(link)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout.decode(errors="replace"))
+        raise RuntimeError("link failed")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,376 @@
+// Single-emitter Newton MLE of theta = {x, y, N, bg} with CRLB and
+// log-likelihood (fit type 'Basic').  Same inputs, outputs and iteration as
+// kernel_MLEFit_XYNB_ (source/cuda/cuda_gaussMLEv2/cuda_gaussMLEv2.cu:16-141).
+//
+// gfx950 design: one work-item per fit, the box staged through LDS so that
+// global loads are contiguous, the pixel-integrated Gaussian evaluated in
+// separable form (2*sz erf pairs and 2*sz exp pairs per Newton iteration
+// instead of 2*sz^2 of each), and the initial-value smoothing filter of
+// kernel_GaussFMaxMin2D (GPUgaussLib.cuh:230-258) applied as two 1-D passes.
+// Per pixel the model, derivative and Newton sums are the reference's
+// expressions evaluated in the reference's order.
+#include "common.hpp"
+
+namespace bamf {
+namespace {
+
+constexpr float kPi = 3.141592f;  // definitions.h:13
+constexpr int kNV = 4;
+constexpr int kBlock = 64;
+
+// 4x4 partial inverse: in-place LU without pivoting, diagonal of the inverse
+// (same algorithm as kernel_MatInvN, MatInvLib.h:15-75).
+__device__ inline void mat_inv4_diag(float* m, float* diag) {
+    constexpr int sz = kNV;
+    float minv[sz * sz];
+    for (int jj = 0; jj < sz; jj++) {
+        for (int ii = 1; ii <= jj; ii++) {
+            float t = 0.f;
+            for (int kk = 0; kk <= ii - 1; kk++) t += m[ii + kk * sz] * m[kk + jj * sz];
+            m[ii + jj * sz] -= t;
+        }
+        for (int ii = jj + 1; ii < sz; ii++) {
+            float t = 0.f;
+            for (int kk = 0; kk <= jj - 1; kk++) t += m[ii + kk * sz] * m[kk + jj * sz];
+            m[ii + jj * sz] = (1 / m[jj + jj * sz]) * (m[ii + jj * sz] - t);
+        }
+    }
+    for (int num = 0; num < sz; num++) {
+        float yy[sz];
+        yy[0] = (num == 0) ? 1.f : 0.f;
+        for (int ii = 1; ii < sz; ii++) {
+            float t = 0.f;
+            for (int jj = 0; jj <= ii - 1; jj++) t += m[ii + jj * sz] * yy[jj];
+            yy[ii] = ((ii == num) ? 1.f : 0.f) - t;
+        }
+        minv[sz - 1 + num * sz] = yy[sz - 1] / m[(sz - 1) + (sz - 1) * sz];
+        for (int ii = sz - 2; ii >= 0; ii--) {
+            float t = 0.f;
+            for (int jj = ii + 1; jj < sz; jj++) t += m[ii + jj * sz] * minv[jj + num * sz];
+            minv[ii + num * sz] = (1 / m[ii + ii * sz]) * (yy[ii] - t);
+        }
+    }
+    for (int ii = 0; ii < sz; ii++) diag[ii] = minv[ii * sz + ii];
+}
+
+// Per-axis terms of the pixel-integrated Gaussian and its first and second
+// derivatives (GPUgaussLib.cuh:14-24, 67-87) for pixel index i.  The
+// reference's derivative is ((-N/sqrt(2pi))/sigma*(a-b))*PSF_other, so the
+// axis factor d1 times the other axis's PSF reproduces it exactly.
+struct AxisTerms {
+    float psf, d1, d2;
+};
+__device__ inline AxisTerms axis_terms(int i, float x, float sigma, float sqrt_norm, float n,
+                                       float c1, float c2) {
+    AxisTerms t;
+    t.psf = 0.5f * (erff((i - x + 1.0f) * sqrt_norm) - erff((i - x) * sqrt_norm));
+    float ta = (i + 1.0f - x) / sigma;
+    float tb = (i - x) / sigma;
+    float a = expf(-0.5f * (ta * ta));
+    float b = expf(-0.5f * (tb * tb));
+    t.d1 = -n / c1 / sigma * (a - b);
+    t.d2 = -n / c1 / c2 * ((i + 1.0f - x) * a - (i - x) * b);
+    return t;
+}
+
+template <int SZ>
+__global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ data, float sigma,
+                                                     int iterations, float* __restrict__ params,
+                                                     float* __restrict__ crlb, float* __restrict__ ll,
+                                                     int nfits) {
+    constexpr int NP = SZ * SZ;
+    // +1 word per box keeps the per-work-item stride odd (no LDS bank conflicts).
+    __shared__ float s_box[kBlock * (NP + 1)];
+    const int tx = threadIdx.x;
+    const int64_t first = (int64_t)blockIdx.x * kBlock;
+    const int nb = (int)min<int64_t>(kBlock, nfits - first);
+    const float* src = data + first * NP;
+    for (int k = tx; k < nb * NP; k += kBlock) {
+        int b = k / NP, p = k - b * NP;
+        s_box[b * (NP + 1) + p] = src[k];
+    }
+    __syncthreads();
+    if (tx >= nb) return;
+    const float* box = s_box + tx * (NP + 1);
+    float d[NP];
+#pragma unroll
+    for (int p = 0; p < NP; p++) d[p] = box[p];
+
+    // --- initial values: centre of mass (GPUgaussLib.cuh:207-227) ---
+    float theta[kNV];
+    {
+        float sx = 0.f, sy = 0.f, ss = 0.f;
+#pragma unroll
+        for (int ii = 0; ii < SZ; ii++)
+#pragma unroll
+            for (int jj = 0; jj < SZ; jj++) {
+                float v = d[SZ * jj + ii];
+                sx += v * ii;
+                sy += v * jj;
+                ss += v;
+            }
+        theta[0] = sx / ss;
+        theta[1] = sy / ss;
+    }
+    // --- filtered max/min (GPUgaussLib.cuh:230-258), separable ---
+    {
+        const float norm = 1.0f / 2.0f / sigma / sigma;
+        float g[SZ];
+#pragma unroll
+        for (int k = 0; k < SZ; k++) g[k] = expf(-(float)(k * k) * norm);
+        float rs[SZ];  // sum_i g(|i-k|)
+#pragma unroll
+        for (int k = 0; k < SZ; k++) {
+            float s = 0.f;
+#pragma unroll
+            for (int i = 0; i < SZ; i++) s += g[i > k ? i - k : k - i];
+            rs[k] = s;
+        }
+        float t[NP];  // t[ii][ll] = sum_jj g(|ll-jj|) d[ii*SZ+jj]
+#pragma unroll
+        for (int ii = 0; ii < SZ; ii++)
+#pragma unroll
+            for (int ll = 0; ll < SZ; ll++) {
+                float s = 0.f;
+#pragma unroll
+                for (int jj = 0; jj < SZ; jj++) s += g[ll > jj ? ll - jj : jj - ll] * d[ii * SZ + jj];
+                t[ii * SZ + ll] = s;
+            }
+        float fmax_ = 0.0f, fmin_ = 10e10f;
+#pragma unroll
+        for (int kk = 0; kk < SZ; kk++)
+#pragma unroll
+            for (int ll = 0; ll < SZ; ll++) {
+                float s = 0.f;
+#pragma unroll
+                for (int ii = 0; ii < SZ; ii++) s += g[ii > kk ? ii - kk : kk - ii] * t[ii * SZ + ll];
+                float f = s / (rs[kk] * rs[ll]);
+                fmax_ = fmaxf(fmax_, f);
+                fmin_ = fminf(fmin_, f);
+            }
+        theta[3] = fmin_;
+        theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
+    }
+
+    const float sqrt_norm = sqrtf(1.0f / 2.0f / sigma / sigma);
+    const float c1 = sqrtf(2.0f * kPi);
+    const float c2 = sigma * sigma * sigma;
+    for (int it = 0; it < iterations; it++) {
+        AxisTerms ax[SZ], ay[SZ];
+#pragma unroll
+        for (int i = 0; i < SZ; i++) {
+            ax[i] = axis_terms(i, theta[0], sigma, sqrt_norm, theta[2], c1, c2);
+            ay[i] = axis_terms(i, theta[1], sigma, sqrt_norm, theta[2], c1, c2);
+        }
+        float num[kNV] = {0.f, 0.f, 0.f, 0.f}, den[kNV] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ii = 0; ii < SZ; ii++)
+#pragma unroll
+            for (int jj = 0; jj < SZ; jj++) {
+                const float px = ax[ii].psf, py = ay[jj].psf;
+                const float model = theta[3] + theta[2] * px * py;
+                const float dat = d[SZ * jj + ii];
+                const float u0 = ax[ii].d1 * py, v0 = ax[ii].d2 * py;
+                const float u1 = ay[jj].d1 * px, v1 = ay[jj].d2 * px;
+                const float u2 = px * py;
+                float cf = 0.f, df = 0.f;
+                if (model > 10e-3f) {
+                    cf = dat / model - 1;
+                    df = dat / (model * model);
+                }
+                cf = fminf(cf, 10e4f);
+                df = fminf(df, 10e4f);
+                num[0] += u0 * cf;
+                den[0] += v0 * cf - (u0 * u0) * df;
+                num[1] += u1 * cf;
+                den[1] += v1 * cf - (u1 * u1) * df;
+                num[2] += u2 * cf;
+                den[2] += -(u2 * u2) * df;
+                num[3] += cf;
+                den[3] += -df;
+            }
+        theta[0] -= fminf(fmaxf(num[0] / den[0], -1e0f), 1e0f);
+        theta[1] -= fminf(fmaxf(num[1] / den[1], -1e0f), 1e0f);
+        theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
+        theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
+        theta[2] = fmaxf(theta[2], 1.0f);
+        theta[3] = fmaxf(theta[3], 0.01f);
+    }
+
+    // --- Fisher information, CRLB and log-likelihood (cuda_gaussMLEv2.cu:105-138) ---
+    float M[kNV * kNV];
+#pragma unroll
+    for (int k = 0; k < kNV * kNV; k++) M[k] = 0.f;
+    float div = 0.f;
+    {
+        AxisTerms ax[SZ], ay[SZ];
+#pragma unroll
+        for (int i = 0; i < SZ; i++) {
+            ax[i] = axis_terms(i, theta[0], sigma, sqrt_norm, theta[2], c1, c2);
+            ay[i] = axis_terms(i, theta[1], sigma, sqrt_norm, theta[2], c1, c2);
+        }
+#pragma unroll
+        for (int ii = 0; ii < SZ; ii++)
+#pragma unroll
+            for (int jj = 0; jj < SZ; jj++) {
+                const float px = ax[ii].psf, py = ay[jj].psf;
+                const float model = theta[3] + theta[2] * px * py;
+                const float dat = d[SZ * jj + ii];
+                float du[kNV] = {ax[ii].d1 * py, ay[jj].d1 * px, px * py, 1.0f};
+                const float rm = 1.0f / model;
+#pragma unroll
+                for (int k = 0; k < kNV; k++)
+#pragma unroll
+                    for (int l = k; l < kNV; l++) M[k * kNV + l] += du[l] * du[k] * rm;
+                if (model > 0) {
+                    if (dat > 0) div += dat * logf(model) - model - dat * logf(dat) + dat;
+                    else div += -model;
+                }
+            }
+#pragma unroll
+        for (int k = 0; k < kNV; k++)
+#pragma unroll
+            for (int l = k + 1; l < kNV; l++) M[l * kNV + k] = M[k * kNV + l];
+    }
+    float diag[kNV];
+    mat_inv4_diag(M, diag);
+    const int64_t i = first + tx;
+#pragma unroll
+    for (int k = 0; k < kNV; k++) {
+        params[(int64_t)nfits * k + i] = theta[k];
+        crlb[(int64_t)nfits * k + i] = diag[k];
+    }
+    ll[i] = div;
+}
+
+// Any box size: same algorithm reading the box from global memory.
+__global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict__ data, float sigma,
+                                                         int sz, int iterations, float* __restrict__ params,
+                                                         float* __restrict__ crlb, float* __restrict__ ll,
+                                                         int nfits) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nfits) return;
+    const float* d = data + i * sz * sz;
+    float theta[kNV];
+    {
+        float sx = 0.f, sy = 0.f, ss = 0.f;
+        for (int ii = 0; ii < sz; ii++)
+            for (int jj = 0; jj < sz; jj++) {
+                float v = d[sz * jj + ii];
+                sx += v * ii;
+                sy += v * jj;
+                ss += v;
+            }
+        theta[0] = sx / ss;
+        theta[1] = sy / ss;
+    }
+    {
+        const float norm = 1.0f / 2.0f / sigma / sigma;
+        float fmax_ = 0.0f, fmin_ = 10e10f;
+        for (int kk = 0; kk < sz; kk++)
+            for (int ll2 = 0; ll2 < sz; ll2++) {
+                float s = 0.f, w = 0.f;
+                for (int ii = 0; ii < sz; ii++) {
+                    float gi = expf(-(float)((ii - kk) * (ii - kk)) * norm);
+                    for (int jj = 0; jj < sz; jj++) {
+                        float gg = gi * expf(-(float)((ll2 - jj) * (ll2 - jj)) * norm);
+                        s += gg * d[ii * sz + jj];
+                        w += gg;
+                    }
+                }
+                float f = s / w;
+                fmax_ = fmaxf(fmax_, f);
+                fmin_ = fminf(fmin_, f);
+            }
+        theta[3] = fmin_;
+        theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
+    }
+    const float sqrt_norm = sqrtf(1.0f / 2.0f / sigma / sigma);
+    const float c1 = sqrtf(2.0f * kPi);
+    const float c2 = sigma * sigma * sigma;
+    for (int it = 0; it < iterations; it++) {
+        float num[kNV] = {0.f, 0.f, 0.f, 0.f}, den[kNV] = {0.f, 0.f, 0.f, 0.f};
+        for (int ii = 0; ii < sz; ii++) {
+            AxisTerms ax = axis_terms(ii, theta[0], sigma, sqrt_norm, theta[2], c1, c2);
+            for (int jj = 0; jj < sz; jj++) {
+                AxisTerms ay = axis_terms(jj, theta[1], sigma, sqrt_norm, theta[2], c1, c2);
+                const float px = ax.psf, py = ay.psf;
+                const float model = theta[3] + theta[2] * px * py;
+                const float dat = d[sz * jj + ii];
+                const float u0 = ax.d1 * py, v0 = ax.d2 * py, u1 = ay.d1 * px, v1 = ay.d2 * px, u2 = px * py;
+                float cf = 0.f, df = 0.f;
+                if (model > 10e-3f) {
+                    cf = dat / model - 1;
+                    df = dat / (model * model);
+                }
+                cf = fminf(cf, 10e4f);
+                df = fminf(df, 10e4f);
+                num[0] += u0 * cf; den[0] += v0 * cf - (u0 * u0) * df;
+                num[1] += u1 * cf; den[1] += v1 * cf - (u1 * u1) * df;
+                num[2] += u2 * cf; den[2] += -(u2 * u2) * df;
+                num[3] += cf;      den[3] += -df;
+            }
+        }
+        theta[0] -= fminf(fmaxf(num[0] / den[0], -1e0f), 1e0f);
+        theta[1] -= fminf(fmaxf(num[1] / den[1], -1e0f), 1e0f);
+        theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
+        theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
+        theta[2] = fmaxf(theta[2], 1.0f);
+        theta[3] = fmaxf(theta[3], 0.01f);
+    }
+    float M[kNV * kNV];
+    for (int k = 0; k < kNV * kNV; k++) M[k] = 0.f;
+    float div = 0.f;
+    for (int ii = 0; ii < sz; ii++) {
+        AxisTerms ax = axis_terms(ii, theta[0], sigma, sqrt_norm, theta[2], c1, c2);
+        for (int jj = 0; jj < sz; jj++) {
+            AxisTerms ay = axis_terms(jj, theta[1], sigma, sqrt_norm, theta[2], c1, c2);
+            const float px = ax.psf, py = ay.psf;
+            const float model = theta[3] + theta[2] * px * py;
+            const float dat = d[sz * jj + ii];
+            float du[kNV] = {ax.d1 * py, ay.d1 * px, px * py, 1.0f};
+            for (int k = 0; k < kNV; k++)
+                for (int l = k; l < kNV; l++) M[k * kNV + l] += du[l] * du[k] / model;
+            if (model > 0) {
+                if (dat > 0) div += dat * logf(model) - model - dat * logf(dat) + dat;
+                else div += -model;
+            }
+        }
+    }
+    for (int k = 0; k < kNV; k++)
+        for (int l = k + 1; l < kNV; l++) M[l * kNV + k] = M[k * kNV + l];
+    float diag[kNV];
+    mat_inv4_diag(M, diag);
+    for (int k = 0; k < kNV; k++) {
+        params[(int64_t)nfits * k + i] = theta[k];
+        crlb[(int64_t)nfits * k + i] = diag[k];
+    }
+    ll[i] = div;
+}
+
+}  // namespace
+}  // namespace bamf
+
+extern "C" int bamf_gauss_mle_xynb(const float* d_data, float psf_sigma, int sz, int iterations,
+                                   float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream) {
+    using namespace bamf;
+    BAMF_CHECK_ARG(nfits >= 0, "nfits must be >= 0 (got %d)", nfits);
+    BAMF_CHECK_ARG(sz >= 1 && sz <= 64, "sz must be in [1, 64] (got %d)", sz);
+    BAMF_CHECK_ARG(iterations >= 0, "iterations must be >= 0");
+    BAMF_CHECK_ARG(psf_sigma > 0.f, "psf_sigma must be > 0");
+    if (nfits == 0) return BAMF_OK;
+    BAMF_CHECK_ARG(d_data && d_params && d_crlb && d_ll, "null device pointer");
+    dim3 grid((unsigned)ceil_div(nfits, kBlock)), block(kBlock);
+    hipStream_t s = as_stream(stream);
+    switch (sz) {
+    case 5: hipLaunchKernelGGL(k_mle_xynb<5>, grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
+    case 6: hipLaunchKernelGGL(k_mle_xynb<6>, grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
+    case 7: hipLaunchKernelGGL(k_mle_xynb<7>, grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
+    case 8: hipLaunchKernelGGL(k_mle_xynb<8>, grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
+    case 9: hipLaunchKernelGGL(k_mle_xynb<9>, grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
+    default:
+        hipLaunchKernelGGL(k_mle_xynb_any, grid, block, 0, s, d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits);
+    }
+    BAMF_LAUNCH_CHECK("k_mle_xynb");
+    return BAMF_OK;
+}

@@ -1,0 +1,202 @@
+/*
+ * Oracle: single-emitter Newton MLE of (x, y, I, bg) with CRLB and
+ * log-likelihood.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Restates kernel_MLEFit_XYNB_ (source/cuda/cuda_gaussMLEv2/cuda_gaussMLEv2.cu:16-141)
+ * and its helpers kernel_IntGauss1D / kernel_DerivativeIntGauss1D
+ * (GPUgaussLib.cuh:14-24, 67-87), kernel_CenterofMass2D (GPUgaussLib.cuh:207-227),
+ * kernel_GaussFMaxMin2D (GPUgaussLib.cuh:230-258) and kernel_MatInvN
+ * (MatInvLib.h:15-75), one fit at a time, in the reference's loop order.
+ * pi is 3.141592f (definitions.h:13).
+ */
+#include <math.h>
+#include <string.h>
+#include "oracle.h"
+
+#define NV 4
+static const float kPi = 3.141592f;
+
+/* GPUgaussLib.cuh:14-24 */
+float orc_int_gauss_1d(int ii, float x, float sigma) {
+    const float norm = 1.0f / 2.0f / sigma / sigma;
+    return 1.0f / 2.0f * (erff((ii - x + 1.0f) * sqrtf(norm)) - erff((ii - x) * sqrtf(norm)));
+}
+
+/* GPUgaussLib.cuh:67-87.  pow(v, 2.0f) is written v*v (both correctly rounded). */
+static void deriv_int_gauss_1d(int ii, float x, float sigma, float n, float psfy,
+                               float* dudt, float* d2udt2) {
+    float ta = (ii + 1.0f - x) / sigma;
+    float tb = (ii - x) / sigma;
+    float a = expf(-1.0f / 2.0f * (ta * ta));
+    float b = expf(-1.0f / 2.0f * (tb * tb));
+    *dudt = -n / sqrtf(2.0f * kPi) / sigma * (a - b) * psfy;
+    if (d2udt2)
+        *d2udt2 = -n / sqrtf(2.0f * kPi) / powf(sigma, 3.0f) * ((ii + 1.0f - x) * a - (ii - x) * b) * psfy;
+}
+
+/* GPUgaussLib.cuh:207-227 */
+void orc_center_of_mass_2d(int sz, const float* data, float* x, float* y) {
+    float tmpx = 0.0f, tmpy = 0.0f, tmpsum = 0.0f;
+    for (int ii = 0; ii < sz; ii++)
+        for (int jj = 0; jj < sz; jj++) {
+            tmpx += data[sz * jj + ii] * ii;
+            tmpy += data[sz * jj + ii] * jj;
+            tmpsum += data[sz * jj + ii];
+        }
+    *x = tmpx / tmpsum;
+    *y = tmpy / tmpsum;
+}
+
+/* GPUgaussLib.cuh:230-258: brute-force normalised Gaussian smoothing; max and min. */
+void orc_gauss_fmaxmin_2d(int sz, float sigma, const float* data, float* maxn, float* minbg) {
+    float filteredpixel, sum;
+    *maxn = 0.0f;
+    *minbg = 10e10f;
+    const float norm = 1.0f / 2.0f / sigma / sigma;
+    for (int kk = 0; kk < sz; kk++)
+        for (int ll = 0; ll < sz; ll++) {
+            filteredpixel = 0.0f;
+            sum = 0.0f;
+            for (int ii = 0; ii < sz; ii++)
+                for (int jj = 0; jj < sz; jj++) {
+                    float di = (float)(ii - kk), dj = (float)(ll - jj);
+                    filteredpixel += expf(-(di * di) * norm) * expf(-(dj * dj) * norm) * data[ii * sz + jj];
+                    sum += expf(-(di * di) * norm) * expf(-(dj * dj) * norm);
+                }
+            filteredpixel /= sum;
+            *maxn = fmaxf(*maxn, filteredpixel);
+            *minbg = fminf(*minbg, filteredpixel);
+        }
+}
+
+/* MatInvLib.h:15-75: in-place LU without pivoting, column solves, diag of inverse. */
+void orc_mat_inv_n(float* m, float* minv, float* diag, int sz) {
+    int ii, jj, kk, num, b;
+    float tmp1 = 0;
+    float yy[25];
+    for (jj = 0; jj < sz; jj++) {
+        for (ii = 0; ii <= jj; ii++)
+            if (ii > 0) {
+                for (kk = 0; kk <= ii - 1; kk++) tmp1 += m[ii + kk * sz] * m[kk + jj * sz];
+                m[ii + jj * sz] -= tmp1;
+                tmp1 = 0;
+            }
+        for (ii = jj + 1; ii < sz; ii++)
+            if (jj > 0) {
+                for (kk = 0; kk <= jj - 1; kk++) tmp1 += m[ii + kk * sz] * m[kk + jj * sz];
+                m[ii + jj * sz] = (1 / m[jj + jj * sz]) * (m[ii + jj * sz] - tmp1);
+                tmp1 = 0;
+            } else {
+                m[ii + jj * sz] = (1 / m[jj + jj * sz]) * m[ii + jj * sz];
+            }
+    }
+    tmp1 = 0;
+    for (num = 0; num < sz; num++) {
+        yy[0] = (num == 0) ? 1.0f : 0.0f;
+        for (ii = 1; ii < sz; ii++) {
+            b = (ii == num) ? 1 : 0;
+            for (jj = 0; jj <= ii - 1; jj++) tmp1 += m[ii + jj * sz] * yy[jj];
+            yy[ii] = b - tmp1;
+            tmp1 = 0;
+        }
+        minv[sz - 1 + num * sz] = yy[sz - 1] / m[(sz - 1) + (sz - 1) * sz];
+        for (ii = sz - 2; ii >= 0; ii--) {
+            for (jj = ii + 1; jj < sz; jj++) tmp1 += m[ii + jj * sz] * minv[jj + num * sz];
+            minv[ii + num * sz] = (1 / m[ii + ii * sz]) * (yy[ii] - tmp1);
+            tmp1 = 0;
+        }
+    }
+    if (diag)
+        for (ii = 0; ii < sz; ii++) diag[ii] = minv[ii * sz + ii];
+}
+
+/* cuda_gaussMLEv2.cu:16-141, one fit per loop iteration instead of per thread. */
+static void mle_one(const float* s_data, float sigma, int sz, int iterations,
+                    float* theta, float* diagout, float* llout) {
+    float m[NV * NV], diag[NV], minv[NV * NV];
+    float dudt[NV], d2udt2[NV], num[NV], den[NV];
+    const float maxjump[NV] = {1e0f, 1e0f, 1e2f, 2e0f};
+    float nmax, model, cf, df, data, psfx, psfy, div;
+    (void)maxjump;
+    memset(m, 0, sizeof(m));
+    memset(minv, 0, sizeof(minv));
+    orc_center_of_mass_2d(sz, s_data, &theta[0], &theta[1]);
+    orc_gauss_fmaxmin_2d(sz, sigma, s_data, &nmax, &theta[3]);
+    theta[2] = fmaxf(0.0f, (nmax - theta[3]) * 2 * 2 * kPi * sigma * sigma);
+
+    for (int kk = 0; kk < iterations; kk++) {
+        memset(num, 0, sizeof(num));
+        memset(den, 0, sizeof(den));
+        for (int ii = 0; ii < sz; ii++)
+            for (int jj = 0; jj < sz; jj++) {
+                psfx = orc_int_gauss_1d(ii, theta[0], sigma);
+                psfy = orc_int_gauss_1d(jj, theta[1], sigma);
+                model = theta[3] + theta[2] * psfx * psfy;
+                data = s_data[sz * jj + ii];
+                deriv_int_gauss_1d(ii, theta[0], sigma, theta[2], psfy, &dudt[0], &d2udt2[0]);
+                deriv_int_gauss_1d(jj, theta[1], sigma, theta[2], psfx, &dudt[1], &d2udt2[1]);
+                dudt[2] = psfx * psfy;
+                d2udt2[2] = 0.0f;
+                dudt[3] = 1.0f;
+                d2udt2[3] = 0.0f;
+                cf = 0.0f;
+                df = 0.0f;
+                if (model > 10e-3f) cf = data / model - 1;
+                if (model > 10e-3f) df = data / (model * model);
+                cf = fminf(cf, 10e4f);
+                df = fminf(df, 10e4f);
+                for (int ll = 0; ll < NV; ll++) {
+                    num[ll] += dudt[ll] * cf;
+                    den[ll] += d2udt2[ll] * cf - (dudt[ll] * dudt[ll]) * df;
+                }
+            }
+        /* cuda_gaussMLEv2.cu:93-100 */
+        theta[0] -= fminf(fmaxf(num[0] / den[0], -1e0f), 1e0f);
+        theta[1] -= fminf(fmaxf(num[1] / den[1], -1e0f), 1e0f);
+        theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
+        theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
+        theta[2] = fmaxf(theta[2], 1.0f);
+        theta[3] = fmaxf(theta[3], 0.01f);
+    }
+
+    /* cuda_gaussMLEv2.cu:105-133: Fisher matrix and log-likelihood. */
+    div = 0.0f;
+    for (int ii = 0; ii < sz; ii++)
+        for (int jj = 0; jj < sz; jj++) {
+            psfx = orc_int_gauss_1d(ii, theta[0], sigma);
+            psfy = orc_int_gauss_1d(jj, theta[1], sigma);
+            model = theta[3] + theta[2] * psfx * psfy;
+            data = s_data[sz * jj + ii];
+            deriv_int_gauss_1d(ii, theta[0], sigma, theta[2], psfy, &dudt[0], NULL);
+            deriv_int_gauss_1d(jj, theta[1], sigma, theta[2], psfx, &dudt[1], NULL);
+            dudt[2] = psfx * psfy;
+            dudt[3] = 1.0f;
+            for (int k = 0; k < NV; k++)
+                for (int l = k; l < NV; l++) {
+                    m[k * NV + l] += dudt[l] * dudt[k] / model;
+                    m[l * NV + k] = m[k * NV + l];
+                }
+            if (model > 0) {
+                if (data > 0)
+                    div += data * logf(model) - model - data * logf(data) + data;
+                else
+                    div += -model;
+            }
+        }
+    orc_mat_inv_n(m, minv, diag, NV);
+    for (int k = 0; k < NV; k++) diagout[k] = diag[k];
+    *llout = div;
+}
+
+void orc_mle_xynb(const float* data, float psf_sigma, int sz, int iterations,
+                  float* params, float* crlb, float* ll, int nfits) {
+    for (int i = 0; i < nfits; i++) {
+        float theta[NV], diag[NV], l;
+        mle_one(data + (size_t)sz * sz * i, psf_sigma, sz, iterations, theta, diag, &l);
+        for (int k = 0; k < NV; k++) {
+            params[(size_t)nfits * k + i] = theta[k];
+            crlb[(size_t)nfits * k + i] = diag[k];
+        }
+        ll[i] = l;
+    }
+}

@@ -1,0 +1,72 @@
+/* Oracle RJMCMC interface (included from oracle.h).  TEST INFRASTRUCTURE ONLY. */
+#ifndef BAMF_ORACLE_RJMCMC_H
+#define BAMF_ORACLE_RJMCMC_H
+#include <stdint.h>
+
+/* Chain parameters; the fields of the MCMC struct read by make_Params
+ * (source/c/cRJMCMC.cpp:14-50).  Same layout as bamf_mcmc_params. */
+typedef struct {
+    float psf_sigma; int32_t n_trials; int32_t n_burnin;
+    float i_std_fg, i_std_bg, x_std_fg, x_std_bg;
+    float p_burnin[8]; float p_trials[8];
+    float rho; float bnd_out; int32_t grid_zoom; float split_std; float icutoff;
+    float bg_std; float abbg_std; float dx; float drift_x; float drift_y; int32_t is_backg;
+} orc_mcmc_params;
+
+/* One record per jump call (burn-in and trials).  Same layout as bamf_step_record. */
+typedef struct {
+    int32_t kind;       /* dispatch: 0 move 1 split 2 merge 3 birth 4 death 5 gsplit 6 gmerge 7 backfore */
+    int32_t jump_type;  /* reference JumpType of the record (1,2,3,10,4,5,6,7), 0 when none */
+    int32_t flags;      /* bit0 accepted, bit1 chain record, bit2 grid update, bit3 bounds test failed */
+    int32_t n;          /* N after the step */
+    float llr, pr;      /* reference single-precision LLR = exp(sum) and PR */
+    float bg;           /* BG after the step */
+    float photons;      /* sum of I after the step (sequential single precision) */
+    double log_pr;      /* log PR in double from the single-precision factors; NaN if undefined */
+    int32_t draws;      /* uniforms consumed by the step */
+    int32_t n_prop;     /* N of the proposed state */
+} orc_step;
+
+enum { ORC_SRC_PHILOX = 0, ORC_SRC_TAPE = 1, ORC_SRC_LIBC = 2 };
+typedef struct {
+    int32_t kind; uint32_t roi; uint64_t seed;
+    const uint32_t* tape; int64_t tape_len;   /* ORC_SRC_TAPE: raw 31-bit ints */
+    uint32_t* record; int64_t record_cap;     /* optional log of the raw ints consumed */
+    int64_t consumed; uint32_t step; uint32_t j; int32_t exhausted;
+} orc_src;
+
+typedef struct {
+    orc_step* steps;                 /* n_burnin + n_trials entries, or NULL */
+    float* pgrid; float* pback;      /* (ysize*zoom) x (xsize*zoom), or NULL */
+    int32_t counters[30];            /* N_*, A_*, N_*Fg, A_*Fg, N_*Bg, A_*Bg  (jump, split, merge, birth, death) */
+    int32_t n; float bg, abg, bbg;   /* final state */
+    float* fin_i; float* fin_x; float* fin_y; int32_t* fin_sig; int32_t fin_cap;
+    /* Chain / Chain_Test records (LibRJMCMC.cpp:290-322), or NULL */
+    int64_t rec_cap, em_cap;
+    int32_t* acc_head; float* acc_val; float* acc_em;
+    int32_t* prop_head; float* prop_val; float* prop_em;
+    int64_t n_rec, n_acc_em, n_prop_em;
+    int32_t status;
+} orc_out;
+
+#define ORC_OK 0
+#define ORC_E_UNSUPPORTED (-3)
+#define ORC_E_NMAX (-4)
+#define ORC_E_TAPE (-5)
+
+int orc_rjmcmc(const float* data, int xsize, int ysize, const float* scmos,
+               const orc_mcmc_params* p,
+               const float* off_pdf, int off_len, const float* sig_pdf, int sig_len,
+               const float* back_pdf, int back_len,
+               int n0, const float* i0, const float* x0, const float* y0, float bg0,
+               orc_src* src, orc_out* out);
+
+/* Pieces exposed for unit tests. */
+float orc_urand_from_raw(uint32_t r);
+float orc_stat_findpdf(const float* pdf, int len, float dx, float v);
+int   orc_stat_tables(const float* pdf, int len, float dx, float* xarr, float* cdf,
+                      float* uniq_x, float* uniq_cdf);   /* returns number of unique entries */
+float orc_stat_genrand(const float* uniq_x, const float* uniq_cdf, int nuniq, float u);
+void  orc_model(int n, const float* I, const float* X, const float* Y, float bg, float abg, float bbg,
+                int xsize, int ysize, float sigma, const float* scmos, float* m);
+#endif

@@ -1,0 +1,58 @@
+"""GPU gaussMLE (bamf_gauss_mle_xynb) against the oracle: (x, y, I, bg) within
+1e-4 relative (BASELINE tolerance), CRLB and log-likelihood within 1e-3."""
+import numpy as np
+import pytest
+
+from bamf_amd import synth
+from bamf_amd.gauss_mle import fit_boxes, gauss_mle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b, floor):
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+@pytest.mark.parametrize("sz,n,seed", [(7, 3000, 11), (5, 500, 12), (9, 500, 13), (11, 200, 14)])
+def test_mle_matches_oracle(oracle, sz, n, seed):
+    boxes, _ = synth.make_mle_boxes(n, sz, seed=seed)
+    pg, cg, lg = fit_boxes(boxes, 1.3, 20)
+    po, co, lo = oracle.mle_xynb(boxes, 1.3, 20)
+    print(f"sz={sz} n={n} seed={seed}", flush=True)
+    for k, floor in zip(range(4), (1.0, 1.0, 1.0, 1.0)):
+        r = rel(pg[k], po[k], floor)
+        assert np.quantile(r, 0.999) < 1e-4 and r.max() < 1e-3, (k, r.max())
+    assert np.quantile(rel(cg, co, 1e-6), 0.999) < 1e-3
+    assert np.quantile(rel(lg, lo, 1.0), 0.999) < 1e-3
+
+
+def test_mle_edge_cases(oracle):
+    """Arrays are parameter-major: p[k, i] is parameter k of box i.
+    Box 0 is all zeros (0/0 centre of mass), box 1 is constant (no emitter, so
+    the position is unidentifiable: only bg and a floor-level I are compared),
+    box 2 is a very bright emitter, boxes 3.. are ordinary; 67 boxes is not a
+    multiple of the work-group size."""
+    boxes = np.zeros((67, 7, 7), np.float32)
+    boxes[1] = 5.0
+    bright, _ = synth.make_mle_boxes(1, 7, seed=70, photons=(5.0, 2e5), noise=False)
+    boxes[2] = bright[0]
+    boxes[3:], _ = synth.make_mle_boxes(64, 7, seed=7)
+    pg, cg, lg = fit_boxes(boxes, 1.3, 20)
+    po, co, lo = oracle.mle_xynb(boxes, 1.3, 20)
+    assert np.array_equal(np.isfinite(po), np.isfinite(pg))
+    assert np.array_equal(np.isnan(pg[:, 0]), np.isnan(po[:, 0]))
+    assert abs(pg[3, 1] - po[3, 1]) < 1e-3 and abs(po[3, 1] - 5.0) < 0.05
+    assert pg[2, 1] < 2.0 and po[2, 1] < 2.0
+    assert np.all(rel(pg[:, 2], po[:, 2], 1.0) < 1e-4)
+    assert np.all(rel(pg[:, 3:], po[:, 3:], 1.0) < 1e-4)
+
+
+def test_gauss_mle_matlab_mirror():
+    boxes, t = synth.make_mle_boxes(256, 7, seed=8, noise=False)
+    stack = np.asfortranarray(np.transpose(boxes, (2, 1, 0)))  # MATLAB (SZ, SZ, N)
+    res, stats = gauss_mle(stack, "Basic", "CCD", 1.3)
+    # gaussMLE.m:183-186: column 1 -> Y; the kernel's x runs along the box's first MATLAB index
+    np.testing.assert_allclose(res["Y"], t["x"], atol=2e-3)
+    np.testing.assert_allclose(res["X"], t["y"], atol=2e-3)
+    np.testing.assert_allclose(res["Photons"], t["I"], rtol=2e-3)
+    assert stats["NFitsPerCall"] == 256
