@@ -1,0 +1,1172 @@
+// Reversible-jump MCMC chains on gfx950: one wavefront per ROI chain,
+// persistent work queue over ROIs.
+//
+// Chain semantics follow class RJMCMC (source/c/LibRJMCMC.cpp:657-1736):
+// the same jump set (move with its four sub-moves, split, merge, birth,
+// death, backFore), the same uniform draw order (urand/randn,
+// LibRJMCMCreturn.hpp:17-26), the same proposal arithmetic in single
+// precision without contraction, the same prior-table lookups and the same
+// posterior-ratio factors, formed in the reference's order.
+//
+// gfx950 mapping
+//   - pixel p of the ROI (column-major, y fastest) lives in lane p % 64, slot
+//     p / 64: data, current model M and proposed model M' stay in VGPRs;
+//   - the emitter sets (current and proposed) are double-buffered in LDS;
+//     accepting a jump swaps the buffers;
+//   - a proposal changes M by a short list of emitter stencils
+//     (+new / -old): lanes 0..31 evaluate the x stencil, lanes 32..63 the y
+//     stencil (two erf per lane), and every lane updates its pixels;
+//   - the log-likelihood ratio sum(M - M' + D log(M'/M)) is reduced across
+//     the wavefront with lane shuffles; the full model is rebuilt from the
+//     emitter set after every `recompute_every` accepted jumps so that the
+//     incremental updates cannot drift;
+//   - every jump draws its uniforms from one counter-based Philox4x32-10
+//     batch: lane i holds draws 4i..4i+3 of the step, and draw j is read with
+//     readlane (replay mode loads the same batch layout from the tape);
+//   - accept: double-precision test log(U) < log PR with log PR summed from
+//     the reference's single-precision factors; when the reference's own
+//     single-precision PR is saturated (inf, 0, NaN or beyond 1e+-30) the
+//     reference's test U < PR decides, so overflow semantics are kept.
+#include <cmath>
+
+#include "common.hpp"
+#include "rjmcmc_dev.hpp"
+
+#pragma clang fp contract(off)
+
+namespace bamf {
+namespace rj {
+namespace {
+
+constexpr int kMaxDelta = 2 * kNMax + 8;
+constexpr double kPiD = 3.141592;  // LibRJMCMCreturn.hpp:2 (a double literal)
+
+enum { C_N = 0, C_A = 5, C_NFG = 10, C_AFG = 15, C_NBG = 20, C_ABG = 25 };
+enum { J_MOVE = 0, J_SPLIT = 1, J_MERGE = 2, J_BIRTH = 3, J_DEATH = 4 };
+enum { ST_OK = 0, ST_NMAX = 1, ST_UNSUPPORTED = 2, ST_TAPE = 3 };
+
+struct Em {
+    float I[kNMax];
+    float X[kNMax];
+    float Y[kNMax];
+    int S[kNMax];
+};
+
+struct WaveLds {
+    Em e[2];
+    uint32_t draw[256];  // the step's draw batch: draw j at [j & 255]
+    int delta[kMaxDelta];
+    float sten[64];
+    int nb[kNMax + 1];
+    int hist[kNMax + 1];
+    int cnt[32];
+};
+
+__device__ __forceinline__ void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ int bcasti(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float bcastf(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int kth_bit(uint64_t m, int k) {
+    for (int i = 0; i < k; i++) m &= m - 1;
+    return __builtin_ctzll(m);
+}
+
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                       uint32_t k1, uint32_t* o) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    o[0] = c0;
+    o[1] = c1;
+    o[2] = c2;
+    o[3] = c3;
+}
+
+// Fills the wave's draw batch b of step `step`: Philox block (step, 64b + lane)
+// gives draws 256b + 4 lane .. +3; in replay mode the same positions of the tape.
+__device__ __noinline__ void refill_batch(uint32_t* draw, int lane, int mode, int step, int b, uint32_t key,
+                                          uint32_t seed_lo, uint32_t seed_hi, const uint32_t* tape, int64_t tpos,
+                                          int64_t tend) {
+    uint32_t o[4];
+    if (mode == BAMF_RJ_REPLAY) {
+        const int64_t base = tpos + (int64_t)b * 256 + 4 * lane;
+#pragma unroll
+        for (int c = 0; c < 4; c++) o[c] = (base + c < tend) ? tape[base + c] : 0u;
+    } else {
+        philox((uint32_t)step, (uint32_t)(b * 64 + lane), seed_hi, 0u, key, seed_lo, o);
+#pragma unroll
+        for (int c = 0; c < 4; c++) o[c] >>= 1;  // 31-bit draws, the range of rand()
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) draw[4 * lane + c] = o[c];
+    wsync();
+}
+
+// LibRJMCMCreturn.hpp:28
+__device__ __forceinline__ int kround(float v) { return (int)floor((double)v + 0.5); }
+
+// LibRJMCMC.cpp:219-246 (first x[j] > I, searched from the grid guess I/dx).
+__device__ __noinline__ float findpdf(const StatDev s, float dx, float I) {
+    if (!(I > 0.0f) || I >= s.x[s.len - 1]) return 0.0f;  // NaN also yields 0, as the reference does
+    int j = (int)(I / dx);
+    j = j < 1 ? 1 : (j > s.len - 1 ? s.len - 1 : j);
+    while (j > 1 && s.x[j - 1] > I) j--;
+    while (j < s.len - 1 && s.x[j] <= I) j++;
+    return (s.pdf[j] * (I - s.x[j - 1]) + s.pdf[j - 1] * (s.x[j] - I)) / (s.x[1] - s.x[0]);
+}
+
+// LibRJMCMC.cpp:248-282 with the uniform already drawn.
+__device__ __noinline__ float genrand(const StatDev s, float u) {
+    if (u <= 0) return 0;
+    if (u >= 1) return s.ux[s.nu - 1];
+    if (s.nu < 2) return 0;
+    int lo = 0, hi = s.nu - 1;  // uc[lo] <= u < uc[hi]; uc strictly increasing, uc[0] = 0, uc[nu-1] = 1
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s.uc[mid] - u > 0) hi = mid;
+        else lo = mid;
+    }
+    return (s.ux[hi - 1] * (s.uc[hi] - u) + s.ux[hi] * (u - s.uc[hi - 1])) / (s.uc[hi] - s.uc[hi - 1]);
+}
+
+// LibRJMCMCreturn.hpp:19-21 (sqrt(2*pi) evaluated in double, then single).
+__device__ __noinline__ float normpdf(float x, float u, float s) {
+    const float a = (float)sqrt(2.0f * kPiD);
+    const float e = expf(-(x - u) * (x - u) / (2 * s * s));
+    return 1.0f / (a * s) * e;
+}
+
+__device__ __noinline__ double log_d(float v) { return log((double)v); }
+
+// Proposal kinds that need the likelihood ratio, with their accept counters.
+enum { EV_NONE = 0, EV_MOVE, EV_SPLIT, EV_MERGE, EV_BIRTH, EV_DEATH };
+
+struct Chain {
+    const ChainArgs& a;
+    WaveLds& L;
+    const int lane;
+    int roi, X, Y, P, ppl;
+    float C, meanN;
+    int bnd;
+    float xb, yb;  // Xsize + Bnd_out, Ysize + Bnd_out
+    int ci;        // current emitter buffer
+    int cN, tN;
+    float cBG, cABG, cBBG, tBG, tABG, tBBG;
+    float d[kMaxPPL], m[kMaxPPL], mt[kMaxPPL];
+    uint32_t adr[kMaxPPL];  // x | y << 8 of each slot's pixel; 0xffff when the slot is empty
+    // draws
+    int dbatch, dj;
+    uint32_t key;
+    int step;
+    int64_t tbase, tlen, tcur;
+    // phase
+    float P1, P2, P3, P4;
+    bool burn;
+    int status;
+    int nacc;
+    double logpost;
+    int ndelta;
+    bool bgdelta;
+    // proposal bookkeeping for the shared evaluation
+    int ev, jt, acc_c1, acc_c2;
+    bool need_test;
+    float f_pri, f_u1, f_u2, f_u3, f_ijp, f_pixprob;
+    int f_jj, f_jm;
+    // step record
+    int r_jt, r_flags, r_nprop, r_pix;
+    float r_llr, r_pr;
+    double r_logpr;
+    int ref_flags, ref_pix;
+
+    __device__ Chain(const ChainArgs& a_, WaveLds& L_, int lane_) : a(a_), L(L_), lane(lane_) {}
+
+    __device__ __forceinline__ Em& cur() { return L.e[ci]; }
+    __device__ __forceinline__ Em& tst() { return L.e[ci ^ 1]; }
+    __device__ __forceinline__ bool slot_ok(int s) const { return (adr[s] & 0xffffu) != 0xffffu; }
+    __device__ __forceinline__ int sx(int s) const { return adr[s] & 0xffu; }
+    __device__ __forceinline__ int sy(int s) const { return (adr[s] >> 8) & 0xffu; }
+
+    // ---------------- draws ----------------
+    __device__ __forceinline__ uint32_t raw() {
+        const int j = dj++;
+        const int b = j >> 8;
+        if (b != dbatch) {
+            refill_batch(L.draw, lane, a.mode, step, b, key, a.seed_lo, a.seed_hi, a.tape, tbase + tcur,
+                         tbase + tlen);
+            dbatch = b;
+        }
+        if (a.mode == BAMF_RJ_REPLAY && tcur + j >= tlen) status = ST_TAPE;
+        return L.draw[j & 255];
+    }
+    __device__ __forceinline__ float urand() { return (float)raw() / 2147483648.0f; }
+    __device__ __forceinline__ float randn() {
+        float t = 0;
+        for (int i = 0; i < 12; i++) t = t + urand();
+        return t - 6.0f;
+    }
+    __device__ __forceinline__ void cnt_inc(int k) {
+        if (lane == 0) L.cnt[k]++;
+    }
+
+    // ---------------- emitter sets ----------------
+    __device__ __forceinline__ void copy_to_test() {
+        Em& c = cur();
+        Em& t = tst();
+        for (int n = lane; n < cN; n += 64) {
+            t.I[n] = c.I[n];
+            t.X[n] = c.X[n];
+            t.Y[n] = c.Y[n];
+            t.S[n] = c.S[n];
+        }
+        tN = cN;
+        tBG = cBG;
+        tABG = cABG;
+        tBBG = cBBG;
+        ndelta = 0;
+        bgdelta = false;
+        wsync();
+    }
+    __device__ __forceinline__ void set_t(int k, float I, float x, float y) {
+        if (lane == 0) {
+            tst().I[k] = I;
+            tst().X[k] = x;
+            tst().Y[k] = y;
+        }
+    }
+    __device__ __forceinline__ bool add_t(float I, float x, float y, int s) {
+        if (tN >= kNMax) {
+            status = ST_NMAX;
+            return false;
+        }
+        if (lane == 0) {
+            tst().I[tN] = I;
+            tst().X[tN] = x;
+            tst().Y[tN] = y;
+            tst().S[tN] = s;
+        }
+        tN++;
+        wsync();
+        return true;
+    }
+    __device__ __forceinline__ void remove_t(int id) {
+        Em& t = tst();
+        wsync();
+        float vi[2], vx[2], vy[2];
+        int vs[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int n = lane + 64 * h;
+            if (n >= id && n < tN - 1) {
+                vi[h] = t.I[n + 1];
+                vx[h] = t.X[n + 1];
+                vy[h] = t.Y[n + 1];
+                vs[h] = t.S[n + 1];
+            }
+        }
+        wsync();
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int n = lane + 64 * h;
+            if (n >= id && n < tN - 1) {
+                t.I[n] = vi[h];
+                t.X[n] = vx[h];
+                t.Y[n] = vy[h];
+                t.S[n] = vs[h];
+            }
+        }
+        tN--;
+        wsync();
+    }
+    __device__ __forceinline__ void delta_push(int which, int k, bool neg) {
+        if (lane == 0) L.delta[ndelta] = k | (which << 8) | ((neg ? 1 : 0) << 9);
+        ndelta++;
+    }
+    // Emitters::test (LibRJMCMC.cpp:492-514) on the proposed set.
+    __device__ __forceinline__ bool test_t() {
+        const Em& t = tst();
+        bool bad = false;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int n = lane + 64 * h;
+            if (n < tN) {
+                const float I = t.I[n], x = t.X[n], y = t.Y[n];
+                bad |= (I < 0) || (x < (float)(-bnd)) || (y < (float)(-bnd)) || (x > (float)(X + bnd)) ||
+                       (y > (float)(Y + bnd));
+            }
+        }
+        return ballot(bad) == 0;
+    }
+    __device__ __forceinline__ int nem_cur() {
+        const Em& c = cur();
+        int k = 0;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int n = lane + 64 * h;
+            k += __popcll(ballot(n < cN && c.S[n] == 1));
+        }
+        return k;
+    }
+    // Neighbours of emitter jj in the current set closer than dmin, as two
+    // 64-bit index masks.  Distance as closest()/closestMulti() compute it:
+    // float differences, squared and summed in double, sqrt in double, then float.
+    __device__ __forceinline__ void near_mask(int jj, float dmin, uint64_t& m0, uint64_t& m1) {
+        const Em& c = cur();
+        const float xj = c.X[jj], yj = c.Y[jj];
+        bool in[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int n = lane + 64 * h;
+            in[h] = false;
+            if (n < cN && n != jj) {
+                const double dx = (double)(c.X[n] - xj), dy = (double)(c.Y[n] - yj);
+                const float dd = (float)sqrt(dx * dx + dy * dy);
+                in[h] = dd < dmin;
+            }
+        }
+        m0 = ballot(in[0]);
+        m1 = ballot(in[1]);
+    }
+    __device__ __forceinline__ int pick(uint64_t m0, uint64_t m1, int k) {
+        const int c0 = __popcll(m0);
+        return k < c0 ? kth_bit(m0, k) : 64 + kth_bit(m1, k - c0);
+    }
+    // LibRJMCMC.cpp:525-558
+    __device__ __forceinline__ int closest(int jj, float dmin) {
+        uint64_t m0, m1;
+        near_mask(jj, dmin, m0, m1);
+        const int num = __popcll(m0) + __popcll(m1);
+        if (num > 0) {
+            const float Ua = urand();
+            const int mm = min((int)(Ua * num), num - 1);
+            return pick(m0, m1, mm);
+        }
+        const float Ua = urand();
+        const int mm = min((int)(Ua * (cN - 1)), cN - 2);
+        return mm < jj ? mm : mm + 1;
+    }
+    // LibRJMCMC.cpp:615-650: writes the chosen indices to L.nb, returns how many.
+    __device__ __forceinline__ int closest_multi(int jj, float dmin) {
+        uint64_t m0, m1;
+        near_mask(jj, dmin, m0, m1);
+        const int num = __popcll(m0) + __popcll(m1);
+        int np = 0;
+        if (num > 0) {
+            float Ua = urand();
+            if (Ua < 1.0f / (float)num) np = num;
+            else np = min((int)(Ua * num), num - 1);
+            for (int n = 0; n < np; n++) {
+                Ua = urand();
+                const int k = min((int)(Ua * (num - n)), num - n - 1);
+                const int idx = pick(m0, m1, k);
+                if (idx < 64) m0 &= ~(1ull << idx);
+                else m1 &= ~(1ull << (idx - 64));
+                if (lane == 0) L.nb[n] = idx;
+            }
+        }
+        wsync();
+        return np;
+    }
+
+    // ---------------- model ----------------
+    // Stencil of one emitter (LibRJMCMC.cpp:15-36): lanes 0..X-1 hold
+    // I*0.5*(erf((x-X+1)C) - erf((x-X)C)), lanes 32..32+Y-1 the y factor.
+    __device__ __forceinline__ void stencil(float I, float xe, float ye) {
+        float v = 0.0f;
+        if (lane < X) {
+            const float xx = (float)lane;
+            v = I * 0.5f * (erff((xx - xe + 1.0f) * C) - erff((xx - xe) * C));
+        } else if (lane >= 32 && lane - 32 < Y) {
+            const float yy = (float)(lane - 32);
+            v = 0.5f * (erff((yy - ye + 1.0f) * C) - erff((yy - ye) * C));
+        }
+        L.sten[lane] = v;
+        wsync();
+    }
+    // Full model of the current set: background plane + sCMOS term, then the
+    // emitters in order (Model::Model, LibRJMCMC.cpp:42-59).
+    __device__ __forceinline__ void rebuild_model() {
+        const int64_t off = a.pix_off[roi];
+#pragma unroll
+        for (int s = 0; s < kMaxPPL; s++)
+            if (s < ppl && slot_ok(s)) {
+                const float sc = a.scmos ? a.scmos[off + lane + 64 * s] : 0.0f;
+                m[s] = cBG + cABG * (float)sx(s) + cBBG * (float)sy(s) + sc;
+            }
+        const Em& c = cur();
+        for (int n = 0; n < cN; n++) {
+            stencil(c.I[n], c.X[n], c.Y[n]);
+#pragma unroll
+            for (int s = 0; s < kMaxPPL; s++)
+                if (s < ppl && slot_ok(s)) m[s] = m[s] + L.sten[sx(s)] * L.sten[32 + sy(s)];
+            wsync();
+        }
+        nacc = 0;
+    }
+    // M' = M + background change + the stencil list; returns the sum over
+    // pixels of M - M' + D log(M'/M) (likelihood_ratio, LibRJMCMC.cpp:1677-1678).
+    __device__ __forceinline__ float proposal_llr_sum() {
+#pragma unroll
+        for (int s = 0; s < kMaxPPL; s++) mt[s] = m[s];
+        if (bgdelta) {
+            const float db = tBG - cBG, da = tABG - cABG, dbb = tBBG - cBBG;
+#pragma unroll
+            for (int s = 0; s < kMaxPPL; s++)
+                if (s < ppl && slot_ok(s)) mt[s] = mt[s] + (db + da * (float)sx(s) + dbb * (float)sy(s));
+        }
+        wsync();
+        for (int e = 0; e < ndelta; e++) {
+            const int code = L.delta[e];
+            const Em& E = ((code >> 8) & 1) ? tst() : cur();
+            const int k = code & 255;
+            const bool neg = (code >> 9) & 1;
+            stencil(E.I[k], E.X[k], E.Y[k]);
+#pragma unroll
+            for (int s = 0; s < kMaxPPL; s++)
+                if (s < ppl && slot_ok(s)) {
+                    const float pr = L.sten[sx(s)] * L.sten[32 + sy(s)];
+                    mt[s] = neg ? mt[s] - pr : mt[s] + pr;
+                }
+            wsync();
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int s = 0; s < kMaxPPL; s++)
+            if (s < ppl && slot_ok(s)) acc = acc + ((m[s] - mt[s]) + d[s] * logf(mt[s] / m[s]));
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        return bcastf(acc, 0);
+    }
+
+    // ---------------- accept ----------------
+    __device__ __forceinline__ bool decide(float pr, double logpr) {
+        const float U = urand();
+        bool acc;
+        if (isfinite(pr) && pr > 1e-30f && pr < 1e30f) {
+            acc = log((double)U) < logpr;
+        } else {
+            acc = U < pr;
+            r_flags |= 16;
+        }
+        r_pr = pr;
+        r_logpr = logpr;
+        if (acc) r_flags |= 1;
+        if (a.mode == BAMF_RJ_REPLAY) {
+            const bool forced = (ref_flags & 1) != 0;
+            if (forced != acc) r_flags |= 32;
+            acc = forced;
+        }
+        return acc;
+    }
+    __device__ __forceinline__ void record() {
+        r_jt = jt;
+        r_flags |= 2;
+        r_nprop = tN;
+    }
+    // update_grids (LibRJMCMC.cpp:1629-1648)
+    __device__ __forceinline__ void update_grids() {
+        r_flags |= 4;
+        if (burn || !a.pgrid) return;
+        const Em& c = cur();
+        const int z = a.p.grid_zoom;
+        const int64_t go = a.grid_off[roi];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int n = lane + 64 * h;
+            if (n < cN) {
+                const int x = kround((float)z * (c.X[n] + a.p.drift_x));
+                const int y = kround((float)z * (c.Y[n] + a.p.drift_y));
+                const float i = c.I[n];
+                if (x >= 0 && x < X * z && y >= 0 && y < Y * z && i > a.p.icutoff) {
+                    float* g = c.S[n] == 1 ? a.pgrid : a.pback;
+                    atomicAdd(&g[go + (int64_t)x * Y * z + y], 1.0f);
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ float pdf_of(const Em& e, int k) {
+        return e.S[k] == 1 ? findpdf(a.sig, a.p.dx, e.I[k]) : findpdf(a.back, a.p.dx, e.I[k]);
+    }
+
+    // ---------------- proposals ----------------
+    // Each prop_* follows the reference jump up to the likelihood ratio and
+    // returns the evaluation kind (EV_NONE when the jump ended early).
+    // LibRJMCMC.cpp:752-916
+    __device__ __forceinline__ int prop_move() {
+        const bamf_mcmc_params& p = a.p;
+        jt = 1;
+        float PrI = 1, IPDF = 1, IPDFtest = 1;
+        cnt_inc(C_N + J_MOVE);
+        copy_to_test();
+        if (cN < 1) return EV_NONE;
+        const Em& E = cur();
+        const Em& T = tst();
+        const float Ua = urand();
+        const int nn = min((int)(Ua * tN), tN - 1);
+        const float Ub = urand();
+        if (Ub < 0.25) {
+            jt = 2;
+            tBG = tBG + p.bg_std * randn();
+            tABG = tABG + p.abbg_std * randn();
+            tBBG = tBBG + p.abbg_std * randn();
+            bgdelta = true;
+        } else if (Ub < 0.50 && Ub >= 0.25) {
+            if (cN < 2) return EV_NONE;
+            jt = 10;
+            const int np = closest_multi(nn, 2 * p.psf_sigma);
+            if (np == 0) return EV_NONE;
+            float I0 = E.I[nn];
+            for (int ii = 0; ii < np; ii++) I0 = I0 + E.I[L.nb[ii]];
+            float II0 = 0, XI0 = 0, YI0 = 0, XII0 = 0, YII0 = 0;
+            for (int ii = 0; ii < np; ii++) {
+                const int k = L.nb[ii];
+                const bool fg = E.S[k] == 1;
+                const float xs = fg ? p.x_std_fg : p.x_std_bg;
+                const float tx = E.X[k] + xs * randn();
+                const float ty = E.Y[k] + xs * randn();
+                const float ti = (I0 / (float)(np + 1)) * (0.15f * randn() + 1);
+                IPDFtest = findpdf(fg ? a.sig : a.back, p.dx, ti);
+                IPDF = findpdf(fg ? a.sig : a.back, p.dx, E.I[k]);
+                set_t(k, ti, tx, ty);
+                PrI = PrI * IPDFtest / IPDF;
+                XI0 = XI0 + E.I[k] * E.X[k];
+                YI0 = YI0 + E.I[k] * E.Y[k];
+                XII0 = XII0 + ti * tx;
+                YII0 = YII0 + ti * ty;
+                II0 = II0 + ti;
+                delta_push(0, k, true);
+                delta_push(1, k, false);
+            }
+            const float tin = I0 - II0;
+            const float txn = (XI0 + E.I[nn] * E.X[nn] - XII0) / tin;
+            const float tyn = (YI0 + E.I[nn] * E.Y[nn] - YII0) / tin;
+            set_t(nn, tin, txn, tyn);
+            const bool fg = E.S[nn] == 1;
+            IPDFtest = findpdf(fg ? a.sig : a.back, p.dx, tin);
+            IPDF = findpdf(fg ? a.sig : a.back, p.dx, E.I[nn]);
+            PrI = PrI * IPDFtest / IPDF;
+            delta_push(0, nn, true);
+            delta_push(1, nn, false);
+        } else if (Ub >= 0.5 && Ub < 0.75) {
+            jt = 3;
+            int np = closest_multi(nn, 2 * p.psf_sigma);
+            if (lane == 0) L.nb[np] = nn;
+            np++;
+            wsync();
+            for (int ii = 0; ii < np; ii++) {
+                const int k = L.nb[ii];
+                const bool fg = E.S[k] == 1;
+                cnt_inc(fg ? C_NFG + J_MOVE : C_NBG + J_MOVE);
+                const float is = fg ? p.i_std_fg : p.i_std_bg, xs = fg ? p.x_std_fg : p.x_std_bg;
+                const float ti = T.I[k] + is * randn();
+                const float tx = T.X[k] + xs * randn();
+                const float ty = T.Y[k] + xs * randn();
+                IPDFtest = findpdf(fg ? a.sig : a.back, p.dx, ti);
+                IPDF = findpdf(fg ? a.sig : a.back, p.dx, E.I[k]);
+                wsync();
+                set_t(k, ti, tx, ty);
+                wsync();
+                PrI = PrI * IPDFtest / IPDF;
+                delta_push(0, k, true);
+                delta_push(1, k, false);
+            }
+        } else {
+            const bool fg = E.S[nn] == 1;
+            cnt_inc(fg ? C_NFG + J_MOVE : C_NBG + J_MOVE);
+            const float is = fg ? p.i_std_fg : p.i_std_bg, xs = fg ? p.x_std_fg : p.x_std_bg;
+            const float ti = T.I[nn] + is * randn();
+            const float tx = T.X[nn] + xs * randn();
+            const float ty = T.Y[nn] + xs * randn();
+            IPDFtest = findpdf(fg ? a.sig : a.back, p.dx, ti);
+            IPDF = findpdf(fg ? a.sig : a.back, p.dx, E.I[nn]);
+            set_t(nn, ti, tx, ty);
+            PrI = IPDFtest / IPDF;
+            delta_push(0, nn, true);
+            delta_push(1, nn, false);
+        }
+        wsync();
+        f_pri = PrI;
+        acc_c1 = C_A + J_MOVE;
+        acc_c2 = E.S[nn] == 1 ? C_AFG + J_MOVE : C_ABG + J_MOVE;
+        need_test = true;
+        return EV_MOVE;
+    }
+    // LibRJMCMC.cpp:918-936
+    __device__ __forceinline__ float fin_move(float sum, float LLR, double& logpr) {
+        const float BgPDF = findpdf(a.off, a.p.dx, cBG);
+        const float BgPDFtest = findpdf(a.off, a.p.dx, tBG);
+        const float PrBg = (BgPDFtest / BgPDF) * expf(-(tBBG * tBBG - cBBG * cBBG) / 2) *
+                           expf(-(tABG * tABG - cABG * cABG) / 2);
+        logpr = (double)sum + log_d(f_pri) + log_d(PrBg);
+        return LLR * f_pri * PrBg;
+    }
+
+    // LibRJMCMC.cpp:972-1028
+    __device__ __forceinline__ int prop_split() {
+        const bamf_mcmc_params& p = a.p;
+        jt = 4;
+        cnt_inc(C_N + J_SPLIT);
+        copy_to_test();
+        if (cN < 1) return EV_NONE;
+        const Em& E = cur();
+        const float Ua = urand();
+        const int jj = min((int)floorf((float)cN * Ua), cN - 1);
+        const float Ijp = E.I[jj], mux = E.X[jj], muy = E.Y[jj];
+        const int BF = E.S[jj];
+        cnt_inc(BF == 1 ? C_NFG + J_SPLIT : C_NBG + J_SPLIT);
+        const float u1 = urand();
+        const float u2 = p.split_std * randn();
+        const float u3 = p.split_std * randn();
+        set_t(jj, u1 * Ijp, mux + u2, muy + u3);
+        if (!add_t((1 - u1) * Ijp, mux - u1 * u2 / (1 - u1), muy - u1 * u3 / (1 - u1), BF)) return EV_NONE;
+        delta_push(0, jj, true);
+        delta_push(1, jj, false);
+        delta_push(1, tN - 1, false);
+        f_jj = jj;
+        f_u1 = u1;
+        f_u2 = u2;
+        f_u3 = u3;
+        f_ijp = Ijp;
+        acc_c1 = C_A + J_SPLIT;
+        acc_c2 = BF == 1 ? C_AFG + J_SPLIT : C_ABG + J_SPLIT;
+        need_test = true;
+        return EV_SPLIT;
+    }
+    // LibRJMCMC.cpp:1030-1070
+    __device__ __forceinline__ float fin_split(float sum, float LLR, double& logpr) {
+        const bamf_mcmc_params& p = a.p;
+        const Em& E = cur();
+        const Em& T = tst();
+        float tmp1 = 1, tmp2 = 1;
+        tmp1 = tmp1 * pdf_of(E, f_jj);
+        tmp2 = tmp2 * pdf_of(T, f_jj);
+        tmp2 = tmp2 * pdf_of(T, tN - 1);
+        const float Norm1 = normpdf(f_u2, 0, p.split_std);
+        const float Norm2 = normpdf(f_u3, 0, p.split_std);
+        const float NEm = (float)nem_cur();
+        const float PrM = meanN / (NEm + 1);
+        const float PrI = tmp2 / tmp1;
+        const float Area = xb * yb;
+        const float PrX = 1 / Area;
+        const float Us = 1 / (Norm1 * Norm2);
+        const double om = (double)(1 - f_u1);
+        const float Jacob = (float)((double)f_ijp / (om * om));
+        const float P12 = P1 / P2;
+        logpr = (double)sum + log_d(PrI) + log_d(PrX) + log_d(Us) + log_d(Jacob) + log_d(P12) + log_d(PrM);
+        return LLR * PrI * PrX * Us * Jacob * P12 * PrM;
+    }
+
+    // LibRJMCMC.cpp:1111-1164
+    __device__ __forceinline__ int prop_merge() {
+        const bamf_mcmc_params& p = a.p;
+        jt = 5;
+        cnt_inc(C_N + J_MERGE);
+        copy_to_test();
+        if (cN < 2) {
+            update_grids();
+            return EV_NONE;
+        }
+        const Em& E = cur();
+        const float Ua = urand();
+        const int jj = min((int)floorf((float)cN * Ua), cN - 1);
+        const int ii = closest(jj, 3 * p.psf_sigma);
+        cnt_inc(E.S[jj] == 1 ? C_NFG + J_MERGE : C_NBG + J_MERGE);
+        {
+            const double dx = (double)(E.X[ii] - E.X[jj]), dy = (double)(E.Y[ii] - E.Y[jj]);
+            const float dd = (float)(dx * dx + dy * dy);
+            if (sqrtf(dd) > 3 * p.psf_sigma) {
+                update_grids();
+                return EV_NONE;
+            }
+        }
+        const float Ijp = E.I[jj] + E.I[ii];
+        const float muxjp = (E.I[jj] * E.X[jj] + E.I[ii] * E.X[ii]) / Ijp;
+        const float muyjp = (E.I[jj] * E.Y[jj] + E.I[ii] * E.Y[ii]) / Ijp;
+        f_u1 = E.I[jj] / Ijp;
+        f_u2 = E.X[jj] - muxjp;
+        f_u3 = E.Y[jj] - muyjp;
+        f_ijp = Ijp;
+        const float PDFfg = findpdf(a.sig, p.dx, Ijp);
+        const float PDFbg = findpdf(a.back, p.dx, Ijp);
+        set_t(jj, Ijp, muxjp, muyjp);
+        if (lane == 0) tst().S[jj] = (PDFbg > PDFfg) ? 0 : 1;
+        remove_t(ii);
+        f_jj = jj;
+        f_jm = ii > jj ? jj : jj - 1;  // the merged emitter's index in the proposed set
+        L.nb[0] = ii;                   // keep ii for the prior ratio (same value from every lane)
+        delta_push(0, jj, true);
+        delta_push(0, ii, true);
+        delta_push(1, f_jm, false);
+        acc_c1 = C_A + J_MERGE;
+        acc_c2 = E.S[jj] == 1 ? C_AFG + J_MERGE : C_ABG + J_MERGE;
+        need_test = true;
+        wsync();
+        return EV_MERGE;
+    }
+    // LibRJMCMC.cpp:1166-1217
+    __device__ __forceinline__ float fin_merge(float sum, float LLR, double& logpr) {
+        const bamf_mcmc_params& p = a.p;
+        const Em& E = cur();
+        const Em& T = tst();
+        const int ii = L.nb[0];
+        float tmp1 = 1, tmp2 = 1;
+        tmp1 = tmp1 * pdf_of(E, f_jj);
+        tmp1 = tmp1 * pdf_of(E, ii);
+        tmp2 = tmp2 * pdf_of(T, f_jm);
+        const float Norm1 = normpdf(f_u2, 0, p.split_std);
+        const float Norm2 = normpdf(f_u3, 0, p.split_std);
+        const float NEm = (float)nem_cur();
+        const float PrI = tmp1 / tmp2;
+        const float Area = xb * yb;
+        const float PrX = 1 / Area;
+        const float Us = 1 / (Norm1 * Norm2);
+        const double om = (double)(1 - f_u1);
+        const float Jacob = (float)((double)f_ijp / (om * om));
+        const float PrM = meanN / NEm;
+        logpr = (double)sum -
+                (log_d(PrI) + log_d(PrX) + log_d(Jacob) + log_d(Us) + log_d(PrM) + log_d(P1) - log_d(P2));
+        return LLR / (PrI * PrX * Jacob * Us * PrM * P1 / P2);
+    }
+
+    // find_Birth (LibRJMCMC.cpp:1688-1736): residual max(0, D - M), prefix
+    // sum in pixel order, r = total * U, first pixel with r <= cumulative.
+    __device__ __forceinline__ int birth_pixel(float u) {
+        float carry = 0.0f;
+        float cum[kMaxPPL];
+#pragma unroll
+        for (int s = 0; s < kMaxPPL; s++) {
+            cum[s] = 0.0f;
+            if (s < ppl) {
+                float v = 0.0f;
+                if (slot_ok(s)) {
+                    const float r = d[s] - m[s];
+                    v = (0 > r) ? 0.0f : r;
+                }
+                for (int o = 1; o < 64; o <<= 1) {
+                    const float t = __shfl_up(v, o);
+                    if (lane >= o) v += t;
+                }
+                cum[s] = carry + v;
+                carry = bcastf(cum[s], 63);
+            }
+        }
+        const float rr = carry * u;
+        int pix = P;
+#pragma unroll
+        for (int s = 0; s < kMaxPPL; s++) {
+            if (s < ppl && pix == P) {
+                const uint64_t mk = ballot(slot_ok(s) && rr <= cum[s]);
+                if (mk) pix = 64 * s + __builtin_ctzll(mk);
+            }
+        }
+        return pix;
+    }
+    // Learned proposal: pixel drawn from the per-ROI proposal map q; returns
+    // the pixel and its probability q[pix] / sum(q).
+    __device__ __forceinline__ int map_pixel(float u, float& prob) {
+        const int64_t off = a.pix_off[roi];
+        float carry = 0.0f;
+        float cum[kMaxPPL], q[kMaxPPL];
+#pragma unroll
+        for (int s = 0; s < kMaxPPL; s++) {
+            cum[s] = 0.0f;
+            q[s] = 0.0f;
+            if (s < ppl) {
+                float v = slot_ok(s) ? a.prop_map[off + lane + 64 * s] : 0.0f;
+                q[s] = v;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const float t = __shfl_up(v, o);
+                    if (lane >= o) v += t;
+                }
+                cum[s] = carry + v;
+                carry = bcastf(cum[s], 63);
+            }
+        }
+        const float rr = carry * u;
+        int pix = P - 1;
+        float qp = 0.0f;
+        bool found = false;
+#pragma unroll
+        for (int s = 0; s < kMaxPPL; s++) {
+            if (s < ppl && !found) {
+                const uint64_t mk = ballot(slot_ok(s) && rr <= cum[s] && q[s] > 0.0f);
+                if (mk) {
+                    const int l = __builtin_ctzll(mk);
+                    pix = 64 * s + l;
+                    qp = bcastf(q[s], l);
+                    found = true;
+                }
+            }
+        }
+        prob = carry > 0.0f ? qp / carry : 0.0f;
+        return pix;
+    }
+
+    // LibRJMCMC.cpp:1258-1295
+    __device__ __forceinline__ int prop_birth() {
+        const bamf_mcmc_params& p = a.p;
+        jt = 6;
+        cnt_inc(C_N + J_BIRTH);
+        copy_to_test();
+        const float ur = urand();
+        float pixprob = 1.0f;
+        int pix;
+        if (a.mode == BAMF_RJ_LEARNED) pix = map_pixel(ur, pixprob);
+        else pix = birth_pixel(ur);
+        r_pix = pix;
+        if (a.mode == BAMF_RJ_REPLAY) pix = ref_pix;
+        const float y = (float)(pix - Y * (pix / Y)) + urand() - 0.5f;
+        const float x = (float)(int)floorf(((float)pix) / Y) + urand() - 0.5f;
+        int BF = 1;
+        if (p.is_backg == 1) BF = kround(urand());
+        cnt_inc(BF == 1 ? C_NFG + J_BIRTH : C_NBG + J_BIRTH);
+        const float I_new = genrand(BF == 1 ? a.sig : a.back, urand());
+        if (!add_t(I_new, x, y, BF)) return EV_NONE;
+        delta_push(1, tN - 1, false);
+        f_pixprob = pixprob;
+        acc_c1 = C_A + J_BIRTH;
+        acc_c2 = BF == 1 ? C_AFG + J_BIRTH : C_ABG + J_BIRTH;
+        need_test = true;
+        return EV_BIRTH;
+    }
+    // LibRJMCMC.cpp:1297-1302
+    __device__ __forceinline__ float fin_birth(float sum, float LLR, double& logpr) {
+        const float NEm = (float)nem_cur();
+        const float PrM = meanN / (NEm + 1);
+        logpr = (double)sum + log_d(P3) - (log_d(P4) + log_d(xb) + log_d(yb) + log_d(f_pixprob)) + log_d(PrM);
+        return LLR * P3 / (P4 * xb * yb * f_pixprob) * PrM;
+    }
+
+    // LibRJMCMC.cpp:1332-1355
+    __device__ __forceinline__ int prop_death() {
+        jt = 7;
+        cnt_inc(C_N + J_DEATH);
+        copy_to_test();
+        if (cN < 1) return EV_NONE;
+        const Em& E = cur();
+        const float Ua = urand();
+        const int jj = min((int)floorf((float)cN * Ua), cN - 1);
+        const int sj = E.S[jj];
+        cnt_inc(sj == 1 ? C_NFG + J_DEATH : C_NBG + J_DEATH);
+        remove_t(jj);
+        delta_push(0, jj, true);
+        acc_c1 = C_A + J_DEATH;
+        acc_c2 = sj == 1 ? C_AFG + J_DEATH : C_ABG + J_DEATH;
+        need_test = false;  // death has no bounds test
+        return EV_DEATH;
+    }
+    // LibRJMCMC.cpp:1356-1360
+    __device__ __forceinline__ float fin_death(float sum, float LLR, double& logpr) {
+        const float NEm = (float)nem_cur();
+        const float PrM = NEm / meanN;
+        const float P43 = P4 / P3;
+        logpr = (double)sum + log_d(P43) + log_d(PrM);
+        return LLR * P43 * PrM;
+    }
+
+    // LibRJMCMC.cpp:1586-1627 (changes the current set in place, no record)
+    __device__ __forceinline__ void do_backfore() {
+        if (cN == 0) return;
+        if (a.p.is_backg == 0) return;
+        Em& E = cur();
+        const float Ua = urand();
+        const int nn = min((int)floorf((float)cN * Ua), cN - 1);
+        const float BackP = findpdf(a.back, a.p.dx, E.I[nn]);
+        const float SignalP = findpdf(a.sig, a.p.dx, E.I[nn]);
+        int s = E.S[nn];
+        if (s == 1) {
+            if (SignalP != 0) {
+                if (BackP / SignalP > urand()) s = 0;
+            } else {
+                s = 0;
+            }
+        } else {
+            if (BackP != 0) {
+                if (SignalP / BackP > urand()) s = 1;
+            } else {
+                s = 1;
+            }
+        }
+        wsync();
+        if (lane == 0) E.S[nn] = s;
+        wsync();
+    }
+
+    __device__ __forceinline__ int kind_of(float rtmp, const float* PV, const float* PVB) {
+        if (rtmp < PV[0]) return 0;
+        if (rtmp >= PV[0] && rtmp < PV[1]) return 1;
+        if (rtmp >= PV[1] && rtmp < PV[2]) return 2;
+        if (rtmp >= PV[2] && rtmp < PV[3]) return 3;
+        if (rtmp >= PV[3] && rtmp < PV[4]) return 4;
+        if (rtmp >= PVB[4] && rtmp < PVB[5]) return 5;
+        if (rtmp >= PVB[5] && rtmp < PVB[6]) return 6;
+        return 7;
+    }
+
+    // One jump: proposal, then (when the jump reaches it) the shared
+    // likelihood evaluation, the posterior ratio, the accept test, the
+    // records and the grid update, in the reference's order.
+    __device__ __forceinline__ int one_step() {
+        const float rtmp = urand();
+        const int kind = burn ? kind_of(rtmp, a.pvb, a.pvb) : kind_of(rtmp, a.pvt, a.pvb);
+        int evk = EV_NONE;
+        switch (kind) {
+            case 0: evk = prop_move(); break;
+            case 1: evk = prop_split(); break;
+            case 2: evk = prop_merge(); break;
+            case 3: evk = prop_birth(); break;
+            case 4: evk = prop_death(); break;
+            case 7: do_backfore(); break;
+            default: status = ST_UNSUPPORTED; break;
+        }
+        if (evk != EV_NONE && status == ST_OK) {
+            if (need_test && !test_t()) {
+                r_flags |= 8;
+            } else {
+                const float sum = proposal_llr_sum();
+                const float LLR = expf(sum);
+                r_llr = LLR;
+                double logpr = 0.0;
+                float PR;
+                switch (evk) {
+                    case EV_MOVE: PR = fin_move(sum, LLR, logpr); break;
+                    case EV_SPLIT: PR = fin_split(sum, LLR, logpr); break;
+                    case EV_MERGE: PR = fin_merge(sum, LLR, logpr); break;
+                    case EV_BIRTH: PR = fin_birth(sum, LLR, logpr); break;
+                    default: PR = fin_death(sum, LLR, logpr); break;
+                }
+                if (decide(PR, logpr)) {
+                    cnt_inc(acc_c1);
+                    cnt_inc(acc_c2);
+                    ci ^= 1;
+                    cN = tN;
+                    cBG = tBG;
+                    cABG = tABG;
+                    cBBG = tBBG;
+#pragma unroll
+                    for (int s = 0; s < kMaxPPL; s++) m[s] = mt[s];
+                    logpost += logpr;
+                    if (++nacc >= a.recompute_every) {
+                        wsync();
+                        rebuild_model();
+                    }
+                }
+            }
+            record();
+            update_grids();
+        }
+        return kind;
+    }
+
+    __device__ __forceinline__ float total_photons_seq() {
+        // sequential single-precision sum (the oracle's order) for step records
+        float ph = 0.0f;
+        const Em& c = cur();
+        for (int k = 0; k < cN; k++) ph = ph + c.I[k];
+        return ph;
+    }
+
+    __device__ __forceinline__ void run(int roi_) {
+        roi = roi_;
+        const bamf_mcmc_params& p = a.p;
+        X = a.xs[roi];
+        Y = a.ys[roi];
+        P = X * Y;
+        ppl = (P + 63) >> 6;
+        C = 1 / (p.psf_sigma * sqrtf(2.0f));
+        const float rho = a.rho ? a.rho[roi] : p.rho;
+        meanN = rho * X * Y;  // cRJMCMC.cpp:116
+        bnd = (int)p.bnd_out;
+        xb = X + p.bnd_out;
+        yb = Y + p.bnd_out;
+        key = a.key ? a.key[roi] : (uint32_t)roi;
+        status = ST_OK;
+        logpost = 0.0;
+        const int64_t off = a.pix_off[roi];
+#pragma unroll
+        for (int s = 0; s < kMaxPPL; s++) {
+            const int pp = lane + 64 * s;
+            const bool ok = s < ppl && pp < P;
+            const int xx = ok ? pp / Y : 0;
+            adr[s] = ok ? (uint32_t)(xx | ((pp - xx * Y) << 8)) : 0xffffu;
+            d[s] = ok ? a.data[off + pp] : 0.0f;
+            m[s] = 1.0f;
+            mt[s] = 1.0f;
+        }
+        // counters start at N = 1, A = 0 (LibRJMCMC.cpp:672-677)
+        if (lane < 32) L.cnt[lane] = (lane < 30 && ((lane / 5) % 2 == 0)) ? 1 : 0;
+        for (int k = lane; k < kNMax + 1; k += 64) L.hist[k] = 0;
+        // initial state (cRJMCMC.cpp:120-133)
+        ci = 0;
+        cN = a.n0 ? min(a.n0[roi], kNMax) : 0;
+        if (a.n0) {
+            const int64_t eo = a.em0_off[roi];
+            for (int n = lane; n < cN; n += 64) {
+                L.e[0].I[n] = a.i0[eo + n];
+                L.e[0].X[n] = a.x0[eo + n];
+                L.e[0].Y[n] = a.y0[eo + n];
+                L.e[0].S[n] = 1;
+            }
+        }
+        const float bg0 = a.bg0 ? a.bg0[roi] : 0.0f;
+        cBG = bg0 == 0 ? 1.0f : bg0;
+        cABG = 0;
+        cBBG = 0;
+        wsync();
+        rebuild_model();
+        if (a.mode == BAMF_RJ_REPLAY) {
+            tbase = a.tape_off[roi];
+            tlen = a.tape_len[roi];
+        } else {
+            tbase = 0;
+            tlen = 0;
+        }
+        tcur = 0;
+        const int nb = p.n_burnin, total = p.n_burnin + p.n_trials;
+        const int64_t rstride = total;
+        for (int s = 0; s < total && status == ST_OK; s++) {
+            step = s;
+            burn = s < nb;
+            if (burn) {
+                P1 = p.p_burnin[1]; P2 = p.p_burnin[2]; P3 = p.p_burnin[3]; P4 = p.p_burnin[4];
+            } else {
+                P1 = a.pvt[1]; P2 = a.pvt[2]; P3 = a.pvt[3]; P4 = a.pvt[4];
+            }
+            dj = 0;
+            dbatch = -1;
+            r_jt = 0;
+            r_flags = 0;
+            r_pix = -1;
+            r_llr = 0;
+            r_pr = 0;
+            r_logpr = __builtin_nan("");
+            r_nprop = -1;
+            if (a.mode == BAMF_RJ_REPLAY) {
+                const bamf_step_record& rr = a.ref_steps[roi * rstride + s];
+                ref_flags = rr.flags;
+                ref_pix = rr.pix;
+            }
+            const int kind = one_step();
+            wsync();
+            if (a.steps) {
+                const float ph = total_photons_seq();
+                if (lane == 0) {
+                    bamf_step_record& o = a.steps[roi * rstride + s];
+                    o.kind = kind;
+                    o.jump_type = r_jt;
+                    o.flags = r_flags;
+                    o.n = cN;
+                    o.llr = r_llr;
+                    o.pr = r_pr;
+                    o.bg = cBG;
+                    o.photons = ph;
+                    o.log_pr = r_logpr;
+                    o.draws = dj;
+                    o.n_prop = r_nprop < 0 ? cN : r_nprop;
+                    o.pix = r_pix;
+                    o.aux = 0;
+                }
+            }
+            if (!burn) {
+                if (lane == 0) L.hist[min(cN, kNMax)]++;
+                const int t = s - nb;
+                if (a.trace_n && t % a.trace_thin == 0 && t / a.trace_thin < a.trace_len) {
+                    const Em& c = cur();
+                    float v = (lane < cN ? c.I[lane] : 0.0f) + (lane + 64 < cN ? c.I[lane + 64] : 0.0f);
+                    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                    v = bcastf(v, 0);
+                    if (lane == 0) {
+                        const int64_t ti = (int64_t)roi * a.trace_len + t / a.trace_thin;
+                        a.trace_n[ti] = (int16_t)cN;
+                        a.trace_ph[ti] = v;
+                    }
+                }
+            }
+            tcur += dj;
+        }
+        wsync();
+        // ---- outputs ----
+        if (a.status && lane == 0) a.status[roi] = status;
+        if (a.counters && lane < 30) a.counters[(int64_t)roi * 30 + lane] = L.cnt[lane];
+        if (a.logpost && lane == 0) a.logpost[roi] = logpost;
+        if (a.hist) {
+            const int hl = a.hist_len;
+            for (int k = lane; k < hl; k += 64) {
+                int v = 0;
+                if (k < hl - 1) {
+                    v = k <= kNMax ? L.hist[k] : 0;
+                } else {
+                    for (int q = k; q <= kNMax; q++) v += L.hist[q];
+                }
+                a.hist[(int64_t)roi * hl + k] = v;
+            }
+        }
+        if (a.fin) {
+            float* f = a.fin + (int64_t)roi * (4 + 4 * a.fin_cap);
+            if (lane == 0) {
+                f[0] = (float)cN;
+                f[1] = cBG;
+                f[2] = cABG;
+                f[3] = cBBG;
+            }
+            const Em& c = cur();
+            for (int n = lane; n < cN && n < a.fin_cap; n += 64) {
+                f[4 + n] = c.I[n];
+                f[4 + a.fin_cap + n] = c.X[n];
+                f[4 + 2 * a.fin_cap + n] = c.Y[n];
+                f[4 + 3 * a.fin_cap + n] = (float)c.S[n];
+            }
+        }
+        wsync();
+    }
+};
+
+__global__ __launch_bounds__(64 * kWavesPerBlock) void k_chain(ChainArgs a) {
+    __shared__ WaveLds lds[kWavesPerBlock];
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    Chain ch(a, lds[w], lane);
+    while (true) {
+        int qi = 0;
+        if (lane == 0) qi = (int)atomicAdd(a.queue, 1u);
+        qi = bcasti(qi, 0);
+        if (qi >= a.n_roi) break;
+        const int roi = a.order ? a.order[qi] : qi;
+        ch.run(roi);
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* p) { p[0] = 0u; }
+
+}  // namespace
+
+int launch_chain(const ChainArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, a.queue);
+    BAMF_LAUNCH_CHECK("k_zero_u32");
+    hipLaunchKernelGGL(k_chain, dim3(grid), dim3(64 * kWavesPerBlock), 0, s, a);
+    BAMF_LAUNCH_CHECK("k_chain");
+    return BAMF_OK;
+}
+
+int chain_occupancy_blocks(int* blocks_per_cu) {
+    int n = 0;
+    BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chain, 64 * kWavesPerBlock, 0));
+    *blocks_per_cu = n;
+    return BAMF_OK;
+}
+
+}  // namespace rj
+}  // namespace bamf

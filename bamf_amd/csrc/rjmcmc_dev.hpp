@@ -1,0 +1,77 @@
+// Device-side argument block of the chain kernel (shared by rjmcmc.hip and
+// the host launcher in rjmcmc_host.cpp).  Not part of the public ABI.
+#pragma once
+#include <cstdint>
+
+#include "../../include/bamf_gpu.h"
+
+namespace bamf {
+namespace rj {
+
+constexpr int kNMax = 128;       // emitters per chain (LDS state capacity)
+constexpr int kWavesPerBlock = 4;
+constexpr int kMaxPPL = 16;      // pixels per lane: ROIs up to 32 x 32
+constexpr int kMaxSide = 32;
+
+// One prior table (Stat, LibRJMCMC.cpp:176-217) on the device.
+struct StatDev {
+    const float* x;    // Xarray (0, dx, 2dx, ... by repeated addition)
+    const float* pdf;  // PDFarray
+    const float* ux;   // UniqXarray
+    const float* uc;   // UniqCDFarray
+    int32_t len;
+    int32_t nu;
+};
+
+struct ChainArgs {
+    bamf_mcmc_params p;
+    float pvb[8];  // cumulative P_Burnin (make_Params)
+    float pvt[8];  // cumulative P_Trials
+    StatDev off, sig, back;
+    int32_t mode;
+    uint32_t seed_lo, seed_hi;
+    // batch
+    int32_t n_roi;
+    const int64_t* pix_off;
+    const int32_t* xs;
+    const int32_t* ys;
+    const float* data;
+    const float* scmos;
+    const float* rho;
+    const float* bg0;
+    const int32_t* n0;
+    const int64_t* em0_off;
+    const float* i0;
+    const float* x0;
+    const float* y0;
+    const uint32_t* key;
+    const int32_t* order;
+    uint32_t* queue;  // work-queue counter (reset before launch)
+    // replay
+    const uint32_t* tape;
+    const int64_t* tape_off;
+    const int64_t* tape_len;
+    const bamf_step_record* ref_steps;
+    // learned birth proposal: per-ROI per-pixel probabilities (same layout as data)
+    const float* prop_map;
+    // outputs
+    int32_t* status;
+    int32_t* counters;
+    float* fin;
+    int32_t fin_cap;
+    int32_t* hist;
+    int32_t hist_len;
+    double* logpost;
+    int16_t* trace_n;
+    float* trace_ph;
+    int32_t trace_len;
+    int32_t trace_thin;
+    bamf_step_record* steps;
+    float* pgrid;
+    float* pback;
+    const int64_t* grid_off;
+    int32_t recompute_every;  // full model rebuild after this many accepted jumps
+};
+
+}  // namespace rj
+}  // namespace bamf

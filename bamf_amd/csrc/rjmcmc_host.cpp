@@ -1,0 +1,230 @@
+// Host side of the chain ABI: prior tables, argument checks, launch sizing.
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "rjmcmc_dev.hpp"
+
+namespace bamf {
+namespace rj {
+int launch_chain(const ChainArgs& a, int grid, hipStream_t s);
+int chain_occupancy_blocks(int* blocks_per_cu);
+}  // namespace rj
+}  // namespace bamf
+
+using namespace bamf;
+using namespace bamf::rj;
+
+namespace {
+
+// Stat::Stat (LibRJMCMC.cpp:176-217): trapezoid CDF clamped at 1, Xarray by
+// repeated addition of DX, and the strictly increasing unique-CDF arrays with
+// the first entry forced to 0 and the last to 1.
+struct HostStat {
+    std::vector<float> x, pdf, ux, uc;
+    void build(const float* p, int len, float dx) {
+        pdf.assign(p, p + len);
+        std::vector<float> cdf(len), xm(len);
+        x.assign(len, 0.0f);
+        cdf[0] = 0;
+        xm[0] = 0;
+        for (int ii = 1; ii < len; ii++) {
+            cdf[ii] = cdf[ii - 1] + (pdf[ii] + pdf[ii - 1]) * dx / 2.0f;
+            if (cdf[ii] > 1.0f) cdf[ii] = 1.0f;
+            xm[ii] = cdf[ii];
+            x[ii] = x[ii - 1] + dx;
+        }
+        for (int ii = len - 2; ii >= 0; ii--)
+            if (cdf[ii] == cdf[ii + 1]) xm[ii + 1] = -1;
+        ux.clear();
+        uc.clear();
+        for (int ii = 0; ii < len; ii++)
+            if (xm[ii] != -1) {
+                ux.push_back(x[ii]);
+                uc.push_back(cdf[ii]);
+            }
+        if (uc.back() != 1) uc.back() = 1;
+        if (uc.front() != 0) uc.front() = 0;
+    }
+    size_t floats() const { return x.size() + pdf.size() + ux.size() + uc.size(); }
+};
+
+}  // namespace
+
+struct bamf_rjmcmc_plan {
+    bamf_mcmc_params params;
+    float pvb[8], pvt[8];
+    HostStat off, sig, back;
+    float* d_tables = nullptr;
+    uint32_t* d_queue = nullptr;
+    StatDev s_off, s_sig, s_back;
+    int device = 0;
+    int cu = 0;
+    int blocks_per_cu = 0;
+};
+
+extern "C" {
+
+int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_pdf, int off_len,
+                            const float* h_sig_pdf, int sig_len, const float* h_back_pdf, int back_len,
+                            bamf_rjmcmc_plan** plan) {
+    BAMF_CHECK_ARG(params && plan, "null argument");
+    BAMF_CHECK_ARG(h_off_pdf && h_sig_pdf && h_back_pdf, "prior tables are required");
+    BAMF_CHECK_ARG(off_len >= 1 && sig_len >= 1 && back_len >= 1, "prior tables must be non-empty");
+    BAMF_CHECK_ARG(params->dx > 0, "dx must be > 0");
+    BAMF_CHECK_ARG(params->psf_sigma > 0, "psf_sigma must be > 0");
+    BAMF_CHECK_ARG(params->n_burnin >= 0 && params->n_trials >= 0, "negative chain length");
+    BAMF_CHECK_ARG(params->grid_zoom >= 1, "grid_zoom must be >= 1");
+    auto* pl = new bamf_rjmcmc_plan();
+    pl->params = *params;
+    // make_Params (cRJMCMC.cpp:37-47): cumulative sums formed from the start for each entry
+    for (int ii = 0; ii < 8; ii++) {
+        float b = 0, t = 0;
+        for (int mm = 0; mm <= ii; mm++) {
+            b = b + params->p_burnin[mm];
+            t = t + params->p_trials[mm];
+        }
+        pl->pvb[ii] = b;
+        pl->pvt[ii] = t;
+    }
+    pl->off.build(h_off_pdf, off_len, params->dx);
+    pl->sig.build(h_sig_pdf, sig_len, params->dx);
+    pl->back.build(h_back_pdf, back_len, params->dx);
+    std::vector<float> flat;
+    auto put = [&](const std::vector<float>& v) {
+        size_t at = flat.size();
+        flat.insert(flat.end(), v.begin(), v.end());
+        return at;
+    };
+    size_t o[12];
+    const HostStat* ss[3] = {&pl->off, &pl->sig, &pl->back};
+    for (int k = 0; k < 3; k++) {
+        o[4 * k + 0] = put(ss[k]->x);
+        o[4 * k + 1] = put(ss[k]->pdf);
+        o[4 * k + 2] = put(ss[k]->ux);
+        o[4 * k + 3] = put(ss[k]->uc);
+    }
+    hipError_t e = hipGetDevice(&pl->device);
+    if (e == hipSuccess) e = hipMalloc(&pl->d_tables, flat.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(pl->d_tables, flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&pl->d_queue, 256);
+    if (e != hipSuccess) {
+        set_error("plan device setup failed: %s", hipGetErrorString(e));
+        if (pl->d_tables) (void)hipFree(pl->d_tables);
+        if (pl->d_queue) (void)hipFree(pl->d_queue);
+        delete pl;
+        return BAMF_E_HIP;
+    }
+    StatDev* sd[3] = {&pl->s_off, &pl->s_sig, &pl->s_back};
+    for (int k = 0; k < 3; k++) {
+        sd[k]->x = pl->d_tables + o[4 * k + 0];
+        sd[k]->pdf = pl->d_tables + o[4 * k + 1];
+        sd[k]->ux = pl->d_tables + o[4 * k + 2];
+        sd[k]->uc = pl->d_tables + o[4 * k + 3];
+        sd[k]->len = (int)ss[k]->x.size();
+        sd[k]->nu = (int)ss[k]->ux.size();
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, pl->device) == hipSuccess) pl->cu = prop.multiProcessorCount;
+    int bpc = 0;
+    if (chain_occupancy_blocks(&bpc) == BAMF_OK) pl->blocks_per_cu = bpc;
+    *plan = pl;
+    return BAMF_OK;
+}
+
+int bamf_rjmcmc_plan_destroy(bamf_rjmcmc_plan* plan) {
+    if (!plan) return BAMF_OK;
+    if (plan->d_tables) (void)hipFree(plan->d_tables);
+    if (plan->d_queue) (void)hipFree(plan->d_queue);
+    delete plan;
+    return BAMF_OK;
+}
+
+int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed) {
+    BAMF_CHECK_ARG(plan && claimed, "null argument");
+    uint32_t v = 0;
+    BAMF_HIP_TRY(hipMemcpy(&v, plan->d_queue, 4, hipMemcpyDeviceToHost));
+    *claimed = v;
+    return BAMF_OK;
+}
+
+int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_t seed, int mode,
+                    const bamf_replay* replay, const bamf_rjmcmc_out* out, void* stream) {
+    BAMF_CHECK_ARG(plan && batch && out, "null argument");
+    BAMF_CHECK_ARG(mode == BAMF_RJ_PLAIN || mode == BAMF_RJ_REPLAY, "unknown mode %d", mode);
+    BAMF_CHECK_ARG(batch->n_roi >= 0, "n_roi must be >= 0");
+    if (batch->n_roi == 0) return BAMF_OK;
+    BAMF_CHECK_ARG(batch->d_pix_offset && batch->d_xsize && batch->d_ysize && batch->d_data,
+                   "batch pixel arrays are required");
+    BAMF_CHECK_ARG(!batch->d_n0 || (batch->d_em0_offset && batch->d_i0 && batch->d_x0 && batch->d_y0),
+                   "initial emitters need offsets and I/X/Y");
+    if (mode == BAMF_RJ_REPLAY)
+        BAMF_CHECK_ARG(replay && replay->d_tape && replay->d_tape_offset && replay->d_tape_len && replay->d_ref_steps,
+                       "replay mode needs the tape and the recorded steps");
+    BAMF_CHECK_ARG(!out->d_final || out->final_cap >= 0, "final_cap must be >= 0");
+    BAMF_CHECK_ARG(!out->d_n_hist || out->hist_len >= 1, "hist_len must be >= 1");
+    BAMF_CHECK_ARG(!out->d_trace_n || (out->d_trace_photons && out->trace_len >= 0 && out->trace_thin >= 1),
+                   "trace needs photons, trace_len >= 0 and trace_thin >= 1");
+    BAMF_CHECK_ARG(!out->d_pgrid || (out->d_pback && out->d_grid_offset), "grids need pback and offsets");
+    const bamf_mcmc_params& p = plan->params;
+    for (int k = 5; k < 7; k++)
+        BAMF_CHECK_ARG(p.p_burnin[k] == 0.0f && p.p_trials[k] == 0.0f,
+                       "gSplit/gMerge (probability entries 5 and 6) are not implemented");
+    ChainArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.p = p;
+    std::memcpy(a.pvb, plan->pvb, sizeof(a.pvb));
+    std::memcpy(a.pvt, plan->pvt, sizeof(a.pvt));
+    a.off = plan->s_off;
+    a.sig = plan->s_sig;
+    a.back = plan->s_back;
+    a.mode = mode;
+    a.seed_lo = (uint32_t)seed;
+    a.seed_hi = (uint32_t)(seed >> 32);
+    a.n_roi = batch->n_roi;
+    a.pix_off = batch->d_pix_offset;
+    a.xs = batch->d_xsize;
+    a.ys = batch->d_ysize;
+    a.data = batch->d_data;
+    a.scmos = batch->d_scmos;
+    a.rho = batch->d_rho;
+    a.bg0 = batch->d_bg0;
+    a.n0 = batch->d_n0;
+    a.em0_off = batch->d_em0_offset;
+    a.i0 = batch->d_i0;
+    a.x0 = batch->d_x0;
+    a.y0 = batch->d_y0;
+    a.key = batch->d_key;
+    a.order = batch->d_order;
+    a.queue = plan->d_queue;
+    if (mode == BAMF_RJ_REPLAY) {
+        a.tape = replay->d_tape;
+        a.tape_off = replay->d_tape_offset;
+        a.tape_len = replay->d_tape_len;
+        a.ref_steps = replay->d_ref_steps;
+    }
+    a.status = out->d_status;
+    a.counters = out->d_counters;
+    a.fin = out->d_final;
+    a.fin_cap = out->final_cap;
+    a.hist = out->d_n_hist;
+    a.hist_len = out->hist_len;
+    a.logpost = out->d_logpost;
+    a.trace_n = out->d_trace_n;
+    a.trace_ph = out->d_trace_photons;
+    a.trace_len = out->trace_len;
+    a.trace_thin = out->trace_thin > 0 ? out->trace_thin : 1;
+    a.steps = out->d_steps;
+    a.pgrid = out->d_pgrid;
+    a.pback = out->d_pback;
+    a.grid_off = out->d_grid_offset;
+    a.recompute_every = 128;
+    const int bpc = plan->blocks_per_cu > 0 ? plan->blocks_per_cu : 1;
+    const int cu = plan->cu > 0 ? plan->cu : 256;
+    int grid = bpc * cu;
+    const int need = (int)ceil_div(batch->n_roi, kWavesPerBlock);
+    if (grid > need) grid = need;
+    return launch_chain(a, grid, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,254 @@
+"""Host mirror of the chain interfaces: @RJ/rjmcmc.m (one ROI, the cRJMCMC
+mex argument layout) and a batched runner over many ROIs.
+
+All chains run on the device through bamf_rjmcmc_run; there is no host
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _hip
+from ._lib import BamfError, check, lib
+from .rjmcmc_abi import LEARNED, PLAIN, REPLAY, STEP_DTYPE, McmcParams, Replay, RjOut, RoiBatch
+
+COUNTER_NAMES = [f"{k}_{j}" for k in ("N", "A", "NFg", "AFg", "NBg", "ABg")
+                 for j in ("jump", "split", "merge", "birth", "death")]
+
+
+def make_params(psf_sigma=1.3, n_burnin=3000, n_trials=3000, p_burnin=(0.4, 0.15, 0.15, 0.15, 0.15, 0, 0, 0),
+                p_trials=(0.6, 0.1, 0.1, 0.1, 0.1, 0, 0, 0), rho=3 / 256, x_std_fg=0.1, i_std_fg=10.0, bg_std=1.0,
+                split_std=0.5, bnd_out=2.0, abbg_std=0.02, dx=10.0, grid_zoom=20, icutoff=0.0, is_backg=0,
+                drift_x=0.0, drift_y=0.0) -> McmcParams:
+    """The MCMC struct as @RJ/rjmcmc.m:49-78 fills it (I_stdBg = IstdFg/2, X_stdBg = XstdFg/2,
+    Split_std 0.5, Bnd_out 2, ABBG_std 0.02, DX = P_dP)."""
+    p = McmcParams()
+    p.psf_sigma = psf_sigma
+    p.n_burnin = n_burnin
+    p.n_trials = n_trials
+    p.i_std_fg = i_std_fg
+    p.i_std_bg = np.float32(i_std_fg) / np.float32(2)
+    p.x_std_fg = x_std_fg
+    p.x_std_bg = np.float32(x_std_fg) / np.float32(2)
+    for i in range(8):
+        p.p_burnin[i] = p_burnin[i]
+        p.p_trials[i] = p_trials[i]
+    p.rho = rho
+    p.bnd_out = bnd_out
+    p.grid_zoom = grid_zoom
+    p.split_std = split_std
+    p.icutoff = icutoff
+    p.bg_std = bg_std
+    p.abbg_std = abbg_std
+    p.dx = dx
+    p.drift_x = drift_x
+    p.drift_y = drift_y
+    p.is_backg = is_backg
+    return p
+
+
+class Plan:
+    """bamf_rjmcmc_plan: parameters plus the three prior tables on the device."""
+
+    def __init__(self, params: McmcParams, off_pdf, sig_pdf, back_pdf):
+        self.params = params
+        self._tables = [np.ascontiguousarray(t, np.float32) for t in (off_pdf, sig_pdf, back_pdf)]
+        h = ctypes.c_void_p()
+        o, s, b = self._tables
+        check(lib().bamf_rjmcmc_plan_create(ctypes.byref(params), o.ctypes.data, len(o), s.ctypes.data, len(s),
+                                            b.ctypes.data, len(b), ctypes.byref(h)), "bamf_rjmcmc_plan_create")
+        self.handle = h
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().bamf_rjmcmc_plan_destroy(self.handle)
+        except Exception:
+            pass
+
+
+@dataclass
+class BatchResult:
+    status: np.ndarray
+    counters: np.ndarray
+    n_final: np.ndarray
+    final: np.ndarray
+    hist: np.ndarray
+    logpost: np.ndarray
+    trace_n: np.ndarray | None
+    trace_photons: np.ndarray | None
+    steps: np.ndarray | None
+    pgrid: np.ndarray | None = None
+    pback: np.ndarray | None = None
+
+
+class RoiSet:
+    """A device-resident batch of ROIs (bamf_roi_batch)."""
+
+    def __init__(self, data: np.ndarray, offsets: np.ndarray, xsize, ysize, rho=None, bg0=None, keys=None,
+                 order=None, scmos=None, n0=None, em0=None):
+        self.n_roi = len(xsize)
+        self.xsize = np.ascontiguousarray(xsize, np.int32)
+        self.ysize = np.ascontiguousarray(ysize, np.int32)
+        self.offsets = np.ascontiguousarray(offsets[: self.n_roi], np.int64)
+        if np.any(self.xsize < 1) or np.any(self.xsize > 32) or np.any(self.ysize < 1) or np.any(self.ysize > 32):
+            raise BamfError("ROI sides must be in [1, 32]")
+        up = _hip.DeviceBuffer.from_array
+        self.d_data = up(np.ascontiguousarray(data, np.float32))
+        self.d_off = up(self.offsets)
+        self.d_xs = up(self.xsize)
+        self.d_ys = up(self.ysize)
+        self.bufs = {}
+        b = RoiBatch()
+        b.n_roi = self.n_roi
+        b.d_pix_offset, b.d_xsize, b.d_ysize, b.d_data = self.d_off.ptr, self.d_xs.ptr, self.d_ys.ptr, self.d_data.ptr
+        for name, arr, dt in (("d_rho", rho, np.float32), ("d_bg0", bg0, np.float32), ("d_key", keys, np.uint32),
+                              ("d_order", order, np.int32), ("d_scmos", scmos, np.float32)):
+            if arr is not None:
+                self.bufs[name] = up(np.ascontiguousarray(arr, dt))
+                setattr(b, name, self.bufs[name].ptr)
+        if n0 is not None:
+            i0, x0, y0, eoff = em0
+            for name, arr, dt in (("d_n0", n0, np.int32), ("d_em0_offset", eoff, np.int64), ("d_i0", i0, np.float32),
+                                  ("d_x0", x0, np.float32), ("d_y0", y0, np.float32)):
+                self.bufs[name] = up(np.ascontiguousarray(arr, dt))
+                setattr(b, name, self.bufs[name].ptr)
+        self.batch = b
+
+    @classmethod
+    def from_synth(cls, rois: dict, **kw):
+        return cls(rois["data"], rois["offset"], rois["side"], rois["side"], **kw)
+
+
+def lpt_order(xsize: np.ndarray, ysize: np.ndarray, expected_n: np.ndarray | None = None) -> np.ndarray:
+    """Longest-processing-time-first queue order: ROIs sorted by estimated cost
+    (pixels, plus the emitter count's share of stencil work), largest first."""
+    cost = (xsize.astype(np.float64) * ysize)
+    if expected_n is not None:
+        cost = cost * (1.0 + 0.05 * np.asarray(expected_n, np.float64))
+    return np.argsort(-cost, kind="stable").astype(np.int32)
+
+
+def run_batch(plan: Plan, rois: RoiSet, seed: int = 0, mode: int = PLAIN, replay=None, *, hist_len: int = 64,
+              final_cap: int = 128, trace_len: int = 0, trace_thin: int = 1, record_steps: bool = False,
+              grids: bool = False, stream=None, sync: bool = True, outs=None) -> BatchResult | dict:
+    """Runs every chain of the batch; returns host copies of the outputs."""
+    n = rois.n_roi
+    p = plan.params
+    total = p.n_burnin + p.n_trials
+    if outs is None:
+        outs = alloc_outputs(n, total, hist_len, final_cap, trace_len, record_steps, grids, rois, p.grid_zoom)
+    o = outs["struct"]
+    o.trace_thin = trace_thin
+    rp = None
+    if mode == REPLAY:
+        rp = replay["struct"]
+    check(lib().bamf_rjmcmc_run(plan.handle, ctypes.byref(rois.batch), int(seed), int(mode),
+                                ctypes.byref(rp) if rp is not None else None, ctypes.byref(o),
+                                stream.vp if stream else None), "bamf_rjmcmc_run")
+    if not sync:
+        return outs
+    _hip.synchronize()
+    return fetch(outs, n, total, hist_len, final_cap, trace_len)
+
+
+def alloc_outputs(n, total, hist_len, final_cap, trace_len, record_steps, grids, rois, zoom):
+    B = _hip.DeviceBuffer
+    d = {"status": B(4 * n), "counters": B(4 * 30 * n), "final": B(4 * n * (4 + 4 * final_cap)),
+         "hist": B(4 * n * hist_len), "logpost": B(8 * n)}
+    o = RjOut()
+    o.d_status, o.d_counters, o.d_final, o.final_cap = d["status"].ptr, d["counters"].ptr, d["final"].ptr, final_cap
+    o.d_n_hist, o.hist_len, o.d_logpost = d["hist"].ptr, hist_len, d["logpost"].ptr
+    if trace_len > 0:
+        d["trace_n"] = B(2 * n * trace_len)
+        d["trace_ph"] = B(4 * n * trace_len)
+        o.d_trace_n, o.d_trace_photons, o.trace_len = d["trace_n"].ptr, d["trace_ph"].ptr, trace_len
+    if record_steps:
+        d["steps"] = B(STEP_DTYPE.itemsize * n * total)
+        o.d_steps = d["steps"].ptr
+    if grids:
+        sizes = rois.xsize.astype(np.int64) * rois.ysize * zoom * zoom
+        goff = np.zeros(n, np.int64)
+        goff[1:] = np.cumsum(sizes)[:-1]
+        d["pgrid"] = B.zeros(4 * int(sizes.sum()))
+        d["pback"] = B.zeros(4 * int(sizes.sum()))
+        d["goff"] = B.from_array(goff)
+        o.d_pgrid, o.d_pback, o.d_grid_offset = d["pgrid"].ptr, d["pback"].ptr, d["goff"].ptr
+        d["grid_sizes"] = sizes
+        d["grid_off_host"] = goff
+    d["struct"] = o
+    return d
+
+
+def fetch(d, n, total, hist_len, final_cap, trace_len) -> BatchResult:
+    fin = d["final"].download(np.float32).reshape(n, 4 + 4 * final_cap)
+    res = BatchResult(
+        status=d["status"].download(np.int32),
+        counters=d["counters"].download(np.int32).reshape(n, 30),
+        n_final=fin[:, 0].astype(np.int32), final=fin,
+        hist=d["hist"].download(np.int32).reshape(n, hist_len),
+        logpost=d["logpost"].download(np.float64),
+        trace_n=d["trace_n"].download(np.int16).reshape(n, trace_len) if "trace_n" in d else None,
+        trace_photons=d["trace_ph"].download(np.float32).reshape(n, trace_len) if "trace_ph" in d else None,
+        steps=d["steps"].download(STEP_DTYPE).reshape(n, total) if "steps" in d else None)
+    if "pgrid" in d:
+        res.pgrid = d["pgrid"].download(np.float32)
+        res.pback = d["pback"].download(np.float32)
+    return res
+
+
+def make_replay(tapes: list[np.ndarray], ref_steps: np.ndarray) -> dict:
+    """Device replay input from per-ROI raw draw tapes and the recorded step records."""
+    lens = np.array([len(t) for t in tapes], np.int64)
+    offs = np.zeros(len(tapes), np.int64)
+    offs[1:] = np.cumsum(lens)[:-1]
+    flat = np.concatenate(tapes).astype(np.uint32) if len(tapes) else np.zeros(1, np.uint32)
+    ref = np.ascontiguousarray(ref_steps)
+    if ref.dtype != STEP_DTYPE:
+        ref = ref.astype(STEP_DTYPE)
+    B = _hip.DeviceBuffer.from_array
+    d = {"tape": B(flat), "off": B(offs), "len": B(lens), "ref": B(ref)}
+    r = Replay()
+    r.d_tape, r.d_tape_offset, r.d_tape_len, r.d_ref_steps = d["tape"].ptr, d["off"].ptr, d["len"].ptr, d["ref"].ptr
+    d["struct"] = r
+    return d
+
+
+def rjmcmc(data: np.ndarray, rjstruct: dict, ysize: int, xsize: int, drift_y: float = 0.0, drift_x: float = 0.0,
+           scmos_var=None, seed: int = 0):
+    """@RJ/rjmcmc.m for one ROI: builds the MCMC struct and initial state as
+    rjmcmc.m:43-101 does and runs one chain.  Returns (Chain, Stat, PImage, PBackg)
+    where Chain holds the per-step N, BG, LLR, PR, JumpType and Accepted."""
+    r = rjstruct
+    p_off = np.asarray(r["P_Offset"], np.float32)
+    bg0 = np.float32(r["P_dP"] * (int(np.argmax(p_off == p_off.max())) + 1 - 1))
+    params = make_params(psf_sigma=r["PSF_Sigma"], n_burnin=r["N_Burnin"], n_trials=r["N_Trials"],
+                         p_burnin=r["P_Burnin"], p_trials=r["P_Trials"], rho=r["Rho"], x_std_fg=r["XstdFg"],
+                         i_std_fg=r["IstdFg"], bg_std=r["BGstd"], dx=r["P_dP"], grid_zoom=r["SRZoom"],
+                         icutoff=r.get("MinPhoton", 0.0), is_backg=r.get("IsBackg", 0), drift_x=drift_x,
+                         drift_y=drift_y)
+    plan = Plan(params, p_off, r["P_EmPhotons"], r["P_BgEmPhotons"])
+    d = np.asarray(data, np.float32)[:ysize, :xsize]
+    flat = np.asarray(d, order="F").ravel(order="F")  # column-major, y fastest
+    sc = None
+    if scmos_var is not None and np.ndim(scmos_var) == 2:
+        sc = np.asarray(scmos_var, np.float32)[:ysize, :xsize].ravel(order="F")
+    rois = RoiSet(flat, np.array([0, flat.size]), [xsize], [ysize], bg0=[bg0], scmos=sc)
+    res = run_batch(plan, rois, seed=seed, record_steps=True, grids=True, hist_len=129)
+    st = res.steps[0]
+    rec = st[(st["flags"] & 2) != 0]
+    chain = {"N": rec["n"], "BG": rec["bg"], "LLR": rec["llr"], "PR": rec["pr"], "JumpType": rec["jump_type"],
+             "Accepted": rec["flags"] & 1}
+    c = res.counters[0].astype(np.float64)
+    stat = {"NTrials": float(params.n_trials + params.n_burnin)}
+    for i, nm in enumerate(("Jump", "Split", "Merge", "Birth", "Death")):
+        stat[f"Accept_{nm}"] = c[5 + i] / c[i]
+        stat[f"Accept_{nm}Fg"] = c[15 + i] / c[10 + i]
+        stat[f"Accept_{nm}Bg"] = c[25 + i] / c[20 + i]
+    z = params.grid_zoom
+    pimg = res.pgrid.reshape(xsize * z, ysize * z).T.copy()
+    pbg = res.pback.reshape(xsize * z, ysize * z).T.copy()
+    return chain, stat, pimg, pbg

@@ -93,6 +93,126 @@ int bamf_find_roi_crop(const float* d_stack, int xsize, int ysize,
                        const int32_t* d_frame, int64_t n, int boxsize,
                        float* d_roi_stack, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Reversible-jump MCMC chains, one per ROI.
+ * Replaces the cRJMCMC mex entry (source/c/cRJMCMC.cpp:52, driving
+ * RJMCMC::start_Chain, source/c/LibRJMCMC.cpp:685-750) for a whole batch of
+ * ROIs per call.  Chain semantics, draw order and the posterior-ratio
+ * formulas are the reference's; see bamf_rjmcmc_run for the modes.
+ * ------------------------------------------------------------------------ */
+
+/* The MCMC struct fields read by make_Params (cRJMCMC.cpp:14-50).  p_burnin
+ * and p_trials are the raw 8-entry jump probabilities in the reference's
+ * index order [move, split, merge, birth, death, gSplit, gMerge, backFore];
+ * the cumulative vectors are formed inside, as make_Params does. */
+typedef struct bamf_mcmc_params {
+    float psf_sigma;
+    int32_t n_trials;
+    int32_t n_burnin;
+    float i_std_fg, i_std_bg, x_std_fg, x_std_bg;
+    float p_burnin[8];
+    float p_trials[8];
+    float rho;        /* emitters per pixel (MeanN = rho * Xsize * Ysize) */
+    float bnd_out;
+    int32_t grid_zoom;
+    float split_std;
+    float icutoff;
+    float bg_std;
+    float abbg_std;
+    float dx;         /* grid step of the three prior tables */
+    float drift_x, drift_y;
+    int32_t is_backg;
+} bamf_mcmc_params;
+
+/* One record per jump call (burn-in and trials), for replay checks. */
+typedef struct bamf_step_record {
+    int32_t kind;      /* dispatch: 0 move 1 split 2 merge 3 birth 4 death 5 gSplit 6 gMerge 7 backFore */
+    int32_t jump_type; /* the reference's recorded JumpType (1,2,3,10,4,5,6,7), 0 when the step records none */
+    int32_t flags;     /* bit0 accepted, bit1 chain record, bit2 grid update, bit3 bounds test failed,
+                          bit4 decided on the single-precision PR (saturated), bit5 replay: own decision differed */
+    int32_t n;         /* N after the step */
+    float llr, pr;     /* single-precision LLR = exp(sum) and PR, formed as the reference forms them */
+    float bg;          /* BG after the step */
+    float photons;     /* sum of I after the step */
+    double log_pr;     /* log PR in double precision from the single-precision factors */
+    int32_t draws;     /* uniforms consumed by the step */
+    int32_t n_prop;    /* N of the proposed state */
+    int32_t pix;       /* birth: sampled pixel index, else -1 */
+    int32_t aux;       /* reserved */
+} bamf_step_record;
+
+/* A batch of ROIs (device pointers).  ROI r has xsize[r] x ysize[r] pixels
+ * (each 1..32), column-major with y fastest (data[y + x*ysize], the layout of
+ * LibRJMCMC.cpp:52), starting at element pix_offset[r] of d_data. */
+typedef struct bamf_roi_batch {
+    int32_t n_roi;
+    const int64_t* d_pix_offset;
+    const int32_t* d_xsize;
+    const int32_t* d_ysize;
+    const float* d_data;
+    const float* d_scmos;        /* same layout as d_data, or NULL for zeros */
+    const float* d_rho;          /* per-ROI emitters per pixel, or NULL for params.rho */
+    const float* d_bg0;          /* per-ROI initial BG, or NULL for 1 (BG = 0 -> 1 as cRJMCMC.cpp:132) */
+    const int32_t* d_n0;         /* per-ROI initial emitter count, or NULL for 0 */
+    const int64_t* d_em0_offset; /* offsets into d_i0/d_x0/d_y0 when d_n0 is given */
+    const float* d_i0;
+    const float* d_x0;
+    const float* d_y0;
+    const uint32_t* d_key;       /* per-ROI random-stream key, or NULL for the ROI index */
+    const int32_t* d_order;      /* work-queue order (a permutation of 0..n_roi-1), or NULL */
+} bamf_roi_batch;
+
+/* Replay input: each chain consumes its recorded draws and follows the
+ * recorded accept decisions and birth pixels, so its state stays on the
+ * recorded trajectory; its own decisions and log ratios are reported in the
+ * step records for comparison. */
+typedef struct bamf_replay {
+    const uint32_t* d_tape;              /* raw 31-bit draws; U = (float)r / 2^31 */
+    const int64_t* d_tape_offset;        /* [n_roi] */
+    const int64_t* d_tape_len;           /* [n_roi] */
+    const bamf_step_record* d_ref_steps; /* [n_roi][n_burnin + n_trials] */
+} bamf_replay;
+
+/* Outputs (device pointers; NULL skips an output). */
+typedef struct bamf_rjmcmc_out {
+    int32_t* d_status;       /* [n_roi]: 0 ok, 1 emitter cap reached, 2 unsupported jump, 3 replay tape exhausted */
+    int32_t* d_counters;     /* [n_roi][30]: N_* A_* N_*Fg A_*Fg N_*Bg A_*Bg for jump, split, merge, birth, death */
+    float* d_final;          /* [n_roi][4 + 4*final_cap]: N, BG, ABG, BBG, then I, X, Y, Signal per emitter */
+    int32_t final_cap;
+    int32_t* d_n_hist;       /* [n_roi][hist_len]: post-burn-in histogram of N (last bin collects N >= hist_len-1) */
+    int32_t hist_len;
+    double* d_logpost;       /* [n_roi]: sum of the log PR of accepted jumps */
+    int16_t* d_trace_n;      /* [n_roi][trace_len]: N at every trace_thin-th post-burn-in step */
+    float* d_trace_photons;  /* [n_roi][trace_len]: total photons at the same steps */
+    int32_t trace_len;
+    int32_t trace_thin;
+    bamf_step_record* d_steps;   /* [n_roi][n_burnin + n_trials] */
+    float* d_pgrid;          /* zoomed posterior images (update_grids, LibRJMCMC.cpp:1629-1648) */
+    float* d_pback;
+    const int64_t* d_grid_offset; /* [n_roi] offset of ROI r's (ysize*zoom) x (xsize*zoom) grid */
+} bamf_rjmcmc_out;
+
+#define BAMF_RJ_PLAIN 0   /* counter-based Philox draws keyed by (seed, key[r], step) */
+#define BAMF_RJ_REPLAY 1  /* recorded draws and decisions (bamf_replay) */
+#define BAMF_RJ_LEARNED 2 /* PLAIN with birth positions drawn from a proposal map (bamf_rjmcmc_set_proposal) */
+
+typedef struct bamf_rjmcmc_plan bamf_rjmcmc_plan;
+
+/* Builds the three prior tables (Stat, LibRJMCMC.cpp:176-217) from host pdf
+ * arrays on the grid 0, dx, 2dx, ... and keeps them on the current device. */
+int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_pdf, int off_len,
+                            const float* h_sig_pdf, int sig_len, const float* h_back_pdf, int back_len,
+                            bamf_rjmcmc_plan** plan);
+int bamf_rjmcmc_plan_destroy(bamf_rjmcmc_plan* plan);
+/* Runs every chain of the batch to completion in one persistent launch
+ * (work queue over ROIs, one wavefront per chain). */
+int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_t seed, int mode,
+                    const bamf_replay* replay, const bamf_rjmcmc_out* out, void* stream);
+/* The counter that a persistent launch uses as its work queue is reset by
+ * bamf_rjmcmc_run itself; this returns how many chains the last run on the
+ * plan claimed (synchronous). */
+int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,8 @@ typedef struct {
     double log_pr;      /* log PR in double from the single-precision factors; NaN if undefined */
     int32_t draws;      /* uniforms consumed by the step */
     int32_t n_prop;     /* N of the proposed state */
+    int32_t pix;        /* birth: residual-sampled pixel index (find_Birth's ii); else -1 */
+    int32_t pad;
 } orc_step;
 
 enum { ORC_SRC_PHILOX = 0, ORC_SRC_TAPE = 1, ORC_SRC_LIBC = 2 };

@@ -44,8 +44,8 @@ class McmcParams(Structure):
 
 STEP_DTYPE = np.dtype([("kind", "<i4"), ("jump_type", "<i4"), ("flags", "<i4"), ("n", "<i4"),
                        ("llr", "<f4"), ("pr", "<f4"), ("bg", "<f4"), ("photons", "<f4"),
-                       ("log_pr", "<f8"), ("draws", "<i4"), ("n_prop", "<i4")])
-assert STEP_DTYPE.itemsize == 48
+                       ("log_pr", "<f8"), ("draws", "<i4"), ("n_prop", "<i4"), ("pix", "<i4"), ("pad", "<i4")])
+assert STEP_DTYPE.itemsize == 56
 
 
 class Src(Structure):

@@ -597,6 +597,7 @@ static void find_birth(ch_t* c, float* x, float* y, float* pixprob) {
         if (r <= tmp[ii]) break;
     }
     (void)Ind;
+    c->st->pix = ii;
     pixprob[0] = 1;
     y[0] = (float)(ii - c->Y * (ii / c->Y)) + urand(c) - 0.5f;
     x[0] = (float)(int)floorf(((float)ii) / c->Y) + urand(c) - 0.5f;
@@ -743,6 +744,7 @@ int orc_rjmcmc(const float* data, int xsize, int ysize, const float* scmos,
         c->st = out->steps ? &out->steps[s] : &dummy;
         memset(c->st, 0, sizeof(orc_step));
         c->st->log_pr = NAN;
+        c->st->pix = -1;
         c->draws = 0;
         src->step = (uint32_t)s;
         src->j = 0;

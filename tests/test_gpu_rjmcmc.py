@@ -1,0 +1,169 @@
+"""GPU chain (bamf_rjmcmc_run) against the oracle chain.
+
+Replay: the GPU chain consumes the oracle's recorded draws and follows its
+recorded decisions and birth pixels.  The proposals are then computed from
+identical states, so every step must match the oracle exactly in what it
+proposes (kind, JumpType, draws, bounds-test outcome, N, BG, photons, final
+emitters, grids, counters).  The GPU's own accept decisions must equal the
+oracle's except at near-ties, and its double-precision log PR must agree with
+the oracle's within 1e-5 relative, with an absolute floor of 64 single-precision
+ulps of sum|M| over the ROI (the rounding the reference's own single-precision
+sum carries).
+
+Free-running: both use the same Philox stream, so chains agree step for step
+until a near-tie decision differs.
+"""
+import numpy as np
+import pytest
+
+from bamf_amd import synth
+from bamf_amd.rjmcmc import PLAIN, REPLAY, Plan, RoiSet, make_replay, run_batch
+from rj_common import BACK_PDF, OFF_PDF, SIG_PDF, params, run_oracle_chains
+
+pytestmark = pytest.mark.gpu
+
+
+def accept_uniform(tape, steps):
+    """The accept uniform of each step (the last draw) as float."""
+    ends = np.cumsum(steps["draws"])
+    u = np.full(len(steps), np.nan)
+    ok = steps["draws"] > 0
+    u[ok] = tape[ends[ok] - 1].astype(np.float64) / 2147483648.0
+    return u
+
+
+def check_replay(rois, gres, ores, tapes, p, data_sum, stats=None):
+    g = gres.steps
+    n_roi = g.shape[0]
+    nd_total = 0
+    rel_errs = []
+    for r in range(n_roi):
+        o = ores[r]["steps"]
+        gs = g[r]
+        assert gres.status[r] == 0, (r, gres.status[r])
+        for f in ("kind", "jump_type", "n", "draws", "n_prop"):
+            assert np.array_equal(gs[f], o[f]), (r, f, np.flatnonzero(gs[f] != o[f])[:5])
+        assert np.array_equal(gs["bg"].view(np.uint32), o["bg"].view(np.uint32)), r
+        assert np.array_equal(gs["photons"].view(np.uint32), o["photons"].view(np.uint32)), r
+        assert np.array_equal(gs["flags"] & 0b1110, o["flags"] & 0b1110), r
+        births = o["kind"] == 3
+        if births.any():
+            assert np.mean(gs["pix"][births] == o["pix"][births]) >= 0.99
+        lp_o, lp_g = o["log_pr"], gs["log_pr"]
+        # tolerance: 1e-5 relative, floor of 16 single-precision ulps of sum|M| ~ sum D
+        tol = 1e-5 * np.abs(lp_o) + 16 * 2.0 ** -23 * max(float(data_sum(r)), 1.0)
+        fin = np.isfinite(lp_o) & (o["pr"] > 1e-30) & (o["pr"] < 1e30) & np.isfinite(o["pr"])
+        assert np.array_equal(np.isfinite(lp_g[fin]), np.ones(fin.sum(), bool)), r
+        err = np.abs(lp_g[fin] - lp_o[fin])
+        assert np.all(err <= tol[fin]), (r, float(np.max(err - tol[fin])))
+        rel_errs.append(err / np.maximum(np.abs(lp_o[fin]), 1e-300))
+        # own decision vs the oracle's: differences only at near-ties
+        dec = (gs["flags"] & 1) != (o["flags"] & 1)
+        u = accept_uniform(tapes[r], o)
+        for s in np.flatnonzero(dec):
+            nd_total += 1
+            gap = abs(np.log(u[s]) - lp_o[s])
+            assert gap <= tol[s] + 1e-6, (r, s, gap, lp_o[s], lp_g[s])
+        assert np.array_equal(gres.counters[r], ores[r]["counters"]), r
+        n = ores[r]["n"]
+        assert gres.n_final[r] == n
+        cap = (gres.final.shape[1] - 4) // 4
+        fi = gres.final[r]
+        assert np.array_equal(fi[4:4 + n].view(np.uint32), ores[r]["I"].view(np.uint32))
+        assert np.array_equal(fi[4 + cap:4 + cap + n].view(np.uint32), ores[r]["X"].view(np.uint32))
+        assert np.array_equal(fi[4 + 2 * cap:4 + 2 * cap + n].view(np.uint32), ores[r]["Y"].view(np.uint32))
+    e = np.concatenate(rel_errs) if rel_errs else np.zeros(1)
+    print(f"log PR |err|/|log PR|: median {np.median(e):.2e} p99 {np.quantile(e, 0.99):.2e} "
+          f"max {e.max():.2e}; own-decision differences (near-ties): {nd_total}", flush=True)
+    return nd_total
+
+
+def test_replay_config1(oracle):
+    """Config-1 distribution (16x16, N ~ Poisson(3) capped at 10, bg 10), 3000 + 3000 jumps."""
+    rois = synth.make_rois(48, seed=101, sides=(16,), mean_n=3.0, n_cap=10)
+    p = params()
+    ores, tapes = run_oracle_chains(oracle, rois, p, seed=7)
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    rs = RoiSet.from_synth(rois, bg0=np.full(48, 10.0, np.float32))
+    rp = make_replay(tapes, np.stack([o["steps"] for o in ores]))
+    print("replay config1: 48 ROIs", flush=True)
+    g = run_batch(plan, rs, mode=REPLAY, replay=rp, record_steps=True, hist_len=129)
+    check_replay(rois, g, ores, tapes, p, lambda r: rois["data"][rois["offset"][r]:rois["offset"][r + 1]].sum())
+
+
+def test_replay_uneven_sizes_and_background(oracle):
+    """Sides 8..32, non-square ROIs, denser ROIs, initial emitters and the signal/background
+    classification (IsBackg = 1, so backFore and the background prior branches run)."""
+    rng = np.random.default_rng(5)
+    n = 20
+    xs = rng.choice([8, 12, 16, 24, 32], n)
+    ys = rng.choice([8, 12, 20, 32], n)
+    offs = np.zeros(n + 1, np.int64)
+    offs[1:] = np.cumsum(xs * ys)
+    data = np.zeros(offs[-1], np.float32)
+    inits = []
+    for r in range(n):
+        k = rng.poisson(0.05 * xs[r] * ys[r] / 4)
+        X, Y = rng.uniform(0, xs[r], k), rng.uniform(0, ys[r], k)
+        I = rng.gamma(5, 200, k)
+        mu = synth.roi_expected(I, X, Y, 15.0, int(xs[r]), int(ys[r]), 1.3)
+        data[offs[r]:offs[r + 1]] = rng.poisson(mu)
+        m = min(k, 2)
+        inits.append((I[:m].astype(np.float32), X[:m].astype(np.float32) + 0.3, Y[:m].astype(np.float32) - 0.2))
+    rois = {"data": data, "offset": offs, "xsize": xs, "ysize": ys}
+    back = synth.gamma_pdf_table(2, 40, 10.0, 1001)
+    p = params(nb=1500, nt=1500, is_backg=1, p_burnin=(0.3, 0.15, 0.15, 0.15, 0.15, 0, 0, 0.1),
+               p_trials=(0.5, 0.1, 0.1, 0.1, 0.1, 0, 0, 0.1))
+    ores, tapes = run_oracle_chains(oracle, rois, p, seed=11, bg0=15.0, back=back, n0=inits, grids=True)
+    plan = Plan(p, OFF_PDF, SIG_PDF, back)
+    eoff = np.zeros(n, np.int64)
+    n0 = np.array([len(t[0]) for t in inits], np.int32)
+    eoff[1:] = np.cumsum(n0)[:-1]
+    em = (np.concatenate([t[0] for t in inits]), np.concatenate([t[1] for t in inits]),
+          np.concatenate([t[2] for t in inits]), eoff)
+    rs = RoiSet(data, offs, xs, ys, bg0=np.full(n, 15.0, np.float32), n0=n0, em0=em)
+    rp = make_replay(tapes, np.stack([o["steps"] for o in ores]))
+    print("replay uneven: 20 ROIs", flush=True)
+    g = run_batch(plan, rs, mode=REPLAY, replay=rp, record_steps=True, grids=True, hist_len=129)
+    check_replay(rois, g, ores, tapes, p, lambda r: data[offs[r]:offs[r + 1]].sum())
+    off = 0
+    for r in range(n):
+        sz = int(xs[r] * ys[r] * 400)
+        assert np.array_equal(g.pgrid[off:off + sz], ores[r]["pgrid"]), r
+        assert np.array_equal(g.pback[off:off + sz], ores[r]["pback"]), r
+        off += sz
+
+
+def test_free_running_matches_oracle_philox(oracle):
+    """Same Philox stream on both sides: identical trajectories up to the first
+    near-tie; most chains never meet one."""
+    rois = synth.make_rois(32, seed=202, sides=(16,), mean_n=3.0, n_cap=10)
+    p = params(nb=2000, nt=2000)
+    ores, _ = run_oracle_chains(oracle, rois, p, seed=99, record=False)
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    rs = RoiSet.from_synth(rois, bg0=np.full(32, 10.0, np.float32))
+    print("free-running: 32 ROIs", flush=True)
+    g = run_batch(plan, rs, seed=99, mode=PLAIN, record_steps=True, hist_len=129)
+    full = 0
+    for r in range(32):
+        o, gs = ores[r]["steps"], g.steps[r]
+        same = (gs["n"] == o["n"]) & (gs["draws"] == o["draws"]) & (gs["kind"] == o["kind"])
+        if same.all():
+            full += 1
+            assert np.array_equal(g.counters[r], ores[r]["counters"])
+        else:
+            s = int(np.argmin(same))
+            # the first difference is a near-tie accept decision or a birth-pixel tie
+            assert (gs["flags"][s] & 1) != (o["flags"][s] & 1) or gs["pix"][s] != o["pix"][s], (r, s)
+    assert full >= 24, full
+
+
+def test_status_and_counters_shape():
+    rois = synth.make_rois(3, seed=3, sides=(8,), mean_n=1.0)
+    p = params(nb=50, nt=50)
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    g = run_batch(plan, RoiSet.from_synth(rois), seed=1, hist_len=16, trace_len=50)
+    assert np.all(g.status == 0)
+    assert np.all(g.hist.sum(1) == 50)
+    assert np.all(g.counters[:, :5] >= 1)
+    assert g.trace_n.shape == (3, 50)
