@@ -30,6 +30,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "erf_fast.hpp"
 #include "rjmcmc_dev.hpp"
 
 #pragma clang fp contract(off)
@@ -120,44 +121,56 @@ __device__ __noinline__ void refill_batch(uint32_t* draw, int lane, int mode, in
 // LibRJMCMCreturn.hpp:28
 __device__ __forceinline__ int kround(float v) { return (int)floor((double)v + 0.5); }
 
-// LibRJMCMC.cpp:219-246 (first x[j] > I, searched from the grid guess I/dx).
-__device__ __noinline__ float findpdf(const StatDev s, float dx, float I) {
-    if (!(I > 0.0f) || I >= s.x[s.len - 1]) return 0.0f;  // NaN also yields 0, as the reference does
-    int j = (int)(I / dx);
+// LibRJMCMC.cpp:219-246: linear interpolation of the pdf table at I between
+// the first grid point above I and the one before it.  When the grid is exact
+// multiples of dx, that index is floor(I/dx) + 1 corrected by one comparison
+// with the exact product; otherwise it is found by search from the same guess.
+__device__ __forceinline__ float findpdf(const StatDev& s, float dx, float I) {
+    if (!(I > 0.0f) || I >= s.x_last) return 0.0f;  // NaN also yields 0, as the reference does
+    int j = (int)(I / dx) + 1;
     j = j < 1 ? 1 : (j > s.len - 1 ? s.len - 1 : j);
-    while (j > 1 && s.x[j - 1] > I) j--;
-    while (j < s.len - 1 && s.x[j] <= I) j++;
-    return (s.pdf[j] * (I - s.x[j - 1]) + s.pdf[j - 1] * (s.x[j] - I)) / (s.x[1] - s.x[0]);
+    float xj, xj1;
+    if (s.x_exact) {
+        if ((float)(j - 1) * dx > I) j--;
+        else if (j < s.len - 1 && (float)j * dx <= I) j++;
+        xj = (float)j * dx;
+        xj1 = (float)(j - 1) * dx;
+    } else {
+        while (j > 1 && s.x[j - 1] > I) j--;
+        while (j < s.len - 1 && s.x[j] <= I) j++;
+        xj = s.x[j];
+        xj1 = s.x[j - 1];
+    }
+    const float dx0 = s.x_exact ? dx : (s.x[1] - s.x[0]);
+    return (s.pdf[j] * (I - xj1) + s.pdf[j - 1] * (xj - I)) / dx0;
 }
 
-// LibRJMCMC.cpp:248-282 with the uniform already drawn.
-__device__ __noinline__ float genrand(const StatDev s, float u) {
+// LibRJMCMC.cpp:248-282 with the uniform already drawn; the first uc > u is
+// found by a forward scan from the guide entry of u.
+__device__ __forceinline__ float genrand(const StatDev& s, float u) {
     if (u <= 0) return 0;
     if (u >= 1) return s.ux[s.nu - 1];
     if (s.nu < 2) return 0;
-    int lo = 0, hi = s.nu - 1;  // uc[lo] <= u < uc[hi]; uc strictly increasing, uc[0] = 0, uc[nu-1] = 1
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s.uc[mid] - u > 0) hi = mid;
-        else lo = mid;
-    }
+    int g = (int)(u * (float)kGuide);
+    g = g > kGuide - 1 ? kGuide - 1 : g;
+    int hi = s.guide[g] + 1;  // uc[guide[g]] <= g / kGuide <= u
+    while (hi < s.nu - 1 && !(s.uc[hi] - u > 0)) hi++;
     return (s.ux[hi - 1] * (s.uc[hi] - u) + s.ux[hi] * (u - s.uc[hi - 1])) / (s.uc[hi] - s.uc[hi - 1]);
 }
 
 // LibRJMCMCreturn.hpp:19-21 (sqrt(2*pi) evaluated in double, then single).
-__device__ __noinline__ float normpdf(float x, float u, float s) {
+__device__ __forceinline__ float normpdf(float x, float u, float s) {
     const float a = (float)sqrt(2.0f * kPiD);
     const float e = expf(-(x - u) * (x - u) / (2 * s * s));
     return 1.0f / (a * s) * e;
 }
 
-__device__ __noinline__ double log_d(float v) { return log((double)v); }
 
 // Proposal kinds that need the likelihood ratio, with their accept counters.
 enum { EV_NONE = 0, EV_MOVE, EV_SPLIT, EV_MERGE, EV_BIRTH, EV_DEATH };
 
 struct Chain {
-    const ChainArgs& a;
+    const ChainArgs* a;  // the kernel argument block (in the kernarg segment)
     WaveLds& L;
     const int lane;
     int roi, X, Y, P, ppl;
@@ -192,12 +205,19 @@ struct Chain {
     float r_llr, r_pr;
     double r_logpr;
     int ref_flags, ref_pix;
+    uint64_t aflops;  // algorithmic work of this chain
 
-    __device__ Chain(const ChainArgs& a_, WaveLds& L_, int lane_) : a(a_), L(L_), lane(lane_) {}
+    __device__ Chain(const ChainArgs* a_, WaveLds& L_, int lane_) : a(a_), L(L_), lane(lane_) {}
 
     __device__ __forceinline__ Em& cur() { return L.e[ci]; }
     __device__ __forceinline__ Em& tst() { return L.e[ci ^ 1]; }
-    __device__ __forceinline__ bool slot_ok(int s) const { return (adr[s] & 0xffffu) != 0xffffu; }
+    // Pixel p = lane + 64 s exists when p < P; the count is laundered per loop
+    // so the sixteen per-slot masks are recomputed instead of kept live.
+    __device__ __forceinline__ int pix_count() const {
+        int n = P;
+        asm volatile("" : "+v"(n));
+        return n;
+    }
     __device__ __forceinline__ int sx(int s) const { return adr[s] & 0xffu; }
     __device__ __forceinline__ int sy(int s) const { return (adr[s] >> 8) & 0xffu; }
 
@@ -206,11 +226,11 @@ struct Chain {
         const int j = dj++;
         const int b = j >> 8;
         if (b != dbatch) {
-            refill_batch(L.draw, lane, a.mode, step, b, key, a.seed_lo, a.seed_hi, a.tape, tbase + tcur,
+            refill_batch(L.draw, lane, a->mode, step, b, key, a->seed_lo, a->seed_hi, a->tape, tbase + tcur,
                          tbase + tlen);
             dbatch = b;
         }
-        if (a.mode == BAMF_RJ_REPLAY && tcur + j >= tlen) status = ST_TAPE;
+        if (a->mode == BAMF_RJ_REPLAY && tcur + j >= tlen) status = ST_TAPE;
         return L.draw[j & 255];
     }
     __device__ __forceinline__ float urand() { return (float)raw() / 2147483648.0f; }
@@ -383,36 +403,46 @@ struct Chain {
     }
 
     // ---------------- model ----------------
-    // Stencil of one emitter (LibRJMCMC.cpp:15-36): lanes 0..X-1 hold
-    // I*0.5*(erf((x-X+1)C) - erf((x-X)C)), lanes 32..32+Y-1 the y factor.
+    // Stencil of one emitter (LibRJMCMC.cpp:15-36): sten[x] = I*0.5*(erf(e_x+1) - erf(e_x)),
+    // sten[32 + y] = 0.5*(erf(e_y+1) - erf(e_y)) with pixel edges e_k = (k - pos)*C.
+    // Each edge's erf is evaluated once (branch-free, one lane per edge) and
+    // shared with the neighbouring pixel through a lane shuffle.
     __device__ __forceinline__ void stencil(float I, float xe, float ye) {
-        float v = 0.0f;
-        if (lane < X) {
-            const float xx = (float)lane;
-            v = I * 0.5f * (erff((xx - xe + 1.0f) * C) - erff((xx - xe) * C));
-        } else if (lane >= 32 && lane - 32 < Y) {
-            const float yy = (float)(lane - 32);
-            v = 0.5f * (erff((yy - ye + 1.0f) * C) - erff((yy - ye) * C));
+        const float l = (float)lane;
+        if (X + Y + 2 <= 64) {
+            // x edges on lanes 0..X, y edges on lanes X+1..X+Y+1
+            const int by = X + 1;
+            const float arg = lane < by ? (l - xe) * C : ((float)(lane - by) - ye) * C;
+            const float e = erf_nr(arg);
+            const float en = __shfl_down(e, 1);
+            if (lane < X) L.sten[lane] = I * 0.5f * (en - e);
+            else if (lane >= by && lane < by + Y) L.sten[32 + lane - by] = 0.5f * (en - e);
+        } else {
+            const float ex = erf_nr((l - xe) * C), ey = erf_nr((l - ye) * C);
+            const float exn = __shfl_down(ex, 1), eyn = __shfl_down(ey, 1);
+            if (lane < X) L.sten[lane] = I * 0.5f * (exn - ex);
+            if (lane < Y) L.sten[32 + lane] = 0.5f * (eyn - ey);
         }
-        L.sten[lane] = v;
         wsync();
     }
     // Full model of the current set: background plane + sCMOS term, then the
     // emitters in order (Model::Model, LibRJMCMC.cpp:42-59).
     __device__ __forceinline__ void rebuild_model() {
-        const int64_t off = a.pix_off[roi];
+        const int np_ = pix_count();
+        const int64_t off = a->pix_off[roi];
 #pragma unroll
         for (int s = 0; s < kMaxPPL; s++)
-            if (s < ppl && slot_ok(s)) {
-                const float sc = a.scmos ? a.scmos[off + lane + 64 * s] : 0.0f;
+            if (lane + 64 * s < np_) {
+                const float sc = a->scmos ? a->scmos[off + lane + 64 * s] : 0.0f;
                 m[s] = cBG + cABG * (float)sx(s) + cBBG * (float)sy(s) + sc;
             }
         const Em& c = cur();
+        aflops += (uint64_t)(4 * P) + (uint64_t)cN * (uint64_t)(2 * P + 8 * (X + Y));
         for (int n = 0; n < cN; n++) {
             stencil(c.I[n], c.X[n], c.Y[n]);
 #pragma unroll
             for (int s = 0; s < kMaxPPL; s++)
-                if (s < ppl && slot_ok(s)) m[s] = m[s] + L.sten[sx(s)] * L.sten[32 + sy(s)];
+                if (lane + 64 * s < np_) m[s] = m[s] + L.sten[sx(s)] * L.sten[32 + sy(s)];
             wsync();
         }
         nacc = 0;
@@ -420,13 +450,15 @@ struct Chain {
     // M' = M + background change + the stencil list; returns the sum over
     // pixels of M - M' + D log(M'/M) (likelihood_ratio, LibRJMCMC.cpp:1677-1678).
     __device__ __forceinline__ float proposal_llr_sum() {
+        const int np_ = pix_count();
+        aflops += (uint64_t)(6 * P) + (uint64_t)ndelta * (uint64_t)(2 * P + 8 * (X + Y)) + (bgdelta ? 5 * P : 0);
 #pragma unroll
         for (int s = 0; s < kMaxPPL; s++) mt[s] = m[s];
         if (bgdelta) {
             const float db = tBG - cBG, da = tABG - cABG, dbb = tBBG - cBBG;
 #pragma unroll
             for (int s = 0; s < kMaxPPL; s++)
-                if (s < ppl && slot_ok(s)) mt[s] = mt[s] + (db + da * (float)sx(s) + dbb * (float)sy(s));
+                if (lane + 64 * s < np_) mt[s] = mt[s] + (db + da * (float)sx(s) + dbb * (float)sy(s));
         }
         wsync();
         for (int e = 0; e < ndelta; e++) {
@@ -437,7 +469,7 @@ struct Chain {
             stencil(E.I[k], E.X[k], E.Y[k]);
 #pragma unroll
             for (int s = 0; s < kMaxPPL; s++)
-                if (s < ppl && slot_ok(s)) {
+                if (lane + 64 * s < np_) {
                     const float pr = L.sten[sx(s)] * L.sten[32 + sy(s)];
                     mt[s] = neg ? mt[s] - pr : mt[s] + pr;
                 }
@@ -446,7 +478,7 @@ struct Chain {
         float acc = 0.0f;
 #pragma unroll
         for (int s = 0; s < kMaxPPL; s++)
-            if (s < ppl && slot_ok(s)) acc = acc + ((m[s] - mt[s]) + d[s] * logf(mt[s] / m[s]));
+            if (lane + 64 * s < np_) acc = acc + ((m[s] - mt[s]) + d[s] * logf(mt[s] / m[s]));
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
         return bcastf(acc, 0);
     }
@@ -464,7 +496,7 @@ struct Chain {
         r_pr = pr;
         r_logpr = logpr;
         if (acc) r_flags |= 1;
-        if (a.mode == BAMF_RJ_REPLAY) {
+        if (a->mode == BAMF_RJ_REPLAY) {
             const bool forced = (ref_flags & 1) != 0;
             if (forced != acc) r_flags |= 32;
             acc = forced;
@@ -479,26 +511,26 @@ struct Chain {
     // update_grids (LibRJMCMC.cpp:1629-1648)
     __device__ __forceinline__ void update_grids() {
         r_flags |= 4;
-        if (burn || !a.pgrid) return;
+        if (burn || !a->pgrid) return;
         const Em& c = cur();
-        const int z = a.p.grid_zoom;
-        const int64_t go = a.grid_off[roi];
+        const int z = a->p.grid_zoom;
+        const int64_t go = a->grid_off[roi];
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int n = lane + 64 * h;
             if (n < cN) {
-                const int x = kround((float)z * (c.X[n] + a.p.drift_x));
-                const int y = kround((float)z * (c.Y[n] + a.p.drift_y));
+                const int x = kround((float)z * (c.X[n] + a->p.drift_x));
+                const int y = kround((float)z * (c.Y[n] + a->p.drift_y));
                 const float i = c.I[n];
-                if (x >= 0 && x < X * z && y >= 0 && y < Y * z && i > a.p.icutoff) {
-                    float* g = c.S[n] == 1 ? a.pgrid : a.pback;
+                if (x >= 0 && x < X * z && y >= 0 && y < Y * z && i > a->p.icutoff) {
+                    float* g = c.S[n] == 1 ? a->pgrid : a->pback;
                     atomicAdd(&g[go + (int64_t)x * Y * z + y], 1.0f);
                 }
             }
         }
     }
     __device__ __forceinline__ float pdf_of(const Em& e, int k) {
-        return e.S[k] == 1 ? findpdf(a.sig, a.p.dx, e.I[k]) : findpdf(a.back, a.p.dx, e.I[k]);
+        return e.S[k] == 1 ? findpdf(a->sig, a->p.dx, e.I[k]) : findpdf(a->back, a->p.dx, e.I[k]);
     }
 
     // ---------------- proposals ----------------
@@ -506,7 +538,7 @@ struct Chain {
     // returns the evaluation kind (EV_NONE when the jump ended early).
     // LibRJMCMC.cpp:752-916
     __device__ __forceinline__ int prop_move() {
-        const bamf_mcmc_params& p = a.p;
+        const bamf_mcmc_params& p = a->p;
         jt = 1;
         float PrI = 1, IPDF = 1, IPDFtest = 1;
         cnt_inc(C_N + J_MOVE);
@@ -538,8 +570,8 @@ struct Chain {
                 const float tx = E.X[k] + xs * randn();
                 const float ty = E.Y[k] + xs * randn();
                 const float ti = (I0 / (float)(np + 1)) * (0.15f * randn() + 1);
-                IPDFtest = findpdf(fg ? a.sig : a.back, p.dx, ti);
-                IPDF = findpdf(fg ? a.sig : a.back, p.dx, E.I[k]);
+                IPDFtest = findpdf(fg ? a->sig : a->back, p.dx, ti);
+                IPDF = findpdf(fg ? a->sig : a->back, p.dx, E.I[k]);
                 set_t(k, ti, tx, ty);
                 PrI = PrI * IPDFtest / IPDF;
                 XI0 = XI0 + E.I[k] * E.X[k];
@@ -555,8 +587,8 @@ struct Chain {
             const float tyn = (YI0 + E.I[nn] * E.Y[nn] - YII0) / tin;
             set_t(nn, tin, txn, tyn);
             const bool fg = E.S[nn] == 1;
-            IPDFtest = findpdf(fg ? a.sig : a.back, p.dx, tin);
-            IPDF = findpdf(fg ? a.sig : a.back, p.dx, E.I[nn]);
+            IPDFtest = findpdf(fg ? a->sig : a->back, p.dx, tin);
+            IPDF = findpdf(fg ? a->sig : a->back, p.dx, E.I[nn]);
             PrI = PrI * IPDFtest / IPDF;
             delta_push(0, nn, true);
             delta_push(1, nn, false);
@@ -574,8 +606,8 @@ struct Chain {
                 const float ti = T.I[k] + is * randn();
                 const float tx = T.X[k] + xs * randn();
                 const float ty = T.Y[k] + xs * randn();
-                IPDFtest = findpdf(fg ? a.sig : a.back, p.dx, ti);
-                IPDF = findpdf(fg ? a.sig : a.back, p.dx, E.I[k]);
+                IPDFtest = findpdf(fg ? a->sig : a->back, p.dx, ti);
+                IPDF = findpdf(fg ? a->sig : a->back, p.dx, E.I[k]);
                 wsync();
                 set_t(k, ti, tx, ty);
                 wsync();
@@ -590,8 +622,8 @@ struct Chain {
             const float ti = T.I[nn] + is * randn();
             const float tx = T.X[nn] + xs * randn();
             const float ty = T.Y[nn] + xs * randn();
-            IPDFtest = findpdf(fg ? a.sig : a.back, p.dx, ti);
-            IPDF = findpdf(fg ? a.sig : a.back, p.dx, E.I[nn]);
+            IPDFtest = findpdf(fg ? a->sig : a->back, p.dx, ti);
+            IPDF = findpdf(fg ? a->sig : a->back, p.dx, E.I[nn]);
             set_t(nn, ti, tx, ty);
             PrI = IPDFtest / IPDF;
             delta_push(0, nn, true);
@@ -606,17 +638,17 @@ struct Chain {
     }
     // LibRJMCMC.cpp:918-936
     __device__ __forceinline__ float fin_move(float sum, float LLR, double& logpr) {
-        const float BgPDF = findpdf(a.off, a.p.dx, cBG);
-        const float BgPDFtest = findpdf(a.off, a.p.dx, tBG);
+        const float BgPDF = findpdf(a->off, a->p.dx, cBG);
+        const float BgPDFtest = findpdf(a->off, a->p.dx, tBG);
         const float PrBg = (BgPDFtest / BgPDF) * expf(-(tBBG * tBBG - cBBG * cBBG) / 2) *
                            expf(-(tABG * tABG - cABG * cABG) / 2);
-        logpr = (double)sum + log_d(f_pri) + log_d(PrBg);
+        logpr = (double)sum + log((double)f_pri * (double)PrBg);
         return LLR * f_pri * PrBg;
     }
 
     // LibRJMCMC.cpp:972-1028
     __device__ __forceinline__ int prop_split() {
-        const bamf_mcmc_params& p = a.p;
+        const bamf_mcmc_params& p = a->p;
         jt = 4;
         cnt_inc(C_N + J_SPLIT);
         copy_to_test();
@@ -647,7 +679,7 @@ struct Chain {
     }
     // LibRJMCMC.cpp:1030-1070
     __device__ __forceinline__ float fin_split(float sum, float LLR, double& logpr) {
-        const bamf_mcmc_params& p = a.p;
+        const bamf_mcmc_params& p = a->p;
         const Em& E = cur();
         const Em& T = tst();
         float tmp1 = 1, tmp2 = 1;
@@ -665,13 +697,13 @@ struct Chain {
         const double om = (double)(1 - f_u1);
         const float Jacob = (float)((double)f_ijp / (om * om));
         const float P12 = P1 / P2;
-        logpr = (double)sum + log_d(PrI) + log_d(PrX) + log_d(Us) + log_d(Jacob) + log_d(P12) + log_d(PrM);
+        logpr = (double)sum + log((double)PrI * (double)PrX * (double)Us * (double)Jacob * (double)P12 * (double)PrM);
         return LLR * PrI * PrX * Us * Jacob * P12 * PrM;
     }
 
     // LibRJMCMC.cpp:1111-1164
     __device__ __forceinline__ int prop_merge() {
-        const bamf_mcmc_params& p = a.p;
+        const bamf_mcmc_params& p = a->p;
         jt = 5;
         cnt_inc(C_N + J_MERGE);
         copy_to_test();
@@ -699,8 +731,8 @@ struct Chain {
         f_u2 = E.X[jj] - muxjp;
         f_u3 = E.Y[jj] - muyjp;
         f_ijp = Ijp;
-        const float PDFfg = findpdf(a.sig, p.dx, Ijp);
-        const float PDFbg = findpdf(a.back, p.dx, Ijp);
+        const float PDFfg = findpdf(a->sig, p.dx, Ijp);
+        const float PDFbg = findpdf(a->back, p.dx, Ijp);
         set_t(jj, Ijp, muxjp, muyjp);
         if (lane == 0) tst().S[jj] = (PDFbg > PDFfg) ? 0 : 1;
         remove_t(ii);
@@ -718,7 +750,7 @@ struct Chain {
     }
     // LibRJMCMC.cpp:1166-1217
     __device__ __forceinline__ float fin_merge(float sum, float LLR, double& logpr) {
-        const bamf_mcmc_params& p = a.p;
+        const bamf_mcmc_params& p = a->p;
         const Em& E = cur();
         const Em& T = tst();
         const int ii = L.nb[0];
@@ -736,14 +768,16 @@ struct Chain {
         const double om = (double)(1 - f_u1);
         const float Jacob = (float)((double)f_ijp / (om * om));
         const float PrM = meanN / NEm;
-        logpr = (double)sum -
-                (log_d(PrI) + log_d(PrX) + log_d(Jacob) + log_d(Us) + log_d(PrM) + log_d(P1) - log_d(P2));
+        logpr = (double)sum - log((double)PrI * (double)PrX * (double)Jacob * (double)Us * (double)PrM * (double)P1 /
+                                  (double)P2);
         return LLR / (PrI * PrX * Jacob * Us * PrM * P1 / P2);
     }
 
     // find_Birth (LibRJMCMC.cpp:1688-1736): residual max(0, D - M), prefix
     // sum in pixel order, r = total * U, first pixel with r <= cumulative.
     __device__ __forceinline__ int birth_pixel(float u) {
+        const int np_ = pix_count();
+        aflops += (uint64_t)(4 * P);
         float carry = 0.0f;
         float cum[kMaxPPL];
 #pragma unroll
@@ -751,7 +785,7 @@ struct Chain {
             cum[s] = 0.0f;
             if (s < ppl) {
                 float v = 0.0f;
-                if (slot_ok(s)) {
+                if (lane + 64 * s < np_) {
                     const float r = d[s] - m[s];
                     v = (0 > r) ? 0.0f : r;
                 }
@@ -768,7 +802,7 @@ struct Chain {
 #pragma unroll
         for (int s = 0; s < kMaxPPL; s++) {
             if (s < ppl && pix == P) {
-                const uint64_t mk = ballot(slot_ok(s) && rr <= cum[s]);
+                const uint64_t mk = ballot(lane + 64 * s < np_ && rr <= cum[s]);
                 if (mk) pix = 64 * s + __builtin_ctzll(mk);
             }
         }
@@ -777,7 +811,8 @@ struct Chain {
     // Learned proposal: pixel drawn from the per-ROI proposal map q; returns
     // the pixel and its probability q[pix] / sum(q).
     __device__ __forceinline__ int map_pixel(float u, float& prob) {
-        const int64_t off = a.pix_off[roi];
+        const int np_ = pix_count();
+        const int64_t off = a->pix_off[roi];
         float carry = 0.0f;
         float cum[kMaxPPL], q[kMaxPPL];
 #pragma unroll
@@ -785,7 +820,7 @@ struct Chain {
             cum[s] = 0.0f;
             q[s] = 0.0f;
             if (s < ppl) {
-                float v = slot_ok(s) ? a.prop_map[off + lane + 64 * s] : 0.0f;
+                float v = (lane + 64 * s < np_) ? a->prop_map[off + lane + 64 * s] : 0.0f;
                 q[s] = v;
                 for (int o = 1; o < 64; o <<= 1) {
                     const float t = __shfl_up(v, o);
@@ -802,7 +837,7 @@ struct Chain {
 #pragma unroll
         for (int s = 0; s < kMaxPPL; s++) {
             if (s < ppl && !found) {
-                const uint64_t mk = ballot(slot_ok(s) && rr <= cum[s] && q[s] > 0.0f);
+                const uint64_t mk = ballot(lane + 64 * s < np_ && rr <= cum[s] && q[s] > 0.0f);
                 if (mk) {
                     const int l = __builtin_ctzll(mk);
                     pix = 64 * s + l;
@@ -817,23 +852,23 @@ struct Chain {
 
     // LibRJMCMC.cpp:1258-1295
     __device__ __forceinline__ int prop_birth() {
-        const bamf_mcmc_params& p = a.p;
+        const bamf_mcmc_params& p = a->p;
         jt = 6;
         cnt_inc(C_N + J_BIRTH);
         copy_to_test();
         const float ur = urand();
         float pixprob = 1.0f;
         int pix;
-        if (a.mode == BAMF_RJ_LEARNED) pix = map_pixel(ur, pixprob);
+        if (a->mode == BAMF_RJ_LEARNED) pix = map_pixel(ur, pixprob);
         else pix = birth_pixel(ur);
         r_pix = pix;
-        if (a.mode == BAMF_RJ_REPLAY) pix = ref_pix;
+        if (a->mode == BAMF_RJ_REPLAY) pix = ref_pix;
         const float y = (float)(pix - Y * (pix / Y)) + urand() - 0.5f;
         const float x = (float)(int)floorf(((float)pix) / Y) + urand() - 0.5f;
         int BF = 1;
         if (p.is_backg == 1) BF = kround(urand());
         cnt_inc(BF == 1 ? C_NFG + J_BIRTH : C_NBG + J_BIRTH);
-        const float I_new = genrand(BF == 1 ? a.sig : a.back, urand());
+        const float I_new = genrand(BF == 1 ? a->sig : a->back, urand());
         if (!add_t(I_new, x, y, BF)) return EV_NONE;
         delta_push(1, tN - 1, false);
         f_pixprob = pixprob;
@@ -846,7 +881,7 @@ struct Chain {
     __device__ __forceinline__ float fin_birth(float sum, float LLR, double& logpr) {
         const float NEm = (float)nem_cur();
         const float PrM = meanN / (NEm + 1);
-        logpr = (double)sum + log_d(P3) - (log_d(P4) + log_d(xb) + log_d(yb) + log_d(f_pixprob)) + log_d(PrM);
+        logpr = (double)sum + log((double)P3 / ((double)P4 * (double)xb * (double)yb * (double)f_pixprob) * (double)PrM);
         return LLR * P3 / (P4 * xb * yb * f_pixprob) * PrM;
     }
 
@@ -873,19 +908,19 @@ struct Chain {
         const float NEm = (float)nem_cur();
         const float PrM = NEm / meanN;
         const float P43 = P4 / P3;
-        logpr = (double)sum + log_d(P43) + log_d(PrM);
+        logpr = (double)sum + log((double)P43 * (double)PrM);
         return LLR * P43 * PrM;
     }
 
     // LibRJMCMC.cpp:1586-1627 (changes the current set in place, no record)
     __device__ __forceinline__ void do_backfore() {
         if (cN == 0) return;
-        if (a.p.is_backg == 0) return;
+        if (a->p.is_backg == 0) return;
         Em& E = cur();
         const float Ua = urand();
         const int nn = min((int)floorf((float)cN * Ua), cN - 1);
-        const float BackP = findpdf(a.back, a.p.dx, E.I[nn]);
-        const float SignalP = findpdf(a.sig, a.p.dx, E.I[nn]);
+        const float BackP = findpdf(a->back, a->p.dx, E.I[nn]);
+        const float SignalP = findpdf(a->sig, a->p.dx, E.I[nn]);
         int s = E.S[nn];
         if (s == 1) {
             if (SignalP != 0) {
@@ -921,7 +956,7 @@ struct Chain {
     // records and the grid update, in the reference's order.
     __device__ __forceinline__ int one_step() {
         const float rtmp = urand();
-        const int kind = burn ? kind_of(rtmp, a.pvb, a.pvb) : kind_of(rtmp, a.pvt, a.pvb);
+        const int kind = burn ? kind_of(rtmp, a->pvb, a->pvb) : kind_of(rtmp, a->pvt, a->pvb);
         int evk = EV_NONE;
         switch (kind) {
             case 0: evk = prop_move(); break;
@@ -959,7 +994,7 @@ struct Chain {
 #pragma unroll
                     for (int s = 0; s < kMaxPPL; s++) m[s] = mt[s];
                     logpost += logpr;
-                    if (++nacc >= a.recompute_every) {
+                    if (++nacc >= a->recompute_every) {
                         wsync();
                         rebuild_model();
                     }
@@ -981,28 +1016,29 @@ struct Chain {
 
     __device__ __forceinline__ void run(int roi_) {
         roi = roi_;
-        const bamf_mcmc_params& p = a.p;
-        X = a.xs[roi];
-        Y = a.ys[roi];
+        const bamf_mcmc_params& p = a->p;
+        X = a->xs[roi];
+        Y = a->ys[roi];
         P = X * Y;
         ppl = (P + 63) >> 6;
         C = 1 / (p.psf_sigma * sqrtf(2.0f));
-        const float rho = a.rho ? a.rho[roi] : p.rho;
+        const float rho = a->rho ? a->rho[roi] : p.rho;
         meanN = rho * X * Y;  // cRJMCMC.cpp:116
         bnd = (int)p.bnd_out;
         xb = X + p.bnd_out;
         yb = Y + p.bnd_out;
-        key = a.key ? a.key[roi] : (uint32_t)roi;
+        key = a->key ? a->key[roi] : (uint32_t)roi;
         status = ST_OK;
         logpost = 0.0;
-        const int64_t off = a.pix_off[roi];
+        aflops = 0;
+        const int64_t off = a->pix_off[roi];
 #pragma unroll
         for (int s = 0; s < kMaxPPL; s++) {
             const int pp = lane + 64 * s;
             const bool ok = s < ppl && pp < P;
             const int xx = ok ? pp / Y : 0;
             adr[s] = ok ? (uint32_t)(xx | ((pp - xx * Y) << 8)) : 0xffffu;
-            d[s] = ok ? a.data[off + pp] : 0.0f;
+            d[s] = ok ? a->data[off + pp] : 0.0f;
             m[s] = 1.0f;
             mt[s] = 1.0f;
         }
@@ -1011,25 +1047,25 @@ struct Chain {
         for (int k = lane; k < kNMax + 1; k += 64) L.hist[k] = 0;
         // initial state (cRJMCMC.cpp:120-133)
         ci = 0;
-        cN = a.n0 ? min(a.n0[roi], kNMax) : 0;
-        if (a.n0) {
-            const int64_t eo = a.em0_off[roi];
+        cN = a->n0 ? min(a->n0[roi], kNMax) : 0;
+        if (a->n0) {
+            const int64_t eo = a->em0_off[roi];
             for (int n = lane; n < cN; n += 64) {
-                L.e[0].I[n] = a.i0[eo + n];
-                L.e[0].X[n] = a.x0[eo + n];
-                L.e[0].Y[n] = a.y0[eo + n];
+                L.e[0].I[n] = a->i0[eo + n];
+                L.e[0].X[n] = a->x0[eo + n];
+                L.e[0].Y[n] = a->y0[eo + n];
                 L.e[0].S[n] = 1;
             }
         }
-        const float bg0 = a.bg0 ? a.bg0[roi] : 0.0f;
+        const float bg0 = a->bg0 ? a->bg0[roi] : 0.0f;
         cBG = bg0 == 0 ? 1.0f : bg0;
         cABG = 0;
         cBBG = 0;
         wsync();
         rebuild_model();
-        if (a.mode == BAMF_RJ_REPLAY) {
-            tbase = a.tape_off[roi];
-            tlen = a.tape_len[roi];
+        if (a->mode == BAMF_RJ_REPLAY) {
+            tbase = a->tape_off[roi];
+            tlen = a->tape_len[roi];
         } else {
             tbase = 0;
             tlen = 0;
@@ -1043,7 +1079,7 @@ struct Chain {
             if (burn) {
                 P1 = p.p_burnin[1]; P2 = p.p_burnin[2]; P3 = p.p_burnin[3]; P4 = p.p_burnin[4];
             } else {
-                P1 = a.pvt[1]; P2 = a.pvt[2]; P3 = a.pvt[3]; P4 = a.pvt[4];
+                P1 = a->pvt[1]; P2 = a->pvt[2]; P3 = a->pvt[3]; P4 = a->pvt[4];
             }
             dj = 0;
             dbatch = -1;
@@ -1054,17 +1090,17 @@ struct Chain {
             r_pr = 0;
             r_logpr = __builtin_nan("");
             r_nprop = -1;
-            if (a.mode == BAMF_RJ_REPLAY) {
-                const bamf_step_record& rr = a.ref_steps[roi * rstride + s];
+            if (a->mode == BAMF_RJ_REPLAY) {
+                const bamf_step_record& rr = a->ref_steps[roi * rstride + s];
                 ref_flags = rr.flags;
                 ref_pix = rr.pix;
             }
             const int kind = one_step();
             wsync();
-            if (a.steps) {
+            if (a->steps) {
                 const float ph = total_photons_seq();
                 if (lane == 0) {
-                    bamf_step_record& o = a.steps[roi * rstride + s];
+                    bamf_step_record& o = a->steps[roi * rstride + s];
                     o.kind = kind;
                     o.jump_type = r_jt;
                     o.flags = r_flags;
@@ -1083,15 +1119,15 @@ struct Chain {
             if (!burn) {
                 if (lane == 0) L.hist[min(cN, kNMax)]++;
                 const int t = s - nb;
-                if (a.trace_n && t % a.trace_thin == 0 && t / a.trace_thin < a.trace_len) {
+                if (a->trace_n && t % a->trace_thin == 0 && t / a->trace_thin < a->trace_len) {
                     const Em& c = cur();
                     float v = (lane < cN ? c.I[lane] : 0.0f) + (lane + 64 < cN ? c.I[lane + 64] : 0.0f);
                     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
                     v = bcastf(v, 0);
                     if (lane == 0) {
-                        const int64_t ti = (int64_t)roi * a.trace_len + t / a.trace_thin;
-                        a.trace_n[ti] = (int16_t)cN;
-                        a.trace_ph[ti] = v;
+                        const int64_t ti = (int64_t)roi * a->trace_len + t / a->trace_thin;
+                        a->trace_n[ti] = (int16_t)cN;
+                        a->trace_ph[ti] = v;
                     }
                 }
             }
@@ -1099,11 +1135,12 @@ struct Chain {
         }
         wsync();
         // ---- outputs ----
-        if (a.status && lane == 0) a.status[roi] = status;
-        if (a.counters && lane < 30) a.counters[(int64_t)roi * 30 + lane] = L.cnt[lane];
-        if (a.logpost && lane == 0) a.logpost[roi] = logpost;
-        if (a.hist) {
-            const int hl = a.hist_len;
+        if (a->status && lane == 0) a->status[roi] = status;
+        if (a->counters && lane < 30) a->counters[(int64_t)roi * 30 + lane] = L.cnt[lane];
+        if (a->logpost && lane == 0) a->logpost[roi] = logpost;
+        if (a->alg_flops && lane == 0) a->alg_flops[roi] = aflops;
+        if (a->hist) {
+            const int hl = a->hist_len;
             for (int k = lane; k < hl; k += 64) {
                 int v = 0;
                 if (k < hl - 1) {
@@ -1111,11 +1148,11 @@ struct Chain {
                 } else {
                     for (int q = k; q <= kNMax; q++) v += L.hist[q];
                 }
-                a.hist[(int64_t)roi * hl + k] = v;
+                a->hist[(int64_t)roi * hl + k] = v;
             }
         }
-        if (a.fin) {
-            float* f = a.fin + (int64_t)roi * (4 + 4 * a.fin_cap);
+        if (a->fin) {
+            float* f = a->fin + (int64_t)roi * (4 + 4 * a->fin_cap);
             if (lane == 0) {
                 f[0] = (float)cN;
                 f[1] = cBG;
@@ -1123,33 +1160,41 @@ struct Chain {
                 f[3] = cBBG;
             }
             const Em& c = cur();
-            for (int n = lane; n < cN && n < a.fin_cap; n += 64) {
+            for (int n = lane; n < cN && n < a->fin_cap; n += 64) {
                 f[4 + n] = c.I[n];
-                f[4 + a.fin_cap + n] = c.X[n];
-                f[4 + 2 * a.fin_cap + n] = c.Y[n];
-                f[4 + 3 * a.fin_cap + n] = (float)c.S[n];
+                f[4 + a->fin_cap + n] = c.X[n];
+                f[4 + 2 * a->fin_cap + n] = c.Y[n];
+                f[4 + 3 * a->fin_cap + n] = (float)c.S[n];
             }
         }
         wsync();
     }
 };
 
-__global__ __launch_bounds__(64 * kWavesPerBlock) void k_chain(ChainArgs a) {
+// The argument block is the kernel's only parameter, so it sits at offset 0 of
+// the kernarg segment; the chain reads it through that segment pointer.
+__global__ __launch_bounds__(64 * kWavesPerBlock, 3) void k_chain(ChainArgs) {
     __shared__ WaveLds lds[kWavesPerBlock];
-    const int w = threadIdx.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int lane = threadIdx.x & 63;
-    Chain ch(a, lds[w], lane);
+    const ChainArgs* ap = (const ChainArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
+    Chain ch(ap, lds[w], lane);
     while (true) {
         int qi = 0;
-        if (lane == 0) qi = (int)atomicAdd(a.queue, 1u);
+        if (lane == 0) qi = (int)atomicAdd(ap->queue, 1u);
         qi = bcasti(qi, 0);
-        if (qi >= a.n_roi) break;
-        const int roi = a.order ? a.order[qi] : qi;
+        if (qi >= ap->n_roi) break;
+        const int roi = ap->order ? ap->order[qi] : qi;
         ch.run(roi);
     }
 }
 
 __global__ void k_zero_u32(uint32_t* p) { p[0] = 0u; }
+
+__global__ void k_erf(const float* x, float* y, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = erf_nr(x[i]);
+}
 
 }  // namespace
 
@@ -1158,6 +1203,12 @@ int launch_chain(const ChainArgs& a, int grid, hipStream_t s) {
     BAMF_LAUNCH_CHECK("k_zero_u32");
     hipLaunchKernelGGL(k_chain, dim3(grid), dim3(64 * kWavesPerBlock), 0, s, a);
     BAMF_LAUNCH_CHECK("k_chain");
+    return BAMF_OK;
+}
+
+int launch_erf(const float* x, float* y, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_erf, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, x, y, n);
+    BAMF_LAUNCH_CHECK("k_erf");
     return BAMF_OK;
 }
 

@@ -15,13 +15,17 @@ constexpr int kMaxSide = 32;
 
 // One prior table (Stat, LibRJMCMC.cpp:176-217) on the device.
 struct StatDev {
-    const float* x;    // Xarray (0, dx, 2dx, ... by repeated addition)
-    const float* pdf;  // PDFarray
-    const float* ux;   // UniqXarray
-    const float* uc;   // UniqCDFarray
+    const float* x;      // Xarray (0, dx, 2dx, ... by repeated addition)
+    const float* pdf;    // PDFarray
+    const float* ux;     // UniqXarray
+    const float* uc;     // UniqCDFarray
+    const int32_t* guide;  // guide[k] = last index with uc <= k / kGuide (genRand search start)
     int32_t len;
     int32_t nu;
+    int32_t x_exact;     // 1 when x[i] == (float)i * dx for every i (O(1) findPDF index)
+    float x_last;        // x[len - 1]
 };
+constexpr int kGuide = 256;
 
 struct ChainArgs {
     bamf_mcmc_params p;
@@ -70,6 +74,7 @@ struct ChainArgs {
     float* pgrid;
     float* pback;
     const int64_t* grid_off;
+    uint64_t* alg_flops;
     int32_t recompute_every;  // full model rebuild after this many accepted jumps
 };
 

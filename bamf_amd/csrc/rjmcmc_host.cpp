@@ -9,6 +9,7 @@ namespace bamf {
 namespace rj {
 int launch_chain(const ChainArgs& a, int grid, hipStream_t s);
 int chain_occupancy_blocks(int* blocks_per_cu);
+int launch_erf(const float* x, float* y, int64_t n, hipStream_t s);
 }  // namespace rj
 }  // namespace bamf
 
@@ -46,7 +47,25 @@ struct HostStat {
         if (uc.back() != 1) uc.back() = 1;
         if (uc.front() != 0) uc.front() = 0;
     }
-    size_t floats() const { return x.size() + pdf.size() + ux.size() + uc.size(); }
+    // guide[k]: the last index whose uc <= k / kGuide (uc[0] = 0, strictly increasing)
+    std::vector<float> guide() const {
+        std::vector<float> g(kGuide);
+        int i = 0;
+        for (int k = 0; k < kGuide; k++) {
+            const float t = (float)k / (float)kGuide;
+            while (i + 1 < (int)uc.size() && uc[i + 1] <= t) i++;
+            int32_t v = i;
+            float f;
+            std::memcpy(&f, &v, 4);
+            g[k] = f;
+        }
+        return g;
+    }
+    bool exact(float dx) const {
+        for (size_t i = 0; i < x.size(); i++)
+            if (x[i] != (float)i * dx) return false;
+        return true;
+    }
 };
 
 }  // namespace
@@ -96,13 +115,14 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
         flat.insert(flat.end(), v.begin(), v.end());
         return at;
     };
-    size_t o[12];
+    size_t o[15];
     const HostStat* ss[3] = {&pl->off, &pl->sig, &pl->back};
     for (int k = 0; k < 3; k++) {
-        o[4 * k + 0] = put(ss[k]->x);
-        o[4 * k + 1] = put(ss[k]->pdf);
-        o[4 * k + 2] = put(ss[k]->ux);
-        o[4 * k + 3] = put(ss[k]->uc);
+        o[5 * k + 0] = put(ss[k]->x);
+        o[5 * k + 1] = put(ss[k]->pdf);
+        o[5 * k + 2] = put(ss[k]->ux);
+        o[5 * k + 3] = put(ss[k]->uc);
+        o[5 * k + 4] = put(ss[k]->guide());
     }
     hipError_t e = hipGetDevice(&pl->device);
     if (e == hipSuccess) e = hipMalloc(&pl->d_tables, flat.size() * sizeof(float));
@@ -117,12 +137,15 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
     }
     StatDev* sd[3] = {&pl->s_off, &pl->s_sig, &pl->s_back};
     for (int k = 0; k < 3; k++) {
-        sd[k]->x = pl->d_tables + o[4 * k + 0];
-        sd[k]->pdf = pl->d_tables + o[4 * k + 1];
-        sd[k]->ux = pl->d_tables + o[4 * k + 2];
-        sd[k]->uc = pl->d_tables + o[4 * k + 3];
+        sd[k]->x = pl->d_tables + o[5 * k + 0];
+        sd[k]->pdf = pl->d_tables + o[5 * k + 1];
+        sd[k]->ux = pl->d_tables + o[5 * k + 2];
+        sd[k]->uc = pl->d_tables + o[5 * k + 3];
+        sd[k]->guide = reinterpret_cast<const int32_t*>(pl->d_tables + o[5 * k + 4]);
         sd[k]->len = (int)ss[k]->x.size();
         sd[k]->nu = (int)ss[k]->ux.size();
+        sd[k]->x_exact = ss[k]->exact(params->dx) ? 1 : 0;
+        sd[k]->x_last = ss[k]->x.back();
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, pl->device) == hipSuccess) pl->cu = prop.multiProcessorCount;
@@ -138,6 +161,13 @@ int bamf_rjmcmc_plan_destroy(bamf_rjmcmc_plan* plan) {
     if (plan->d_queue) (void)hipFree(plan->d_queue);
     delete plan;
     return BAMF_OK;
+}
+
+int bamf_math_erf(const float* d_x, float* d_y, int64_t n, void* stream) {
+    BAMF_CHECK_ARG(n >= 0, "n must be >= 0");
+    if (n == 0) return BAMF_OK;
+    BAMF_CHECK_ARG(d_x && d_y, "null device pointer");
+    return launch_erf(d_x, d_y, n, as_stream(stream));
 }
 
 int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed) {
@@ -218,6 +248,7 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     a.pgrid = out->d_pgrid;
     a.pback = out->d_pback;
     a.grid_off = out->d_grid_offset;
+    a.alg_flops = out->d_alg_flops;
     a.recompute_every = 128;
     const int bpc = plan->blocks_per_cu > 0 ? plan->blocks_per_cu : 1;
     const int cu = plan->cu > 0 ? plan->cu : 256;

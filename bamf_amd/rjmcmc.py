@@ -83,6 +83,7 @@ class BatchResult:
     steps: np.ndarray | None
     pgrid: np.ndarray | None = None
     pback: np.ndarray | None = None
+    alg_flops: np.ndarray | None = None
 
 
 class RoiSet:
@@ -158,10 +159,11 @@ def run_batch(plan: Plan, rois: RoiSet, seed: int = 0, mode: int = PLAIN, replay
 def alloc_outputs(n, total, hist_len, final_cap, trace_len, record_steps, grids, rois, zoom):
     B = _hip.DeviceBuffer
     d = {"status": B(4 * n), "counters": B(4 * 30 * n), "final": B(4 * n * (4 + 4 * final_cap)),
-         "hist": B(4 * n * hist_len), "logpost": B(8 * n)}
+         "hist": B(4 * n * hist_len), "logpost": B(8 * n), "flops": B(8 * n)}
     o = RjOut()
     o.d_status, o.d_counters, o.d_final, o.final_cap = d["status"].ptr, d["counters"].ptr, d["final"].ptr, final_cap
     o.d_n_hist, o.hist_len, o.d_logpost = d["hist"].ptr, hist_len, d["logpost"].ptr
+    o.d_alg_flops = d["flops"].ptr
     if trace_len > 0:
         d["trace_n"] = B(2 * n * trace_len)
         d["trace_ph"] = B(4 * n * trace_len)
@@ -194,6 +196,7 @@ def fetch(d, n, total, hist_len, final_cap, trace_len) -> BatchResult:
         trace_n=d["trace_n"].download(np.int16).reshape(n, trace_len) if "trace_n" in d else None,
         trace_photons=d["trace_ph"].download(np.float32).reshape(n, trace_len) if "trace_ph" in d else None,
         steps=d["steps"].download(STEP_DTYPE).reshape(n, total) if "steps" in d else None)
+    res.alg_flops = d["flops"].download(np.uint64)
     if "pgrid" in d:
         res.pgrid = d["pgrid"].download(np.float32)
         res.pback = d["pback"].download(np.float32)

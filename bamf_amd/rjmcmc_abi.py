@@ -40,7 +40,7 @@ class RjOut(Structure):
                 ("d_n_hist", c_void_p), ("hist_len", c_int32), ("d_logpost", c_void_p),
                 ("d_trace_n", c_void_p), ("d_trace_photons", c_void_p), ("trace_len", c_int32),
                 ("trace_thin", c_int32), ("d_steps", c_void_p), ("d_pgrid", c_void_p), ("d_pback", c_void_p),
-                ("d_grid_offset", c_void_p)]
+                ("d_grid_offset", c_void_p), ("d_alg_flops", c_void_p)]
 
 
 PLAIN, REPLAY, LEARNED = 0, 1, 2
@@ -53,5 +53,6 @@ def declare(L: ctypes.CDLL) -> None:
     L.bamf_rjmcmc_run.argtypes = [c_void_p, POINTER(RoiBatch), c_uint64, c_int, POINTER(Replay), POINTER(RjOut),
                                   c_void_p]
     L.bamf_rjmcmc_last_claimed.argtypes = [c_void_p, POINTER(c_int64)]
+    L.bamf_math_erf.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
     if hasattr(L, "bamf_rjmcmc_set_proposal"):
         L.bamf_rjmcmc_set_proposal.argtypes = [c_void_p, c_void_p]

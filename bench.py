@@ -1,0 +1,238 @@
+"""Benchmark: ROI posteriors/sec of batched RJMCMC on one MI355X.
+
+Workload (default, the batched uneven plain configuration): 20,000 synthetic
+ROIs with side drawn uniformly from {8, 12, 16, 24, 32} px, emitter density
+uniform in 0.5-5 /um^2 at 100 nm pixels, intensity Gamma(5, 200), background
+uniform 5-30, sigma 1.3 px; 10,000 jumps per chain (5,000 burn-in + 5,000
+post-burn-in) with P_Burnin = [.4 .15 .15 .15 .15 0 0 0] and
+P_Trials = [.6 .1 .1 .1 .1 0 0 0].
+
+A step is one batched run of every chain of the workload (one persistent
+launch).  An ROI posterior is ESS = 200 of an ROI's post-burn-in chain, the
+ESS being the smaller of the emitter-count and total-photon traces' ESS; value
+= sum over ROIs of ESS / 200, per second of the timed steps (inputs resident in
+HBM): the rate at which posteriors of that ESS are produced.  The strict count
+(ROIs that reach ESS >= 200 within the fixed chain length) is reported too.  The CPU baseline is the oracle chain (a C restatement of the
+reference's RJMCMC class) on a sample of the same ROIs over all host cores,
+timed before the GPU is touched.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+SEED = 20261203
+DX = 10.0
+
+
+def workload(name: str):
+    from bamf_amd import synth
+
+    if name == "batched_uneven":
+        rois = synth.make_rois(20000, seed=SEED + 2, sides=(8, 12, 16, 24, 32), density=(0.5, 5.0),
+                               bg=(5.0, 30.0))
+        nb, nt = 5000, 5000
+        rho = 2.75 * 0.01  # mean density 2.75 / um^2 at 0.01 um^2 per pixel
+    elif name == "plain_16":
+        rois = synth.make_rois(256, seed=SEED + 1, sides=(16,), mean_n=3.0, n_cap=10, bg=(10.0, 10.0))
+        nb, nt = 3000, 3000
+        rho = 3.0 / 256
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    sig = synth.gamma_pdf_table(5, 200, DX, 1001)
+    off = synth.gamma_pdf_table(4, 5, DX, 1001)  # offset prior: mean 20, sd 10 photons/px
+    back = np.zeros(1001, np.float32)
+    bg0 = float(DX * int(np.argmax(off)))  # @RJ/rjmcmc.m:47
+    return rois, nb, nt, rho, sig, off, back, bg0
+
+
+def params(nb, nt, rho):
+    from bamf_amd.rjmcmc import make_params
+
+    return make_params(n_burnin=nb, n_trials=nt, rho=rho, p_burnin=(0.4, 0.15, 0.15, 0.15, 0.15, 0, 0, 0),
+                       p_trials=(0.6, 0.1, 0.1, 0.1, 0.1, 0, 0, 0))
+
+
+# ---------------------------------------------------------------- CPU baseline
+_CPU = {}
+
+
+def _cpu_worker(idx):
+    from oracle import pyoracle as po
+
+    rois, p, off, sig, back, bg0 = _CPU["args"]
+    op = po.McmcParams.from_buffer_copy(p)
+    out = []
+    for r in idx:
+        o = rois["offset"]
+        s = int(rois["side"][r])
+        res = po.rjmcmc(rois["data"][o[r]:o[r + 1]], s, s, op, off, sig, back, bg0=bg0, seed=SEED, roi=int(r),
+                        record=False)
+        st = res["steps"][p.n_burnin:]
+        out.append((int(r), st["n"].copy(), st["photons"].copy()))
+    return out
+
+
+def cpu_baseline(rois, p, off, sig, back, bg0, sample, cores, thin):
+    import multiprocessing as mp
+
+    from bamf_amd.ess import roi_posteriors
+    from oracle import pyoracle as po
+
+    po.build()
+    rng = np.random.default_rng(SEED)
+    idx = np.sort(rng.choice(len(rois["side"]), size=sample, replace=False))
+    _CPU["args"] = (rois, p, off, sig, back, bg0)
+    chunks = [idx[i::cores * 4] for i in range(cores * 4)]
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        parts = pool.map(_cpu_worker, chunks)
+    dt = time.perf_counter() - t0
+    res = sorted([x for part in parts for x in part])
+    tn = np.stack([r[1][::thin] for r in res])
+    tp = np.stack([r[2][::thin] for r in res])
+    ess, ok = roi_posteriors(tn, tp)
+    jumps = sample * (p.n_burnin + p.n_trials)
+    return {"value": float(ess.sum() / 200.0 / dt), "unit": "ROI posteriors/s", "cores": cores, "kind": "port",
+            "sample": f"{sample} of the workload's ROIs (seeded random subset), full {p.n_burnin}+{p.n_trials}-jump "
+                      f"chains of the oracle C chain, {cores} forked worker processes, {dt:.1f} s wall",
+            "chains_per_s": float(sample / dt), "jumps_per_s": float(jumps / dt),
+            "strict_posteriors_per_s": float(ok.sum() / dt), "median_ess": float(np.median(ess))}
+
+
+# ---------------------------------------------------------------- accuracy
+def localization_rmse_nm(final, truth, cap, pixel_nm=100.0):
+    """RMSE of final-state emitters matched one-to-one to the truth (greedy
+    nearest pairs within 1 px), in nm."""
+    d2 = []
+    for r, (I, X, Y) in enumerate(truth):
+        n = int(final[r, 0])
+        if n == 0 or len(X) == 0:
+            continue
+        fx = final[r, 4 + cap:4 + cap + n]
+        fy = final[r, 4 + 2 * cap:4 + 2 * cap + n]
+        dd = (fx[:, None] - X[None, :]) ** 2 + (fy[:, None] - Y[None, :]) ** 2
+        used_f, used_t = set(), set()
+        for k in np.argsort(dd, axis=None):
+            i, j = divmod(int(k), dd.shape[1])
+            if dd[i, j] > 1.0:
+                break
+            if i in used_f or j in used_t:
+                continue
+            used_f.add(i)
+            used_t.add(j)
+            d2.append(dd[i, j])
+    return float(math.sqrt(np.mean(d2)) * pixel_nm) if d2 else float("nan")
+
+
+def read_traffic(profile: Path):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    if not profile.exists():
+        return None
+    vals = {}
+    for line in profile.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="batched_uneven", choices=["batched_uneven", "plain_16"])
+    ap.add_argument("--cpu-sample", type=int, default=512)
+    ap.add_argument("--cpu-cores", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--thin", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (ROIs shard with no exchange; "
+              "see the design notes). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    rois, nb, nt, rho, sig, off, back, bg0 = workload(args.workload)
+    p = params(nb, nt, rho)
+    cpu = None
+    if not args.no_cpu:
+        cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
+        print(f"[bench] cpu baseline: {args.cpu_sample} ROIs on {cores} cores", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(rois, p, off, sig, back, bg0, args.cpu_sample, cores, args.thin)
+        print(f"[bench] cpu baseline done: {cpu['chains_per_s']:.1f} chains/s", file=sys.stderr, flush=True)
+
+    from bamf_amd import _hip
+    from bamf_amd.ess import roi_posteriors
+    from bamf_amd.rjmcmc import Plan, RoiSet, alloc_outputs, fetch, lpt_order, run_batch
+
+    n = len(rois["side"])
+    plan = Plan(p, off, sig, back)
+    order = lpt_order(rois["side"], rois["side"], rois["n_true"])
+    rs = RoiSet.from_synth(rois, bg0=np.full(n, bg0, np.float32), order=order)
+    trace_len = nt // args.thin
+    cap = 128
+    outs = alloc_outputs(n, nb + nt, 129, cap, trace_len, False, False, rs, p.grid_zoom)
+    stream = _hip.Stream()
+    ev0, ev1 = _hip.Event(), _hip.Event()
+    for w in range(args.warmup):
+        run_batch(plan, rs, seed=SEED + 1000 + w, trace_thin=args.thin, stream=stream, sync=False, outs=outs,
+                  hist_len=129, final_cap=cap, trace_len=trace_len)
+    stream.synchronize()
+    print(f"[bench] warmup done ({args.warmup})", file=sys.stderr, flush=True)
+    _hip.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for k in range(args.steps):
+        run_batch(plan, rs, seed=SEED + k, trace_thin=args.thin, stream=stream, sync=False, outs=outs, hist_len=129,
+                  final_cap=cap, trace_len=trace_len)
+    ev1.record(stream)
+    ev1.synchronize()
+    _hip.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_ms(ev1) / args.steps
+    print(f"[bench] timed {args.steps} steps: {ms:.1f} ms/step (wall {wall:.2f} s)", file=sys.stderr, flush=True)
+    res = fetch(outs, n, nb + nt, 129, cap, trace_len)
+    assert np.all(res.status == 0), np.unique(res.status, return_counts=True)
+    ess, ok = roi_posteriors(res.trace_n, res.trace_photons)
+    value = float(ess.sum() / 200.0 / (ms / 1e3))
+    flops = float(res.alg_flops.astype(np.float64).sum())
+    achieved = flops / (ms / 1e3) / 1e12
+    rmse = localization_rmse_nm(res.final, rois["truth"], cap)
+    traffic = read_traffic(ROOT / "profiles" / f"rjmcmc_chain_{args.workload}_pmc.txt")
+    line = {
+        "metric": "ROI posteriors/sec at fixed ESS & RMSE: plain vs learned-proposal RJMCMC",
+        "value": value, "unit": "ROI posteriors/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (accept test f64)", "data": "synthetic (seeded forward-model ROIs)",
+        "config": {"workload": args.workload, "n_roi": n, "roi_sides": sorted(set(int(s) for s in rois["side"])),
+                   "jumps_per_chain": nb + nt, "burnin": nb, "ess_target": 200, "proposal": "plain",
+                   "parallelism": "1 GPU, one wavefront per chain, persistent work queue"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "kernel": "k_chain", "alg_flop_per_launch": flops},
+        "cpu_baseline": cpu,
+        "quality": {"strict_posteriors_per_s": float(ok.sum() / (ms / 1e3)), "median_ess": float(np.median(ess)),
+                    "rmse_nm_final_state": rmse, "chains_per_s": n / (ms / 1e3),
+                    "jumps_per_s": n * (nb + nt) / (ms / 1e3)},
+    }
+    if cpu is not None:
+        line["gpu_over_cpu"] = value / cpu["value"] if cpu["value"] > 0 else None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

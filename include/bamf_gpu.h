@@ -190,6 +190,7 @@ typedef struct bamf_rjmcmc_out {
     float* d_pgrid;          /* zoomed posterior images (update_grids, LibRJMCMC.cpp:1629-1648) */
     float* d_pback;
     const int64_t* d_grid_offset; /* [n_roi] offset of ROI r's (ysize*zoom) x (xsize*zoom) grid */
+    uint64_t* d_alg_flops;   /* [n_roi]: algorithmic floating-point operations of the chain (see DESIGN) */
 } bamf_rjmcmc_out;
 
 #define BAMF_RJ_PLAIN 0   /* counter-based Philox draws keyed by (seed, key[r], step) */
@@ -212,6 +213,10 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
  * bamf_rjmcmc_run itself; this returns how many chains the last run on the
  * plan claimed (synchronous). */
 int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed);
+
+/* The branch-free erf the chain's PSF stencils use, evaluated elementwise
+ * (a self-check of the device math; no counterpart in the reference). */
+int bamf_math_erf(const float* d_x, float* d_y, int64_t n, void* stream);
 
 #ifdef __cplusplus
 }

@@ -167,3 +167,20 @@ def test_status_and_counters_shape():
     assert np.all(g.hist.sum(1) == 50)
     assert np.all(g.counters[:, :5] >= 1)
     assert g.trace_n.shape == (3, 50)
+
+
+def test_branch_free_erf_accuracy():
+    """The stencil's erf against float64 erf over the range the stencils use."""
+    from scipy.special import erf
+
+    from bamf_amd import _hip
+    from bamf_amd._lib import check, lib
+
+    x = np.concatenate([np.linspace(-30, 30, 200001), [0.0, -0.0, 1e-30, -1e-30]]).astype(np.float32)
+    dx = _hip.DeviceBuffer.from_array(x)
+    dy = _hip.DeviceBuffer(x.nbytes)
+    check(lib().bamf_math_erf(dx.vp, dy.vp, len(x), None), "bamf_math_erf")
+    _hip.synchronize()
+    y = dy.download(np.float32)
+    err = np.abs(y.astype(np.float64) - erf(x.astype(np.float64)))
+    assert err.max() < 1e-6, err.max()
