@@ -74,6 +74,13 @@ def synchronize() -> None:
     check(_load().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
+def copy_d2d(dst: "DeviceBuffer", src: "DeviceBuffer", nbytes: int, stream: "Stream | None" = None) -> None:
+    """Asynchronous device-to-device copy on `stream`."""
+    if nbytes > min(dst.nbytes, src.nbytes):
+        raise ValueError("copy larger than its buffers")
+    check(_load().hipMemcpyAsync(dst.vp, src.vp, nbytes, D2D, stream.vp if stream else None), "hipMemcpyAsync")
+
+
 class DeviceBuffer:
     """An owned hipMalloc allocation."""
 

@@ -149,6 +149,21 @@ def read_traffic(profile: Path):
     return None
 
 
+def copy_calibration(mib: int) -> int:
+    """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of its dispatches
+    give the counters' bytes-per-KiB scale for the traffic figure."""
+    from bamf_amd import _hip
+
+    n = mib << 20
+    src, dst = _hip.DeviceBuffer.zeros(n), _hip.DeviceBuffer(n)
+    s = _hip.Stream()
+    for _ in range(3):
+        _hip.copy_d2d(dst, src, n, s)
+    s.synchronize()
+    print(json.dumps({"copy_bytes": n, "dispatches": 3}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,12 +174,16 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--thin", type=int, default=5)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many MiB (the HBM counter calibration run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (ROIs shard with no exchange; "
               "see the design notes). Run with --gpus 1.", file=sys.stderr)
         return 2
 
+    if args.copy_mib:
+        return copy_calibration(args.copy_mib)
     rois, nb, nt, rho, sig, off, back, bg0 = workload(args.workload)
     p = params(nb, nt, rho)
     cpu = None
