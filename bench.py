@@ -34,6 +34,9 @@ sys.path.insert(0, str(ROOT))
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 SEED = 20261203
+# Algorithmic FLOP of one XYNB fit (7 x 7, 20 iterations, separable form):
+# init 2.0 k, 20 x 1.95 k per iteration, CRLB + LL 2.55 k, 4 x 4 inverse 0.12 k.
+MLE_FLOP_PER_FIT = 43_800
 DX = 10.0
 
 
@@ -164,13 +167,150 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
+# ---------------------------------------------------------------- front end
+def _time_gpu(fn, stream, warmup, steps):
+    from bamf_amd import _hip
+
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    ev0, ev1 = _hip.Event(), _hip.Event()
+    ev0.record(stream)
+    for _ in range(steps):
+        fn()
+    ev1.record(stream)
+    ev1.synchronize()
+    return ev0.elapsed_ms(ev1) / steps
+
+
+def _cpu_pool(fn, chunks, cores):
+    import multiprocessing as mp
+
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        pool.map(fn, chunks)
+    return time.perf_counter() - t0
+
+
+_FE = {}
+
+
+def _cpu_findroi(fr):
+    from oracle import pyoracle as po
+
+    st, s1, s2, box, mv = _FE["args"]
+    for f in fr:
+        po.find_roi(st[f], st.shape[2], st.shape[1], 1, s1, s2, box, mv)
+
+
+def _cpu_mle(rng_):
+    from oracle import pyoracle as po
+
+    boxes, sigma = _FE["args"]
+    po.mle_xynb(boxes[rng_[0]:rng_[1]], sigma, 20)
+
+
+def bench_find_roi(args):
+    """FindROI over 2000 frames of 256 x 256 (Poisson emitters at 0.3 /um^2,
+    sigma 1.3, bg 10), sigma1 1.3, sigma2 2.6, Boxsize 7, default Minval."""
+    from bamf_amd import _hip, synth
+    from bamf_amd.find_roi import FindRoiDevice, default_minval
+
+    nf, size = 2000, 256
+    frames = synth.make_frames(nf, size, seed=SEED + 3)
+    s1, s2, box, mv = 1.3, 2.6, 7, default_minval()
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+
+        po.build()
+        cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
+        sample = args.cpu_sample_frames
+        _FE["args"] = (frames[:sample], s1, s2, box, mv)
+        dt = _cpu_pool(_cpu_findroi, [list(range(sample))[i::cores] for i in range(cores)], cores)
+        print(f"[bench] cpu baseline: {sample} frames in {dt:.1f} s", file=sys.stderr, flush=True)
+        cpu = {"value": sample * size * size / dt, "unit": "pixels/s", "cores": cores, "kind": "port",
+               "sample": f"first {sample} frames, oracle C FindROI (restated findROI.m + cuda_FindROI.cu), "
+                         f"{cores} forked workers, {dt:.1f} s wall"}
+    d = _hip.DeviceBuffer.from_array(frames)
+    dev = FindRoiDevice(size, size, nf, capacity=4_000_000)
+    s = _hip.Stream()
+    ms = _time_gpu(lambda: dev.run(d, s1, s2, box, mv, stream=s), s, args.warmup, args.steps)
+    n, *_ = dev.results()
+    px = nf * size * size
+    alg_bytes = 4.125 * px  # stack read once + 1 bit per pixel of the Res map
+    return {"metric": "FindROI pixels/s (2000 x 256 x 256 stack, DoG 1.3/2.6, Boxsize 7)", "value": px / (ms / 1e3),
+            "unit": "pixels/s", "ms_per_step": ms, "dtype": "f32", "higher_is_better": True,
+            "config": {"workload": "find_roi", "frames": nf, "size": size, "detections": int(n)},
+            "roofline": {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS,
+                         "unit": "GB/s", "frac": None, "kernel": "whole pipeline (one bamf_find_roi call)",
+                         "traffic": read_traffic(ROOT / "profiles" / "findroi_pipeline_find_roi_2000x256x256_pmc.txt"),
+                         "alg_bytes_per_launch": alg_bytes},
+            "cpu_baseline": cpu}
+
+
+def bench_gauss_mle(args):
+    """gaussMLE XYNB: 100,000 single-emitter 7 x 7 boxes, sigma 1.3, 20 iterations."""
+    from bamf_amd import _hip, synth
+    from bamf_amd.gauss_mle import gauss_mle_device
+
+    n, sz, sigma = 100_000, 7, 1.3
+    boxes, _ = synth.make_mle_boxes(n, sz, seed=SEED + 4, sigma=sigma)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+
+        po.build()
+        cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
+        sample = args.cpu_sample_fits
+        _FE["args"] = (boxes[:sample], sigma)
+        step = (sample + cores - 1) // cores
+        dt = _cpu_pool(_cpu_mle, [(i, min(sample, i + step)) for i in range(0, sample, step)], cores)
+        print(f"[bench] cpu baseline: {sample} fits in {dt:.1f} s", file=sys.stderr, flush=True)
+        cpu = {"value": sample / dt, "unit": "fits/s", "cores": cores, "kind": "port",
+               "sample": f"first {sample} boxes, oracle C kernel_MLEFit_XYNB_ restatement, {cores} forked workers, "
+                         f"{dt:.1f} s wall"}
+    d = _hip.DeviceBuffer.from_array(boxes)
+    dp, dc, dl = _hip.DeviceBuffer(16 * n), _hip.DeviceBuffer(16 * n), _hip.DeviceBuffer(4 * n)
+    s = _hip.Stream()
+    ms = _time_gpu(lambda: gauss_mle_device(d, sigma, sz, 20, n, dp, dc, dl, stream=s), s, args.warmup, args.steps)
+    flops = MLE_FLOP_PER_FIT * n
+    achieved = flops / (ms / 1e3) / 1e12
+    return {"metric": "gaussMLE XYNB fits/s (7 x 7 boxes, 20 iterations)", "value": n / (ms / 1e3), "unit": "fits/s",
+            "ms_per_step": ms, "dtype": "f32", "higher_is_better": True,
+            "config": {"workload": "gauss_mle", "fits": n, "box": sz, "iterations": 20},
+            "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "frac": achieved / PEAK_FP32_TFLOPS, "kernel": "k_mle_xynb",
+                         "traffic": read_traffic(ROOT / "profiles" / "mle_xynb_gauss_mle_100k_7x7_pmc.txt"),
+                         "alg_flop_per_launch": flops},
+            "cpu_baseline": cpu}
+
+
+def front_end_line(args):
+    res = bench_find_roi(args) if args.workload == "find_roi" else bench_gauss_mle(args)
+    print(f"[bench] {args.workload}: {res['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    if res["roofline"]["bound"] == "hbm":
+        r = res["roofline"]
+        r["achieved"] = r["alg_bytes_per_launch"] / (res["ms_per_step"] / 1e3) / 1e9
+        r["frac"] = r["achieved"] / r["peak"]
+    line = {"metric": res["metric"], "value": res["value"], "unit": res["unit"], "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": res["dtype"], "data": "synthetic (seeded forward-model frames/boxes)",
+            "config": res["config"], "roofline": res["roofline"], "cpu_baseline": res["cpu_baseline"]}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batched_uneven", choices=["batched_uneven", "plain_16"])
+    ap.add_argument("--workload", default="batched_uneven",
+                    choices=["batched_uneven", "plain_16", "find_roi", "gauss_mle"])
     ap.add_argument("--cpu-sample", type=int, default=512)
+    ap.add_argument("--cpu-sample-frames", type=int, default=2000)
+    ap.add_argument("--cpu-sample-fits", type=int, default=100000)
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--thin", type=int, default=5)
@@ -184,6 +324,8 @@ def main():
 
     if args.copy_mib:
         return copy_calibration(args.copy_mib)
+    if args.workload in ("find_roi", "gauss_mle"):
+        return front_end_line(args)
     rois, nb, nt, rho, sig, off, back, bg0 = workload(args.workload)
     p = params(nb, nt, rho)
     cpu = None
