@@ -17,16 +17,17 @@
 //     (+new / -old): lanes 0..31 evaluate the x stencil, lanes 32..63 the y
 //     stencil (two erf per lane), and every lane updates its pixels;
 //   - the log-likelihood ratio sum(M - M' + D log(M'/M)) is reduced across
-//     the wavefront with lane shuffles; the full model is rebuilt from the
+//     the wavefront with DPP row scans; the full model is rebuilt from the
 //     emitter set after every `recompute_every` accepted jumps so that the
 //     incremental updates cannot drift;
-//   - every jump draws its uniforms from one counter-based Philox4x32-10
-//     batch: lane i holds draws 4i..4i+3 of the step, and draw j is read with
-//     readlane (replay mode loads the same batch layout from the tape);
-//   - accept: double-precision test log(U) < log PR with log PR summed from
-//     the reference's single-precision factors; when the reference's own
-//     single-precision PR is saturated (inf, 0, NaN or beyond 1e+-30) the
-//     reference's test U < PR decides, so overflow semantics are kept.
+//   - draws live in a 64-position register window: lane l holds uniform
+//     wbase + l of the step (a counter-based Philox4x32-10 word, or the tape
+//     in replay mode) and the Irwin-Hall sum of the twelve uniforms starting
+//     there, so urand() and randn() are one readlane each.  Every jump but
+//     the group moves fits in one window; the group moves re-centre it;
+//   - accept: the reference's single-precision test U < PR
+//     (LibRJMCMC.cpp:939); the double-precision log PR is formed only when
+//     step records are requested.
 #include <cmath>
 
 #include "common.hpp"
@@ -44,7 +45,7 @@ constexpr double kPiD = 3.141592;  // LibRJMCMCreturn.hpp:2 (a double literal)
 
 enum { C_N = 0, C_A = 5, C_NFG = 10, C_AFG = 15, C_NBG = 20, C_ABG = 25 };
 enum { J_MOVE = 0, J_SPLIT = 1, J_MERGE = 2, J_BIRTH = 3, J_DEATH = 4 };
-enum { ST_OK = 0, ST_NMAX = 1, ST_UNSUPPORTED = 2, ST_TAPE = 3 };
+enum { ST_OK = 0, ST_NMAX = 1, ST_UNSUPPORTED = 2, ST_TAPE = 3, ST_DRAWS = 4 };
 
 struct Em {
     float I[kNMax];
@@ -55,7 +56,6 @@ struct Em {
 
 struct WaveLds {
     Em e[2];
-    uint32_t draw[256];  // the step's draw batch: draw j at [j & 255]
     int delta[kMaxDelta];
     float sten[64];
     int nb[kNMax + 1];
@@ -73,6 +73,24 @@ __device__ __forceinline__ float bcastf(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// DPP move of a float (lanes whose source row is masked off or out of range read 0).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+// Inclusive prefix sum over the wavefront: Hillis-Steele within each row of
+// 16 lanes (row_shr 1, 2, 4, 8), then the row totals carried forward with
+// row_bcast:15 and row_bcast:31.
+__device__ __forceinline__ float wave_scan(float v) {
+    v = v + dpp_f<0x111, 0xf>(v);
+    v = v + dpp_f<0x112, 0xf>(v);
+    v = v + dpp_f<0x114, 0xf>(v);
+    v = v + dpp_f<0x118, 0xf>(v);
+    v = v + dpp_f<0x142, 0xa>(v);
+    v = v + dpp_f<0x143, 0xc>(v);
+    return v;
+}
 __device__ __forceinline__ int kth_bit(uint64_t m, int k) {
     for (int i = 0; i < k; i++) m &= m - 1;
     return __builtin_ctzll(m);
@@ -96,26 +114,6 @@ __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, ui
     o[1] = c1;
     o[2] = c2;
     o[3] = c3;
-}
-
-// Fills the wave's draw batch b of step `step`: Philox block (step, 64b + lane)
-// gives draws 256b + 4 lane .. +3; in replay mode the same positions of the tape.
-__device__ __noinline__ void refill_batch(uint32_t* draw, int lane, int mode, int step, int b, uint32_t key,
-                                          uint32_t seed_lo, uint32_t seed_hi, const uint32_t* tape, int64_t tpos,
-                                          int64_t tend) {
-    uint32_t o[4];
-    if (mode == BAMF_RJ_REPLAY) {
-        const int64_t base = tpos + (int64_t)b * 256 + 4 * lane;
-#pragma unroll
-        for (int c = 0; c < 4; c++) o[c] = (base + c < tend) ? tape[base + c] : 0u;
-    } else {
-        philox((uint32_t)step, (uint32_t)(b * 64 + lane), seed_hi, 0u, key, seed_lo, o);
-#pragma unroll
-        for (int c = 0; c < 4; c++) o[c] >>= 1;  // 31-bit draws, the range of rand()
-    }
-#pragma unroll
-    for (int c = 0; c < 4; c++) draw[4 * lane + c] = o[c];
-    wsync();
 }
 
 // LibRJMCMCreturn.hpp:28
@@ -182,8 +180,9 @@ struct Chain {
     float cBG, cABG, cBBG, tBG, tABG, tBBG;
     float d[kMaxPPL], m[kMaxPPL], mt[kMaxPPL];
     uint32_t adr[kMaxPPL];  // x | y << 8 of each slot's pixel; 0xffff when the slot is empty
-    // draws
-    int dbatch, dj;
+    // draws: window of 64 positions starting at wbase; dj = next position
+    int wbase, dj;
+    float wu, wr;  // uniform at wbase + lane, Irwin-Hall sum of the 12 uniforms from there
     uint32_t key;
     int step;
     int64_t tbase, tlen, tcur;
@@ -205,6 +204,8 @@ struct Chain {
     float r_llr, r_pr;
     double r_logpr;
     int ref_flags, ref_pix;
+    double lp_prod;  // product of the posterior-ratio factors other than the LLR, in double
+    bool lp_neg;     // merge divides by the product
     uint64_t aflops;  // algorithmic work of this chain
 
     __device__ Chain(const ChainArgs* a_, WaveLds& L_, int lane_) : a(a_), L(L_), lane(lane_) {}
@@ -222,22 +223,46 @@ struct Chain {
     __device__ __forceinline__ int sy(int s) const { return (adr[s] >> 8) & 0xffu; }
 
     // ---------------- draws ----------------
-    __device__ __forceinline__ uint32_t raw() {
-        const int j = dj++;
-        const int b = j >> 8;
-        if (b != dbatch) {
-            refill_batch(L.draw, lane, a->mode, step, b, key, a->seed_lo, a->seed_hi, a->tape, tbase + tcur,
-                         tbase + tlen);
-            dbatch = b;
+    // Window of draws wbase .. wbase + 63 of this step.  Draw j is word j % 4
+    // of Philox4x32-10 with counter (step, j / 4, seed_hi, 0) and key
+    // (roi key, seed_lo), shifted to 31 bits (the range of rand()); replay
+    // mode reads the tape instead (0 past its end, flagged at the step end).
+    // urand = (float)raw / 2^31 (LibRJMCMCreturn.hpp:17); the twelve-term
+    // sum of randn (LibRJMCMCreturn.hpp:23-26) is formed left to right from
+    // each position, so randn() is that sum minus 6, bit for bit.
+    __device__ __forceinline__ void fill_window(int base) {
+        wbase = base;
+        const uint32_t q = (uint32_t)(base + lane);
+        uint32_t w;
+        if (a->mode == BAMF_RJ_REPLAY) {
+            w = (tcur + (int64_t)q < tlen) ? a->tape[tbase + tcur + q] : 0u;
+        } else {
+            uint32_t o[4];
+            philox((uint32_t)step, q >> 2, a->seed_hi, 0u, key, a->seed_lo, o);
+            const uint32_t c = q & 3u;
+            w = (c == 0 ? o[0] : c == 1 ? o[1] : c == 2 ? o[2] : o[3]) >> 1;
         }
-        if (a->mode == BAMF_RJ_REPLAY && tcur + j >= tlen) status = ST_TAPE;
-        return L.draw[j & 255];
+        wu = (float)w / 2147483648.0f;
+        float t = wu;
+#pragma unroll
+        for (int i = 1; i < 12; i++) t = t + __shfl_down(wu, i);
+        wr = t;
     }
-    __device__ __forceinline__ float urand() { return (float)raw() / 2147483648.0f; }
+    // Re-centres the window when fewer than n positions remain in it.
+    __device__ __forceinline__ void need(int n) {
+        if (dj - wbase + n > 64) fill_window(dj);
+    }
+    __device__ __forceinline__ float urand() {
+        const int k = dj - wbase;
+        if (k > 63) status = ST_DRAWS;
+        dj++;
+        return bcastf(wu, k & 63);
+    }
     __device__ __forceinline__ float randn() {
-        float t = 0;
-        for (int i = 0; i < 12; i++) t = t + urand();
-        return t - 6.0f;
+        const int k = dj - wbase;
+        if (k > 52) status = ST_DRAWS;
+        dj += 12;
+        return bcastf(wr, k & 63) - 6.0f;
     }
     __device__ __forceinline__ void cnt_inc(int k) {
         if (lane == 0) L.cnt[k]++;
@@ -390,6 +415,7 @@ struct Chain {
             if (Ua < 1.0f / (float)num) np = num;
             else np = min((int)(Ua * num), num - 1);
             for (int n = 0; n < np; n++) {
+                need(1);
                 Ua = urand();
                 const int k = min((int)(Ua * (num - n)), num - n - 1);
                 const int idx = pick(m0, m1, k);
@@ -479,22 +505,17 @@ struct Chain {
 #pragma unroll
         for (int s = 0; s < kMaxPPL; s++)
             if (lane + 64 * s < np_) acc = acc + ((m[s] - mt[s]) + d[s] * logf(mt[s] / m[s]));
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-        return bcastf(acc, 0);
+        return bcastf(wave_scan(acc), 63);
     }
 
     // ---------------- accept ----------------
-    __device__ __forceinline__ bool decide(float pr, double logpr) {
+    // Accept = (urand() < PR) (LibRJMCMC.cpp:939 and the other jumps).
+    __device__ __forceinline__ bool decide(float pr) {
+        need(1);
         const float U = urand();
-        bool acc;
-        if (isfinite(pr) && pr > 1e-30f && pr < 1e30f) {
-            acc = log((double)U) < logpr;
-        } else {
-            acc = U < pr;
-            r_flags |= 16;
-        }
+        bool acc = U < pr;
+        if (!(isfinite(pr) && pr > 1e-30f && pr < 1e30f)) r_flags |= 16;
         r_pr = pr;
-        r_logpr = logpr;
         if (acc) r_flags |= 1;
         if (a->mode == BAMF_RJ_REPLAY) {
             const bool forced = (ref_flags & 1) != 0;
@@ -567,6 +588,7 @@ struct Chain {
                 const int k = L.nb[ii];
                 const bool fg = E.S[k] == 1;
                 const float xs = fg ? p.x_std_fg : p.x_std_bg;
+                need(36);
                 const float tx = E.X[k] + xs * randn();
                 const float ty = E.Y[k] + xs * randn();
                 const float ti = (I0 / (float)(np + 1)) * (0.15f * randn() + 1);
@@ -603,6 +625,7 @@ struct Chain {
                 const bool fg = E.S[k] == 1;
                 cnt_inc(fg ? C_NFG + J_MOVE : C_NBG + J_MOVE);
                 const float is = fg ? p.i_std_fg : p.i_std_bg, xs = fg ? p.x_std_fg : p.x_std_bg;
+                need(36);
                 const float ti = T.I[k] + is * randn();
                 const float tx = T.X[k] + xs * randn();
                 const float ty = T.Y[k] + xs * randn();
@@ -637,12 +660,13 @@ struct Chain {
         return EV_MOVE;
     }
     // LibRJMCMC.cpp:918-936
-    __device__ __forceinline__ float fin_move(float sum, float LLR, double& logpr) {
+    __device__ __forceinline__ float fin_move(float sum, float LLR) {
         const float BgPDF = findpdf(a->off, a->p.dx, cBG);
         const float BgPDFtest = findpdf(a->off, a->p.dx, tBG);
         const float PrBg = (BgPDFtest / BgPDF) * expf(-(tBBG * tBBG - cBBG * cBBG) / 2) *
                            expf(-(tABG * tABG - cABG * cABG) / 2);
-        logpr = (double)sum + log((double)f_pri * (double)PrBg);
+        lp_prod = (double)f_pri * (double)PrBg;
+        lp_neg = false;
         return LLR * f_pri * PrBg;
     }
 
@@ -678,7 +702,7 @@ struct Chain {
         return EV_SPLIT;
     }
     // LibRJMCMC.cpp:1030-1070
-    __device__ __forceinline__ float fin_split(float sum, float LLR, double& logpr) {
+    __device__ __forceinline__ float fin_split(float sum, float LLR) {
         const bamf_mcmc_params& p = a->p;
         const Em& E = cur();
         const Em& T = tst();
@@ -697,7 +721,8 @@ struct Chain {
         const double om = (double)(1 - f_u1);
         const float Jacob = (float)((double)f_ijp / (om * om));
         const float P12 = P1 / P2;
-        logpr = (double)sum + log((double)PrI * (double)PrX * (double)Us * (double)Jacob * (double)P12 * (double)PrM);
+        lp_prod = (double)PrI * (double)PrX * (double)Us * (double)Jacob * (double)P12 * (double)PrM;
+        lp_neg = false;
         return LLR * PrI * PrX * Us * Jacob * P12 * PrM;
     }
 
@@ -749,7 +774,7 @@ struct Chain {
         return EV_MERGE;
     }
     // LibRJMCMC.cpp:1166-1217
-    __device__ __forceinline__ float fin_merge(float sum, float LLR, double& logpr) {
+    __device__ __forceinline__ float fin_merge(float sum, float LLR) {
         const bamf_mcmc_params& p = a->p;
         const Em& E = cur();
         const Em& T = tst();
@@ -768,8 +793,8 @@ struct Chain {
         const double om = (double)(1 - f_u1);
         const float Jacob = (float)((double)f_ijp / (om * om));
         const float PrM = meanN / NEm;
-        logpr = (double)sum - log((double)PrI * (double)PrX * (double)Jacob * (double)Us * (double)PrM * (double)P1 /
-                                  (double)P2);
+        lp_prod = (double)PrI * (double)PrX * (double)Jacob * (double)Us * (double)PrM * (double)P1 / (double)P2;
+        lp_neg = true;
         return LLR / (PrI * PrX * Jacob * Us * PrM * P1 / P2);
     }
 
@@ -789,10 +814,7 @@ struct Chain {
                     const float r = d[s] - m[s];
                     v = (0 > r) ? 0.0f : r;
                 }
-                for (int o = 1; o < 64; o <<= 1) {
-                    const float t = __shfl_up(v, o);
-                    if (lane >= o) v += t;
-                }
+                v = wave_scan(v);
                 cum[s] = carry + v;
                 carry = bcastf(cum[s], 63);
             }
@@ -822,10 +844,7 @@ struct Chain {
             if (s < ppl) {
                 float v = (lane + 64 * s < np_) ? a->prop_map[off + lane + 64 * s] : 0.0f;
                 q[s] = v;
-                for (int o = 1; o < 64; o <<= 1) {
-                    const float t = __shfl_up(v, o);
-                    if (lane >= o) v += t;
-                }
+                v = wave_scan(v);
                 cum[s] = carry + v;
                 carry = bcastf(cum[s], 63);
             }
@@ -878,10 +897,11 @@ struct Chain {
         return EV_BIRTH;
     }
     // LibRJMCMC.cpp:1297-1302
-    __device__ __forceinline__ float fin_birth(float sum, float LLR, double& logpr) {
+    __device__ __forceinline__ float fin_birth(float sum, float LLR) {
         const float NEm = (float)nem_cur();
         const float PrM = meanN / (NEm + 1);
-        logpr = (double)sum + log((double)P3 / ((double)P4 * (double)xb * (double)yb * (double)f_pixprob) * (double)PrM);
+        lp_prod = (double)P3 / ((double)P4 * (double)xb * (double)yb * (double)f_pixprob) * (double)PrM;
+        lp_neg = false;
         return LLR * P3 / (P4 * xb * yb * f_pixprob) * PrM;
     }
 
@@ -904,11 +924,12 @@ struct Chain {
         return EV_DEATH;
     }
     // LibRJMCMC.cpp:1356-1360
-    __device__ __forceinline__ float fin_death(float sum, float LLR, double& logpr) {
+    __device__ __forceinline__ float fin_death(float sum, float LLR) {
         const float NEm = (float)nem_cur();
         const float PrM = NEm / meanN;
         const float P43 = P4 / P3;
-        logpr = (double)sum + log((double)P43 * (double)PrM);
+        lp_prod = (double)P43 * (double)PrM;
+        lp_neg = false;
         return LLR * P43 * PrM;
     }
 
@@ -974,16 +995,24 @@ struct Chain {
                 const float sum = proposal_llr_sum();
                 const float LLR = expf(sum);
                 r_llr = LLR;
-                double logpr = 0.0;
                 float PR;
                 switch (evk) {
-                    case EV_MOVE: PR = fin_move(sum, LLR, logpr); break;
-                    case EV_SPLIT: PR = fin_split(sum, LLR, logpr); break;
-                    case EV_MERGE: PR = fin_merge(sum, LLR, logpr); break;
-                    case EV_BIRTH: PR = fin_birth(sum, LLR, logpr); break;
-                    default: PR = fin_death(sum, LLR, logpr); break;
+                    case EV_MOVE: PR = fin_move(sum, LLR); break;
+                    case EV_SPLIT: PR = fin_split(sum, LLR); break;
+                    case EV_MERGE: PR = fin_merge(sum, LLR); break;
+                    case EV_BIRTH: PR = fin_birth(sum, LLR); break;
+                    default: PR = fin_death(sum, LLR); break;
                 }
-                if (decide(PR, logpr)) {
+                const bool acc = decide(PR);
+                // log PR in double: the sum plus the log of the factors'
+                // product (kept for step records and the accepted-jump total)
+                double logpr = __builtin_nan("");
+                if (acc || a->steps) {
+                    const double l = log(lp_prod);
+                    logpr = (double)sum + (lp_neg ? -l : l);
+                }
+                r_logpr = logpr;
+                if (acc) {
                     cnt_inc(acc_c1);
                     cnt_inc(acc_c2);
                     ci ^= 1;
@@ -1082,7 +1111,7 @@ struct Chain {
                 P1 = a->pvt[1]; P2 = a->pvt[2]; P3 = a->pvt[3]; P4 = a->pvt[4];
             }
             dj = 0;
-            dbatch = -1;
+            fill_window(0);
             r_jt = 0;
             r_flags = 0;
             r_pix = -1;
@@ -1122,8 +1151,7 @@ struct Chain {
                 if (a->trace_n && t % a->trace_thin == 0 && t / a->trace_thin < a->trace_len) {
                     const Em& c = cur();
                     float v = (lane < cN ? c.I[lane] : 0.0f) + (lane + 64 < cN ? c.I[lane + 64] : 0.0f);
-                    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-                    v = bcastf(v, 0);
+                    v = bcastf(wave_scan(v), 63);
                     if (lane == 0) {
                         const int64_t ti = (int64_t)roi * a->trace_len + t / a->trace_thin;
                         a->trace_n[ti] = (int16_t)cN;
@@ -1131,6 +1159,7 @@ struct Chain {
                     }
                 }
             }
+            if (a->mode == BAMF_RJ_REPLAY && tcur + dj > tlen) status = ST_TAPE;
             tcur += dj;
         }
         wsync();
