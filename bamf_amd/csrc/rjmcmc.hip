@@ -45,7 +45,7 @@ constexpr double kPiD = 3.141592;  // LibRJMCMCreturn.hpp:2 (a double literal)
 
 enum { C_N = 0, C_A = 5, C_NFG = 10, C_AFG = 15, C_NBG = 20, C_ABG = 25 };
 enum { J_MOVE = 0, J_SPLIT = 1, J_MERGE = 2, J_BIRTH = 3, J_DEATH = 4 };
-enum { ST_OK = 0, ST_NMAX = 1, ST_UNSUPPORTED = 2, ST_TAPE = 3, ST_DRAWS = 4 };
+enum { ST_OK = 0, ST_NMAX = 1, ST_UNSUPPORTED = 2, ST_TAPE = 3, ST_DRAWS = 4, ST_SIZE = 5 };
 
 struct Em {
     float I[kNMax];
@@ -54,6 +54,7 @@ struct Em {
     int S[kNMax];
 };
 
+template <int PPL>
 struct WaveLds {
     Em e[2];
     int delta[kMaxDelta];
@@ -61,6 +62,7 @@ struct WaveLds {
     int nb[kNMax + 1];
     int hist[kNMax + 1];
     int cnt[32];
+    float dpx[64 * PPL];  // the ROI's data in the pixel-slot layout: slot s at [64 s + lane]
 };
 
 __device__ __forceinline__ void wsync() {
@@ -73,6 +75,12 @@ __device__ __forceinline__ float bcastf(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// Pixel slots of an X x Y ROI: columns padded to a power of two, 64 / Ypad per slot.
+__host__ __device__ constexpr int slots_for(int X, int Y) {
+    return Y <= 1 ? (X + 63) / 64 : Y <= 2 ? (X + 31) / 32 : Y <= 4 ? (X + 15) / 16 : Y <= 8 ? (X + 7) / 8
+         : Y <= 16 ? (X + 3) / 4 : Y <= 32 ? (X + 1) / 2 : X * ((Y + 63) / 64);
+}
 
 // DPP move of a float (lanes whose source row is masked off or out of range read 0).
 template <int CTRL, int ROW_MASK>
@@ -167,9 +175,10 @@ __device__ __forceinline__ float normpdf(float x, float u, float s) {
 // Proposal kinds that need the likelihood ratio, with their accept counters.
 enum { EV_NONE = 0, EV_MOVE, EV_SPLIT, EV_MERGE, EV_BIRTH, EV_DEATH };
 
+template <int PPL>
 struct Chain {
     const ChainArgs* a;  // the kernel argument block (in the kernarg segment)
-    WaveLds& L;
+    WaveLds<PPL>& L;
     const int lane;
     int roi, X, Y, P, ppl;
     float C, meanN;
@@ -178,8 +187,14 @@ struct Chain {
     int ci;        // current emitter buffer
     int cN, tN;
     float cBG, cABG, cBBG, tBG, tABG, tBBG;
-    float d[kMaxPPL], m[kMaxPPL], mt[kMaxPPL];
-    uint32_t adr[kMaxPPL];  // x | y << 8 of each slot's pixel; 0xffff when the slot is empty
+    float m[PPL], mt[PPL];
+    // Pixel layout: the column height is padded to Ypad = 2^lgY >= Y and a
+    // slot holds cps = 64 / Ypad whole columns: lane l of slot s is pixel
+    // (x = s * cps + (l >> lgY), y = l & (Ypad - 1)), present when x < X and
+    // y < Y.  Lane order within a slot is then the reference's pixel order
+    // (x * Y + y), and a lane's y never changes between slots.
+    int lgY, cps;
+    int xsub, ysub;
     // draws: window of 64 positions starting at wbase; dj = next position
     int wbase, dj;
     float wu, wr;  // uniform at wbase + lane, Irwin-Hall sum of the 12 uniforms from there
@@ -194,6 +209,10 @@ struct Chain {
     double logpost;
     int ndelta;
     bool bgdelta;
+    int sxa, sxb;      // x range of the last stencil's nonzero columns (sxa > sxb: none)
+    int lo_s, hi_s;    // pixel slots the proposal changes
+    bool e_ok;         // the current set passes Emitters::test
+    bool t_bad;        // a value written into the proposed set fails it
     // proposal bookkeeping for the shared evaluation
     int ev, jt, acc_c1, acc_c2;
     bool need_test;
@@ -208,19 +227,13 @@ struct Chain {
     bool lp_neg;     // merge divides by the product
     uint64_t aflops;  // algorithmic work of this chain
 
-    __device__ Chain(const ChainArgs* a_, WaveLds& L_, int lane_) : a(a_), L(L_), lane(lane_) {}
+    __device__ Chain(const ChainArgs* a_, WaveLds<PPL>& L_, int lane_) : a(a_), L(L_), lane(lane_) {}
 
     __device__ __forceinline__ Em& cur() { return L.e[ci]; }
     __device__ __forceinline__ Em& tst() { return L.e[ci ^ 1]; }
-    // Pixel p = lane + 64 s exists when p < P; the count is laundered per loop
-    // so the sixteen per-slot masks are recomputed instead of kept live.
-    __device__ __forceinline__ int pix_count() const {
-        int n = P;
-        asm volatile("" : "+v"(n));
-        return n;
-    }
-    __device__ __forceinline__ int sx(int s) const { return adr[s] & 0xffu; }
-    __device__ __forceinline__ int sy(int s) const { return (adr[s] >> 8) & 0xffu; }
+    __device__ __forceinline__ int px(int s) const { return s * cps + xsub; }
+    __device__ __forceinline__ bool has(int s) const { return ysub < Y && px(s) < X; }
+    __device__ __forceinline__ int pidx(int s) const { return px(s) * Y + ysub; }
 
     // ---------------- draws ----------------
     // Window of draws wbase .. wbase + 63 of this step.  Draw j is word j % 4
@@ -279,6 +292,7 @@ struct Chain {
             t.S[n] = c.S[n];
         }
         tN = cN;
+        t_bad = false;
         tBG = cBG;
         tABG = cABG;
         tBBG = cBBG;
@@ -286,7 +300,13 @@ struct Chain {
         bgdelta = false;
         wsync();
     }
+    // One emitter's part of Emitters::test (LibRJMCMC.cpp:492-514).
+    __device__ __forceinline__ bool bad_val(float I, float x, float y) const {
+        return (I < 0) || (x < (float)(-bnd)) || (y < (float)(-bnd)) || (x > (float)(X + bnd)) ||
+               (y > (float)(Y + bnd));
+    }
     __device__ __forceinline__ void set_t(int k, float I, float x, float y) {
+        t_bad |= bad_val(I, x, y);
         if (lane == 0) {
             tst().I[k] = I;
             tst().X[k] = x;
@@ -298,6 +318,7 @@ struct Chain {
             status = ST_NMAX;
             return false;
         }
+        t_bad |= bad_val(I, x, y);
         if (lane == 0) {
             tst().I[tN] = I;
             tst().X[tN] = x;
@@ -341,8 +362,12 @@ struct Chain {
         if (lane == 0) L.delta[ndelta] = k | (which << 8) | ((neg ? 1 : 0) << 9);
         ndelta++;
     }
-    // Emitters::test (LibRJMCMC.cpp:492-514) on the proposed set.
+    // Emitters::test (LibRJMCMC.cpp:492-514) on the proposed set: only the
+    // values the proposal wrote can fail it once the current set is known to
+    // pass; otherwise every emitter is checked.
     __device__ __forceinline__ bool test_t() {
+        if (t_bad) return false;
+        if (e_ok) return true;
         const Em& t = tst();
         bool bad = false;
 #pragma unroll
@@ -429,46 +454,42 @@ struct Chain {
     }
 
     // ---------------- model ----------------
-    // Stencil of one emitter (LibRJMCMC.cpp:15-36): sten[x] = I*0.5*(erf(e_x+1) - erf(e_x)),
-    // sten[32 + y] = 0.5*(erf(e_y+1) - erf(e_y)) with pixel edges e_k = (k - pos)*C.
-    // Each edge's erf is evaluated once (branch-free, one lane per edge) and
-    // shared with the neighbouring pixel through a lane shuffle.
-    __device__ __forceinline__ void stencil(float I, float xe, float ye) {
+    // Stencil of one emitter (LibRJMCMC.cpp:15-36): N_X[x] = I*0.5*(erf(e_x+1) - erf(e_x)) into
+    // sten[x] (lanes x < X), and this lane's N_Y[y] = 0.5*(erf(e_y+1) - erf(e_y)) returned, with
+    // pixel edges e_k = (k - pos)*C.  erf is the branch-free one, two edges per lane.
+    __device__ __forceinline__ float stencil(float I, float xe, float ye) {
         const float l = (float)lane;
-        if (X + Y + 2 <= 64) {
-            // x edges on lanes 0..X, y edges on lanes X+1..X+Y+1
-            const int by = X + 1;
-            const float arg = lane < by ? (l - xe) * C : ((float)(lane - by) - ye) * C;
-            const float e = erf_nr(arg);
-            const float en = __shfl_down(e, 1);
-            if (lane < X) L.sten[lane] = I * 0.5f * (en - e);
-            else if (lane >= by && lane < by + Y) L.sten[32 + lane - by] = 0.5f * (en - e);
-        } else {
-            const float ex = erf_nr((l - xe) * C), ey = erf_nr((l - ye) * C);
-            const float exn = __shfl_down(ex, 1), eyn = __shfl_down(ey, 1);
-            if (lane < X) L.sten[lane] = I * 0.5f * (exn - ex);
-            if (lane < Y) L.sten[32 + lane] = 0.5f * (eyn - ey);
-        }
+        const float vx = 0.5f * (erf_nr(((float)(lane + 1) - xe) * C) - erf_nr((l - xe) * C));
+        const float vy = 0.5f * (erf_nr(((float)(ysub + 1) - ye) * C) - erf_nr(((float)ysub - ye) * C));
+        // columns x >= X and rows y >= Y (padding lanes) get exact zeros, so
+        // the pixel loops need no lane masks
+        L.sten[lane] = lane < X ? I * vx : 0.0f;
+        // columns where the x factor (before the intensity) is nonzero; a
+        // zero factor makes the whole column's contribution exactly zero
+        const uint64_t nz = ballot(lane < X && vx != 0.0f);
+        sxa = nz ? (int)__builtin_ctzll(nz) : 1;
+        sxb = nz ? 63 - (int)__builtin_clzll(nz) : 0;
         wsync();
+        return ysub < Y ? vy : 0.0f;
     }
     // Full model of the current set: background plane + sCMOS term, then the
     // emitters in order (Model::Model, LibRJMCMC.cpp:42-59).
     __device__ __forceinline__ void rebuild_model() {
-        const int np_ = pix_count();
         const int64_t off = a->pix_off[roi];
 #pragma unroll
-        for (int s = 0; s < kMaxPPL; s++)
-            if (lane + 64 * s < np_) {
-                const float sc = a->scmos ? a->scmos[off + lane + 64 * s] : 0.0f;
-                m[s] = cBG + cABG * (float)sx(s) + cBBG * (float)sy(s) + sc;
+        for (int s = 0; s < PPL; s++)
+            if (s < ppl) {
+                const bool ok = has(s);
+                const float sc = (a->scmos && ok) ? a->scmos[off + pidx(s)] : 0.0f;
+                m[s] = ok ? cBG + cABG * (float)px(s) + cBBG * (float)ysub + sc : 1.0f;
             }
         const Em& c = cur();
         aflops += (uint64_t)(4 * P) + (uint64_t)cN * (uint64_t)(2 * P + 8 * (X + Y));
         for (int n = 0; n < cN; n++) {
-            stencil(c.I[n], c.X[n], c.Y[n]);
+            const float vy = stencil(c.I[n], c.X[n], c.Y[n]);
 #pragma unroll
-            for (int s = 0; s < kMaxPPL; s++)
-                if (lane + 64 * s < np_) m[s] = m[s] + L.sten[sx(s)] * L.sten[32 + sy(s)];
+            for (int s = 0; s < PPL; s++)
+                if (s < ppl) m[s] = m[s] + L.sten[px(s)] * vy;
             wsync();
         }
         nacc = 0;
@@ -476,15 +497,18 @@ struct Chain {
     // M' = M + background change + the stencil list; returns the sum over
     // pixels of M - M' + D log(M'/M) (likelihood_ratio, LibRJMCMC.cpp:1677-1678).
     __device__ __forceinline__ float proposal_llr_sum() {
-        const int np_ = pix_count();
         aflops += (uint64_t)(6 * P) + (uint64_t)ndelta * (uint64_t)(2 * P + 8 * (X + Y)) + (bgdelta ? 5 * P : 0);
 #pragma unroll
-        for (int s = 0; s < kMaxPPL; s++) mt[s] = m[s];
+        for (int s = 0; s < PPL; s++) mt[s] = m[s];
+        lo_s = ppl;
+        hi_s = -1;
         if (bgdelta) {
             const float db = tBG - cBG, da = tABG - cABG, dbb = tBBG - cBBG;
 #pragma unroll
-            for (int s = 0; s < kMaxPPL; s++)
-                if (lane + 64 * s < np_) mt[s] = mt[s] + (db + da * (float)sx(s) + dbb * (float)sy(s));
+            for (int s = 0; s < PPL; s++)
+                if (s < ppl && has(s)) mt[s] = mt[s] + (db + da * (float)px(s) + dbb * (float)ysub);
+            lo_s = 0;
+            hi_s = ppl - 1;
         }
         wsync();
         for (int e = 0; e < ndelta; e++) {
@@ -492,19 +516,32 @@ struct Chain {
             const Em& E = ((code >> 8) & 1) ? tst() : cur();
             const int k = code & 255;
             const bool neg = (code >> 9) & 1;
-            stencil(E.I[k], E.X[k], E.Y[k]);
+            const float vy = stencil(E.I[k], E.X[k], E.Y[k]);
+            // slots holding columns sxa..sxb
+            const int s0 = sxa >> (6 - lgY), s1 = sxb >> (6 - lgY);
+            lo_s = min(lo_s, s0);
+            hi_s = max(hi_s, s1);
 #pragma unroll
-            for (int s = 0; s < kMaxPPL; s++)
-                if (lane + 64 * s < np_) {
-                    const float pr = L.sten[sx(s)] * L.sten[32 + sy(s)];
+            for (int s = 0; s < PPL; s++)
+                if (s >= s0 && s <= s1) {
+                    const float pr = L.sten[px(s)] * vy;
                     mt[s] = neg ? mt[s] - pr : mt[s] + pr;
                 }
             wsync();
         }
+        // M - M' + D log(M'/M); slots outside lo_s..hi_s, and padding lanes
+        // (D = 0, M = M' = 1), have M' == M and add exactly 0
         float acc = 0.0f;
 #pragma unroll
-        for (int s = 0; s < kMaxPPL; s++)
-            if (lane + 64 * s < np_) acc = acc + ((m[s] - mt[s]) + d[s] * logf(mt[s] / m[s]));
+        for (int s = 0; s < PPL; s++)
+            if (s >= lo_s && s <= hi_s) {
+                // M'/M from the hardware reciprocal and one residual correction
+                // (the correctly rounded quotient up to rare last-bit ties)
+                const float r = __builtin_amdgcn_rcpf(m[s]);
+                const float q0 = mt[s] * r;
+                const float q = fmaf(fmaf(-q0, m[s], mt[s]), r, q0);
+                acc = acc + ((m[s] - mt[s]) + L.dpx[64 * s + lane] * logf(q));
+            }
         return bcastf(wave_scan(acc), 63);
     }
 
@@ -801,31 +838,27 @@ struct Chain {
     // find_Birth (LibRJMCMC.cpp:1688-1736): residual max(0, D - M), prefix
     // sum in pixel order, r = total * U, first pixel with r <= cumulative.
     __device__ __forceinline__ int birth_pixel(float u) {
-        const int np_ = pix_count();
         aflops += (uint64_t)(4 * P);
         float carry = 0.0f;
-        float cum[kMaxPPL];
+        float cum[PPL];
 #pragma unroll
-        for (int s = 0; s < kMaxPPL; s++) {
+        for (int s = 0; s < PPL; s++) {
             cum[s] = 0.0f;
             if (s < ppl) {
-                float v = 0.0f;
-                if (lane + 64 * s < np_) {
-                    const float r = d[s] - m[s];
-                    v = (0 > r) ? 0.0f : r;
-                }
+                const float r = L.dpx[64 * s + lane] - m[s];
+                float v = (0 > r) ? 0.0f : r;
                 v = wave_scan(v);
                 cum[s] = carry + v;
                 carry = bcastf(cum[s], 63);
             }
         }
         const float rr = carry * u;
-        int pix = P;
+        int pix = P;  // (padding lanes add 0: D - M = -1)
 #pragma unroll
-        for (int s = 0; s < kMaxPPL; s++) {
+        for (int s = 0; s < PPL; s++) {
             if (s < ppl && pix == P) {
-                const uint64_t mk = ballot(lane + 64 * s < np_ && rr <= cum[s]);
-                if (mk) pix = 64 * s + __builtin_ctzll(mk);
+                const uint64_t mk = ballot(has(s) && rr <= cum[s]);
+                if (mk) pix = bcasti(pidx(s), __builtin_ctzll(mk));
             }
         }
         return pix;
@@ -833,16 +866,15 @@ struct Chain {
     // Learned proposal: pixel drawn from the per-ROI proposal map q; returns
     // the pixel and its probability q[pix] / sum(q).
     __device__ __forceinline__ int map_pixel(float u, float& prob) {
-        const int np_ = pix_count();
         const int64_t off = a->pix_off[roi];
         float carry = 0.0f;
-        float cum[kMaxPPL], q[kMaxPPL];
+        float cum[PPL], q[PPL];
 #pragma unroll
-        for (int s = 0; s < kMaxPPL; s++) {
+        for (int s = 0; s < PPL; s++) {
             cum[s] = 0.0f;
             q[s] = 0.0f;
             if (s < ppl) {
-                float v = (lane + 64 * s < np_) ? a->prop_map[off + lane + 64 * s] : 0.0f;
+                float v = has(s) ? a->prop_map[off + pidx(s)] : 0.0f;
                 q[s] = v;
                 v = wave_scan(v);
                 cum[s] = carry + v;
@@ -854,12 +886,12 @@ struct Chain {
         float qp = 0.0f;
         bool found = false;
 #pragma unroll
-        for (int s = 0; s < kMaxPPL; s++) {
+        for (int s = 0; s < PPL; s++) {
             if (s < ppl && !found) {
-                const uint64_t mk = ballot(lane + 64 * s < np_ && rr <= cum[s] && q[s] > 0.0f);
+                const uint64_t mk = ballot(has(s) && rr <= cum[s] && q[s] > 0.0f);
                 if (mk) {
                     const int l = __builtin_ctzll(mk);
-                    pix = 64 * s + l;
+                    pix = bcasti(pidx(s), l);
                     qp = bcastf(q[s], l);
                     found = true;
                 }
@@ -1021,7 +1053,9 @@ struct Chain {
                     cABG = tABG;
                     cBBG = tBBG;
 #pragma unroll
-                    for (int s = 0; s < kMaxPPL; s++) m[s] = mt[s];
+                    for (int s = 0; s < PPL; s++)
+                        if (s >= lo_s && s <= hi_s) m[s] = mt[s];
+                    if (need_test) e_ok = true;  // the proposed set passed the test
                     logpost += logpr;
                     if (++nacc >= a->recompute_every) {
                         wsync();
@@ -1048,8 +1082,17 @@ struct Chain {
         const bamf_mcmc_params& p = a->p;
         X = a->xs[roi];
         Y = a->ys[roi];
+        if (X < 1 || Y < 1 || X > kMaxSide || Y > kMaxSide || slots_for(X, Y) > PPL) {
+            if (a->status && lane == 0) a->status[roi] = ST_SIZE;
+            return;
+        }
         P = X * Y;
-        ppl = (P + 63) >> 6;
+        lgY = 32 - __builtin_clz((unsigned)(Y - 1) | 1u);  // ceil(log2(Y)), 0 for Y = 1
+        if (Y == 1) lgY = 0;
+        cps = 64 >> lgY;
+        ppl = slots_for(X, Y);
+        xsub = lane >> lgY;
+        ysub = lane & ((1 << lgY) - 1);
         C = 1 / (p.psf_sigma * sqrtf(2.0f));
         const float rho = a->rho ? a->rho[roi] : p.rho;
         meanN = rho * X * Y;  // cRJMCMC.cpp:116
@@ -1062,12 +1105,9 @@ struct Chain {
         aflops = 0;
         const int64_t off = a->pix_off[roi];
 #pragma unroll
-        for (int s = 0; s < kMaxPPL; s++) {
-            const int pp = lane + 64 * s;
-            const bool ok = s < ppl && pp < P;
-            const int xx = ok ? pp / Y : 0;
-            adr[s] = ok ? (uint32_t)(xx | ((pp - xx * Y) << 8)) : 0xffffu;
-            d[s] = ok ? a->data[off + pp] : 0.0f;
+        for (int s = 0; s < PPL; s++) {
+            const bool ok = s < ppl && has(s);
+            if (s < ppl) L.dpx[64 * s + lane] = ok ? a->data[off + pidx(s)] : 0.0f;
             m[s] = 1.0f;
             mt[s] = 1.0f;
         }
@@ -1086,6 +1126,8 @@ struct Chain {
                 L.e[0].S[n] = 1;
             }
         }
+        e_ok = false;  // initial emitters are checked in full by the first test
+        t_bad = false;
         const float bg0 = a->bg0 ? a->bg0[roi] : 0.0f;
         cBG = bg0 == 0 ? 1.0f : bg0;
         cABG = 0;
@@ -1202,23 +1244,65 @@ struct Chain {
 
 // The argument block is the kernel's only parameter, so it sits at offset 0 of
 // the kernarg segment; the chain reads it through that segment pointer.
-__global__ __launch_bounds__(64 * kWavesPerBlock, 3) void k_chain(ChainArgs) {
-    __shared__ WaveLds lds[kWavesPerBlock];
+// One instantiation per ROI size class (PPL pixel slots per lane), each
+// draining its own class list through an atomic queue.
+template <int PPL, int MINB>
+__global__ __launch_bounds__(64 * kWavesPerBlock, MINB) void k_chain(ChainArgs) {
+    __shared__ WaveLds<PPL> lds[kWavesPerBlock];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int lane = threadIdx.x & 63;
     const ChainArgs* ap = (const ChainArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
-    Chain ch(ap, lds[w], lane);
+    Chain<PPL> ch(ap, lds[w], lane);
+    const int n = *ap->list_count;
     while (true) {
         int qi = 0;
         if (lane == 0) qi = (int)atomicAdd(ap->queue, 1u);
         qi = bcasti(qi, 0);
-        if (qi >= ap->n_roi) break;
-        const int roi = ap->order ? ap->order[qi] : qi;
-        ch.run(roi);
+        if (qi >= n) break;
+        ch.run(ap->list[qi]);
     }
 }
 
-__global__ void k_zero_u32(uint32_t* p) { p[0] = 0u; }
+// Stable split of the ROI order into the size classes (P <= 64 * kClassPPL[c]
+// for the first class c that fits); also resets the class queues.  One
+// work-group walks the order in tiles of 1024 with a work-group scan.
+__global__ __launch_bounds__(1024) void k_classify(int n_roi, const int32_t* order, const int32_t* xs,
+                                                   const int32_t* ys, int32_t* list0, int32_t* list1,
+                                                   int32_t* counts, uint32_t* queues) {
+    __shared__ int sc[2][1024];
+    __shared__ int base[2];
+    const int t = threadIdx.x;
+    if (t < 2) base[t] = 0;
+    __syncthreads();
+    for (int i0 = 0; i0 < n_roi; i0 += 1024) {
+        const int i = i0 + t;
+        int roi = -1, c = -1;
+        if (i < n_roi) {
+            roi = order ? order[i] : i;
+            c = slots_for(xs[roi], ys[roi]) <= kClassPPL[0] ? 0 : 1;
+        }
+        sc[0][t] = c == 0;
+        sc[1][t] = c == 1;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const int v0 = t >= o ? sc[0][t - o] : 0, v1 = t >= o ? sc[1][t - o] : 0;
+            __syncthreads();
+            sc[0][t] += v0;
+            sc[1][t] += v1;
+            __syncthreads();
+        }
+        if (c == 0) list0[base[0] + sc[0][t] - 1] = roi;
+        if (c == 1) list1[base[1] + sc[1][t] - 1] = roi;
+        __syncthreads();
+        if (t < 2) base[t] += sc[t][1023];
+        __syncthreads();
+    }
+    if (t < 2) {
+        counts[t] = base[t];
+        queues[t] = 0u;
+    }
+}
+
 
 __global__ void k_erf(const float* x, float* y, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1227,10 +1311,16 @@ __global__ void k_erf(const float* x, float* y, int64_t n) {
 
 }  // namespace
 
-int launch_chain(const ChainArgs& a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, a.queue);
-    BAMF_LAUNCH_CHECK("k_zero_u32");
-    hipLaunchKernelGGL(k_chain, dim3(grid), dim3(64 * kWavesPerBlock), 0, s, a);
+int launch_classify(int n_roi, const int32_t* order, const int32_t* xs, const int32_t* ys, int32_t* list0,
+                    int32_t* list1, int32_t* counts, uint32_t* queues, hipStream_t s) {
+    hipLaunchKernelGGL(k_classify, dim3(1), dim3(1024), 0, s, n_roi, order, xs, ys, list0, list1, counts, queues);
+    BAMF_LAUNCH_CHECK("k_classify");
+    return BAMF_OK;
+}
+
+int launch_chain(const ChainArgs& a, int cls, int grid, hipStream_t s) {
+    if (cls == 0) hipLaunchKernelGGL((k_chain<kClassPPL[0], kClassMinBlocks[0]>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, a);
+    else hipLaunchKernelGGL((k_chain<kClassPPL[1], kClassMinBlocks[1]>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, a);
     BAMF_LAUNCH_CHECK("k_chain");
     return BAMF_OK;
 }
@@ -1241,9 +1331,14 @@ int launch_erf(const float* x, float* y, int64_t n, hipStream_t s) {
     return BAMF_OK;
 }
 
-int chain_occupancy_blocks(int* blocks_per_cu) {
+int chain_occupancy_blocks(int cls, int* blocks_per_cu) {
     int n = 0;
-    BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chain, 64 * kWavesPerBlock, 0));
+    if (cls == 0)
+        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chain<kClassPPL[0], kClassMinBlocks[0]>,
+                                                                   64 * kWavesPerBlock, 0));
+    else
+        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chain<kClassPPL[1], kClassMinBlocks[1]>,
+                                                                   64 * kWavesPerBlock, 0));
     *blocks_per_cu = n;
     return BAMF_OK;
 }

@@ -12,6 +12,11 @@ constexpr int kNMax = 128;       // emitters per chain (LDS state capacity)
 constexpr int kWavesPerBlock = 4;
 constexpr int kMaxPPL = 16;      // pixels per lane: ROIs up to 32 x 32
 constexpr int kMaxSide = 32;
+// ROI size classes: each has its own kernel instantiation with PPL pixel
+// slots per lane (fewer registers, more resident chains for small ROIs).
+constexpr int kClasses = 2;
+constexpr int kClassPPL[kClasses] = {4, 16};
+constexpr int kClassMinBlocks[kClasses] = {3, 3};  // work-groups per CU the register budget targets
 
 // One prior table (Stat, LibRJMCMC.cpp:176-217) on the device.
 struct StatDev {
@@ -49,8 +54,10 @@ struct ChainArgs {
     const float* x0;
     const float* y0;
     const uint32_t* key;
-    const int32_t* order;
-    uint32_t* queue;  // work-queue counter (reset before launch)
+    const int32_t* order;       // caller's ROI order (read by the classifier only)
+    const int32_t* list;        // this launch's class list of ROI indices
+    const int32_t* list_count;  // its length (device)
+    uint32_t* queue;            // work-queue counter (reset by the classifier)
     // replay
     const uint32_t* tape;
     const int64_t* tape_off;

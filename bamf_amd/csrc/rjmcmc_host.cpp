@@ -7,8 +7,10 @@
 
 namespace bamf {
 namespace rj {
-int launch_chain(const ChainArgs& a, int grid, hipStream_t s);
-int chain_occupancy_blocks(int* blocks_per_cu);
+int launch_classify(int n_roi, const int32_t* order, const int32_t* xs, const int32_t* ys, int32_t* list0,
+                    int32_t* list1, int32_t* counts, uint32_t* queues, hipStream_t s);
+int launch_chain(const ChainArgs& a, int cls, int grid, hipStream_t s);
+int chain_occupancy_blocks(int cls, int* blocks_per_cu);
 int launch_erf(const float* x, float* y, int64_t n, hipStream_t s);
 }  // namespace rj
 }  // namespace bamf
@@ -75,12 +77,28 @@ struct bamf_rjmcmc_plan {
     float pvb[8], pvt[8];
     HostStat off, sig, back;
     float* d_tables = nullptr;
-    uint32_t* d_queue = nullptr;
+    uint32_t* d_queue = nullptr;  // [0..1] class queues, [2..3] class counts (int32)
+    int32_t* d_lists = nullptr;   // class lists, list_cap entries each
+    int64_t list_cap = 0;
+    hipStream_t side = nullptr;   // second stream: the classes run concurrently
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     StatDev s_off, s_sig, s_back;
     int device = 0;
     int cu = 0;
-    int blocks_per_cu = 0;
+    int blocks_per_cu[kClasses] = {};
 };
+
+namespace {
+void plan_release(bamf_rjmcmc_plan* pl) {
+    if (pl->d_tables) (void)hipFree(pl->d_tables);
+    if (pl->d_queue) (void)hipFree(pl->d_queue);
+    if (pl->d_lists) (void)hipFree(pl->d_lists);
+    if (pl->ev_fork) (void)hipEventDestroy(pl->ev_fork);
+    if (pl->ev_join) (void)hipEventDestroy(pl->ev_join);
+    if (pl->side) (void)hipStreamDestroy(pl->side);
+    delete pl;
+}
+}  // namespace
 
 extern "C" {
 
@@ -128,11 +146,12 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
     if (e == hipSuccess) e = hipMalloc(&pl->d_tables, flat.size() * sizeof(float));
     if (e == hipSuccess) e = hipMemcpy(pl->d_tables, flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&pl->d_queue, 256);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
         set_error("plan device setup failed: %s", hipGetErrorString(e));
-        if (pl->d_tables) (void)hipFree(pl->d_tables);
-        if (pl->d_queue) (void)hipFree(pl->d_queue);
-        delete pl;
+        plan_release(pl);
         return BAMF_E_HIP;
     }
     StatDev* sd[3] = {&pl->s_off, &pl->s_sig, &pl->s_back};
@@ -149,17 +168,18 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, pl->device) == hipSuccess) pl->cu = prop.multiProcessorCount;
-    int bpc = 0;
-    if (chain_occupancy_blocks(&bpc) == BAMF_OK) pl->blocks_per_cu = bpc;
+    for (int c = 0; c < kClasses; c++) {
+        int bpc = 0;
+        if (chain_occupancy_blocks(c, &bpc) == BAMF_OK) pl->blocks_per_cu[c] = bpc;
+    }
     *plan = pl;
     return BAMF_OK;
 }
 
 int bamf_rjmcmc_plan_destroy(bamf_rjmcmc_plan* plan) {
     if (!plan) return BAMF_OK;
-    if (plan->d_tables) (void)hipFree(plan->d_tables);
-    if (plan->d_queue) (void)hipFree(plan->d_queue);
-    delete plan;
+    (void)hipStreamSynchronize(plan->side);
+    plan_release(plan);
     return BAMF_OK;
 }
 
@@ -172,9 +192,11 @@ int bamf_math_erf(const float* d_x, float* d_y, int64_t n, void* stream) {
 
 int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed) {
     BAMF_CHECK_ARG(plan && claimed, "null argument");
-    uint32_t v = 0;
-    BAMF_HIP_TRY(hipMemcpy(&v, plan->d_queue, 4, hipMemcpyDeviceToHost));
-    *claimed = v;
+    uint32_t q[4] = {0, 0, 0, 0};
+    BAMF_HIP_TRY(hipMemcpy(q, plan->d_queue, sizeof(q), hipMemcpyDeviceToHost));
+    int64_t n = 0;
+    for (int c = 0; c < kClasses; c++) n += q[c] < q[2 + c] ? q[c] : q[2 + c];
+    *claimed = n;
     return BAMF_OK;
 }
 
@@ -226,7 +248,6 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     a.y0 = batch->d_y0;
     a.key = batch->d_key;
     a.order = batch->d_order;
-    a.queue = plan->d_queue;
     if (mode == BAMF_RJ_REPLAY) {
         a.tape = replay->d_tape;
         a.tape_off = replay->d_tape_offset;
@@ -250,12 +271,38 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     a.grid_off = out->d_grid_offset;
     a.alg_flops = out->d_alg_flops;
     a.recompute_every = 128;
-    const int bpc = plan->blocks_per_cu > 0 ? plan->blocks_per_cu : 1;
+    hipStream_t s = as_stream(stream);
+    if (plan->list_cap < batch->n_roi) {
+        // grow the class lists (hipFree waits for work that may still use them)
+        if (plan->d_lists) BAMF_HIP_TRY(hipFree(plan->d_lists));
+        plan->d_lists = nullptr;
+        plan->list_cap = 0;
+        BAMF_HIP_TRY(hipMalloc(&plan->d_lists, sizeof(int32_t) * (size_t)kClasses * (size_t)batch->n_roi));
+        plan->list_cap = batch->n_roi;
+    }
+    int32_t* counts = reinterpret_cast<int32_t*>(plan->d_queue + 2);
+    int rc = launch_classify(batch->n_roi, batch->d_order, batch->d_xsize, batch->d_ysize, plan->d_lists,
+                             plan->d_lists + plan->list_cap, counts, plan->d_queue, s);
+    if (rc != BAMF_OK) return rc;
     const int cu = plan->cu > 0 ? plan->cu : 256;
-    int grid = bpc * cu;
     const int need = (int)ceil_div(batch->n_roi, kWavesPerBlock);
-    if (grid > need) grid = need;
-    return launch_chain(a, grid, as_stream(stream));
+    // the large-ROI class goes on the caller's stream first; the small class
+    // runs beside it on the side stream and the caller's stream joins it
+    BAMF_HIP_TRY(hipEventRecord(plan->ev_fork, s));
+    BAMF_HIP_TRY(hipStreamWaitEvent(plan->side, plan->ev_fork, 0));
+    for (int c = kClasses - 1; c >= 0; c--) {
+        ChainArgs ac = a;
+        ac.list = plan->d_lists + (int64_t)c * plan->list_cap;
+        ac.list_count = counts + c;
+        ac.queue = plan->d_queue + c;
+        int grid = (plan->blocks_per_cu[c] > 0 ? plan->blocks_per_cu[c] : 1) * cu;
+        if (grid > need) grid = need;
+        rc = launch_chain(ac, c, grid, c == kClasses - 1 ? s : plan->side);
+        if (rc != BAMF_OK) return rc;
+    }
+    BAMF_HIP_TRY(hipEventRecord(plan->ev_join, plan->side));
+    BAMF_HIP_TRY(hipStreamWaitEvent(s, plan->ev_join, 0));
+    return BAMF_OK;
 }
 
 }  // extern "C"

@@ -176,7 +176,8 @@ typedef struct bamf_replay {
 /* Outputs (device pointers; NULL skips an output). */
 typedef struct bamf_rjmcmc_out {
     int32_t* d_status;       /* [n_roi]: 0 ok, 1 emitter cap reached, 2 unsupported jump, 3 replay tape exhausted,
-                                4 internal draw-window overrun (a defect; never expected) */
+                                4 internal draw-window overrun (a defect; never expected),
+                                5 ROI side outside 1..32 */
     int32_t* d_counters;     /* [n_roi][30]: N_* A_* N_*Fg A_*Fg N_*Bg A_*Bg for jump, split, merge, birth, death */
     float* d_final;          /* [n_roi][4 + 4*final_cap]: N, BG, ABG, BBG, then I, X, Y, Signal per emitter */
     int32_t final_cap;
