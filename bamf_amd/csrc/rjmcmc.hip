@@ -25,9 +25,10 @@
 //     in replay mode) and the Irwin-Hall sum of the twelve uniforms starting
 //     there, so urand() and randn() are one readlane each.  Every jump but
 //     the group moves fits in one window; the group moves re-centre it;
-//   - accept: the reference's single-precision test U < PR
-//     (LibRJMCMC.cpp:939); the double-precision log PR is formed only when
-//     step records are requested.
+//   - accept: the test U < PR decided in double precision (log U < log PR)
+//     with a single-precision pre-test outside a narrow band around PR, so
+//     the double-precision logs are formed for near-ties, accepted jumps
+//     (the running log posterior) and step records only.
 #include <cmath>
 
 #include "common.hpp"
@@ -546,12 +547,39 @@ struct Chain {
     }
 
     // ---------------- accept ----------------
-    // Accept = (urand() < PR) (LibRJMCMC.cpp:939 and the other jumps).
-    __device__ __forceinline__ bool decide(float pr) {
+    // log PR in double: the LLR's exponent plus the log of the other factors'
+    // product formed in double from their single-precision values.
+    __device__ __forceinline__ double logpr_of(float sum) const {
+        const double l = log(lp_prod);
+        return (double)sum + (lp_neg ? -l : l);
+    }
+    // The accept test U < PR (LibRJMCMC.cpp:939 and the other jumps), decided
+    // in double precision as log U < log PR.  A single-precision estimate of
+    // both logs (hardware log2, error far below 1e-5 (1 + |log PR|)) settles
+    // every case outside that band; the double-precision logs are formed only
+    // inside it.  When the single-precision PR is saturated (inf, 0, NaN or
+    // beyond 1e+-30) the reference's own test decides, keeping its overflow
+    // semantics.
+    __device__ __forceinline__ bool decide(float pr, float sum) {
         need(1);
         const float U = urand();
-        bool acc = U < pr;
-        if (!(isfinite(pr) && pr > 1e-30f && pr < 1e30f)) r_flags |= 16;
+        bool acc;
+        if (isfinite(pr) && pr > 1e-30f && pr < 1e30f) {
+            const float fp = (float)lp_prod;
+            int v = -1;  // -1 undecided, else the decision
+            if (fp > 1e-30f && fp < 1e30f) {
+                const float lf = __logf(fp);
+                const float lpf = sum + (lp_neg ? -lf : lf);
+                const float lu = __logf(U);  // -inf for U = 0
+                const float band = 1e-5f * (1.0f + fabsf(lpf));
+                if (lu < lpf - band) v = 1;
+                else if (lu > lpf + band) v = 0;
+            }
+            acc = v >= 0 ? v == 1 : log((double)U) < logpr_of(sum);
+        } else {
+            acc = U < pr;
+            r_flags |= 16;
+        }
         r_pr = pr;
         if (acc) r_flags |= 1;
         if (a->mode == BAMF_RJ_REPLAY) {
@@ -1035,14 +1063,10 @@ struct Chain {
                     case EV_BIRTH: PR = fin_birth(sum, LLR); break;
                     default: PR = fin_death(sum, LLR); break;
                 }
-                const bool acc = decide(PR);
-                // log PR in double: the sum plus the log of the factors'
-                // product (kept for step records and the accepted-jump total)
+                const bool acc = decide(PR, sum);
+                // kept for step records and the accepted-jump total
                 double logpr = __builtin_nan("");
-                if (acc || a->steps) {
-                    const double l = log(lp_prod);
-                    logpr = (double)sum + (lp_neg ? -l : l);
-                }
+                if (acc || a->steps) logpr = logpr_of(sum);
                 r_logpr = logpr;
                 if (acc) {
                     cnt_inc(acc_c1);

@@ -319,7 +319,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (ROIs shard with no exchange; "
-              "see the design notes). Run with --gpus 1.", file=sys.stderr)
+              "a multi-GPU run would give each GPU its own share of the ROIs). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     if args.copy_mib:

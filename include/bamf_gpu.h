@@ -192,7 +192,10 @@ typedef struct bamf_rjmcmc_out {
     float* d_pgrid;          /* zoomed posterior images (update_grids, LibRJMCMC.cpp:1629-1648) */
     float* d_pback;
     const int64_t* d_grid_offset; /* [n_roi] offset of ROI r's (ysize*zoom) x (xsize*zoom) grid */
-    uint64_t* d_alg_flops;   /* [n_roi]: algorithmic floating-point operations of the chain (see DESIGN) */
+    uint64_t* d_alg_flops;   /* [n_roi]: algorithmic floating-point operations of the chain: per jump
+                                6P (likelihood ratio) + 2P + 8(X+Y) per changed emitter + 5P for a
+                                background move + 4P for a birth pixel draw; 4P + N(2P + 8(X+Y)) per
+                                model rebuild (P = X*Y pixels) */
 } bamf_rjmcmc_out;
 
 #define BAMF_RJ_PLAIN 0   /* counter-based Philox draws keyed by (seed, key[r], step) */
@@ -207,13 +210,15 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
                             const float* h_sig_pdf, int sig_len, const float* h_back_pdf, int back_len,
                             bamf_rjmcmc_plan** plan);
 int bamf_rjmcmc_plan_destroy(bamf_rjmcmc_plan* plan);
-/* Runs every chain of the batch to completion in one persistent launch
- * (work queue over ROIs, one wavefront per chain). */
+/* Runs every chain of the batch to completion: one wavefront per chain, ROIs
+ * split into two size classes (up to 4 and up to 16 pixel slots per lane),
+ * each drained by a persistent launch through its own work queue; the two
+ * launches run concurrently (the small class on an internal stream that
+ * `stream` joins), so the call is asynchronous on `stream` as a whole. */
 int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_t seed, int mode,
                     const bamf_replay* replay, const bamf_rjmcmc_out* out, void* stream);
-/* The counter that a persistent launch uses as its work queue is reset by
- * bamf_rjmcmc_run itself; this returns how many chains the last run on the
- * plan claimed (synchronous). */
+/* The work-queue counters are reset by bamf_rjmcmc_run itself; this returns
+ * how many chains the last run on the plan claimed (synchronous). */
 int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed);
 
 /* The branch-free erf the chain's PSF stencils use, evaluated elementwise

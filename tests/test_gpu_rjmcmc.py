@@ -184,3 +184,19 @@ def test_branch_free_erf_accuracy():
     y = dy.download(np.float32)
     err = np.abs(y.astype(np.float64) - erf(x.astype(np.float64)))
     assert err.max() < 1e-6, err.max()
+
+
+def test_birth_death_stationary_law_gpu():
+    """The GPU chains' N under an exactly flat likelihood follows the
+    birth-death law fixed by the reference's birth and death ratios
+    (derivation in test_oracle_rjmcmc.py)."""
+    from test_oracle_rjmcmc import BD_BG, BD_SIDE, birth_death_setup, check_birth_death
+
+    n_roi, nt = 64, 20000
+    p, data, sig, off, back, law = birth_death_setup(n_trials=nt)
+    plan = Plan(p, off, sig, back)
+    rs = RoiSet(np.tile(data, n_roi), np.arange(n_roi + 1) * data.size, [BD_SIDE] * n_roi, [BD_SIDE] * n_roi,
+                bg0=np.full(n_roi, BD_BG, np.float32))
+    g = run_batch(plan, rs, seed=11, trace_len=nt, trace_thin=1, hist_len=129)
+    assert np.all(g.status == 0)
+    check_birth_death(g.trace_n.astype(np.int64).ravel(), law)

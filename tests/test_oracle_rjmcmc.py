@@ -158,3 +158,88 @@ def test_fixed_dimension_posterior_matches_mle(oracle):
     assert abs(xs.mean() - (mle_x + 0.0)) < 4 * se_x / math.sqrt(ess_guess) + 0.02
     assert abs(ys.mean() - (mle_y + 0.0)) < 4 * se_y / math.sqrt(ess_guess) + 0.02
     assert 0.6 < xs.std() / se_x < 1.5 and 0.6 < ys.std() / se_y < 1.5
+
+
+def test_philox_known_answers(oracle):
+    """Philox4x32-10 against the known-answer vectors published with Random123
+    (Salmon et al., SC'11; kat_vectors, philox4x32 R=10)."""
+    kat = [([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+           ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+           ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+            [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1])]
+    for ctr, key, want in kat:
+        assert [int(v) for v in oracle.philox(ctr, key)] == want
+
+
+# ---- birth/death balance -------------------------------------------------
+# With emitter intensities far below one ulp of the background, M' == M
+# exactly, so LLR = 1 and N performs a birth-death walk whose stationary law
+# follows from the jump ratios alone (LibRJMCMC.cpp:1302 and :1360):
+#   birth from N:   PR_b = P3 / (P4 * (X+Bnd)(Y+Bnd) * PixProb) * MeanN / (N + 1)
+#   death from N+1: PR_d = (P4 / P3) * (N + 1) / MeanN
+# with P = PV_Trials (the cumulative vector) after burn-in.  Detailed balance
+# of the walk, with each kind proposed with probability 1/2, gives
+#   pi(N+1) / pi(N) = min(1, PR_b(N)) / min(1, PR_d(N+1)).
+BD_SIDE, BD_BG, BD_MEANN, BD_BND = 8, 100.0, 100.0, 2.0
+BD_AREA = (BD_SIDE + BD_BND) ** 2
+
+
+def birth_death_law(mean_n, area, p3, p4, nmax=128, n_plus=1.0):
+    pi = np.zeros(nmax + 1)
+    pi[0] = 1.0
+    for n in range(nmax):
+        prb = p3 / (p4 * area) * mean_n / (n + n_plus)
+        prd = (p4 / p3) * (n + 1) / mean_n
+        pi[n + 1] = pi[n] * min(1.0, prb) / min(1.0, prd)
+    return pi / pi.sum()
+
+
+def birth_death_setup(n_trials=300000):
+    from bamf_amd.rjmcmc import make_params
+
+    dx = 1e-7  # intensities in (0, 1e-6): far below ulp(100) / 2
+    sig = np.ones(11, np.float32)
+    off = np.ones(11, np.float32)
+    back = np.zeros(11, np.float32)
+    p = make_params(n_burnin=2000, n_trials=n_trials, rho=BD_MEANN / BD_SIDE ** 2, dx=dx, bnd_out=BD_BND,
+                    p_burnin=(0, 0, 0, 0.5, 0.5, 0, 0, 0), p_trials=(0, 0, 0, 0.5, 0.5, 0, 0, 0))
+    data = np.full(BD_SIDE * BD_SIDE, BD_BG, np.float32)
+    law = birth_death_law(BD_MEANN, BD_AREA, 0.5, 1.0)  # PV_Trials[3], PV_Trials[4]
+    return p, data, sig, off, back, law
+
+
+def check_birth_death(n_trace, law):
+    """Mean and total variation of the post-burn-in N against the law; the
+    tolerances are about three standard errors of these chain lengths."""
+    emp = np.bincount(n_trace, minlength=129)[:129] / n_trace.size
+    mean_law = float((np.arange(129) * law).sum())
+    assert abs(n_trace.mean() - mean_law) < 0.1, (n_trace.mean(), mean_law)
+    assert 0.5 * np.abs(emp - law).sum() < 0.03
+    return mean_law
+
+
+def test_birth_death_stationary_law(oracle):
+    p, data, sig, off, back, law = birth_death_setup()
+    op = oracle.McmcParams.from_buffer_copy(p)
+    trace = []
+    for seed in (1, 2):
+        r = oracle.rjmcmc(data, BD_SIDE, BD_SIDE, op, off, sig, back, bg0=BD_BG, seed=seed, record=True)
+        assert r["status"] == 0
+        st = r["steps"][p.n_burnin:]
+        assert np.all(st["llr"][(st["flags"] & 2) != 0] == 1.0)  # the likelihood is exactly flat
+        trace.append(st["n"])
+        # acceptance per N against min(1, PR) of the two ratios
+        kind, acc = st["kind"], (st["flags"] & 1) != 0
+        nprev = np.concatenate([[st["n"][0]], st["n"][:-1]])
+        for n in range(2, 8):
+            for k, pr in ((3, 0.5 / BD_AREA * BD_MEANN / (n + 1)), (4, 2.0 * n / BD_MEANN)):
+                sel = (kind == k) & (nprev == n)
+                want = min(1.0, pr)
+                assert abs(acc[sel].mean() - want) < 4 * np.sqrt(want * (1 - want) / sel.sum()) + 1e-3
+    mean_law = check_birth_death(np.concatenate(trace), law)
+    # the check separates the law from plausible slips in the ratios
+    for wrong in (birth_death_law(BD_MEANN, BD_AREA, 0.5, 0.5),            # raw P instead of PV_Trials
+                  birth_death_law(BD_MEANN, BD_AREA, 0.5, 1.0, n_plus=2.0),  # MeanN / (N + 2)
+                  birth_death_law(BD_MEANN, 1.0, 0.5, 1.0),                  # area term dropped
+                  birth_death_law(BD_MEANN * 1.1, BD_AREA, 0.5, 1.0)):       # MeanN off by 10 %
+        assert abs(float((np.arange(129) * wrong).sum()) - mean_law) > 0.3
