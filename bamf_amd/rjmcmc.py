@@ -95,6 +95,7 @@ class RoiSet:
         self.xsize = np.ascontiguousarray(xsize, np.int32)
         self.ysize = np.ascontiguousarray(ysize, np.int32)
         self.offsets = np.ascontiguousarray(offsets[: self.n_roi], np.int64)
+        self.total_pixels = int(np.max(self.offsets + self.xsize.astype(np.int64) * self.ysize)) if self.n_roi else 0
         if np.any(self.xsize < 1) or np.any(self.xsize > 32) or np.any(self.ysize < 1) or np.any(self.ysize > 32):
             raise BamfError("ROI sides must be in [1, 32]")
         up = _hip.DeviceBuffer.from_array

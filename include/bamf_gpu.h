@@ -221,6 +221,44 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
  * how many chains the last run on the plan claimed (synchronous). */
 int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed);
 
+/* ------------------------------------------------------------------------
+ * Learned birth proposal (no counterpart in the reference, whose birth
+ * draws the pixel from the data residual and fixes PixProb = 1,
+ * LibRJMCMC.cpp:1688-1736, 1302).
+ *
+ * A convolutional network (3x3 convolutions 1->32, 32->32, 32->32, 32->32
+ * with ReLU, then a 1x1 head 32->2: occupancy logit and log-intensity) runs
+ * on bfloat16 matrix cores with fp32 accumulation, forward and backward.
+ * Parameters are one fp32 array of bamf_net_param_count() values:
+ *   W1[32][9], b1[32], then for layers 2..4 W[32][288] (index tap*32 + c,
+ *   tap = 3*(dh+1) + (dw+1)), b[32], then Wh[2][32], bh[2].
+ * Images are batch x h x w floats (row = ROI x, column = ROI y, i.e. the
+ * ROI layout data[y + x*Ysize] for an h = Xsize, w = Ysize image); the
+ * network normalises each image as (sqrt(D + 3/8) - mean) / 2.
+ * ------------------------------------------------------------------------ */
+typedef struct bamf_net bamf_net;
+int bamf_net_param_count(void);
+/* h_params: initial fp32 parameters (host).  The network keeps fp32 masters,
+ * Adam moments and bfloat16 operand copies on the current device. */
+int bamf_net_create(int max_batch, int h, int w, const float* h_params, bamf_net** net);
+int bamf_net_destroy(bamf_net* net);
+int bamf_net_get_params(bamf_net* net, float* h_params);
+/* d_out: batch x h x w x 2 floats (occupancy logit, log-intensity). */
+int bamf_net_forward(bamf_net* net, const float* d_images, int batch, float* d_out, void* stream);
+/* Loss L = mean over pixels of BCE(logit, occ) + occ * (mu - logI)^2 and its
+ * gradient with respect to every parameter (d_grad: param_count floats,
+ * d_loss: the two loss terms); nothing is updated. */
+int bamf_net_gradients(bamf_net* net, const float* d_images, const float* d_occ, const float* d_logi, int batch,
+                       float* d_grad, float* d_loss, void* stream);
+/* One Adam step (beta 0.9 / 0.999) on the same loss. */
+int bamf_net_train_step(bamf_net* net, const float* d_images, const float* d_occ, const float* d_logi, int batch,
+                        float lr, float* d_loss, void* stream);
+/* Per-pixel birth proposal of every ROI of a batch (network built for a
+ * 32 x 32 canvas): d_q (ROI layout, each ROI summing to 1) =
+ * (1 - eps) sigmoid(logit) / sum + eps / P, d_mu = predicted log-intensity. */
+int bamf_net_proposal_maps(bamf_net* net, const bamf_roi_batch* batch, float eps, float* d_q, float* d_mu,
+                           void* stream);
+
 /* The branch-free erf the chain's PSF stencils use, evaluated elementwise
  * (a self-check of the device math; no counterpart in the reference). */
 int bamf_math_erf(const float* d_x, float* d_y, int64_t n, void* stream);
