@@ -217,7 +217,7 @@ struct Chain {
     // proposal bookkeeping for the shared evaluation
     int ev, jt, acc_c1, acc_c2;
     bool need_test;
-    float f_pri, f_u1, f_u2, f_u3, f_ijp, f_pixprob;
+    float f_pri, f_u1, f_u2, f_u3, f_ijp, f_pixprob, f_iratio;
     int f_jj, f_jm;
     // step record
     int r_jt, r_flags, r_nprop, r_pix;
@@ -619,6 +619,14 @@ struct Chain {
         return e.S[k] == 1 ? findpdf(a->sig, a->p.dx, e.I[k]) : findpdf(a->back, a->p.dx, e.I[k]);
     }
 
+    // The network's log-intensity at the pixel a birth would propose (x, y)
+    // from (pixel (floor(x + 1/2), floor(y + 1/2))); -inf outside the ROI.
+    __device__ __forceinline__ float map_mu(float x, float y) const {
+        const int px = (int)floorf(x + 0.5f), py = (int)floorf(y + 0.5f);
+        if (!(px >= 0 && px < X && py >= 0 && py < Y)) return -INFINITY;
+        return a->prop_mu[a->pix_off[roi] + px * Y + py];
+    }
+
     // ---------------- proposals ----------------
     // Each prop_* follows the reference jump up to the likelihood ratio and
     // returns the evaluation kind (EV_NONE when the jump ended early).
@@ -707,13 +715,33 @@ struct Chain {
             const bool fg = E.S[nn] == 1;
             cnt_inc(fg ? C_NFG + J_MOVE : C_NBG + J_MOVE);
             const float is = fg ? p.i_std_fg : p.i_std_bg, xs = fg ? p.x_std_fg : p.x_std_bg;
-            const float ti = T.I[nn] + is * randn();
+            // Learned mode: with probability beta the intensity is redrawn
+            // from the network's log-uniform window at the emitter's pixel
+            // (an independence proposal; its own Metropolis-Hastings kernel,
+            // chosen independently of the state), the position walks as usual.
+            const bool indep = a->mode == BAMF_RJ_LEARNED && urand() < a->l_beta;
+            float ti, qratio = 1.0f;
+            if (indep) {
+                const float w = a->l_halfwidth;
+                const float mu0 = map_mu(T.X[nn], T.Y[nn]);
+                ti = expf(mu0 + w * (2.0f * urand() - 1.0f));
+                if (!(mu0 > -INFINITY)) return EV_NONE;  // outside the ROI: no proposal
+            } else {
+                ti = T.I[nn] + is * randn();
+            }
             const float tx = T.X[nn] + xs * randn();
             const float ty = T.Y[nn] + xs * randn();
+            if (indep) {
+                // q(I | x') / q(I' | x) of the log-uniform windows: I' / I when
+                // the old intensity lies in the window at the new pixel, else 0
+                const float mu1 = map_mu(tx, ty);
+                const float I0 = T.I[nn];
+                qratio = (I0 > 0.0f && fabsf(logf(I0) - mu1) <= a->l_halfwidth) ? ti / I0 : 0.0f;
+            }
             IPDFtest = findpdf(fg ? a->sig : a->back, p.dx, ti);
             IPDF = findpdf(fg ? a->sig : a->back, p.dx, E.I[nn]);
             set_t(nn, ti, tx, ty);
-            PrI = IPDFtest / IPDF;
+            PrI = IPDFtest / IPDF * qratio;  // qratio = 1 outside the independence kernel
             delta_push(0, nn, true);
             delta_push(1, nn, false);
         }
@@ -938,7 +966,8 @@ struct Chain {
         const float ur = urand();
         float pixprob = 1.0f;
         int pix;
-        if (a->mode == BAMF_RJ_LEARNED) pix = map_pixel(ur, pixprob);
+        const bool learned = a->mode == BAMF_RJ_LEARNED;
+        if (learned) pix = map_pixel(ur, pixprob);
         else pix = birth_pixel(ur);
         r_pix = pix;
         if (a->mode == BAMF_RJ_REPLAY) pix = ref_pix;
@@ -947,10 +976,29 @@ struct Chain {
         int BF = 1;
         if (p.is_backg == 1) BF = kround(urand());
         cnt_inc(BF == 1 ? C_NFG + J_BIRTH : C_NBG + J_BIRTH);
-        const float I_new = genrand(BF == 1 ? a->sig : a->back, urand());
+        float I_new, iratio = 1.0f;
+        if (learned) {
+            // intensity from the mixture alpha * prior + (1 - alpha) *
+            // log-uniform on [mu - w, mu + w] around the network's value at
+            // the pixel; the birth ratio carries prior(I) / q(I) and PixProb
+            // is the pixel's proposal probability relative to uniform (P q)
+            const float um = urand(), ui = urand();
+            const float mu = a->prop_mu[a->pix_off[roi] + pix];
+            const float w = a->l_halfwidth;
+            I_new = um < a->l_alpha ? genrand(a->sig, ui) : expf(mu + w * (2.0f * ui - 1.0f));
+            const float prior = findpdf(a->sig, p.dx, I_new) * a->l_zinv;
+            const float lg = logf(I_new);
+            const float win = (I_new > 0.0f && fabsf(lg - mu) <= w) ? 1.0f / (2.0f * w * I_new) : 0.0f;
+            const float q = a->l_alpha * prior + (1.0f - a->l_alpha) * win;
+            iratio = prior / q;
+            pixprob = (float)P * pixprob;
+        } else {
+            I_new = genrand(BF == 1 ? a->sig : a->back, urand());
+        }
         if (!add_t(I_new, x, y, BF)) return EV_NONE;
         delta_push(1, tN - 1, false);
         f_pixprob = pixprob;
+        f_iratio = iratio;
         acc_c1 = C_A + J_BIRTH;
         acc_c2 = BF == 1 ? C_AFG + J_BIRTH : C_ABG + J_BIRTH;
         need_test = true;
@@ -960,9 +1008,10 @@ struct Chain {
     __device__ __forceinline__ float fin_birth(float sum, float LLR) {
         const float NEm = (float)nem_cur();
         const float PrM = meanN / (NEm + 1);
-        lp_prod = (double)P3 / ((double)P4 * (double)xb * (double)yb * (double)f_pixprob) * (double)PrM;
+        lp_prod = (double)P3 / ((double)P4 * (double)xb * (double)yb * (double)f_pixprob) * (double)PrM *
+                  (double)f_iratio;
         lp_neg = false;
-        return LLR * P3 / (P4 * xb * yb * f_pixprob) * PrM;
+        return LLR * P3 / (P4 * xb * yb * f_pixprob) * PrM * f_iratio;  // f_iratio = 1 outside learned mode
     }
 
     // LibRJMCMC.cpp:1332-1355

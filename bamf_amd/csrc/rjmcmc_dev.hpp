@@ -63,8 +63,13 @@ struct ChainArgs {
     const int64_t* tape_off;
     const int64_t* tape_len;
     const bamf_step_record* ref_steps;
-    // learned birth proposal: per-ROI per-pixel probabilities (same layout as data)
+    // learned birth proposal: per-ROI per-pixel probabilities and log-intensity
+    // centres (same layout as data), the prior weight of the intensity mixture,
+    // the log-uniform half-width and 1 / (integral of the signal pdf table)
     const float* prop_map;
+    const float* prop_mu;
+    float l_alpha, l_halfwidth, l_zinv;
+    float l_beta;  // probability of the independence intensity proposal in a single-emitter move
     // outputs
     int32_t* status;
     int32_t* counters;

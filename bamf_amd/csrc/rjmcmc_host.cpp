@@ -83,6 +83,10 @@ struct bamf_rjmcmc_plan {
     hipStream_t side = nullptr;   // second stream: the classes run concurrently
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     StatDev s_off, s_sig, s_back;
+    double sig_integral = 0.0;  // trapezoid integral of the signal pdf table
+    const float* d_q = nullptr;   // learned proposal maps (caller-owned)
+    const float* d_mu = nullptr;
+    float l_alpha = 1.0f, l_halfwidth = 0.5f, l_beta = 0.0f;
     int device = 0;
     int cu = 0;
     int blocks_per_cu[kClasses] = {};
@@ -127,6 +131,9 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
     pl->off.build(h_off_pdf, off_len, params->dx);
     pl->sig.build(h_sig_pdf, sig_len, params->dx);
     pl->back.build(h_back_pdf, back_len, params->dx);
+    for (size_t i = 0; i + 1 < pl->sig.x.size(); i++)
+        pl->sig_integral += 0.5 * ((double)pl->sig.pdf[i] + (double)pl->sig.pdf[i + 1]) *
+                            ((double)pl->sig.x[i + 1] - (double)pl->sig.x[i]);
     std::vector<float> flat;
     auto put = [&](const std::vector<float>& v) {
         size_t at = flat.size();
@@ -190,6 +197,21 @@ int bamf_math_erf(const float* d_x, float* d_y, int64_t n, void* stream) {
     return launch_erf(d_x, d_y, n, as_stream(stream));
 }
 
+int bamf_rjmcmc_set_proposal(bamf_rjmcmc_plan* plan, const float* d_q, const float* d_mu, float alpha,
+                             float halfwidth, float beta) {
+    BAMF_CHECK_ARG(plan && d_q && d_mu, "null argument");
+    BAMF_CHECK_ARG(alpha >= 0.0f && alpha <= 1.0f, "alpha must be in [0, 1]");
+    BAMF_CHECK_ARG(halfwidth > 0.0f, "halfwidth must be > 0");
+    BAMF_CHECK_ARG(beta >= 0.0f && beta <= 1.0f, "beta must be in [0, 1]");
+    BAMF_CHECK_ARG(plan->sig_integral > 0.0, "the signal pdf table integrates to 0");
+    plan->d_q = d_q;
+    plan->d_mu = d_mu;
+    plan->l_alpha = alpha;
+    plan->l_halfwidth = halfwidth;
+    plan->l_beta = beta;
+    return BAMF_OK;
+}
+
 int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed) {
     BAMF_CHECK_ARG(plan && claimed, "null argument");
     uint32_t q[4] = {0, 0, 0, 0};
@@ -203,7 +225,12 @@ int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed) {
 int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_t seed, int mode,
                     const bamf_replay* replay, const bamf_rjmcmc_out* out, void* stream) {
     BAMF_CHECK_ARG(plan && batch && out, "null argument");
-    BAMF_CHECK_ARG(mode == BAMF_RJ_PLAIN || mode == BAMF_RJ_REPLAY, "unknown mode %d", mode);
+    BAMF_CHECK_ARG(mode == BAMF_RJ_PLAIN || mode == BAMF_RJ_REPLAY || mode == BAMF_RJ_LEARNED, "unknown mode %d",
+                   mode);
+    if (mode == BAMF_RJ_LEARNED) {
+        BAMF_CHECK_ARG(plan->d_q && plan->d_mu, "learned mode needs bamf_rjmcmc_set_proposal first");
+        BAMF_CHECK_ARG(plan->params.is_backg == 0, "learned mode supports IsBackg = 0 only");
+    }
     BAMF_CHECK_ARG(batch->n_roi >= 0, "n_roi must be >= 0");
     if (batch->n_roi == 0) return BAMF_OK;
     BAMF_CHECK_ARG(batch->d_pix_offset && batch->d_xsize && batch->d_ysize && batch->d_data,
@@ -270,6 +297,14 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     a.pback = out->d_pback;
     a.grid_off = out->d_grid_offset;
     a.alg_flops = out->d_alg_flops;
+    if (mode == BAMF_RJ_LEARNED) {
+        a.prop_map = plan->d_q;
+        a.prop_mu = plan->d_mu;
+        a.l_alpha = plan->l_alpha;
+        a.l_halfwidth = plan->l_halfwidth;
+        a.l_beta = plan->l_beta;
+        a.l_zinv = (float)(1.0 / plan->sig_integral);
+    }
     a.recompute_every = 128;
     hipStream_t s = as_stream(stream);
     if (plan->list_cap < batch->n_roi) {

@@ -61,6 +61,14 @@ class Plan:
         check(lib().bamf_rjmcmc_plan_create(ctypes.byref(params), o.ctypes.data, len(o), s.ctypes.data, len(s),
                                             b.ctypes.data, len(b), ctypes.byref(h)), "bamf_rjmcmc_plan_create")
         self.handle = h
+        self._proposal = None
+
+    def set_proposal(self, d_q, d_mu, alpha: float = 0.2, halfwidth: float = 0.35, beta: float = 0.5) -> None:
+        """Learned proposal maps (device buffers in the batch's pixel layout)."""
+        check(lib().bamf_rjmcmc_set_proposal(self.handle, d_q.vp, d_mu.vp, ctypes.c_float(alpha),
+                                             ctypes.c_float(halfwidth), ctypes.c_float(beta)),
+              "bamf_rjmcmc_set_proposal")
+        self._proposal = (d_q, d_mu)  # keep the buffers alive while the plan uses them
 
     def __del__(self):
         try:

@@ -54,5 +54,4 @@ def declare(L: ctypes.CDLL) -> None:
                                   c_void_p]
     L.bamf_rjmcmc_last_claimed.argtypes = [c_void_p, POINTER(c_int64)]
     L.bamf_math_erf.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
-    if hasattr(L, "bamf_rjmcmc_set_proposal"):
-        L.bamf_rjmcmc_set_proposal.argtypes = [c_void_p, c_void_p]
+    L.bamf_rjmcmc_set_proposal.argtypes = [c_void_p, c_void_p, c_void_p, c_float, c_float, c_float]

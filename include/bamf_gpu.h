@@ -200,7 +200,7 @@ typedef struct bamf_rjmcmc_out {
 
 #define BAMF_RJ_PLAIN 0   /* counter-based Philox draws keyed by (seed, key[r], step) */
 #define BAMF_RJ_REPLAY 1  /* recorded draws and decisions (bamf_replay) */
-#define BAMF_RJ_LEARNED 2 /* PLAIN with birth positions drawn from a proposal map (bamf_rjmcmc_set_proposal) */
+#define BAMF_RJ_LEARNED 2 /* PLAIN with births drawn from a learned proposal (bamf_rjmcmc_set_proposal) */
 
 typedef struct bamf_rjmcmc_plan bamf_rjmcmc_plan;
 
@@ -217,6 +217,23 @@ int bamf_rjmcmc_plan_destroy(bamf_rjmcmc_plan* plan);
  * `stream` joins), so the call is asynchronous on `stream` as a whole. */
 int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_t seed, int mode,
                     const bamf_replay* replay, const bamf_rjmcmc_out* out, void* stream);
+/* Learned birth proposal for BAMF_RJ_LEARNED runs of this plan (no
+ * counterpart in the reference): d_q[pixel] is each ROI's birth-pixel
+ * probability (summing to 1 over the ROI, the batch's pixel layout; see
+ * bamf_net_proposal_maps) and d_mu[pixel] a log-intensity centre.  A birth
+ * draws the pixel from q (PixProb = P q[pixel] in the birth ratio,
+ * LibRJMCMC.cpp:1302, so a uniform q reproduces the reference's PixProb = 1)
+ * and the intensity from alpha * prior + (1 - alpha) * log-uniform on
+ * [mu - halfwidth, mu + halfwidth], the ratio gaining prior(I) / q(I) with the
+ * signal prior normalised by its table's trapezoid integral.  With
+ * probability beta a single-emitter move (LibRJMCMC.cpp:882-909) redraws
+ * the intensity from the log-uniform window at the emitter's pixel instead
+ * of the random walk (an independence Metropolis-Hastings kernel: the ratio
+ * gains q(I | x') / q(I' | x) = I' / I, or 0 when I lies outside the window at
+ * the new position); the position still walks.  The maps are caller-owned and
+ * must stay valid while runs use them. */
+int bamf_rjmcmc_set_proposal(bamf_rjmcmc_plan* plan, const float* d_q, const float* d_mu, float alpha,
+                             float halfwidth, float beta);
 /* The work-queue counters are reset by bamf_rjmcmc_run itself; this returns
  * how many chains the last run on the plan claimed (synchronous). */
 int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed);

@@ -200,3 +200,95 @@ def test_birth_death_stationary_law_gpu():
     g = run_batch(plan, rs, seed=11, trace_len=nt, trace_thin=1, hist_len=129)
     assert np.all(g.status == 0)
     check_birth_death(g.trace_n.astype(np.int64).ravel(), law)
+
+
+def test_learned_birth_keeps_birth_death_law():
+    """Learned births with a non-uniform pixel map and an intensity mixture:
+    the birth ratio's PixProb = P q(pixel) and prior(I)/q(I) factors restore
+    the flat-likelihood N law of the reference's birth/death pair."""
+    from bamf_amd import _hip
+    from bamf_amd.rjmcmc import LEARNED
+    from test_oracle_rjmcmc import BD_BG, BD_SIDE, birth_death_setup, check_birth_death
+
+    n_roi, nt = 64, 20000
+    p, data, sig, off, back, law = birth_death_setup(n_trials=nt)
+    plan = Plan(p, off, sig, back)
+    P = BD_SIDE * BD_SIDE
+    rng = np.random.default_rng(5)
+    q = rng.uniform(0.8, 1.25, (n_roi, P))
+    q = (q / q.sum(axis=1, keepdims=True)).astype(np.float32)
+    mu = np.full((n_roi, P), np.log(5e-7), np.float32)  # log-uniform part inside the prior's support
+    dq, dmu = _hip.DeviceBuffer.from_array(q.ravel()), _hip.DeviceBuffer.from_array(mu.ravel())
+    plan.set_proposal(dq, dmu, alpha=0.8, halfwidth=0.5, beta=0.0)
+    rs = RoiSet(np.tile(data, n_roi), np.arange(n_roi + 1) * data.size, [BD_SIDE] * n_roi, [BD_SIDE] * n_roi,
+                bg0=np.full(n_roi, BD_BG, np.float32))
+    g = run_batch(plan, rs, seed=13, mode=LEARNED, trace_len=nt, trace_thin=1, hist_len=129)
+    assert np.all(g.status == 0)
+    check_birth_death(g.trace_n.astype(np.int64).ravel(), law)
+
+
+def test_trained_proposal_gain():
+    """What the trained network gains: on the same ROIs, the photon trace of
+    learned chains (independence intensity proposals from the network's
+    log-intensity map) mixes faster than the reference's random walk."""
+    from bamf_amd import proposal
+    from bamf_amd.ess import ess_batch
+    from bamf_amd.rjmcmc import LEARNED
+
+    prm, _ = proposal.train(600, batch=256, lr=3e-3, seed=2)
+    rois = synth.make_rois(256, seed=31, sides=(16,), density=(0.5, 5.0), bg=(5.0, 30.0))
+    p = params(nb=2000, nt=4000, rho=0.0275)
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    rs = RoiSet.from_synth(rois, bg0=np.full(256, 10.0, np.float32))
+    dq, dmu = proposal.proposal_maps(prm, rs)
+    plan.set_proposal(dq, dmu, alpha=0.2, halfwidth=0.35, beta=0.5)
+    res = {}
+    for name, mode in (("plain", PLAIN), ("learned", LEARNED)):
+        g = run_batch(plan, rs, seed=17, mode=mode, trace_len=800, trace_thin=5)
+        assert np.all(g.status == 0)
+        c = g.counters.sum(axis=0)
+        en = ess_batch(g.trace_n.astype(np.float64))
+        ep = ess_batch(g.trace_photons.astype(np.float64))
+        res[name] = dict(birth_acc=c[5 + 3] / c[3], move_acc=c[5] / c[0], ess_n=float(np.median(en)),
+                         ess_ph=float(np.median(ep)), ess_min=float(np.median(np.minimum(en, ep))))
+    print("plain vs learned:", res, flush=True)
+    assert res["learned"]["ess_ph"] > 1.5 * res["plain"]["ess_ph"]
+
+
+def test_learned_intensity_kernel_keeps_prior():
+    """The independence intensity kernel of learned moves under an exactly flat
+    likelihood: an emitter's intensity must be distributed as the prior
+    (uniform on (0, 1e-6)) restricted to the log-uniform window it moves in.
+    Dropping the kernel's Jacobian I'/I would give the log-uniform law instead."""
+    from scipy import stats
+
+    from bamf_amd import _hip
+    from bamf_amd.rjmcmc import LEARNED, make_params
+    from test_oracle_rjmcmc import BD_SIDE
+
+    # background 1e-5 (inside the offset table, so the background factor of
+    # the move ratio is 1) and intensities near 5e-14, far below ulp(1e-5) / 2
+    n_roi, side, bg = 1024, BD_SIDE, 1e-5
+    p = make_params(n_burnin=1000, n_trials=20000, rho=1.0 / side ** 2, dx=1e-7,
+                    p_burnin=(1, 0, 0, 0, 0, 0, 0, 0), p_trials=(1, 0, 0, 0, 0, 0, 0, 0))
+    plan = Plan(p, np.ones(201, np.float32), np.ones(11, np.float32), np.zeros(11, np.float32))
+    P = side * side
+    mu0, w = np.log(5e-14), 0.5
+    dq = _hip.DeviceBuffer.from_array(np.full(n_roi * P, 1.0 / P, np.float32))
+    dmu = _hip.DeviceBuffer.from_array(np.full(n_roi * P, mu0, np.float32))
+    plan.set_proposal(dq, dmu, alpha=1.0, halfwidth=w, beta=0.5)
+    data = np.full(n_roi * P, bg, np.float32)
+    em = (np.full(n_roi, 5e-14, np.float32), np.full(n_roi, 4.0, np.float32), np.full(n_roi, 4.0, np.float32),
+          np.arange(n_roi, dtype=np.int64))
+    rs = RoiSet(data, np.arange(n_roi + 1) * P, [side] * n_roi, [side] * n_roi,
+                bg0=np.full(n_roi, bg, np.float32), n0=np.ones(n_roi, np.int32), em0=em)
+    g = run_batch(plan, rs, seed=21, mode=LEARNED, hist_len=8)
+    assert np.all(g.status == 0) and np.all(g.n_final == 1)
+    I = g.final[:, 4].astype(np.float64)
+    lo, hi = np.exp(mu0 - w), np.exp(mu0 + w)
+    assert np.all((I >= lo * 0.999) & (I <= hi * 1.001))
+    ks = stats.kstest(I, stats.uniform(loc=lo, scale=hi - lo).cdf)
+    ks_log = stats.kstest(np.log(I), stats.uniform(loc=mu0 - w, scale=2 * w).cdf)
+    print("KS uniform p", ks.pvalue, "KS log-uniform p", ks_log.pvalue, flush=True)
+    assert ks.pvalue > 0.01
+    assert ks_log.pvalue < 1e-4

@@ -198,7 +198,7 @@ def birth_death_setup(n_trials=300000):
     from bamf_amd.rjmcmc import make_params
 
     dx = 1e-7  # intensities in (0, 1e-6): far below ulp(100) / 2
-    sig = np.ones(11, np.float32)
+    sig = np.full(11, 1e6, np.float32)  # the density of U(0, 1e-6) (its table integrates to 1)
     off = np.ones(11, np.float32)
     back = np.zeros(11, np.float32)
     p = make_params(n_burnin=2000, n_trials=n_trials, rho=BD_MEANN / BD_SIDE ** 2, dx=dx, bnd_out=BD_BND,
