@@ -51,3 +51,18 @@ def roi_posteriors(trace_n: np.ndarray, trace_photons: np.ndarray, ess_target: f
     the mask of ROIs that reach the target."""
     e = np.minimum(ess_batch(trace_n.astype(np.float64)), ess_batch(trace_photons.astype(np.float64)))
     return e, e >= ess_target
+
+
+def fixed_ess_rate(ess: np.ndarray, seconds: np.ndarray, burn_frac: float, target: float = 200.0) -> float:
+    """ROI posteriors per second at a fixed ESS.
+
+    ROI i's share of the measured run time, seconds[i], is rescaled to the
+    time its chain needs to reach `target` ESS: the burn-in part is kept and
+    the post-burn-in part scales by target / ESS_i (the ESS of a stationary
+    chain grows linearly with its length).  The rate is the number of ROIs
+    over the sum of those times, so ROIs that mix slowly weigh in as the
+    longer runs they would need; ESS is floored at 1.
+    """
+    e = np.maximum(np.asarray(ess, np.float64), 1.0)
+    t = np.asarray(seconds, np.float64) * (burn_frac + (1.0 - burn_frac) * target / e)
+    return float(len(e) / t.sum())

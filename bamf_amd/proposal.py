@@ -162,14 +162,32 @@ def train(steps: int, batch: int = 256, lr: float = 2e-3, seed: int = 0, side: i
     return p, hist
 
 
+# Multiply-adds of one 32 x 32 canvas through the network (x 2 = FLOP):
+# 1024 pixels x (9 x 32 + 3 x 9 x 32 x 32 + 2 x 32).
+NET_FLOP_PER_CANVAS = 2 * 1024 * (9 * C + 3 * 9 * C * C + 2 * C)
+
+
+class MapMaker:
+    """Proposal maps for ROI batches with a network built once (32 x 32 canvas)."""
+
+    def __init__(self, params: np.ndarray, max_batch: int = 1024):
+        self.net = Network(params, max_batch, 32, 32)
+
+    def run(self, rois, dq, dmu, eps: float = 0.05, stream=None) -> None:
+        check(lib().bamf_net_proposal_maps(self.net.ptr, ctypes.byref(rois.batch), ctypes.c_float(eps), dq.vp,
+                                           dmu.vp, stream.vp if stream else None), "bamf_net_proposal_maps")
+
+    def close(self):
+        self.net.close()
+
+
 def proposal_maps(params: np.ndarray, rois, eps: float = 0.05, max_batch: int = 1024):
     """Birth-proposal maps (q, mu) for a device ROI batch (rjmcmc.RoiSet),
     as device buffers in the batch's pixel layout."""
-    net = Network(params, max_batch, 32, 32)
+    mm = MapMaker(params, max_batch)
     total = int(rois.total_pixels)
     dq, dmu = _hip.DeviceBuffer(total * 4), _hip.DeviceBuffer(total * 4)
-    check(lib().bamf_net_proposal_maps(net.ptr, ctypes.byref(rois.batch), ctypes.c_float(eps), dq.vp, dmu.vp, None),
-          "bamf_net_proposal_maps")
+    mm.run(rois, dq, dmu, eps)
     _hip.synchronize()
-    net.close()
+    mm.close()
     return dq, dmu

@@ -7,12 +7,16 @@ uniform 5-30, sigma 1.3 px; 10,000 jumps per chain (5,000 burn-in + 5,000
 post-burn-in) with P_Burnin = [.4 .15 .15 .15 .15 0 0 0] and
 P_Trials = [.6 .1 .1 .1 .1 0 0 0].
 
-A step is one batched run of every chain of the workload (one persistent
-launch).  An ROI posterior is ESS = 200 of an ROI's post-burn-in chain, the
-ESS being the smaller of the emitter-count and total-photon traces' ESS; value
-= sum over ROIs of ESS / 200, per second of the timed steps (inputs resident in
-HBM): the rate at which posteriors of that ESS are produced.  The strict count
-(ROIs that reach ESS >= 200 within the fixed chain length) is reported too.  The CPU baseline is the oracle chain (a C restatement of the
+A step is one batched run of every chain of the workload (the persistent
+launches of both ROI size classes).  An ROI posterior is an ROI's
+post-burn-in chain at ESS = 200, the ESS being the smaller of the
+emitter-count and total-photon traces' ESS (Geyer).  value = ROI posteriors
+per second at that fixed ESS: each ROI's share of the timed step (inputs
+resident in HBM) is rescaled to the length its chain needs to reach ESS 200
+(burn-in kept, the rest scaled by 200 / ESS), and value = ROIs / total
+(bamf_amd.ess.fixed_ess_rate).  The strict count (ROIs reaching ESS 200
+within the fixed chain length) and the summed-ESS rate are reported too, as
+is the learned-proposal chain on the same ROIs.  The CPU baseline is the oracle chain (a C restatement of the
 reference's RJMCMC class) on a sample of the same ROIs over all host cores,
 timed before the GPU is touched.
 """
@@ -81,17 +85,19 @@ def _cpu_worker(idx):
     for r in idx:
         o = rois["offset"]
         s = int(rois["side"][r])
+        t0 = time.perf_counter()
         res = po.rjmcmc(rois["data"][o[r]:o[r + 1]], s, s, op, off, sig, back, bg0=bg0, seed=SEED, roi=int(r),
                         record=False)
+        dt = time.perf_counter() - t0
         st = res["steps"][p.n_burnin:]
-        out.append((int(r), st["n"].copy(), st["photons"].copy()))
+        out.append((int(r), st["n"].copy(), st["photons"].copy(), dt))
     return out
 
 
 def cpu_baseline(rois, p, off, sig, back, bg0, sample, cores, thin):
     import multiprocessing as mp
 
-    from bamf_amd.ess import roi_posteriors
+    from bamf_amd.ess import fixed_ess_rate, roi_posteriors
     from oracle import pyoracle as po
 
     po.build()
@@ -109,11 +115,15 @@ def cpu_baseline(rois, p, off, sig, back, bg0, sample, cores, thin):
     tp = np.stack([r[2][::thin] for r in res])
     ess, ok = roi_posteriors(tn, tp)
     jumps = sample * (p.n_burnin + p.n_trials)
-    return {"value": float(ess.sum() / 200.0 / dt), "unit": "ROI posteriors/s", "cores": cores, "kind": "port",
+    # each ROI's share of the wall time: its own chain time over the worker count
+    share = np.array([r[3] for r in res]) * (dt / sum(r[3] for r in res))
+    burn = p.n_burnin / (p.n_burnin + p.n_trials)
+    return {"value": fixed_ess_rate(ess, share, burn), "unit": "ROI posteriors/s", "cores": cores, "kind": "port",
             "sample": f"{sample} of the workload's ROIs (seeded random subset), full {p.n_burnin}+{p.n_trials}-jump "
                       f"chains of the oracle C chain, {cores} forked worker processes, {dt:.1f} s wall",
             "chains_per_s": float(sample / dt), "jumps_per_s": float(jumps / dt),
-            "strict_posteriors_per_s": float(ok.sum() / dt), "median_ess": float(np.median(ess))}
+            "strict_posteriors_per_s": float(ok.sum() / dt), "median_ess": float(np.median(ess)),
+            "ess_sum_rate": float(ess.sum() / 200.0 / dt)}
 
 
 # ---------------------------------------------------------------- accuracy
@@ -314,6 +324,9 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--thin", type=int, default=5)
+    ap.add_argument("--proposal", default="both", choices=["plain", "learned", "both"],
+                    help="chain proposal(s) timed: the value is the plain chain's, learned is reported beside it")
+    ap.add_argument("--train-steps", type=int, default=3000, help="Adam steps of 256 ROIs for the learned proposal")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (the HBM counter calibration run)")
     args = ap.parse_args()
@@ -335,9 +348,9 @@ def main():
         cpu = cpu_baseline(rois, p, off, sig, back, bg0, args.cpu_sample, cores, args.thin)
         print(f"[bench] cpu baseline done: {cpu['chains_per_s']:.1f} chains/s", file=sys.stderr, flush=True)
 
-    from bamf_amd import _hip
-    from bamf_amd.ess import roi_posteriors
-    from bamf_amd.rjmcmc import Plan, RoiSet, alloc_outputs, fetch, lpt_order, run_batch
+    from bamf_amd import _hip, proposal
+    from bamf_amd.ess import fixed_ess_rate, roi_posteriors
+    from bamf_amd.rjmcmc import LEARNED, PLAIN, Plan, RoiSet, alloc_outputs, fetch, lpt_order, run_batch
 
     n = len(rois["side"])
     plan = Plan(p, off, sig, back)
@@ -347,32 +360,45 @@ def main():
     cap = 128
     outs = alloc_outputs(n, nb + nt, 129, cap, trace_len, False, False, rs, p.grid_zoom)
     stream = _hip.Stream()
-    ev0, ev1 = _hip.Event(), _hip.Event()
-    for w in range(args.warmup):
-        run_batch(plan, rs, seed=SEED + 1000 + w, trace_thin=args.thin, stream=stream, sync=False, outs=outs,
-                  hist_len=129, final_cap=cap, trace_len=trace_len)
-    stream.synchronize()
-    print(f"[bench] warmup done ({args.warmup})", file=sys.stderr, flush=True)
-    _hip.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for k in range(args.steps):
-        run_batch(plan, rs, seed=SEED + k, trace_thin=args.thin, stream=stream, sync=False, outs=outs, hist_len=129,
-                  final_cap=cap, trace_len=trace_len)
-    ev1.record(stream)
-    ev1.synchronize()
-    _hip.synchronize()
-    wall = time.perf_counter() - t0
-    ms = ev0.elapsed_ms(ev1) / args.steps
-    print(f"[bench] timed {args.steps} steps: {ms:.1f} ms/step (wall {wall:.2f} s)", file=sys.stderr, flush=True)
-    res = fetch(outs, n, nb + nt, 129, cap, trace_len)
-    assert np.all(res.status == 0), np.unique(res.status, return_counts=True)
-    ess, ok = roi_posteriors(res.trace_n, res.trace_photons)
-    value = float(ess.sum() / 200.0 / (ms / 1e3))
+
+    def timed(mode, pre=None, pre_events=None):
+        """Warm-up, then args.steps timed steps (HIP events on the launch stream)."""
+        ev0, ev1 = _hip.Event(), _hip.Event()
+        for w in range(args.warmup):
+            if pre:
+                pre()
+            run_batch(plan, rs, seed=SEED + 1000 + w, mode=mode, trace_thin=args.thin, stream=stream, sync=False,
+                      outs=outs, hist_len=129, final_cap=cap, trace_len=trace_len)
+        stream.synchronize()
+        _hip.synchronize()
+        ev0.record(stream)
+        for k in range(args.steps):
+            if pre:
+                pre()
+            run_batch(plan, rs, seed=SEED + k, mode=mode, trace_thin=args.thin, stream=stream, sync=False,
+                      outs=outs, hist_len=129, final_cap=cap, trace_len=trace_len)
+        ev1.record(stream)
+        ev1.synchronize()
+        ms = ev0.elapsed_ms(ev1) / args.steps
+        res = fetch(outs, n, nb + nt, 129, cap, trace_len)
+        assert np.all(res.status == 0), np.unique(res.status, return_counts=True)
+        ess, ok = roi_posteriors(res.trace_n, res.trace_photons)
+        rmse = localization_rmse_nm(res.final, rois["truth"], cap)
+        print(f"[bench] {('plain', 'replay', 'learned')[mode]}: {ms:.1f} ms/step, mean ESS {ess.mean():.2f}, "
+              f"RMSE {rmse:.2f} nm", file=sys.stderr, flush=True)
+        # each ROI's share of the launch: its algorithmic work's share
+        cost = res.alg_flops.astype(np.float64)
+        share = (ms / 1e3) * cost / cost.sum()
+        return ms, res, ess, ok, rmse, fixed_ess_rate(ess, share, nb / (nb + nt))
+
+    ms, res, ess, ok, rmse, value = timed(PLAIN)
     flops = float(res.alg_flops.astype(np.float64).sum())
     achieved = flops / (ms / 1e3) / 1e12
-    rmse = localization_rmse_nm(res.final, rois["truth"], cap)
     traffic = read_traffic(ROOT / "profiles" / f"rjmcmc_chain_{args.workload}_pmc.txt")
+    plain = {"value": value, "ms_per_step": ms, "strict_posteriors_per_s": float(ok.sum() / (ms / 1e3)),
+             "ess_sum_rate": float(ess.sum() / 200.0 / (ms / 1e3)),
+             "mean_ess": float(ess.mean()), "median_ess": float(np.median(ess)), "rmse_nm_final_state": rmse,
+             "chains_per_s": n / (ms / 1e3), "jumps_per_s": n * (nb + nt) / (ms / 1e3)}
     line = {
         "metric": "ROI posteriors/sec at fixed ESS & RMSE: plain vs learned-proposal RJMCMC",
         "value": value, "unit": "ROI posteriors/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -380,17 +406,42 @@ def main():
         "dtype": "f32 (accept test f64)", "data": "synthetic (seeded forward-model ROIs)",
         "config": {"workload": args.workload, "n_roi": n, "roi_sides": sorted(set(int(s) for s in rois["side"])),
                    "jumps_per_chain": nb + nt, "burnin": nb, "ess_target": 200, "proposal": "plain",
-                   "parallelism": "1 GPU, one wavefront per chain, persistent work queue"},
+                   "parallelism": "1 GPU, one wavefront per chain, persistent work queues per ROI size class"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "k_chain", "alg_flop_per_launch": flops},
         "cpu_baseline": cpu,
-        "quality": {"strict_posteriors_per_s": float(ok.sum() / (ms / 1e3)), "median_ess": float(np.median(ess)),
-                    "rmse_nm_final_state": rmse, "chains_per_s": n / (ms / 1e3),
-                    "jumps_per_s": n * (nb + nt) / (ms / 1e3)},
+        "quality": plain,
     }
     if cpu is not None:
         line["gpu_over_cpu"] = value / cpu["value"] if cpu["value"] > 0 else None
+    if args.proposal in ("learned", "both"):
+        t0 = time.perf_counter()
+        prm, hist = proposal.train(args.train_steps, batch=256, lr=3e-3, seed=SEED % 1000)
+        train_s = time.perf_counter() - t0
+        print(f"[bench] network trained: {args.train_steps} x 256 ROIs in {train_s:.1f} s", file=sys.stderr,
+              flush=True)
+        mm = proposal.MapMaker(prm)
+        dq, dmu = _hip.DeviceBuffer(rs.total_pixels * 4), _hip.DeviceBuffer(rs.total_pixels * 4)
+        plan.set_proposal(dq, dmu, alpha=0.2, halfwidth=0.35, beta=1.0)
+        em0, em1 = _hip.Event(), _hip.Event()
+        em0.record(stream)
+        mm.run(rs, dq, dmu, stream=stream)
+        em1.record(stream)
+        em1.synchronize()
+        map_ms = em0.elapsed_ms(em1)
+        lms, lres, less, lok, lrmse, lvalue = timed(LEARNED, pre=lambda: mm.run(rs, dq, dmu, stream=stream))
+        net_tflops = proposal.NET_FLOP_PER_CANVAS * n / (map_ms / 1e3) / 1e12
+        line["learned"] = {
+            "value": lvalue, "ms_per_step": lms, "maps_ms": map_ms, "train_s": train_s,
+            "train_rois": args.train_steps * 256, "strict_posteriors_per_s": float(lok.sum() / (lms / 1e3)),
+            "ess_sum_rate": float(less.sum() / 200.0 / (lms / 1e3)),
+            "mean_ess": float(less.mean()), "median_ess": float(np.median(less)), "rmse_nm_final_state": lrmse,
+            "rmse_vs_plain": lrmse / rmse - 1.0 if rmse else None,
+            "network_roofline": {"bound": "mfma", "achieved": net_tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                                 "frac": net_tflops / 2500.0, "kernel": "proposal maps (bf16 MFMA conv)"}}
+        line["learned_over_plain"] = lvalue / value if value > 0 else None
+        mm.close()
     print(json.dumps(line), flush=True)
     return 0
 

@@ -33,3 +33,15 @@ def test_constant_trace_and_criterion():
     tp = ar1(0.1, 500, 2, 3) + 1000
     ess, ok = roi_posteriors(tn, tp, 200)
     assert ok[0] and not ok[1]
+
+
+def test_fixed_ess_rate():
+    from bamf_amd.ess import fixed_ess_rate
+
+    # two ROIs sharing 1 s equally, burn-in half: ESS 200 needs no extension
+    assert abs(fixed_ess_rate(np.array([200.0, 200.0]), np.array([0.5, 0.5]), 0.5) - 2.0) < 1e-12
+    # ESS 100 doubles the post-burn-in half: 0.5 * (0.5 + 1) each
+    assert abs(fixed_ess_rate(np.array([100.0, 100.0]), np.array([0.5, 0.5]), 0.5) - 2 / 1.5) < 1e-12
+    # a stuck chain (ESS floored at 1) dominates
+    r = fixed_ess_rate(np.array([200.0, 0.1]), np.array([0.5, 0.5]), 0.0)
+    assert abs(r - 2 / (0.5 + 100.0)) < 1e-12
