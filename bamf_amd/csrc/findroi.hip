@@ -33,11 +33,36 @@ constexpr int kChunk = 32;
 
 struct Coeffs {
     float b0, b1, b2, b3, B;
+    float rb0;  // 1 / b0 correctly rounded (set by with_rcp on the host)
 };
 
+// The correctly rounded reciprocal of b: the float nearest 1/(double)b, or
+// the neighbour whose product with b (exact in double) is closer to 1.
+inline Coeffs with_rcp(Coeffs c) {
+    const float y = (float)(1.0 / (double)c.b0);
+    float best = y;
+    double err = std::fabs(1.0 - (double)y * (double)c.b0);
+    for (float z : {std::nextafter(y, 0.0f), std::nextafter(y, 2.0f * y)}) {
+        const double e = std::fabs(1.0 - (double)z * (double)c.b0);
+        if (e < err) {
+            err = e;
+            best = z;
+        }
+    }
+    c.rb0 = best;
+    return c;
+}
+
 __device__ inline float iir_step(float w0, float w1, float w2, float w3, const Coeffs& c) {
-    // cuda_FindROI.cu:28 — temp = w0*B + (b1*w1 + b2*w2 + b3*w3) / b0
-    return w0 * c.B + (c.b1 * w1 + c.b2 * w2 + c.b3 * w3) / c.b0;
+    // cuda_FindROI.cu:28 — temp = w0*B + (b1*w1 + b2*w2 + b3*w3) / b0.  The
+    // quotient is the correctly rounded x / b0 formed from the correctly
+    // rounded reciprocal with one residual correction (Markstein's theorem:
+    // q0 = RN(x y), r = x - q0 b0 exact by fma, RN(q0 + r y) = RN(x / b0) when
+    // y = RN(1 / b0)), three instructions instead of the division sequence.
+    const float x = c.b1 * w1 + c.b2 * w2 + c.b3 * w3;
+    const float q0 = x * c.rb0;
+    const float q = fmaf(fmaf(-q0, c.b0, x), c.rb0, q0);
+    return w0 * c.B + q;
 }
 
 // Row recursion (the reference's gaussX) for NS sigmas at once.  in may alias out0.
@@ -387,7 +412,7 @@ __global__ void k_crop(const float* stack, int xsize, int ysize, const int32_t* 
 Coeffs make_coeffs(float sigma) {
     float o[5];
     bamf_gauss_coeffs(sigma, o);
-    return Coeffs{o[0], o[1], o[2], o[3], o[4]};
+    return with_rcp(Coeffs{o[0], o[1], o[2], o[3], o[4], 0.0f});
 }
 
 int launch_rows(const float* in, float* o0, float* o1, int xsize, int64_t nrows, Coeffs c0, Coeffs c1,
@@ -427,7 +452,7 @@ int bamf_find_roi_gauss_x(float* d, int xsize, int ysize, int nframes, float b0,
     BAMF_CHECK_ARG(xsize > 0 && ysize > 0 && nframes >= 0, "bad sizes");
     if (nframes == 0) return BAMF_OK;
     BAMF_CHECK_ARG(d, "null device pointer");
-    Coeffs c{b0, b1, b2, b3, B};
+    const Coeffs c = with_rcp(Coeffs{b0, b1, b2, b3, B, 0.0f});
     return launch_rows(d, d, nullptr, xsize, (int64_t)ysize * nframes, c, c, 1, as_stream(stream));
 }
 
@@ -437,7 +462,7 @@ int bamf_find_roi_gauss_y(float* d, int xsize, int ysize, int nframes, float b0,
     if (nframes == 0) return BAMF_OK;
     BAMF_CHECK_ARG(d, "null device pointer");
     BAMF_CHECK_ARG(nframes <= 65535, "nframes must be <= 65535 per call");
-    Coeffs c{b0, b1, b2, b3, B};
+    const Coeffs c = with_rcp(Coeffs{b0, b1, b2, b3, B, 0.0f});
     dim3 grid((unsigned)ceil_div(xsize, 64), (unsigned)nframes);
     hipLaunchKernelGGL(k_cols_iir<false>, grid, dim3(64), 0, as_stream(stream), d, d, xsize, ysize, c, c);
     BAMF_LAUNCH_CHECK("k_cols_iir");

@@ -10,6 +10,7 @@
 // Per pixel the model, derivative and Newton sums are the reference's
 // expressions evaluated in the reference's order.
 #include "common.hpp"
+#include "erf_fast.hpp"
 
 namespace bamf {
 namespace {
@@ -63,11 +64,12 @@ struct AxisTerms {
 __device__ inline AxisTerms axis_terms(int i, float x, float sigma, float sqrt_norm, float n,
                                        float c1, float c2) {
     AxisTerms t;
-    t.psf = 0.5f * (erff((i - x + 1.0f) * sqrt_norm) - erff((i - x) * sqrt_norm));
+    // branch-free erf: the fits of a wavefront evaluate different ranges
+    t.psf = 0.5f * (erf_nr((i - x + 1.0f) * sqrt_norm) - erf_nr((i - x) * sqrt_norm));
     float ta = (i + 1.0f - x) / sigma;
     float tb = (i - x) / sigma;
-    float a = expf(-0.5f * (ta * ta));
-    float b = expf(-0.5f * (tb * tb));
+    float a = __expf(-0.5f * (ta * ta));
+    float b = __expf(-0.5f * (tb * tb));
     t.d1 = -n / c1 / sigma * (a - b);
     t.d2 = -n / c1 / c2 * ((i + 1.0f - x) * a - (i - x) * b);
     return t;
@@ -175,8 +177,9 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
                 const float u2 = px * py;
                 float cf = 0.f, df = 0.f;
                 if (model > 10e-3f) {
-                    cf = dat / model - 1;
-                    df = dat / (model * model);
+                    const float r = __builtin_amdgcn_rcpf(model);  // 1 ulp, for two divisions
+                    cf = dat * r - 1;
+                    df = dat * r * r;
                 }
                 cf = fminf(cf, 10e4f);
                 df = fminf(df, 10e4f);
@@ -217,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
                 const float model = theta[3] + theta[2] * px * py;
                 const float dat = d[SZ * jj + ii];
                 float du[kNV] = {ax[ii].d1 * py, ay[jj].d1 * px, px * py, 1.0f};
-                const float rm = 1.0f / model;
+                const float rm = __builtin_amdgcn_rcpf(model);
 #pragma unroll
                 for (int k = 0; k < kNV; k++)
 #pragma unroll
@@ -300,8 +303,9 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
                 const float u0 = ax.d1 * py, v0 = ax.d2 * py, u1 = ay.d1 * px, v1 = ay.d2 * px, u2 = px * py;
                 float cf = 0.f, df = 0.f;
                 if (model > 10e-3f) {
-                    cf = dat / model - 1;
-                    df = dat / (model * model);
+                    const float r = __builtin_amdgcn_rcpf(model);  // 1 ulp, for two divisions
+                    cf = dat * r - 1;
+                    df = dat * r * r;
                 }
                 cf = fminf(cf, 10e4f);
                 df = fminf(df, 10e4f);
