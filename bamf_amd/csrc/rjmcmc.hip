@@ -134,7 +134,7 @@ __device__ __forceinline__ int kround(float v) { return (int)floor((double)v + 0
 // with the exact product; otherwise it is found by search from the same guess.
 __device__ __forceinline__ float findpdf(const StatDev& s, float dx, float I) {
     if (!(I > 0.0f) || I >= s.x_last) return 0.0f;  // NaN also yields 0, as the reference does
-    int j = (int)(I / dx) + 1;
+    int j = (int)(I * s.inv_dx) + 1;  // a guess within one of floor(I / dx) + 1, corrected below
     j = j < 1 ? 1 : (j > s.len - 1 ? s.len - 1 : j);
     float xj, xj1;
     if (s.x_exact) {
@@ -148,8 +148,14 @@ __device__ __forceinline__ float findpdf(const StatDev& s, float dx, float I) {
         xj = s.x[j];
         xj1 = s.x[j - 1];
     }
-    const float dx0 = s.x_exact ? dx : (s.x[1] - s.x[0]);
-    return (s.pdf[j] * (I - xj1) + s.pdf[j - 1] * (xj - I)) / dx0;
+    const float num = s.pdf[j] * (I - xj1) + s.pdf[j - 1] * (xj - I);
+    if (s.x_exact) {
+        // num / dx correctly rounded from the correctly rounded 1/dx and one
+        // residual step (Markstein)
+        const float q0 = num * s.inv_dx;
+        return fmaf(fmaf(-q0, dx, num), s.inv_dx, q0);
+    }
+    return num / (s.x[1] - s.x[0]);
 }
 
 // LibRJMCMC.cpp:248-282 with the uniform already drawn; the first uc > u is
@@ -460,8 +466,24 @@ struct Chain {
     // pixel edges e_k = (k - pos)*C.  erf is the branch-free one, two edges per lane.
     __device__ __forceinline__ float stencil(float I, float xe, float ye) {
         const float l = (float)lane;
-        const float vx = 0.5f * (erf_nr(((float)(lane + 1) - xe) * C) - erf_nr((l - xe) * C));
-        const float vy = 0.5f * (erf_nr(((float)(ysub + 1) - ye) * C) - erf_nr(((float)ysub - ye) * C));
+        // each edge's erf once: the next x edge arrives by a lane shift, the
+        // lane's two y edges by permutes from the lanes that evaluated them
+        float vx, ey0, ey1;
+        if (X + Y + 2 <= 64) {
+            // one erf per lane: x edges 0..X on lanes 0..X, y edges 0..Y on lanes X+1..X+Y+1
+            const int by = X + 1;
+            const float e = erf_nr(lane < by ? (l - xe) * C : ((float)(lane - by) - ye) * C);
+            vx = 0.5f * (__shfl_down(e, 1) - e);
+            const int src = by + (ysub < Y ? ysub : 0);
+            ey0 = __shfl(e, src);
+            ey1 = __shfl(e, src + 1);
+        } else {
+            const float ex = erf_nr((l - xe) * C), ey = erf_nr((l - ye) * C);
+            vx = 0.5f * (__shfl_down(ex, 1) - ex);
+            ey0 = __shfl(ey, ysub);
+            ey1 = __shfl(ey, ysub + 1);
+        }
+        const float vy = 0.5f * (ey1 - ey0);
         // columns x >= X and rows y >= Y (padding lanes) get exact zeros, so
         // the pixel loops need no lane masks
         L.sten[lane] = lane < X ? I * vx : 0.0f;
@@ -541,7 +563,8 @@ struct Chain {
                 const float r = __builtin_amdgcn_rcpf(m[s]);
                 const float q0 = mt[s] * r;
                 const float q = fmaf(fmaf(-q0, m[s], mt[s]), r, q0);
-                acc = acc + ((m[s] - mt[s]) + L.dpx[64 * s + lane] * logf(q));
+                const float lq = __builtin_amdgcn_logf(q) * 0.693147180559945f;  // hardware log2
+                acc = acc + ((m[s] - mt[s]) + L.dpx[64 * s + lane] * lq);
             }
         return bcastf(wave_scan(acc), 63);
     }

@@ -29,6 +29,7 @@ struct StatDev {
     int32_t nu;
     int32_t x_exact;     // 1 when x[i] == (float)i * dx for every i (O(1) findPDF index)
     float x_last;        // x[len - 1]
+    float inv_dx;        // 1 / dx, correctly rounded
 };
 constexpr int kGuide = 256;
 

@@ -1,4 +1,5 @@
 // Host side of the chain ABI: prior tables, argument checks, launch sizing.
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -69,6 +70,22 @@ struct HostStat {
         return true;
     }
 };
+
+// The correctly rounded reciprocal of b (the float nearest 1/b): the nearest
+// float to 1/(double)b or a neighbour, judged by |1 - y b| exactly in double.
+float rcp_rn(float b) {
+    const float y = (float)(1.0 / (double)b);
+    float best = y;
+    double err = std::fabs(1.0 - (double)y * (double)b);
+    for (float z : {std::nextafter(y, 0.0f), std::nextafter(y, 2.0f * y)}) {
+        const double e = std::fabs(1.0 - (double)z * (double)b);
+        if (e < err) {
+            err = e;
+            best = z;
+        }
+    }
+    return best;
+}
 
 }  // namespace
 
@@ -172,6 +189,7 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
         sd[k]->nu = (int)ss[k]->ux.size();
         sd[k]->x_exact = ss[k]->exact(params->dx) ? 1 : 0;
         sd[k]->x_last = ss[k]->x.back();
+        sd[k]->inv_dx = rcp_rn(params->dx);
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, pl->device) == hipSuccess) pl->cu = prop.multiProcessorCount;
