@@ -204,7 +204,9 @@ struct Chain {
     int xsub, ysub;
     // draws: window of 64 positions starting at wbase; dj = next position
     int wbase, dj;
-    float wu, wr;  // uniform at wbase + lane, Irwin-Hall sum of the 12 uniforms from there
+    float wu, wr;
+    uint32_t pw[4];  // Philox block of window position `lane` for steps 4 pblk .. 4 pblk + 3
+    int pblk;  // uniform at wbase + lane, Irwin-Hall sum of the 12 uniforms from there
     uint32_t key;
     int step;
     int64_t tbase, tlen, tcur;
@@ -243,10 +245,12 @@ struct Chain {
     __device__ __forceinline__ int pidx(int s) const { return px(s) * Y + ysub; }
 
     // ---------------- draws ----------------
-    // Window of draws wbase .. wbase + 63 of this step.  Draw j is word j % 4
-    // of Philox4x32-10 with counter (step, j / 4, seed_hi, 0) and key
-    // (roi key, seed_lo), shifted to 31 bits (the range of rand()); replay
-    // mode reads the tape instead (0 past its end, flagged at the step end).
+    // Window of draws wbase .. wbase + 63 of this step.  Draw j of step s is
+    // word s % 4 of Philox4x32-10 with counter (s / 4, j, seed_hi, 0) and key
+    // (roi key, seed_lo), shifted to 31 bits (the range of rand()): the block
+    // lane l computes for window position l serves four consecutive steps and
+    // is kept for the three after it.  Replay mode reads the tape instead (0
+    // past its end, flagged at the step end).
     // urand = (float)raw / 2^31 (LibRJMCMCreturn.hpp:17); the twelve-term
     // sum of randn (LibRJMCMCreturn.hpp:23-26) is formed left to right from
     // each position, so randn() is that sum minus 6, bit for bit.
@@ -257,9 +261,24 @@ struct Chain {
         if (a->mode == BAMF_RJ_REPLAY) {
             w = (tcur + (int64_t)q < tlen) ? a->tape[tbase + tcur + q] : 0u;
         } else {
+            const uint32_t blk = (uint32_t)step >> 2;
             uint32_t o[4];
-            philox((uint32_t)step, q >> 2, a->seed_hi, 0u, key, a->seed_lo, o);
-            const uint32_t c = q & 3u;
+            if (base == 0 && blk == (uint32_t)pblk) {
+                o[0] = pw[0];
+                o[1] = pw[1];
+                o[2] = pw[2];
+                o[3] = pw[3];
+            } else {
+                philox(blk, q, a->seed_hi, 0u, key, a->seed_lo, o);
+                if (base == 0) {
+                    pw[0] = o[0];
+                    pw[1] = o[1];
+                    pw[2] = o[2];
+                    pw[3] = o[3];
+                    pblk = (int)blk;
+                }
+            }
+            const uint32_t c = (uint32_t)step & 3u;
             w = (c == 0 ? o[0] : c == 1 ? o[1] : c == 2 ? o[2] : o[3]) >> 1;
         }
         wu = (float)w / 2147483648.0f;
@@ -1238,6 +1257,7 @@ struct Chain {
             tlen = 0;
         }
         tcur = 0;
+        pblk = -1;
         const int nb = p.n_burnin, total = p.n_burnin + p.n_trials;
         const int64_t rstride = total;
         for (int s = 0; s < total && status == ST_OK; s++) {

@@ -25,7 +25,9 @@ extern "C" {
 
 /* ---- Philox4x32-10 (Salmon et al., SC'11), published algorithm ---------- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
-/* Raw 31-bit draw j of step `step` of chain `roi` (same layout as the GPU). */
+/* Raw 31-bit draw j of step `step` of chain `roi` (same layout as the GPU):
+ * word step % 4 of Philox4x32-10 at counter (step / 4, j, seed_hi, 0), key
+ * (roi, seed_lo), shifted right by one. */
 uint32_t orc_chain_raw(uint64_t seed, uint32_t roi, uint32_t step, uint32_t j);
 
 /* ---- gaussMLE XYNB (cuda_gaussMLEv2.cu:16-141) --------------------------- */

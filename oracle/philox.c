@@ -37,9 +37,10 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 }
 
 uint32_t orc_chain_raw(uint64_t seed, uint32_t roi, uint32_t step, uint32_t j) {
-    uint32_t ctr[4] = {step, j >> 2, (uint32_t)(seed >> 32), 0u};
+    /* one Philox block serves draw j of four consecutive steps */
+    uint32_t ctr[4] = {step >> 2, j, (uint32_t)(seed >> 32), 0u};
     uint32_t key[2] = {roi, (uint32_t)seed};
     uint32_t out[4];
     orc_philox4x32_10(ctr, key, out);
-    return out[j & 3u] >> 1;
+    return out[step & 3u] >> 1;
 }

@@ -32,8 +32,8 @@ def test_urand_mapping(oracle):
 
 def test_chain_raw_is_philox_word(oracle):
     for step, j in [(0, 0), (5, 3), (7, 6), (123, 1023)]:
-        w = oracle.philox([step, j >> 2, 0, 0], [9, 42])
-        assert oracle.lib().orc_chain_raw(42, 9, step, j) == w[j & 3] >> 1
+        w = oracle.philox([step >> 2, j, 0, 0], [9, 42])
+        assert oracle.lib().orc_chain_raw(42, 9, step, j) == w[step & 3] >> 1
 
 
 def test_stat_tables_uniform_pdf(oracle):
