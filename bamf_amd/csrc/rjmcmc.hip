@@ -64,6 +64,7 @@ struct WaveLds {
     int hist[kNMax + 1];
     int cnt[32];
     float dpx[64 * PPL];  // the ROI's data in the pixel-slot layout: slot s at [64 s + lane]
+    int64_t rec[3];       // chain-record counts: records, accepted-state entries, proposed-state entries
 };
 
 __device__ __forceinline__ void wsync() {
@@ -232,6 +233,7 @@ struct Chain {
     float r_llr, r_pr;
     double r_logpr;
     int ref_flags, ref_pix;
+    bool applied;                     // the decision the step applied (the recorded one in replay mode)
     double lp_prod;  // product of the posterior-ratio factors other than the LLR, in double
     bool lp_neg;     // merge divides by the product
     uint64_t aflops;  // algorithmic work of this chain
@@ -631,10 +633,61 @@ struct Chain {
         }
         return acc;
     }
+    // Chain and Chain_Test entries (RJMCMCChain::addEmitters, LibRJMCMC.cpp:290-322):
+    // the current state and the proposed one (the current buffer when the
+    // jump was accepted), with the jump's LLR, PR, type and decision.
+    __device__ __forceinline__ void chain_entry(bamf_chain_rec* recs, float* ems, int which, const Em& e, int n,
+                                                float bg, float abg, float bbg) {
+        const int64_t r0 = (int64_t)roi * a->rec_cap, e0 = (int64_t)roi * a->em_cap;
+        const int64_t nrec = L.rec[0];
+        int64_t nem = L.rec[which];
+        if (lane == 0 && nrec < a->rec_cap) {
+            bamf_chain_rec& h = recs[r0 + nrec];
+            h.n = n;
+            h.jump_type = jt;
+            h.accepted = applied ? 1 : 0;
+            h.em_index = (int32_t)nem;
+            h.llr = r_llr;
+            h.pr = r_pr;
+            h.bg = bg;
+            h.abg = abg;
+            h.bbg = bbg;
+        }
+        if (n < 1) {
+            if (lane == 0 && nem < a->em_cap) {
+                float* q = ems + 4 * (e0 + nem);
+                q[0] = -100.0f;
+                q[1] = -100.0f;
+                q[2] = 0.0f;
+                q[3] = -1.0f;
+            }
+            nem += 1;
+        } else {
+            for (int k = lane; k < n; k += 64)
+                if (nem + k < a->em_cap) {
+                    float* q = ems + 4 * (e0 + nem + k);
+                    q[0] = e.X[k] + a->p.drift_x;
+                    q[1] = e.Y[k] + a->p.drift_y;
+                    q[2] = e.I[k];
+                    q[3] = (float)e.S[k];
+                }
+            nem += n;
+        }
+        wsync();
+        if (lane == 0) L.rec[which] = nem;
+        wsync();
+    }
     __device__ __forceinline__ void record() {
         r_jt = jt;
         r_flags |= 2;
         r_nprop = tN;
+        if (a->acc_rec) {
+            const bool acc = applied;
+            chain_entry(a->acc_rec, a->acc_em, 1, cur(), cN, cBG, cABG, cBBG);
+            chain_entry(a->prop_rec, a->prop_em, 2, acc ? cur() : tst(), acc ? cN : tN, tBG, tABG, tBBG);
+            if (lane == 0) L.rec[0] += 1;
+            wsync();
+        }
     }
     // update_grids (LibRJMCMC.cpp:1629-1648)
     __device__ __forceinline__ void update_grids() {
@@ -1139,6 +1192,7 @@ struct Chain {
             case 7: do_backfore(); break;
             default: status = ST_UNSUPPORTED; break;
         }
+        applied = false;
         if (evk != EV_NONE && status == ST_OK) {
             if (need_test && !test_t()) {
                 r_flags |= 8;
@@ -1155,6 +1209,7 @@ struct Chain {
                     default: PR = fin_death(sum, LLR); break;
                 }
                 const bool acc = decide(PR, sum);
+                applied = acc;
                 // kept for step records and the accepted-jump total
                 double logpr = __builtin_nan("");
                 if (acc || a->steps) logpr = logpr_of(sum);
@@ -1258,6 +1313,7 @@ struct Chain {
         }
         tcur = 0;
         pblk = -1;
+        if (lane < 3) L.rec[lane] = 0;
         const int nb = p.n_burnin, total = p.n_burnin + p.n_trials;
         const int64_t rstride = total;
         for (int s = 0; s < total && status == ST_OK; s++) {
@@ -1326,6 +1382,11 @@ struct Chain {
         if (a->counters && lane < 30) a->counters[(int64_t)roi * 30 + lane] = L.cnt[lane];
         if (a->logpost && lane == 0) a->logpost[roi] = logpost;
         if (a->alg_flops && lane == 0) a->alg_flops[roi] = aflops;
+        if (a->acc_rec && lane == 0) {
+            a->rec_count[3 * (int64_t)roi + 0] = L.rec[0];
+            a->rec_count[3 * (int64_t)roi + 1] = L.rec[1];
+            a->rec_count[3 * (int64_t)roi + 2] = L.rec[2];
+        }
         if (a->hist) {
             const int hl = a->hist_len;
             for (int k = lane; k < hl; k += 64) {

@@ -87,6 +87,12 @@ struct ChainArgs {
     float* pgrid;
     float* pback;
     const int64_t* grid_off;
+    bamf_chain_rec* acc_rec;
+    bamf_chain_rec* prop_rec;
+    float* acc_em;
+    float* prop_em;
+    int64_t* rec_count;
+    int32_t rec_cap, em_cap;
     uint64_t* alg_flops;
     int32_t recompute_every;  // full model rebuild after this many accepted jumps
 };

@@ -315,6 +315,17 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     a.pback = out->d_pback;
     a.grid_off = out->d_grid_offset;
     a.alg_flops = out->d_alg_flops;
+    BAMF_CHECK_ARG((!out->d_acc_rec && !out->d_prop_rec && !out->d_acc_em && !out->d_prop_em) ||
+                       (out->d_acc_rec && out->d_prop_rec && out->d_acc_em && out->d_prop_em && out->d_rec_count &&
+                        out->rec_cap >= 0 && out->em_cap >= 0),
+                   "chain records need all four arrays, the counts and non-negative capacities");
+    a.acc_rec = out->d_acc_rec;
+    a.prop_rec = out->d_prop_rec;
+    a.acc_em = out->d_acc_em;
+    a.prop_em = out->d_prop_em;
+    a.rec_count = out->d_rec_count;
+    a.rec_cap = out->rec_cap;
+    a.em_cap = out->em_cap;
     if (mode == BAMF_RJ_LEARNED) {
         a.prop_map = plan->d_q;
         a.prop_mu = plan->d_mu;

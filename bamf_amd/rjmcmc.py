@@ -13,7 +13,7 @@ import numpy as np
 
 from . import _hip
 from ._lib import BamfError, check, lib
-from .rjmcmc_abi import LEARNED, PLAIN, REPLAY, STEP_DTYPE, McmcParams, Replay, RjOut, RoiBatch
+from .rjmcmc_abi import CHAIN_REC_DTYPE, LEARNED, PLAIN, REPLAY, STEP_DTYPE, McmcParams, Replay, RjOut, RoiBatch
 
 COUNTER_NAMES = [f"{k}_{j}" for k in ("N", "A", "NFg", "AFg", "NBg", "ABg")
                  for j in ("jump", "split", "merge", "birth", "death")]
@@ -144,13 +144,17 @@ def lpt_order(xsize: np.ndarray, ysize: np.ndarray, expected_n: np.ndarray | Non
 
 def run_batch(plan: Plan, rois: RoiSet, seed: int = 0, mode: int = PLAIN, replay=None, *, hist_len: int = 64,
               final_cap: int = 128, trace_len: int = 0, trace_thin: int = 1, record_steps: bool = False,
-              grids: bool = False, stream=None, sync: bool = True, outs=None) -> BatchResult | dict:
-    """Runs every chain of the batch; returns host copies of the outputs."""
+              grids: bool = False, chain_caps: tuple | None = None, stream=None, sync: bool = True,
+              outs=None) -> BatchResult | dict:
+    """Runs every chain of the batch; returns host copies of the outputs.
+    chain_caps = (records, emitter entries) per ROI turns the Chain /
+    ProposedChain records on."""
     n = rois.n_roi
     p = plan.params
     total = p.n_burnin + p.n_trials
     if outs is None:
-        outs = alloc_outputs(n, total, hist_len, final_cap, trace_len, record_steps, grids, rois, p.grid_zoom)
+        outs = alloc_outputs(n, total, hist_len, final_cap, trace_len, record_steps, grids, rois, p.grid_zoom,
+                             chain_caps)
     o = outs["struct"]
     o.trace_thin = trace_thin
     rp = None
@@ -165,7 +169,7 @@ def run_batch(plan: Plan, rois: RoiSet, seed: int = 0, mode: int = PLAIN, replay
     return fetch(outs, n, total, hist_len, final_cap, trace_len)
 
 
-def alloc_outputs(n, total, hist_len, final_cap, trace_len, record_steps, grids, rois, zoom):
+def alloc_outputs(n, total, hist_len, final_cap, trace_len, record_steps, grids, rois, zoom, chain_caps=None):
     B = _hip.DeviceBuffer
     d = {"status": B(4 * n), "counters": B(4 * 30 * n), "final": B(4 * n * (4 + 4 * final_cap)),
          "hist": B(4 * n * hist_len), "logpost": B(8 * n), "flops": B(8 * n)}
@@ -190,6 +194,17 @@ def alloc_outputs(n, total, hist_len, final_cap, trace_len, record_steps, grids,
         o.d_pgrid, o.d_pback, o.d_grid_offset = d["pgrid"].ptr, d["pback"].ptr, d["goff"].ptr
         d["grid_sizes"] = sizes
         d["grid_off_host"] = goff
+    if chain_caps is not None:
+        rc, ec = (int(v) for v in chain_caps)
+        for k in ("acc_rec", "prop_rec"):
+            d[k] = B(CHAIN_REC_DTYPE.itemsize * n * max(rc, 1))
+        for k in ("acc_em", "prop_em"):
+            d[k] = B(16 * n * max(ec, 1))
+        d["rec_count"] = B(8 * 3 * n)
+        o.d_acc_rec, o.d_prop_rec = d["acc_rec"].ptr, d["prop_rec"].ptr
+        o.d_acc_em, o.d_prop_em, o.d_rec_count = d["acc_em"].ptr, d["prop_em"].ptr, d["rec_count"].ptr
+        o.rec_cap, o.em_cap = rc, ec
+        d["chain_caps"] = (rc, ec)
     d["struct"] = o
     return d
 
@@ -209,6 +224,13 @@ def fetch(d, n, total, hist_len, final_cap, trace_len) -> BatchResult:
     if "pgrid" in d:
         res.pgrid = d["pgrid"].download(np.float32)
         res.pback = d["pback"].download(np.float32)
+    if "chain_caps" in d:
+        rc, ec = d["chain_caps"]
+        res.chain = {"count": d["rec_count"].download(np.int64).reshape(n, 3),
+                     "acc_rec": d["acc_rec"].download(CHAIN_REC_DTYPE)[:n * rc].reshape(n, rc),
+                     "prop_rec": d["prop_rec"].download(CHAIN_REC_DTYPE)[:n * rc].reshape(n, rc),
+                     "acc_em": d["acc_em"].download(np.float32)[:n * ec * 4].reshape(n, ec, 4),
+                     "prop_em": d["prop_em"].download(np.float32)[:n * ec * 4].reshape(n, ec, 4)}
     return res
 
 

@@ -40,8 +40,16 @@ class RjOut(Structure):
                 ("d_n_hist", c_void_p), ("hist_len", c_int32), ("d_logpost", c_void_p),
                 ("d_trace_n", c_void_p), ("d_trace_photons", c_void_p), ("trace_len", c_int32),
                 ("trace_thin", c_int32), ("d_steps", c_void_p), ("d_pgrid", c_void_p), ("d_pback", c_void_p),
-                ("d_grid_offset", c_void_p), ("d_alg_flops", c_void_p)]
+                ("d_grid_offset", c_void_p),
+                ("d_acc_rec", c_void_p), ("d_prop_rec", c_void_p), ("d_acc_em", c_void_p), ("d_prop_em", c_void_p),
+                ("d_rec_count", c_void_p), ("rec_cap", c_int32), ("em_cap", c_int32),
+                ("d_alg_flops", c_void_p)]
 
+
+# bamf_chain_rec
+CHAIN_REC_DTYPE = np.dtype([("n", "<i4"), ("jump_type", "<i4"), ("accepted", "<i4"), ("em_index", "<i4"),
+                            ("llr", "<f4"), ("pr", "<f4"), ("bg", "<f4"), ("abg", "<f4"), ("bbg", "<f4")])
+assert CHAIN_REC_DTYPE.itemsize == 36
 
 PLAIN, REPLAY, LEARNED = 0, 1, 2
 

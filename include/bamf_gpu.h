@@ -173,6 +173,18 @@ typedef struct bamf_replay {
     const bamf_step_record* d_ref_steps; /* [n_roi][n_burnin + n_trials] */
 } bamf_replay;
 
+/* One entry of the Chain / ProposedChain records (RJMCMCChain::addEmitters,
+ * LibRJMCMC.cpp:290-322): the state after a jump that reached its accept
+ * test or failed its bounds test, and the proposed state of that jump. */
+typedef struct bamf_chain_rec {
+    int32_t n;          /* emitters in the state */
+    int32_t jump_type;  /* JumpType */
+    int32_t accepted;
+    int32_t em_index;   /* index of its first emitter entry among this ROI's entries */
+    float llr, pr;      /* the jump's LLR and PR (both records carry the same values) */
+    float bg, abg, bbg;
+} bamf_chain_rec;
+
 /* Outputs (device pointers; NULL skips an output). */
 typedef struct bamf_rjmcmc_out {
     int32_t* d_status;       /* [n_roi]: 0 ok, 1 emitter cap reached, 2 unsupported jump, 3 replay tape exhausted,
@@ -192,6 +204,18 @@ typedef struct bamf_rjmcmc_out {
     float* d_pgrid;          /* zoomed posterior images (update_grids, LibRJMCMC.cpp:1629-1648) */
     float* d_pback;
     const int64_t* d_grid_offset; /* [n_roi] offset of ROI r's (ysize*zoom) x (xsize*zoom) grid */
+    /* Chain records (all four arrays or none): ROI r's records at
+     * [r * rec_cap ..), its emitter entries (x + DriftX, y + DriftY, I, Signal; one
+     * (-100, -100, 0, -1) entry for an empty state) at [r * em_cap ..) in units
+     * of 4 floats.  d_rec_count[3 r + 0..2] = records, accepted-state entries,
+     * proposed-state entries (totals; entries beyond the capacities are not written). */
+    bamf_chain_rec* d_acc_rec;
+    bamf_chain_rec* d_prop_rec;
+    float* d_acc_em;
+    float* d_prop_em;
+    int64_t* d_rec_count;
+    int32_t rec_cap;
+    int32_t em_cap;
     uint64_t* d_alg_flops;   /* [n_roi]: algorithmic floating-point operations of the chain: per jump
                                 6P (likelihood ratio) + 2P + 8(X+Y) per changed emitter + 5P for a
                                 background move + 4P for a birth pixel draw; 4P + N(2P + 8(X+Y)) per

@@ -294,3 +294,48 @@ def test_learned_intensity_kernel_keeps_prior():
     print("KS uniform p", ks.pvalue, "KS log-uniform p", ks_log.pvalue, flush=True)
     assert ks.pvalue > 0.01
     assert ks_log.pvalue < 1e-4
+
+
+def test_chain_records_match_oracle(oracle):
+    """Chain / ProposedChain records (RJMCMCChain::addEmitters) of replayed
+    chains: heads, background terms and emitter entries equal the oracle's;
+    LLR and PR agree to single-precision rounding."""
+    rois = synth.make_rois(10, seed=41, sides=(8, 16, 24), mean_n=3.0)
+    p = params(nb=600, nt=600, drift_x=0.25, drift_y=-0.5)
+    op = oracle.McmcParams.from_buffer_copy(p)
+    n = len(rois["side"])
+    ores, tapes = [], []
+    for r in range(n):
+        o = rois["offset"]
+        s = int(rois["side"][r])
+        res = oracle.rjmcmc(rois["data"][o[r]:o[r + 1]], s, s, op, OFF_PDF, SIG_PDF, BACK_PDF, bg0=10.0, seed=7,
+                            roi=r, record=True, chain=True, rec_cap=1300, em_cap=20000)
+        assert res["status"] == 0
+        ores.append(res)
+        tapes.append(res["tape"])
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    rs = RoiSet.from_synth(rois, bg0=np.full(n, 10.0, np.float32))
+    rp = make_replay(tapes, np.stack([o["steps"] for o in ores]))
+    g = run_batch(plan, rs, mode=REPLAY, replay=rp, record_steps=True, chain_caps=(1300, 20000))
+    assert np.all(g.status == 0)
+    ch = g.chain
+    for r in range(n):
+        oc = ores[r]["chain"]
+        nrec = len(oc["acc_head"])
+        assert ch["count"][r, 0] == nrec
+        assert ch["count"][r, 1] == len(oc["acc_em"]) and ch["count"][r, 2] == len(oc["prop_em"])
+        for side, head, val, em in (("acc", "acc_head", "acc_val", "acc_em"), ("prop", "prop_head", "prop_val", "prop_em")):
+            gr = ch[f"{side}_rec"][r, :nrec]
+            oh, ov = oc[head], oc[val]
+            assert np.array_equal(gr["n"], oh[:, 0]), (r, side)
+            assert np.array_equal(gr["jump_type"], oh[:, 1]), (r, side)
+            assert np.array_equal(gr["accepted"], oh[:, 2]), (r, side)
+            assert np.array_equal(gr["em_index"], oh[:, 3]), (r, side)
+            for k, f in ((2, "bg"), (3, "abg"), (4, "bbg")):
+                assert np.array_equal(gr[f].view(np.uint32), ov[:, k].view(np.uint32)), (r, side, f)
+            for k, f in ((0, "llr"), (1, "pr")):
+                a, b = gr[f].astype(np.float64), ov[:, k].astype(np.float64)
+                fin = np.isfinite(b) & (np.abs(b) < 1e30) & (np.abs(b) > 1e-30)
+                assert np.all(np.abs(a[fin] - b[fin]) <= 1e-3 * np.abs(b[fin])), (r, side, f)
+            ne = len(oc[em])
+            assert np.array_equal(ch[f"{side}_em"][r, :ne].view(np.uint32), oc[em].view(np.uint32)), (r, side)
