@@ -18,7 +18,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libbamf_gpu.so"
-HIP_SOURCES = ["runtime.cpp", "mle.hip", "findroi.hip", "rjmcmc.hip", "rjmcmc_host.cpp", "net.hip"]
+HIP_SOURCES = ["runtime.cpp", "mle.hip", "findroi.hip", "rjmcmc.hip", "rjmcmc_host.cpp", "net.hip", "gateway.cpp"]
 ARCH = os.environ.get("BAMF_OFFLOAD_ARCH", "gfx950")
 
 
@@ -70,6 +70,16 @@ def build(verbose: bool = False, jobs: int = 4) -> Path:
         sys.stderr.write(r.stdout.decode(errors="replace"))
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    # the standalone C++ bench harness, linked against the library ($ORIGIN rpath)
+    exe = PKG / "bamf_bench"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-Wno-unused-command-line-argument",
+           f"-I{ROOT / 'include'}", str(CSRC / "bench_main.cpp"), "-o", str(exe) + ".tmp", f"-L{PKG}", "-lbamf_gpu",
+           "-Wl,-rpath,$ORIGIN"]
+    _, r = run(cmd)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout.decode(errors="replace"))
+        raise RuntimeError("bench harness build failed")
+    os.replace(str(exe) + ".tmp", exe)
     return LIB
 
 

@@ -62,4 +62,8 @@ def declare(L: ctypes.CDLL) -> None:
                                   c_void_p]
     L.bamf_rjmcmc_last_claimed.argtypes = [c_void_p, POINTER(c_int64)]
     L.bamf_math_erf.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
+    L.bamf_crjmcmc.argtypes = [POINTER(McmcParams), c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                               c_float, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_uint64,
+                               c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
+                               c_void_p, c_void_p]
     L.bamf_rjmcmc_set_proposal.argtypes = [c_void_p, c_void_p, c_void_p, c_float, c_float, c_float]

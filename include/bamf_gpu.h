@@ -300,6 +300,29 @@ int bamf_net_train_step(bamf_net* net, const float* d_images, const float* d_occ
 int bamf_net_proposal_maps(bamf_net* net, const bamf_roi_batch* batch, float eps, float* d_q, float* d_mu,
                            void* stream);
 
+/* cRJMCMC on host arrays: the argument and output layout of the mex entry
+ * (source/c/cRJMCMC.cpp:52-305) for one ROI, run on the device.
+ *   mcmc                the MCMC struct (make_Params fields);
+ *   data                single Ysize x Xsize, column-major (data[y + x*Ysize]);
+ *   n0, i0, x0, y0, bg0 E_active (N, I, X, Y, BG; BG = 0 becomes 1, :132-133);
+ *   off/sig/back_pdf    OffSetPDF, SignalPDF, BackPDF with their lengths;
+ *   scmos               sCMOScorr (same layout as data) or NULL for zeros;
+ *   opt_model           must be 0 (the analytical PSF);
+ *   seed                the Philox stream replacing the process-global rand();
+ *   pgrid, pback        (Ysize*Zoom) x (Xsize*Zoom) floats, column-major, or NULL;
+ *   acc_rec/prop_rec,   Chain and ProposedChain records and their emitter
+ *   acc_em/prop_em      entries (capacities rec_cap, em_cap; counts[0..2] =
+ *                       records, accepted entries, proposed entries), or NULL;
+ *                       the mex layer drops the last record as :174,229 do;
+ *   stat                the 16 Stat fields (NTrials, Accept_* ratios, :283-305).
+ * Synchronous; returns BAMF_E_CAPACITY when the chain reached the emitter cap. */
+int bamf_crjmcmc(const bamf_mcmc_params* mcmc, const float* data, int xsize, int ysize, int n0, const float* i0,
+                 const float* x0, const float* y0, float bg0, const float* off_pdf, int off_len,
+                 const float* sig_pdf, int sig_len, const float* back_pdf, int back_len, const float* scmos,
+                 int opt_model, uint64_t seed, float* pgrid, float* pback, bamf_chain_rec* acc_rec,
+                 bamf_chain_rec* prop_rec, float* acc_em, float* prop_em, int32_t rec_cap, int32_t em_cap,
+                 int64_t counts[3], double stat[16]);
+
 /* The branch-free erf the chain's PSF stencils use, evaluated elementwise
  * (a self-check of the device math; no counterpart in the reference). */
 int bamf_math_erf(const float* d_x, float* d_y, int64_t n, void* stream);
