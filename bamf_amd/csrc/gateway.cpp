@@ -1,8 +1,12 @@
-// Host-array gateway with cRJMCMC's argument and output layout
-// (source/c/cRJMCMC.cpp:52-305) over the device chain: one ROI per call,
-// host memory in and out, the device work done by bamf_rjmcmc_run.
-// A MATLAB mex function (csrc/mex/cRJMCMC_gpu.cpp) only maps mxArrays onto
-// this entry point.
+// Host-array gateways with the argument and output layouts of the three
+// MATLAB-level entry points, each over the device entry points:
+//   bamf_crjmcmc        cRJMCMC (source/c/cRJMCMC.cpp:52-305), one ROI per call;
+//   bamf_gauss_mle_host gaussMLE(Data, 'Basic', 'CCD', Sigma) (@RJ/gaussMLE.m:64-211);
+//   bamf_find_roi_host  findROI(Image, Sigma1, Sigma2, Boxsize, Minval) (@RJ/findROI.m:1-163).
+// The MATLAB mex functions under csrc/mex only map mxArrays onto these.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -19,6 +23,7 @@ struct DevBufs {
     T* alloc(size_t count) {
         void* p = nullptr;
         if (err == hipSuccess) err = hipMalloc(&p, count * sizeof(T) > 0 ? count * sizeof(T) : 1);
+        if (err != hipSuccess) p = nullptr;
         if (err == hipSuccess) ptrs.push_back(p);
         return static_cast<T*>(p);
     }
@@ -149,6 +154,131 @@ int bamf_crjmcmc(const bamf_mcmc_params* mcmc, const float* data, int xsize, int
                 stat[1 + 5 * g + k] = (double)acc / (double)n;
             }
     }
+    return BAMF_OK;
+}
+
+int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, float* y, float* x,
+                        float* photons, float* bg, float* y_se, float* x_se, float* photons_se, float* bg_se,
+                        float* ll, double* fit_seconds) {
+    BAMF_CHECK_ARG(data && y && x && photons && bg && ll, "null argument");
+    BAMF_CHECK_ARG(sz >= 1 && nfits >= 0 && iterations >= 0, "bad size");
+    if (nfits == 0) {
+        if (fit_seconds) *fit_seconds = 0;
+        return BAMF_OK;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    // Calls of at most 2^24 fits each: gaussMLE.m:123-131 splits by free
+    // device memory the same way; fits are independent, so the split changes
+    // nothing in the results.
+    const int64_t chunk = std::min<int64_t>(nfits, int64_t(1) << 24);
+    DevBufs b;
+    float* d_data = b.alloc<float>((size_t)chunk * sz * sz);
+    float* d_p = b.alloc<float>(4 * (size_t)chunk);
+    float* d_c = b.alloc<float>(4 * (size_t)chunk);
+    float* d_l = b.alloc<float>((size_t)chunk);
+    std::vector<float> hp(4 * (size_t)chunk), hc(4 * (size_t)chunk);
+    float* out[4] = {y, x, photons, bg};
+    float* se[4] = {y_se, x_se, photons_se, bg_se};
+    for (int64_t f0 = 0; f0 < nfits && b.err == hipSuccess; f0 += chunk) {
+        const int n = (int)std::min<int64_t>(chunk, nfits - f0);
+        b.err = hipMemcpy(d_data, data + (size_t)f0 * sz * sz, (size_t)n * sz * sz * 4, hipMemcpyHostToDevice);
+        if (b.err != hipSuccess) break;
+        const int rc = bamf_gauss_mle_xynb(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr);
+        if (rc != BAMF_OK) return rc;
+        b.err = hipMemcpy(hp.data(), d_p, 4 * (size_t)n * 4, hipMemcpyDeviceToHost);
+        if (b.err == hipSuccess) b.err = hipMemcpy(hc.data(), d_c, 4 * (size_t)n * 4, hipMemcpyDeviceToHost);
+        if (b.err == hipSuccess) b.err = hipMemcpy(ll + f0, d_l, (size_t)n * 4, hipMemcpyDeviceToHost);
+        if (b.err != hipSuccess) break;
+        // gaussMLE.m:183-190: parameter k = 0..3 is Y, X, Photons, BG; SE = sqrt(CRLB)
+        for (int k = 0; k < 4; k++) {
+            std::memcpy(out[k] + f0, hp.data() + (size_t)k * n, (size_t)n * 4);
+            if (se[k])
+                for (int i = 0; i < n; i++) se[k][f0 + i] = std::sqrt(hc[(size_t)k * n + i]);
+        }
+    }
+    if (b.err != hipSuccess) {
+        set_error("gaussMLE gateway: %s", hipGetErrorString(b.err));
+        return BAMF_E_HIP;
+    }
+    if (fit_seconds) fit_seconds[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return BAMF_OK;
+}
+
+int bamf_find_roi_host(const float* image, int ysize, int xsize, int nframes, float sigma1, float sigma2,
+                       int boxsize, float minval, int64_t capacity, int32_t* x_corner, int32_t* y_corner,
+                       int32_t* frame_num, float* roi_stack, uint8_t* res, int64_t* count) {
+    BAMF_CHECK_ARG(image && count, "null argument");
+    BAMF_CHECK_ARG(xsize >= 1 && ysize >= 1 && nframes >= 0 && boxsize >= 1 && capacity >= 0, "bad size");
+    *count = 0;
+    if (nframes == 0) return BAMF_OK;
+    const int64_t frame = (int64_t)xsize * ysize;
+    // frames per device pass: findROI.m:98-104 sizes its chunks as 16 bytes
+    // per pixel of device memory; here the chunk is bounded by the free
+    // memory and by 2^31 pixels.  The filters act frame by frame, so the
+    // chunking does not change any result; detections keep find() order
+    // because chunks are consecutive frame ranges.
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        set_error("findROI gateway: hipMemGetInfo failed");
+        return BAMF_E_HIP;
+    }
+    const size_t per_frame = (size_t)frame * 4 * 4 + (size_t)frame / 8 + 64;
+    int64_t fpc = (int64_t)((free_b / 2) / per_frame);
+    fpc = std::min<int64_t>(fpc, ((int64_t)1 << 31) / frame);
+    fpc = std::max<int64_t>(1, std::min<int64_t>(fpc, nframes));
+    const size_t ws_bytes = bamf_find_roi_workspace_bytes(xsize, ysize, (int)fpc);
+    const int64_t npix_max = frame * fpc;
+    DevBufs b;
+    float* d_stack = b.alloc<float>((size_t)npix_max);
+    void* d_ws = b.alloc<uint8_t>(ws_bytes);
+    uint32_t* d_bits = res ? b.alloc<uint32_t>((size_t)(npix_max + 31) / 32) : nullptr;
+    const int64_t cap = std::max<int64_t>(capacity, 1);
+    int32_t* d_row = b.alloc<int32_t>((size_t)cap);
+    int32_t* d_col = b.alloc<int32_t>((size_t)cap);
+    int32_t* d_frm = b.alloc<int32_t>((size_t)cap);
+    int64_t* d_count = b.alloc<int64_t>(1);
+    float* d_crop = roi_stack ? b.alloc<float>((size_t)cap * boxsize * boxsize) : nullptr;
+    std::vector<uint32_t> bits;
+    int64_t total = 0;
+    for (int f0 = 0; f0 < nframes && b.err == hipSuccess; f0 += (int)fpc) {
+        const int nf = (int)std::min<int64_t>(fpc, nframes - f0);
+        const int64_t npix = frame * nf;
+        b.err = hipMemcpy(d_stack, image + (size_t)f0 * frame, (size_t)npix * 4, hipMemcpyHostToDevice);
+        if (b.err != hipSuccess) break;
+        const int64_t room = std::max<int64_t>(capacity - total, 0);
+        int rc = bamf_find_roi(d_stack, xsize, ysize, nf, sigma1, sigma2, boxsize, minval, d_ws, ws_bytes, d_bits,
+                               nullptr, d_row, d_col, d_frm, room, d_count, nullptr);
+        if (rc != BAMF_OK) return rc;
+        int64_t n = 0;
+        b.err = hipMemcpy(&n, d_count, 8, hipMemcpyDeviceToHost);
+        if (b.err != hipSuccess) break;
+        const int64_t m = std::min(n, room);
+        if (m > 0) {
+            if (y_corner) b.err = hipMemcpy(y_corner + total, d_row, m * 4, hipMemcpyDeviceToHost);
+            if (b.err == hipSuccess && x_corner) b.err = hipMemcpy(x_corner + total, d_col, m * 4, hipMemcpyDeviceToHost);
+            if (b.err == hipSuccess && frame_num) {
+                b.err = hipMemcpy(frame_num + total, d_frm, m * 4, hipMemcpyDeviceToHost);
+                for (int64_t i = 0; i < m; i++) frame_num[total + i] += f0;
+            }
+            if (b.err == hipSuccess && roi_stack) {
+                rc = bamf_find_roi_crop(d_stack, xsize, ysize, d_row, d_col, d_frm, m, boxsize, d_crop, nullptr);
+                if (rc != BAMF_OK) return rc;
+                b.err = hipMemcpy(roi_stack + (size_t)total * boxsize * boxsize, d_crop,
+                                  (size_t)m * boxsize * boxsize * 4, hipMemcpyDeviceToHost);
+            }
+        }
+        if (b.err == hipSuccess && res) {
+            bits.resize((size_t)(npix + 31) / 32);
+            b.err = hipMemcpy(bits.data(), d_bits, bits.size() * 4, hipMemcpyDeviceToHost);
+            for (int64_t i = 0; i < npix; i++) res[(size_t)f0 * frame + i] = (bits[i >> 5] >> (i & 31)) & 1u;
+        }
+        total += n;
+    }
+    if (b.err != hipSuccess) {
+        set_error("findROI gateway: %s", hipGetErrorString(b.err));
+        return BAMF_E_HIP;
+    }
+    *count = total;
     return BAMF_OK;
 }
 

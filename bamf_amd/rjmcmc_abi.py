@@ -66,4 +66,7 @@ def declare(L: ctypes.CDLL) -> None:
                                c_float, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_uint64,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
                                c_void_p, c_void_p]
+    L.bamf_gauss_mle_host.argtypes = [c_void_p, c_int, c_int64, c_float, c_int] + [c_void_p] * 10
+    L.bamf_find_roi_host.argtypes = [c_void_p, c_int, c_int, c_int, c_float, c_float, c_int, c_float, c_int64,
+                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.bamf_rjmcmc_set_proposal.argtypes = [c_void_p, c_void_p, c_void_p, c_float, c_float, c_float]

@@ -323,6 +323,36 @@ int bamf_crjmcmc(const bamf_mcmc_params* mcmc, const float* data, int xsize, int
                  bamf_chain_rec* prop_rec, float* acc_em, float* prop_em, int32_t rec_cap, int32_t em_cap,
                  int64_t counts[3], double stat[16]);
 
+/* gaussMLE(Data, 'Basic', 'CCD', Sigma) on host arrays (@RJ/gaussMLE.m:64-211,
+ * the fit type and camera the hot path uses):
+ *   data        single (sz, sz, nfits), column-major: the layout gaussMLE.m
+ *               hands to kernel_MLEFit_XYNB_ (gaussMLE.m:154);
+ *   y .. bg     Results.Y, X, Photons, BG (gaussMLE.m:183-186), nfits each;
+ *   *_se        Results.*_SE = sqrt(CRLB) (:187-190), or NULL;
+ *   ll          Results.LogLikelihood;
+ *   fit_seconds Statistics.FitTime (host wall clock), or NULL.
+ * Synchronous. */
+int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, float* y, float* x,
+                        float* photons, float* bg, float* y_se, float* x_se, float* photons_se, float* bg_se,
+                        float* ll, double* fit_seconds);
+
+/* findROI(Image, Sigma1, Sigma2, Boxsize, Minval) on host arrays
+ * (@RJ/findROI.m:1-163):
+ *   image       single (ysize, xsize, nframes), column-major (MATLAB memory);
+ *   x_corner,   SMD.XBoxCorner, SMD.YBoxCorner, SMD.FrameNum (1-based), in
+ *   y_corner,   find() order; capacity entries each (any may be NULL);
+ *   frame_num
+ *   roi_stack   ROIStack, (boxsize, boxsize, capacity) column-major, or NULL;
+ *   res         Res as 0/1 bytes, ysize*xsize*nframes, or NULL;
+ *   count       receives the number of detections; when it exceeds capacity
+ *               only the first capacity are written, and the caller calls
+ *               again with a larger capacity.
+ * The stack goes to the device in frame chunks sized to free device memory
+ * (findROI.m:98-123 chunks the same way).  Synchronous. */
+int bamf_find_roi_host(const float* image, int ysize, int xsize, int nframes, float sigma1, float sigma2,
+                       int boxsize, float minval, int64_t capacity, int32_t* x_corner, int32_t* y_corner,
+                       int32_t* frame_num, float* roi_stack, uint8_t* res, int64_t* count);
+
 /* The branch-free erf the chain's PSF stencils use, evaluated elementwise
  * (a self-check of the device math; no counterpart in the reference). */
 int bamf_math_erf(const float* d_x, float* d_y, int64_t n, void* stream);

@@ -101,3 +101,49 @@ def test_bench_harness_runs():
     assert r.returncode == 0, r.stderr.decode()
     line = json.loads(r.stdout.decode().strip().splitlines()[-1])
     assert line["status_nonzero"] == 0 and line["n_roi"] == 300 and line["ms_per_step"] > 0
+
+
+def test_gauss_mle_gateway_matches_device_entry():
+    from bamf_amd.gauss_mle import fit_boxes
+
+    boxes = np.ascontiguousarray(synth.make_mle_boxes(3000, 7, seed=17)[0], np.float32)
+    n = boxes.shape[0]
+    p, c, ll = fit_boxes(boxes, 1.3, 20)
+    outs = [np.zeros(n, np.float32) for _ in range(9)]
+    secs = np.zeros(1, np.float64)
+    # (n, jj, ii) C order is the (sz, sz, N) column-major MATLAB stack
+    rc = lib().bamf_gauss_mle_host(ptr(boxes), 7, n, ctypes.c_float(1.3), 20, *[ptr(o) for o in outs], ptr(secs))
+    assert rc == 0, lib().bamf_last_error()
+    for k in range(4):
+        assert np.array_equal(outs[k].view(np.uint32), p[k].view(np.uint32))
+        assert np.array_equal(outs[4 + k], np.sqrt(c[k]))
+    assert np.array_equal(outs[8].view(np.uint32), ll.view(np.uint32))
+    assert secs[0] > 0
+
+
+def test_find_roi_gateway_matches_device_entry():
+    from bamf_amd.find_roi import default_minval, find_roi
+
+    frames = synth.make_frames(6, 64, seed=23, density_um2=1.0)  # (f, y, x), x contiguous
+    img = np.ascontiguousarray(np.transpose(frames, (1, 2, 0)))  # any layout: read as MATLAB (Y, X, Z)
+    smd, stack, res = find_roi(img)
+    flat = np.asarray(img, order="F").ravel(order="F").astype(np.float32)
+    ys, xs, nf = img.shape
+    for cap in (4096, 3):
+        xc, yc, fr = (np.zeros(cap, np.int32) for _ in range(3))
+        st = np.zeros(cap * 49, np.float32)
+        rs = np.zeros(flat.size, np.uint8)
+        cnt = np.zeros(1, np.int64)
+        rc = lib().bamf_find_roi_host(ptr(flat), ys, xs, nf, ctypes.c_float(1.3), ctypes.c_float(2.6), 7,
+                                      ctypes.c_float(default_minval()), cap, ptr(xc), ptr(yc), ptr(fr), ptr(st),
+                                      ptr(rs), ptr(cnt))
+        assert rc == 0, lib().bamf_last_error()
+        n = int(cnt[0])
+        assert n == len(smd["XBoxCorner"]) and n > 3
+        m = min(n, cap)
+        assert np.array_equal(xc[:m], smd["XBoxCorner"][:m].astype(np.int32))
+        assert np.array_equal(yc[:m], smd["YBoxCorner"][:m].astype(np.int32))
+        assert np.array_equal(fr[:m], smd["FrameNum"][:m].astype(np.int32))
+        want = np.asarray(stack, order="F").ravel(order="F")[:m * 49]
+        assert np.array_equal(st[:m * 49], want)
+        assert np.array_equal(rs, np.asarray(res, order="F").ravel(order="F").astype(np.uint8))
