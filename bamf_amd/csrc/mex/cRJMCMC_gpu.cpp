@@ -23,8 +23,11 @@ float fieldf(const mxArray* s, const char* f) {
 
 int longest_side(const mxArray* a) { return (int)(mxGetM(a) > mxGetN(a) ? mxGetM(a) : mxGetN(a)); }
 
-// Chain struct array of NLoop - 1 entries (cRJMCMC.cpp:159-229)
-mxArray* chain_struct(const std::vector<bamf_chain_rec>& rec, const std::vector<float>& em, int64_t nrec) {
+// Chain struct array of NLoop entries, the first NLoop - 1 filled
+// (cRJMCMC.cpp:159-229).  LLR, PR, JumpType and Accepted come from `meta`:
+// the accepted chain's records for both outputs, as cRJMCMC.cpp:265-272 does.
+mxArray* chain_struct(const std::vector<bamf_chain_rec>& rec, const std::vector<bamf_chain_rec>& meta,
+                      const std::vector<float>& em, int64_t nrec) {
     static const char* names[12] = {"N", "X", "Y", "Photons", "Signal", "LLR", "PR", "JumpType", "Accepted",
                                     "BG", "ABG", "BBG"};
     const mwSize n = (mwSize)(nrec > 0 ? nrec : 0);
@@ -41,10 +44,11 @@ mxArray* chain_struct(const std::vector<bamf_chain_rec>& rec, const std::vector<
             for (int i = 0; i < nn; i++) d[i] = (double)e[4 * i + src[f]];
             mxSetFieldByNumber(s, k, 1 + f, v);
         }
-        mxSetFieldByNumber(s, k, 5, mxCreateDoubleScalar((double)h.llr));
-        mxSetFieldByNumber(s, k, 6, mxCreateDoubleScalar((double)h.pr));
-        mxSetFieldByNumber(s, k, 7, mxCreateDoubleScalar((double)h.jump_type));
-        mxSetFieldByNumber(s, k, 8, mxCreateDoubleScalar((double)h.accepted));
+        const bamf_chain_rec& m = meta[k];
+        mxSetFieldByNumber(s, k, 5, mxCreateDoubleScalar((double)m.llr));
+        mxSetFieldByNumber(s, k, 6, mxCreateDoubleScalar((double)m.pr));
+        mxSetFieldByNumber(s, k, 7, mxCreateDoubleScalar((double)m.jump_type));
+        mxSetFieldByNumber(s, k, 8, mxCreateDoubleScalar((double)m.accepted));
         mxSetFieldByNumber(s, k, 9, mxCreateDoubleScalar((double)h.bg));
         mxSetFieldByNumber(s, k, 10, mxCreateDoubleScalar((double)h.abg));
         mxSetFieldByNumber(s, k, 11, mxCreateDoubleScalar((double)h.bbg));
@@ -107,8 +111,8 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
         prop_em.data(), rec_cap, em_cap, counts, stat);
     if (rc != BAMF_OK) mexErrMsgIdAndTxt("bamf:run", "%s", bamf_last_error());
     if (counts[1] > em_cap || counts[2] > em_cap) mexErrMsgIdAndTxt("bamf:cap", "chain record capacity exceeded");
-    plhs[2] = chain_struct(acc, acc_em, counts[0]);
-    plhs[3] = chain_struct(prop, prop_em, counts[0]);
+    plhs[2] = chain_struct(acc, acc, acc_em, counts[0]);
+    plhs[3] = chain_struct(prop, acc, prop_em, counts[0]);
     static const char* sn[16] = {"NTrials", "Accept_Jump", "Accept_Split", "Accept_Merge", "Accept_Birth",
                                  "Accept_Death", "Accept_JumpFg", "Accept_SplitFg", "Accept_MergeFg",
                                  "Accept_BirthFg", "Accept_DeathFg", "Accept_JumpBg", "Accept_SplitBg",
