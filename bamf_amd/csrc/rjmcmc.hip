@@ -181,9 +181,12 @@ __device__ __forceinline__ float normpdf(float x, float u, float s) {
 
 
 // Proposal kinds that need the likelihood ratio, with their accept counters.
-enum { EV_NONE = 0, EV_MOVE, EV_SPLIT, EV_MERGE, EV_BIRTH, EV_DEATH };
+enum { EV_NONE = 0, EV_MOVE, EV_SPLIT, EV_MERGE, EV_BIRTH, EV_DEATH, EV_GSPLIT, EV_GMERGE };
 
-template <int PPL>
+// GJ: the gSplit/gMerge jumps are compiled in (they are reachable only when
+// P_Burnin entries 5 or 6 are nonzero; without them the hot loop carries none
+// of their code).
+template <int PPL, bool GJ>
 struct Chain {
     const ChainArgs* a;  // the kernel argument block (in the kernarg segment)
     WaveLds<PPL>& L;
@@ -212,7 +215,7 @@ struct Chain {
     int step;
     int64_t tbase, tlen, tcur;
     // phase
-    float P1, P2, P3, P4;
+    float P1, P2, P3, P4, P56;  // P[1..4] and P[5] / P[6] of the phase
     bool burn;
     int status;
     int nacc;
@@ -986,6 +989,138 @@ struct Chain {
         return LLR / (PrI * PrX * Jacob * Us * PrM * P1 / P2);
     }
 
+    // gSplit, LibRJMCMC.cpp:1389-1436: the picked emitter and a random subset
+    // of its neighbours within 2 sigma give up the fraction u3 of their
+    // intensity to a new emitter near their centroid.  No bounds test and no
+    // acceptance counters.
+    __device__ __forceinline__ int prop_gsplit() {
+        const bamf_mcmc_params& p = a->p;
+        jt = 8;
+        copy_to_test();
+        if (cN < 2) return EV_NONE;
+        const Em& E = cur();
+        const float Ua = urand();
+        const int jj = min((int)floorf((float)cN * Ua), cN - 1);
+        int np = closest_multi(jj, 2 * p.psf_sigma);
+        if (lane == 0) L.nb[np] = jj;
+        np++;
+        wsync();
+        if (np == 1) return EV_NONE;
+        float I0 = 0, X0 = 0, Y0 = 0;
+        for (int n = 0; n < np; n++) {
+            const int k = L.nb[n];
+            I0 = I0 + E.I[k];
+            X0 = X0 + E.X[k];
+            Y0 = Y0 + E.Y[k];
+        }
+        need(25);
+        const float u1 = 0.3f * randn();
+        const float u2 = 0.3f * randn();
+        const float u3 = (1 / (float)np) * urand();
+        const float fnp = (float)np;
+        // the intensity prior ratio needs no likelihood: formed here; the
+        // proposed set's Signal is read at n, not at Closest[n] (LibRJMCMC.cpp:1447)
+        float PrI = 1;
+        for (int n = 0; n < np; n++) {
+            const int k = L.nb[n];
+            const float ti = (1 - u3) * E.I[k];
+            const float tmpCurrent = pdf_of(E, k);
+            const float tmpProposed = findpdf(E.S[n] == 1 ? a->sig : a->back, p.dx, ti);
+            PrI = PrI * tmpProposed / tmpCurrent;
+            set_t(k, ti, (E.X[k] - u3 * (X0 / fnp + u1)) / (1 - u3), (E.Y[k] - u3 * (Y0 / fnp + u2)) / (1 - u3));
+            delta_push(0, k, true);
+            delta_push(1, k, false);
+        }
+        wsync();
+        const float In = u3 * I0;
+        if (!add_t(In, X0 / fnp + u1, Y0 / fnp + u2, 1)) return EV_NONE;
+        delta_push(1, tN - 1, false);
+        f_pri = PrI * findpdf(a->sig, p.dx, In);
+        f_u1 = u1;
+        f_u2 = u2;
+        f_u3 = u3;
+        f_ijp = I0;
+        f_jj = np;
+        acc_c1 = acc_c2 = -1;
+        need_test = false;
+        return EV_GSPLIT;
+    }
+    // LibRJMCMC.cpp:1454-1472
+    __device__ __forceinline__ float fin_gsplit(float sum, float LLR) {
+        const float PDFu1 = normpdf(f_u1, 0.0f, 0.4f);
+        const float PDFu2 = normpdf(f_u2, 0.0f, 0.4f);
+        const float PDFu3 = (float)f_jj;
+        const float Us = 1 / (PDFu1 * PDFu2 * PDFu3);
+        const float NEm = (float)nem_cur();
+        const float PrM = meanN / (NEm + 1);
+        const float PrX = 1 / (xb * yb);
+        const float Jacob = f_ijp / powf(1 - f_u3, (float)(f_jj + 1));
+        lp_prod = (double)f_pri * (double)PrX * (double)Us * (double)Jacob * (double)PrM * (double)P56;
+        lp_neg = false;
+        return LLR * f_pri * PrX * Us * Jacob * PrM * P56;
+    }
+
+    // gMerge, LibRJMCMC.cpp:1487-1551: the picked emitter is absorbed by a
+    // random subset of its neighbours within 3 sigma.  No bounds test and no
+    // acceptance counters.
+    __device__ __forceinline__ int prop_gmerge() {
+        const bamf_mcmc_params& p = a->p;
+        jt = 9;
+        copy_to_test();
+        if (cN < 3) return EV_NONE;
+        const Em& E = cur();
+        const float Ua = urand();
+        const int jj = min((int)floorf((float)cN * Ua), cN - 1);
+        const int np = closest_multi(jj, 3 * p.psf_sigma);
+        if (np == 0) return EV_NONE;
+        float I0 = E.I[jj];
+        for (int n = 0; n < np; n++) I0 = I0 + E.I[L.nb[n]];
+        const float U3 = E.I[jj] / I0;
+        const float xj = E.X[jj], yj = E.Y[jj];
+        float XX0 = 0, YY0 = 0, PrI = 1;
+        delta_push(0, jj, true);
+        for (int n = 0; n < np; n++) {
+            const int k = L.nb[n];
+            const float ti = E.I[k] / (1 - U3);
+            const float tx = U3 * xj + (1 - U3) * E.X[k];
+            const float ty = U3 * yj + (1 - U3) * E.Y[k];
+            XX0 = XX0 + U3 * xj + (1 - U3) * E.X[k];
+            YY0 = YY0 + U3 * yj + (1 - U3) * E.Y[k];
+            const bool fg = E.S[k] == 1;
+            const float tmpProposed = findpdf(fg ? a->sig : a->back, p.dx, ti);
+            const float tmpCurrent = pdf_of(E, k);
+            PrI = PrI * (tmpCurrent / tmpProposed);
+            set_t(k, ti, tx, ty);
+            delta_push(0, k, true);
+            delta_push(1, k > jj ? k - 1 : k, false);  // its index once jj is removed
+        }
+        f_pri = PrI * pdf_of(E, jj);
+        f_u1 = xj - XX0 / (float)np;
+        f_u2 = yj - YY0 / (float)np;
+        f_u3 = U3;
+        f_ijp = I0;
+        f_jj = np;
+        remove_t(jj);
+        acc_c1 = acc_c2 = -1;
+        need_test = false;
+        return EV_GMERGE;
+    }
+    // LibRJMCMC.cpp:1553-1571
+    __device__ __forceinline__ float fin_gmerge(float sum, float LLR) {
+        const float PDFu1 = normpdf(f_u1, 0.0f, 0.4f);
+        const float PDFu2 = normpdf(f_u2, 0.0f, 0.4f);
+        const float PDFu3 = (float)f_jj;
+        const float Us = 1 / (PDFu1 * PDFu2 * PDFu3);
+        const float NEm = (float)nem_cur();
+        const float PrM = NEm / meanN;
+        const float PrX = 1 / (xb * yb);
+        const float Jacob = f_ijp / powf(1 - f_u3, (float)(f_jj + 1));
+        lp_prod = (double)f_pri * (double)PrX * (double)Us * (double)Jacob * (double)PrM * (double)P56;
+        lp_neg = true;
+        if (PDFu1 == 0 || PDFu2 == 0) return -1.0f;
+        return LLR / (f_pri * PrX * Us * Jacob * PrM * P56);
+    }
+
     // find_Birth (LibRJMCMC.cpp:1688-1736): residual max(0, D - M), prefix
     // sum in pixel order, r = total * U, first pixel with r <= cumulative.
     __device__ __forceinline__ int birth_pixel(float u) {
@@ -1190,7 +1325,10 @@ struct Chain {
             case 3: evk = prop_birth(); break;
             case 4: evk = prop_death(); break;
             case 7: do_backfore(); break;
-            default: status = ST_UNSUPPORTED; break;
+            default:  // 5 gSplit, 6 gMerge
+                if constexpr (GJ) evk = kind == 5 ? prop_gsplit() : prop_gmerge();
+                else status = ST_UNSUPPORTED;
+                break;
         }
         applied = false;
         if (evk != EV_NONE && status == ST_OK) {
@@ -1206,7 +1344,15 @@ struct Chain {
                     case EV_SPLIT: PR = fin_split(sum, LLR); break;
                     case EV_MERGE: PR = fin_merge(sum, LLR); break;
                     case EV_BIRTH: PR = fin_birth(sum, LLR); break;
-                    default: PR = fin_death(sum, LLR); break;
+                    default:
+                        if constexpr (GJ) {
+                            PR = evk == EV_GSPLIT   ? fin_gsplit(sum, LLR)
+                                 : evk == EV_GMERGE ? fin_gmerge(sum, LLR)
+                                                    : fin_death(sum, LLR);
+                        } else {
+                            PR = fin_death(sum, LLR);
+                        }
+                        break;
                 }
                 const bool acc = decide(PR, sum);
                 applied = acc;
@@ -1215,8 +1361,10 @@ struct Chain {
                 if (acc || a->steps) logpr = logpr_of(sum);
                 r_logpr = logpr;
                 if (acc) {
-                    cnt_inc(acc_c1);
-                    cnt_inc(acc_c2);
+                    if (!GJ || acc_c1 >= 0) {  // gSplit/gMerge keep no counters
+                        cnt_inc(acc_c1);
+                        cnt_inc(acc_c2);
+                    }
                     ci ^= 1;
                     cN = tN;
                     cBG = tBG;
@@ -1225,7 +1373,11 @@ struct Chain {
 #pragma unroll
                     for (int s = 0; s < PPL; s++)
                         if (s >= lo_s && s <= hi_s) m[s] = mt[s];
-                    if (need_test) e_ok = true;  // the proposed set passed the test
+                    // the proposed set passed the test, or (no test: death, and
+                    // gSplit/gMerge) passes it when the current set does and no
+                    // value written into it fails it; death writes none
+                    if constexpr (GJ) e_ok = need_test || (e_ok && !t_bad);
+                    else if (need_test) e_ok = true;
                     logpost += logpr;
                     if (++nacc >= a->recompute_every) {
                         wsync();
@@ -1321,8 +1473,10 @@ struct Chain {
             burn = s < nb;
             if (burn) {
                 P1 = p.p_burnin[1]; P2 = p.p_burnin[2]; P3 = p.p_burnin[3]; P4 = p.p_burnin[4];
+                P56 = p.p_burnin[5] / p.p_burnin[6];
             } else {
                 P1 = a->pvt[1]; P2 = a->pvt[2]; P3 = a->pvt[3]; P4 = a->pvt[4];
+                P56 = a->pvt[5] / a->pvt[6];
             }
             dj = 0;
             fill_window(0);
@@ -1423,13 +1577,13 @@ struct Chain {
 // the kernarg segment; the chain reads it through that segment pointer.
 // One instantiation per ROI size class (PPL pixel slots per lane), each
 // draining its own class list through an atomic queue.
-template <int PPL, int MINB>
+template <int PPL, int MINB, bool GJ>
 __global__ __launch_bounds__(64 * kWavesPerBlock, MINB) void k_chain(ChainArgs) {
     __shared__ WaveLds<PPL> lds[kWavesPerBlock];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int lane = threadIdx.x & 63;
     const ChainArgs* ap = (const ChainArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
-    Chain<PPL> ch(ap, lds[w], lane);
+    Chain<PPL, GJ> ch(ap, lds[w], lane);
     const int n = *ap->list_count;
     while (true) {
         int qi = 0;
@@ -1495,9 +1649,12 @@ int launch_classify(int n_roi, const int32_t* order, const int32_t* xs, const in
     return BAMF_OK;
 }
 
-int launch_chain(const ChainArgs& a, int cls, int grid, hipStream_t s) {
-    if (cls == 0) hipLaunchKernelGGL((k_chain<kClassPPL[0], kClassMinBlocks[0]>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, a);
-    else hipLaunchKernelGGL((k_chain<kClassPPL[1], kClassMinBlocks[1]>), dim3(grid), dim3(64 * kWavesPerBlock), 0, s, a);
+int launch_chain(const ChainArgs& a, int cls, bool group_jumps, int grid, hipStream_t s) {
+    const dim3 g(grid), b(64 * kWavesPerBlock);
+    if (cls == 0 && !group_jumps) hipLaunchKernelGGL((k_chain<kClassPPL[0], kClassMinBlocks[0], false>), g, b, 0, s, a);
+    else if (cls == 0) hipLaunchKernelGGL((k_chain<kClassPPL[0], kClassMinBlocks[0], true>), g, b, 0, s, a);
+    else if (!group_jumps) hipLaunchKernelGGL((k_chain<kClassPPL[1], kClassMinBlocks[1], false>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_chain<kClassPPL[1], kClassMinBlocks[1], true>), g, b, 0, s, a);
     BAMF_LAUNCH_CHECK("k_chain");
     return BAMF_OK;
 }
@@ -1508,14 +1665,21 @@ int launch_erf(const float* x, float* y, int64_t n, hipStream_t s) {
     return BAMF_OK;
 }
 
-int chain_occupancy_blocks(int cls, int* blocks_per_cu) {
+int chain_occupancy_blocks(int cls, bool group_jumps, int* blocks_per_cu) {
     int n = 0;
-    if (cls == 0)
-        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chain<kClassPPL[0], kClassMinBlocks[0]>,
-                                                                   64 * kWavesPerBlock, 0));
+    const int b = 64 * kWavesPerBlock;
+    if (cls == 0 && !group_jumps)
+        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, k_chain<kClassPPL[0], kClassMinBlocks[0], false>, b, 0));
+    else if (cls == 0)
+        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, k_chain<kClassPPL[0], kClassMinBlocks[0], true>, b, 0));
+    else if (!group_jumps)
+        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, k_chain<kClassPPL[1], kClassMinBlocks[1], false>, b, 0));
     else
-        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_chain<kClassPPL[1], kClassMinBlocks[1]>,
-                                                                   64 * kWavesPerBlock, 0));
+        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, k_chain<kClassPPL[1], kClassMinBlocks[1], true>, b, 0));
     *blocks_per_cu = n;
     return BAMF_OK;
 }

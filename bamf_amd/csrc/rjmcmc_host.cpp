@@ -10,8 +10,8 @@ namespace bamf {
 namespace rj {
 int launch_classify(int n_roi, const int32_t* order, const int32_t* xs, const int32_t* ys, int32_t* list0,
                     int32_t* list1, int32_t* counts, uint32_t* queues, hipStream_t s);
-int launch_chain(const ChainArgs& a, int cls, int grid, hipStream_t s);
-int chain_occupancy_blocks(int cls, int* blocks_per_cu);
+int launch_chain(const ChainArgs& a, int cls, bool group_jumps, int grid, hipStream_t s);
+int chain_occupancy_blocks(int cls, bool group_jumps, int* blocks_per_cu);
 int launch_erf(const float* x, float* y, int64_t n, hipStream_t s);
 }  // namespace rj
 }  // namespace bamf
@@ -106,7 +106,7 @@ struct bamf_rjmcmc_plan {
     float l_alpha = 1.0f, l_halfwidth = 0.5f, l_beta = 0.0f;
     int device = 0;
     int cu = 0;
-    int blocks_per_cu[kClasses] = {};
+    int blocks_per_cu[kClasses][2] = {};  // [class][gSplit/gMerge compiled in]
 };
 
 namespace {
@@ -193,10 +193,11 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, pl->device) == hipSuccess) pl->cu = prop.multiProcessorCount;
-    for (int c = 0; c < kClasses; c++) {
-        int bpc = 0;
-        if (chain_occupancy_blocks(c, &bpc) == BAMF_OK) pl->blocks_per_cu[c] = bpc;
-    }
+    for (int c = 0; c < kClasses; c++)
+        for (int g = 0; g < 2; g++) {
+            int bpc = 0;
+            if (chain_occupancy_blocks(c, g == 1, &bpc) == BAMF_OK) pl->blocks_per_cu[c][g] = bpc;
+        }
     *plan = pl;
     return BAMF_OK;
 }
@@ -264,9 +265,6 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
                    "trace needs photons, trace_len >= 0 and trace_thin >= 1");
     BAMF_CHECK_ARG(!out->d_pgrid || (out->d_pback && out->d_grid_offset), "grids need pback and offsets");
     const bamf_mcmc_params& p = plan->params;
-    for (int k = 5; k < 7; k++)
-        BAMF_CHECK_ARG(p.p_burnin[k] == 0.0f && p.p_trials[k] == 0.0f,
-                       "gSplit/gMerge (probability entries 5 and 6) are not implemented");
     ChainArgs a;
     std::memset(&a, 0, sizeof(a));
     a.p = p;
@@ -350,6 +348,10 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     if (rc != BAMF_OK) return rc;
     const int cu = plan->cu > 0 ? plan->cu : 256;
     const int need = (int)ceil_div(batch->n_roi, kWavesPerBlock);
+    // gSplit/gMerge are dispatched on P_Burnin's cumulative vector in both
+    // phases (LibRJMCMC.cpp:705-710, 737-742): reachable only when its
+    // entries 5 or 6 are nonzero
+    const bool gj = plan->params.p_burnin[5] != 0.0f || plan->params.p_burnin[6] != 0.0f;
     // the large-ROI class goes on the caller's stream first; the small class
     // runs beside it on the side stream and the caller's stream joins it
     BAMF_HIP_TRY(hipEventRecord(plan->ev_fork, s));
@@ -359,9 +361,10 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
         ac.list = plan->d_lists + (int64_t)c * plan->list_cap;
         ac.list_count = counts + c;
         ac.queue = plan->d_queue + c;
-        int grid = (plan->blocks_per_cu[c] > 0 ? plan->blocks_per_cu[c] : 1) * cu;
+        const int bpc = plan->blocks_per_cu[c][gj ? 1 : 0];
+        int grid = (bpc > 0 ? bpc : 1) * cu;
         if (grid > need) grid = need;
-        rc = launch_chain(ac, c, grid, c == kClasses - 1 ? s : plan->side);
+        rc = launch_chain(ac, c, gj, grid, c == kClasses - 1 ? s : plan->side);
         if (rc != BAMF_OK) return rc;
     }
     BAMF_HIP_TRY(hipEventRecord(plan->ev_join, plan->side));

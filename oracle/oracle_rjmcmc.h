@@ -71,4 +71,8 @@ int   orc_stat_tables(const float* pdf, int len, float dx, float* xarr, float* c
 float orc_stat_genrand(const float* uniq_x, const float* uniq_cdf, int nuniq, float u);
 void  orc_model(int n, const float* I, const float* X, const float* Y, float bg, float abg, float bbg,
                 int xsize, int ysize, float sigma, const float* scmos, float* m);
+void  orc_gsplit_map(int np, const float* I, const float* X, const float* Y, const int* idx, float u1, float u2,
+                     float u3, float* I1, float* X1, float* Y1, float* In, float* Xn, float* Yn, float* I0);
+void  orc_gmerge_map(int np, const float* I, const float* X, const float* Y, int jj, const int* idx, float* I1,
+                     float* X1, float* Y1, float* U1, float* U2, float* U3, float* I0);
 #endif

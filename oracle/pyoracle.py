@@ -100,6 +100,10 @@ def _declare(L):
     L.orc_stat_genrand.restype = c_float
     L.orc_model.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float, c_int, c_int, c_float,
                             c_void_p, c_void_p]
+    L.orc_gsplit_map.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.orc_gmerge_map.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p]
 
 
 # ---------------------------------------------------------------------------
@@ -158,6 +162,27 @@ def model(I, X, Y, bg, abg, bbg, xsize, ysize, sigma, scmos=None):
     lib().orc_model(len(I), I.ctypes.data, X.ctypes.data, Y.ctypes.data, bg, abg, bbg, xsize, ysize, sigma,
                     np.ascontiguousarray(scmos, np.float32).ctypes.data, m.ctypes.data)
     return m
+
+
+def gsplit_map(I, X, Y, idx, u1, u2, u3):
+    """gSplit's proposal map: (members' new I, X, Y in idx order, new emitter (I, X, Y), group I0)."""
+    I, X, Y = (np.ascontiguousarray(v, np.float32) for v in (I, X, Y))
+    idx = np.ascontiguousarray(idx, np.int32)
+    n = len(idx)
+    o = [np.zeros(n, np.float32) for _ in range(3)] + [np.zeros(1, np.float32) for _ in range(4)]
+    lib().orc_gsplit_map(n, _f32p(I), _f32p(X), _f32p(Y), idx.ctypes.data, float(u1), float(u2), float(u3),
+                         *[v.ctypes.data for v in o])
+    return o[0], o[1], o[2], (float(o[3][0]), float(o[4][0]), float(o[5][0])), float(o[6][0])
+
+
+def gmerge_map(I, X, Y, jj, idx):
+    """gMerge's proposal map: (members' new I, X, Y in idx order, (U1, U2, U3), group I0)."""
+    I, X, Y = (np.ascontiguousarray(v, np.float32) for v in (I, X, Y))
+    idx = np.ascontiguousarray(idx, np.int32)
+    n = len(idx)
+    o = [np.zeros(n, np.float32) for _ in range(3)] + [np.zeros(1, np.float32) for _ in range(4)]
+    lib().orc_gmerge_map(n, _f32p(I), _f32p(X), _f32p(Y), int(jj), idx.ctypes.data, *[v.ctypes.data for v in o])
+    return o[0], o[1], o[2], (float(o[3][0]), float(o[4][0]), float(o[5][0])), float(o[6][0])
 
 
 def stat_tables(pdf: np.ndarray, dx: float):

@@ -15,8 +15,9 @@
  *    float, while distance() rounds the squared sum to float before sqrtf;
  *  - pi is the double literal 3.141592 (LibRJMCMCreturn.hpp:2);
  *  - no FMA contraction.
- * gSplit/gMerge (LibRJMCMC.cpp:1389-1584) are not restated: a chain whose
- * dispatch reaches them stops with ORC_E_UNSUPPORTED.
+ * gSplit/gMerge (LibRJMCMC.cpp:1389-1584): their proposal maps are pinned
+ * (numerical Jacobian, split-then-merge round trip); the assembly of their
+ * posterior ratios is parity unpinned, like split/merge's.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -690,14 +691,160 @@ static void do_backfore(ch_t* c) {
     }
 }
 
+/* gSplit's proposal map (LibRJMCMC.cpp:1424-1431): the np group members
+ * (idx[], the picked emitter last) move away from the new emitter, which
+ * takes the fraction u3 of the group's intensity.  Writes the members' new
+ * values to I1/X1/Y1[k] (k = 0..np-1, in idx order) and the new emitter to
+ * *In, *Xn, *Yn.  I0/X0/Y0 are the group sums in idx order. */
+void orc_gsplit_map(int np, const float* I, const float* X, const float* Y, const int* idx, float u1, float u2,
+                    float u3, float* I1, float* X1, float* Y1, float* In, float* Xn, float* Yn, float* I0_out) {
+    float I0 = 0, X0 = 0, Y0 = 0;
+    for (int nn = 0; nn < np; nn++) {
+        I0 = I0 + I[idx[nn]];
+        X0 = X0 + X[idx[nn]];
+        Y0 = Y0 + Y[idx[nn]];
+    }
+    for (int nn = 0; nn < np; nn++) {
+        int k = idx[nn];
+        X1[nn] = (X[k] - u3 * (X0 / (float)np + u1)) / (1 - u3);
+        Y1[nn] = (Y[k] - u3 * (Y0 / (float)np + u2)) / (1 - u3);
+        I1[nn] = (1 - u3) * I[k];
+    }
+    *In = u3 * I0;
+    *Xn = X0 / (float)np + u1;
+    *Yn = Y0 / (float)np + u2;
+    *I0_out = I0;
+}
+
+/* gMerge's proposal map (LibRJMCMC.cpp:1509-1528): emitter jj is absorbed by
+ * the np members idx[]; writes their new values to I1/X1/Y1[k] and returns
+ * U1, U2, U3 and the group intensity I0 (jj first, then idx order). */
+void orc_gmerge_map(int np, const float* I, const float* X, const float* Y, int jj, const int* idx, float* I1,
+                    float* X1, float* Y1, float* U1, float* U2, float* U3, float* I0_out) {
+    float I0 = I[jj];
+    for (int nn = 0; nn < np; nn++) I0 = I0 + I[idx[nn]];
+    float u3 = I[jj] / I0, XX0 = 0, YY0 = 0;
+    for (int nn = 0; nn < np; nn++) {
+        int k = idx[nn];
+        I1[nn] = I[k] / (1 - u3);
+        X1[nn] = u3 * X[jj] + (1 - u3) * X[k];
+        XX0 = XX0 + u3 * X[jj] + (1 - u3) * X[k];
+        Y1[nn] = u3 * Y[jj] + (1 - u3) * Y[k];
+        YY0 = YY0 + u3 * Y[jj] + (1 - u3) * Y[k];
+    }
+    *U1 = X[jj] - XX0 / (float)np;
+    *U2 = Y[jj] - YY0 / (float)np;
+    *U3 = u3;
+    *I0_out = I0;
+}
+
+/* LibRJMCMC.cpp:1389-1485.  No acceptance counters, no bounds test. */
+static void do_gsplit(ch_t* c) {
+    int JumpType = 8, Accept = 0;
+    float LLR = 0, PR = 0, PrI = 1, PrX = 1, Us = 1, Jacob = 1, PrM = 0, sum = 0;
+    em_t* E = &c->cur;
+    em_t* T = &c->T;
+    const orc_mcmc_params* p = &c->p;
+    if (E->n < 2) return;
+    *T = *E;
+    float Ua = urand(c);
+    int jj = imin((int)floorf((float)E->n * Ua), E->n - 1);
+    int cl[ORC_NMAX + 1];
+    int np = em_closest_multi(c, E, jj, 2 * p->psf_sigma, cl);
+    cl[np++] = jj;
+    if (np == 1) return;
+    float u1 = (float)0.3 * randn(c);
+    float u2 = (float)0.3 * randn(c);
+    float u3 = (float)(1 / (float)np) * urand(c);
+    float I1[ORC_NMAX + 1], X1[ORC_NMAX + 1], Y1[ORC_NMAX + 1], In, Xn, Yn, I0;
+    orc_gsplit_map(np, E->I, E->X, E->Y, cl, u1, u2, u3, I1, X1, Y1, &In, &Xn, &Yn, &I0);
+    for (int nn = 0; nn < np; nn++) { T->X[cl[nn]] = X1[nn]; T->Y[cl[nn]] = Y1[nn]; T->I[cl[nn]] = I1[nn]; }
+    if (!em_add(c, T, In, Xn, Yn, 1)) return;
+    LLR = likelihood_ratio(c, T, &sum);
+    float tmpCurrent, tmpProposed;
+    for (int nn = 0; nn < np; nn++) {
+        tmpCurrent = pdf_of(c, E, cl[nn]);
+        /* the proposed set's Signal is read at nn, not at Closest[nn] (LibRJMCMC.cpp:1447) */
+        tmpProposed = T->sig[nn] == 1 ? findpdf(&c->sig, T->I[cl[nn]]) : findpdf(&c->back, T->I[cl[nn]]);
+        PrI = PrI * tmpProposed / tmpCurrent;
+    }
+    tmpProposed = pdf_of(c, T, T->n - 1);
+    PrI = PrI * (tmpProposed);
+    float PDFu1 = normpdf(u1, 0.0f, 0.4f);
+    float PDFu2 = normpdf(u2, 0.0f, 0.4f);
+    float PDFu3 = (float)np;
+    Us = 1 / (PDFu1 * PDFu2 * PDFu3);
+    float NEm = nem(E);
+    PrM = c->meanN / (NEm + 1);
+    float Area = (float)((c->X + p->bnd_out) * (c->Y + p->bnd_out));
+    PrX = 1 / Area;
+    Jacob = I0 / powf(1 - u3, (float)(np + 1));
+    float P56 = c->P[5] / c->P[6];
+    PR = LLR * PrI * PrX * Us * Jacob * PrM * P56;
+    c->st->log_pr = (double)sum + lg(PrI) + lg(PrX) + lg(Us) + lg(Jacob) + lg(PrM) + lg(P56);
+    Accept = (urand(c) < PR);
+    if (Accept) accept_state(c);
+    record(c, T, LLR, PR, JumpType, Accept);
+    update_grids(c);
+}
+
+/* LibRJMCMC.cpp:1487-1584.  No acceptance counters, no bounds test. */
+static void do_gmerge(ch_t* c) {
+    int JumpType = 9, Accept = 0;
+    float LLR = 0, PR = 0, PrI = 1, PrX = 1, Us = 1, Jacob = 1, PrM = 0, sum = 0;
+    em_t* E = &c->cur;
+    em_t* T = &c->T;
+    const orc_mcmc_params* p = &c->p;
+    if (E->n < 3) return;
+    *T = *E;
+    float Ua = urand(c);
+    int jj = imin((int)floorf((float)E->n * Ua), E->n - 1);
+    int cl[ORC_NMAX];
+    int np = em_closest_multi(c, E, jj, 3 * p->psf_sigma, cl);
+    if (np == 0) return;
+    float I1[ORC_NMAX], X1[ORC_NMAX], Y1[ORC_NMAX], U1, U2, U3, I0;
+    orc_gmerge_map(np, E->I, E->X, E->Y, jj, cl, I1, X1, Y1, &U1, &U2, &U3, &I0);
+    for (int nn = 0; nn < np; nn++) { T->I[cl[nn]] = I1[nn]; T->X[cl[nn]] = X1[nn]; T->Y[cl[nn]] = Y1[nn]; }
+    float tmpProposed, tmpCurrent;
+    for (int nn = 0; nn < np; nn++) {
+        tmpProposed = pdf_of(c, T, cl[nn]);
+        tmpCurrent = pdf_of(c, E, cl[nn]);
+        PrI = PrI * (tmpCurrent / tmpProposed);
+    }
+    tmpCurrent = pdf_of(c, E, jj);
+    PrI = PrI * tmpCurrent;
+    em_remove(T, jj);
+    LLR = likelihood_ratio(c, T, &sum);
+    float PDFu1 = normpdf(U1, 0.0f, 0.4f);
+    float PDFu2 = normpdf(U2, 0.0f, 0.4f);
+    float PDFu3 = (float)np;
+    Us = 1 / (PDFu1 * PDFu2 * PDFu3);
+    float NEm = nem(E);
+    PrM = NEm / c->meanN;
+    float Area = (float)((c->X + p->bnd_out) * (c->Y + p->bnd_out));
+    PrX = 1 / Area;
+    Jacob = I0 / powf(1 - U3, (float)(np + 1));
+    float P56 = c->P[5] / c->P[6];
+    PR = LLR / (PrI * PrX * Us * Jacob * PrM * P56);
+    c->st->log_pr = (double)sum - (lg(PrI) + lg(PrX) + lg(Us) + lg(Jacob) + lg(PrM) + lg(P56));
+    if (PDFu1 == 0 || PDFu2 == 0) {
+        PR = -1;
+        c->st->log_pr = -INFINITY;
+    }
+    Accept = (urand(c) < PR);
+    if (Accept) accept_state(c);
+    record(c, T, LLR, PR, JumpType, Accept);
+    update_grids(c);
+}
+
 static void dispatch(ch_t* c, float rtmp, const float* PV, const float* PVB) {
     if (rtmp < PV[0]) { c->st->kind = 0; do_move(c); }
     else if (rtmp >= PV[0] && rtmp < PV[1]) { c->st->kind = 1; do_split(c); }
     else if (rtmp >= PV[1] && rtmp < PV[2]) { c->st->kind = 2; do_merge(c); }
     else if (rtmp >= PV[2] && rtmp < PV[3]) { c->st->kind = 3; do_birth(c); }
     else if (rtmp >= PV[3] && rtmp < PV[4]) { c->st->kind = 4; do_death(c); }
-    else if (rtmp >= PVB[4] && rtmp < PVB[5]) { c->st->kind = 5; c->err = ORC_E_UNSUPPORTED; }
-    else if (rtmp >= PVB[5] && rtmp < PVB[6]) { c->st->kind = 6; c->err = ORC_E_UNSUPPORTED; }
+    else if (rtmp >= PVB[4] && rtmp < PVB[5]) { c->st->kind = 5; do_gsplit(c); }
+    else if (rtmp >= PVB[5] && rtmp < PVB[6]) { c->st->kind = 6; do_gmerge(c); }
     else { c->st->kind = 7; do_backfore(c); }
 }
 

@@ -134,6 +134,34 @@ def test_replay_uneven_sizes_and_background(oracle):
         off += sz
 
 
+def test_replay_group_jumps(oracle):
+    """gSplit / gMerge (P_Burnin entries 5 and 6, also reached in the trials
+    phase through P_Burnin's cumulative vector) on clustered ROIs."""
+    from test_oracle_gsplit import GROUP_P, clustered_rois
+
+    rng = np.random.default_rng(8)
+    rois, inits = clustered_rois(rng, 24)
+    n = 24
+    p = params(nb=1500, nt=1500, **GROUP_P)
+    ores, tapes = run_oracle_chains(oracle, rois, p, seed=17, n0=inits)
+    kinds = np.concatenate([o["steps"]["kind"][(o["steps"]["flags"] & 2) != 0] for o in ores])
+    assert np.sum(kinds == 5) > 200 and np.sum(kinds == 6) > 200
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    n0 = np.array([len(t[0]) for t in inits], np.int32)
+    eoff = np.zeros(n, np.int64)
+    eoff[1:] = np.cumsum(n0)[:-1]
+    em = (np.concatenate([t[0] for t in inits]), np.concatenate([t[1] for t in inits]),
+          np.concatenate([t[2] for t in inits]), eoff)
+    rs = RoiSet(rois["data"], rois["offset"], rois["xsize"], rois["ysize"], bg0=np.full(n, 10.0, np.float32),
+                n0=n0, em0=em)
+    rp = make_replay(tapes, np.stack([o["steps"] for o in ores]))
+    print("replay group jumps: 24 ROIs", flush=True)
+    g = run_batch(plan, rs, mode=REPLAY, replay=rp, record_steps=True, hist_len=129)
+    data = rois["data"]
+    offs = rois["offset"]
+    check_replay(rois, g, ores, tapes, p, lambda r: data[offs[r]:offs[r + 1]].sum())
+
+
 def test_free_running_matches_oracle_philox(oracle):
     """Same Philox stream on both sides: identical trajectories up to the first
     near-tie; most chains never meet one."""
