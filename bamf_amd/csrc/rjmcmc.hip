@@ -185,8 +185,10 @@ enum { EV_NONE = 0, EV_MOVE, EV_SPLIT, EV_MERGE, EV_BIRTH, EV_DEATH, EV_GSPLIT, 
 
 // GJ: the gSplit/gMerge jumps are compiled in (they are reachable only when
 // P_Burnin entries 5 or 6 are nonzero; without them the hot loop carries none
-// of their code).
-template <int PPL, bool GJ>
+// of their code).  SEG: the launch runs a segment of the chain (state loaded
+// from and saved to HBM, seed read from device memory when given); the
+// one-launch variant carries none of that.
+template <int PPL, bool GJ, bool SEG>
 struct Chain {
     const ChainArgs* a;  // the kernel argument block (in the kernarg segment)
     WaveLds<PPL>& L;
@@ -211,7 +213,7 @@ struct Chain {
     float wu, wr;
     uint32_t pw[4];  // Philox block of window position `lane` for steps 4 pblk .. 4 pblk + 3
     int pblk;  // uniform at wbase + lane, Irwin-Hall sum of the 12 uniforms from there
-    uint32_t key;
+    uint32_t key, sd_lo, sd_hi;
     int step;
     int64_t tbase, tlen, tcur;
     // phase
@@ -274,7 +276,8 @@ struct Chain {
                 o[2] = pw[2];
                 o[3] = pw[3];
             } else {
-                philox(blk, q, a->seed_hi, 0u, key, a->seed_lo, o);
+                if constexpr (SEG) philox(blk, q, sd_hi, 0u, key, sd_lo, o);
+                else philox(blk, q, a->seed_hi, 0u, key, a->seed_lo, o);
                 if (base == 0) {
                     pw[0] = o[0];
                     pw[1] = o[1];
@@ -1399,6 +1402,61 @@ struct Chain {
         return ph;
     }
 
+    // ---------------- segments ----------------
+    __device__ __forceinline__ void save_state(SegState& g) {
+        const Em& c = cur();
+        for (int n = lane; n < cN; n += 64) {
+            g.I[n] = c.I[n];
+            g.X[n] = c.X[n];
+            g.Y[n] = c.Y[n];
+            g.S[n] = c.S[n];
+        }
+#pragma unroll
+        for (int s = 0; s < PPL; s++)
+            if (s < ppl) g.m[64 * s + lane] = m[s];
+        if (lane < 32) g.cnt[lane] = L.cnt[lane];
+        for (int k = lane; k < kNMax + 1; k += 64) g.hist[k] = L.hist[k];
+        if (lane < 3) g.rec[lane] = L.rec[lane];
+        if (lane == 0) {
+            g.logpost = logpost;
+            g.aflops = aflops;
+            g.tcur = tcur;
+            g.status = status;
+            g.n = cN;
+            g.e_ok = e_ok ? 1 : 0;
+            g.nacc = nacc;
+            g.bg = cBG;
+            g.abg = cABG;
+            g.bbg = cBBG;
+        }
+    }
+    __device__ __forceinline__ void load_state(const SegState& g) {
+        logpost = g.logpost;
+        aflops = g.aflops;
+        tcur = g.tcur;
+        status = g.status;
+        cN = g.n;
+        e_ok = g.e_ok != 0;
+        nacc = g.nacc;
+        cBG = g.bg;
+        cABG = g.abg;
+        cBBG = g.bbg;
+        Em& c = L.e[0];
+        for (int n = lane; n < cN; n += 64) {
+            c.I[n] = g.I[n];
+            c.X[n] = g.X[n];
+            c.Y[n] = g.Y[n];
+            c.S[n] = g.S[n];
+        }
+#pragma unroll
+        for (int s = 0; s < PPL; s++)
+            if (s < ppl) m[s] = g.m[64 * s + lane];
+        if (lane < 32) L.cnt[lane] = g.cnt[lane];
+        for (int k = lane; k < kNMax + 1; k += 64) L.hist[k] = g.hist[k];
+        if (lane < 3) L.rec[lane] = g.rec[lane];
+        wsync();
+    }
+
     __device__ __forceinline__ void run(int roi_) {
         roi = roi_;
         const bamf_mcmc_params& p = a->p;
@@ -1422,9 +1480,16 @@ struct Chain {
         xb = X + p.bnd_out;
         yb = Y + p.bnd_out;
         key = a->key ? a->key[roi] : (uint32_t)roi;
-        status = ST_OK;
-        logpost = 0.0;
-        aflops = 0;
+        const int nb = p.n_burnin, total = p.n_burnin + p.n_trials;
+        int s_begin = 0, s_end = total;
+        SegState* ss = nullptr;
+        if constexpr (SEG) {
+            sd_lo = a->seed_dev ? a->seed_dev[0] : a->seed_lo;
+            sd_hi = a->seed_dev ? a->seed_dev[1] : a->seed_hi;
+            s_begin = a->seg_begin;
+            s_end = min(a->seg_end, total);
+            ss = a->seg_state ? a->seg_state + roi : nullptr;
+        }
         const int64_t off = a->pix_off[roi];
 #pragma unroll
         for (int s = 0; s < PPL; s++) {
@@ -1433,29 +1498,38 @@ struct Chain {
             m[s] = 1.0f;
             mt[s] = 1.0f;
         }
-        // counters start at N = 1, A = 0 (LibRJMCMC.cpp:672-677)
-        if (lane < 32) L.cnt[lane] = (lane < 30 && ((lane / 5) % 2 == 0)) ? 1 : 0;
-        for (int k = lane; k < kNMax + 1; k += 64) L.hist[k] = 0;
-        // initial state (cRJMCMC.cpp:120-133)
         ci = 0;
-        cN = a->n0 ? min(a->n0[roi], kNMax) : 0;
-        if (a->n0) {
-            const int64_t eo = a->em0_off[roi];
-            for (int n = lane; n < cN; n += 64) {
-                L.e[0].I[n] = a->i0[eo + n];
-                L.e[0].X[n] = a->x0[eo + n];
-                L.e[0].Y[n] = a->y0[eo + n];
-                L.e[0].S[n] = 1;
-            }
-        }
-        e_ok = false;  // initial emitters are checked in full by the first test
         t_bad = false;
-        const float bg0 = a->bg0 ? a->bg0[roi] : 0.0f;
-        cBG = bg0 == 0 ? 1.0f : bg0;
-        cABG = 0;
-        cBBG = 0;
-        wsync();
-        rebuild_model();
+        if (SEG && s_begin > 0) {
+            load_state(*ss);
+        } else {
+            status = ST_OK;
+            logpost = 0.0;
+            aflops = 0;
+            // counters start at N = 1, A = 0 (LibRJMCMC.cpp:672-677)
+            if (lane < 32) L.cnt[lane] = (lane < 30 && ((lane / 5) % 2 == 0)) ? 1 : 0;
+            for (int k = lane; k < kNMax + 1; k += 64) L.hist[k] = 0;
+            // initial state (cRJMCMC.cpp:120-133)
+            cN = a->n0 ? min(a->n0[roi], kNMax) : 0;
+            if (a->n0) {
+                const int64_t eo = a->em0_off[roi];
+                for (int n = lane; n < cN; n += 64) {
+                    L.e[0].I[n] = a->i0[eo + n];
+                    L.e[0].X[n] = a->x0[eo + n];
+                    L.e[0].Y[n] = a->y0[eo + n];
+                    L.e[0].S[n] = 1;
+                }
+            }
+            e_ok = false;  // initial emitters are checked in full by the first test
+            const float bg0 = a->bg0 ? a->bg0[roi] : 0.0f;
+            cBG = bg0 == 0 ? 1.0f : bg0;
+            cABG = 0;
+            cBBG = 0;
+            wsync();
+            rebuild_model();
+            tcur = 0;
+            if (lane < 3) L.rec[lane] = 0;
+        }
         if (a->mode == BAMF_RJ_REPLAY) {
             tbase = a->tape_off[roi];
             tlen = a->tape_len[roi];
@@ -1463,12 +1537,9 @@ struct Chain {
             tbase = 0;
             tlen = 0;
         }
-        tcur = 0;
         pblk = -1;
-        if (lane < 3) L.rec[lane] = 0;
-        const int nb = p.n_burnin, total = p.n_burnin + p.n_trials;
         const int64_t rstride = total;
-        for (int s = 0; s < total && status == ST_OK; s++) {
+        for (int s = s_begin; s < s_end && status == ST_OK; s++) {
             step = s;
             burn = s < nb;
             if (burn) {
@@ -1531,6 +1602,12 @@ struct Chain {
             tcur += dj;
         }
         wsync();
+        if constexpr (SEG) {
+            if (s_end < total) {  // a later segment continues this chain
+                save_state(*ss);
+                return;
+            }
+        }
         // ---- outputs ----
         if (a->status && lane == 0) a->status[roi] = status;
         if (a->counters && lane < 30) a->counters[(int64_t)roi * 30 + lane] = L.cnt[lane];
@@ -1577,13 +1654,13 @@ struct Chain {
 // the kernarg segment; the chain reads it through that segment pointer.
 // One instantiation per ROI size class (PPL pixel slots per lane), each
 // draining its own class list through an atomic queue.
-template <int PPL, int MINB, bool GJ>
+template <int PPL, int MINB, bool GJ, bool SEG>
 __global__ __launch_bounds__(64 * kWavesPerBlock, MINB) void k_chain(ChainArgs) {
     __shared__ WaveLds<PPL> lds[kWavesPerBlock];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int lane = threadIdx.x & 63;
     const ChainArgs* ap = (const ChainArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
-    Chain<PPL, GJ> ch(ap, lds[w], lane);
+    Chain<PPL, GJ, SEG> ch(ap, lds[w], lane);
     const int n = *ap->list_count;
     while (true) {
         int qi = 0;
@@ -1595,11 +1672,12 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, MINB) void k_chain(ChainArgs) 
 }
 
 // Stable split of the ROI order into the size classes (P <= 64 * kClassPPL[c]
-// for the first class c that fits); also resets the class queues.  One
-// work-group walks the order in tiles of 1024 with a work-group scan.
+// for the first class c that fits); also resets the nq work-queue counters
+// (two per segment).  One work-group walks the order in tiles of 1024 with a
+// work-group scan.
 __global__ __launch_bounds__(1024) void k_classify(int n_roi, const int32_t* order, const int32_t* xs,
                                                    const int32_t* ys, int32_t* list0, int32_t* list1,
-                                                   int32_t* counts, uint32_t* queues) {
+                                                   int32_t* counts, uint32_t* queues, int nq) {
     __shared__ int sc[2][1024];
     __shared__ int base[2];
     const int t = threadIdx.x;
@@ -1628,10 +1706,14 @@ __global__ __launch_bounds__(1024) void k_classify(int n_roi, const int32_t* ord
         if (t < 2) base[t] += sc[t][1023];
         __syncthreads();
     }
-    if (t < 2) {
-        counts[t] = base[t];
-        queues[t] = 0u;
-    }
+    if (t < 2) counts[t] = base[t];
+    for (int q = t; q < nq; q += 1024) queues[q] = 0u;
+}
+
+// The seed of a graph replay, written ahead of the graph on the launch stream.
+__global__ void k_set_seed(uint32_t* d, uint32_t lo, uint32_t hi) {
+    d[0] = lo;
+    d[1] = hi;
 }
 
 
@@ -1643,18 +1725,32 @@ __global__ void k_erf(const float* x, float* y, int64_t n) {
 }  // namespace
 
 int launch_classify(int n_roi, const int32_t* order, const int32_t* xs, const int32_t* ys, int32_t* list0,
-                    int32_t* list1, int32_t* counts, uint32_t* queues, hipStream_t s) {
-    hipLaunchKernelGGL(k_classify, dim3(1), dim3(1024), 0, s, n_roi, order, xs, ys, list0, list1, counts, queues);
+                    int32_t* list1, int32_t* counts, uint32_t* queues, int nq, hipStream_t s) {
+    hipLaunchKernelGGL(k_classify, dim3(1), dim3(1024), 0, s, n_roi, order, xs, ys, list0, list1, counts, queues,
+                       nq);
     BAMF_LAUNCH_CHECK("k_classify");
     return BAMF_OK;
 }
 
-int launch_chain(const ChainArgs& a, int cls, bool group_jumps, int grid, hipStream_t s) {
-    const dim3 g(grid), b(64 * kWavesPerBlock);
-    if (cls == 0 && !group_jumps) hipLaunchKernelGGL((k_chain<kClassPPL[0], kClassMinBlocks[0], false>), g, b, 0, s, a);
-    else if (cls == 0) hipLaunchKernelGGL((k_chain<kClassPPL[0], kClassMinBlocks[0], true>), g, b, 0, s, a);
-    else if (!group_jumps) hipLaunchKernelGGL((k_chain<kClassPPL[1], kClassMinBlocks[1], false>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_chain<kClassPPL[1], kClassMinBlocks[1], true>), g, b, 0, s, a);
+int launch_set_seed(uint32_t* d, uint32_t lo, uint32_t hi, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_seed, dim3(1), dim3(1), 0, s, d, lo, hi);
+    BAMF_LAUNCH_CHECK("k_set_seed");
+    return BAMF_OK;
+}
+
+using ChainKernelFn = void (*)(ChainArgs);
+
+// Every instantiation: [class][gSplit/gMerge][segmented].
+template <int C>
+ChainKernelFn chain_kernel(bool gj, bool seg) {
+    constexpr int P = kClassPPL[C], M = kClassMinBlocks[C];
+    return gj ? (seg ? k_chain<P, M, true, true> : k_chain<P, M, true, false>)
+              : (seg ? k_chain<P, M, false, true> : k_chain<P, M, false, false>);
+}
+
+int launch_chain(const ChainArgs& a, int cls, bool group_jumps, bool segmented, int grid, hipStream_t s) {
+    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(group_jumps, segmented) : chain_kernel<1>(group_jumps, segmented);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64 * kWavesPerBlock), 0, s, a);
     BAMF_LAUNCH_CHECK("k_chain");
     return BAMF_OK;
 }
@@ -1665,21 +1761,11 @@ int launch_erf(const float* x, float* y, int64_t n, hipStream_t s) {
     return BAMF_OK;
 }
 
-int chain_occupancy_blocks(int cls, bool group_jumps, int* blocks_per_cu) {
+int chain_occupancy_blocks(int cls, bool group_jumps, bool segmented, int* blocks_per_cu) {
     int n = 0;
-    const int b = 64 * kWavesPerBlock;
-    if (cls == 0 && !group_jumps)
-        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, k_chain<kClassPPL[0], kClassMinBlocks[0], false>, b, 0));
-    else if (cls == 0)
-        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, k_chain<kClassPPL[0], kClassMinBlocks[0], true>, b, 0));
-    else if (!group_jumps)
-        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, k_chain<kClassPPL[1], kClassMinBlocks[1], false>, b, 0));
-    else
-        BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &n, k_chain<kClassPPL[1], kClassMinBlocks[1], true>, b, 0));
+    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(group_jumps, segmented) : chain_kernel<1>(group_jumps, segmented);
+    BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(k),
+                                                               64 * kWavesPerBlock, 0));
     *blocks_per_cu = n;
     return BAMF_OK;
 }

@@ -33,6 +33,25 @@ struct StatDev {
 };
 constexpr int kGuide = 256;
 
+// Segmented runs: a launch runs steps [seg_begin, seg_end) of every chain;
+// between launches each chain's state waits in HBM.  Everything a later
+// step reads is kept, so a segmented run equals the one-launch run bit for bit.
+constexpr int kMaxSegments = 64;
+struct SegState {
+    double logpost;
+    uint64_t aflops;
+    int64_t tcur;
+    int64_t rec[3];
+    int32_t status, n, e_ok, nacc;
+    float bg, abg, bbg, pad;
+    int32_t cnt[32];
+    int32_t hist[kNMax + 1];
+    int32_t pad2[3];
+    float I[kNMax], X[kNMax], Y[kNMax];
+    int32_t S[kNMax];
+    float m[64 * kMaxPPL];  // the model, in the lane-slot layout
+};
+
 struct ChainArgs {
     bamf_mcmc_params p;
     float pvb[8];  // cumulative P_Burnin (make_Params)
@@ -95,6 +114,10 @@ struct ChainArgs {
     int32_t rec_cap, em_cap;
     uint64_t* alg_flops;
     int32_t recompute_every;  // full model rebuild after this many accepted jumps
+    // segments (seg_begin = 0 and seg_end >= total: the whole chain in one launch)
+    int32_t seg_begin, seg_end;
+    SegState* seg_state;       // one per ROI; read when seg_begin > 0, written when seg_end < total
+    const uint32_t* seed_dev;  // when set, {seed_lo, seed_hi} are read from here (graph replays)
 };
 
 }  // namespace rj

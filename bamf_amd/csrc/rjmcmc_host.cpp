@@ -1,4 +1,5 @@
 // Host side of the chain ABI: prior tables, argument checks, launch sizing.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -9,9 +10,10 @@
 namespace bamf {
 namespace rj {
 int launch_classify(int n_roi, const int32_t* order, const int32_t* xs, const int32_t* ys, int32_t* list0,
-                    int32_t* list1, int32_t* counts, uint32_t* queues, hipStream_t s);
-int launch_chain(const ChainArgs& a, int cls, bool group_jumps, int grid, hipStream_t s);
-int chain_occupancy_blocks(int cls, bool group_jumps, int* blocks_per_cu);
+                    int32_t* list1, int32_t* counts, uint32_t* queues, int nq, hipStream_t s);
+int launch_set_seed(uint32_t* d, uint32_t lo, uint32_t hi, hipStream_t s);
+int launch_chain(const ChainArgs& a, int cls, bool group_jumps, bool segmented, int grid, hipStream_t s);
+int chain_occupancy_blocks(int cls, bool group_jumps, bool segmented, int* blocks_per_cu);
 int launch_erf(const float* x, float* y, int64_t n, hipStream_t s);
 }  // namespace rj
 }  // namespace bamf
@@ -94,7 +96,9 @@ struct bamf_rjmcmc_plan {
     float pvb[8], pvt[8];
     HostStat off, sig, back;
     float* d_tables = nullptr;
-    uint32_t* d_queue = nullptr;  // [0..1] class queues, [2..3] class counts (int32)
+    // work-queue counters: class c of segment k at [2k + c]; then the two
+    // class counts (int32) at kCountsAt and the graph seed at kSeedAt
+    uint32_t* d_queue = nullptr;
     int32_t* d_lists = nullptr;   // class lists, list_cap entries each
     int64_t list_cap = 0;
     hipStream_t side = nullptr;   // second stream: the classes run concurrently
@@ -106,8 +110,21 @@ struct bamf_rjmcmc_plan {
     float l_alpha = 1.0f, l_halfwidth = 0.5f, l_beta = 0.0f;
     int device = 0;
     int cu = 0;
-    int blocks_per_cu[kClasses][2] = {};  // [class][gSplit/gMerge compiled in]
+    int blocks_per_cu[kClasses][2][2] = {};  // [class][gSplit/gMerge compiled in][segmented]
+    // segmented runs and their graph
+    int n_segments = 1;
+    bool use_graph = false;
+    SegState* d_seg = nullptr;
+    int64_t seg_cap = 0;
+    int last_nseg = 1;
+    hipStream_t cap = nullptr;  // capture stream
+    hipGraphExec_t gexec = nullptr;
+    ChainArgs graph_key;
+    int graph_nseg = 0;
 };
+
+constexpr int kCountsAt = 2 * kMaxSegments;
+constexpr int kSeedAt = kCountsAt + 4;
 
 namespace {
 void plan_release(bamf_rjmcmc_plan* pl) {
@@ -117,8 +134,54 @@ void plan_release(bamf_rjmcmc_plan* pl) {
     if (pl->ev_fork) (void)hipEventDestroy(pl->ev_fork);
     if (pl->ev_join) (void)hipEventDestroy(pl->ev_join);
     if (pl->side) (void)hipStreamDestroy(pl->side);
+    if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
+    if (pl->cap) (void)hipStreamDestroy(pl->cap);
+    if (pl->d_seg) (void)hipFree(pl->d_seg);
     delete pl;
 }
+void drop_graph(bamf_rjmcmc_plan* pl) {
+    if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
+    pl->gexec = nullptr;
+    pl->graph_nseg = 0;
+}
+
+// The launch sequence of one run: the classifier (which also resets the work
+// queues), then for each size class its segments in order on its own stream,
+// the small-ROI class on the side stream beside the caller's.
+int enqueue_run(bamf_rjmcmc_plan* plan, const ChainArgs& a, int nseg, bool segmented, hipStream_t s) {
+    const int total = a.p.n_burnin + a.p.n_trials;
+    int32_t* counts = reinterpret_cast<int32_t*>(plan->d_queue + kCountsAt);
+    int rc = launch_classify(a.n_roi, a.order, a.xs, a.ys, plan->d_lists, plan->d_lists + plan->list_cap, counts,
+                             plan->d_queue, 2 * nseg, s);
+    if (rc != BAMF_OK) return rc;
+    const int cu = plan->cu > 0 ? plan->cu : 256;
+    const int need = (int)ceil_div(a.n_roi, kWavesPerBlock);
+    // gSplit/gMerge are dispatched on P_Burnin's cumulative vector in both
+    // phases (LibRJMCMC.cpp:705-710, 737-742): reachable only when its
+    // entries 5 or 6 are nonzero
+    const bool gj = a.p.p_burnin[5] != 0.0f || a.p.p_burnin[6] != 0.0f;
+    BAMF_HIP_TRY(hipEventRecord(plan->ev_fork, s));
+    BAMF_HIP_TRY(hipStreamWaitEvent(plan->side, plan->ev_fork, 0));
+    for (int c = kClasses - 1; c >= 0; c--) {
+        const int bpc = plan->blocks_per_cu[c][gj ? 1 : 0][segmented ? 1 : 0];
+        int grid = (bpc > 0 ? bpc : 1) * cu;
+        if (grid > need) grid = need;
+        for (int k = 0; k < nseg; k++) {
+            ChainArgs ac = a;
+            ac.list = plan->d_lists + (int64_t)c * plan->list_cap;
+            ac.list_count = counts + c;
+            ac.queue = plan->d_queue + 2 * k + c;
+            ac.seg_begin = (int)((int64_t)total * k / nseg);
+            ac.seg_end = k == nseg - 1 ? total : (int)((int64_t)total * (k + 1) / nseg);
+            rc = launch_chain(ac, c, gj, segmented, grid, c == kClasses - 1 ? s : plan->side);
+            if (rc != BAMF_OK) return rc;
+        }
+    }
+    BAMF_HIP_TRY(hipEventRecord(plan->ev_join, plan->side));
+    BAMF_HIP_TRY(hipStreamWaitEvent(s, plan->ev_join, 0));
+    return BAMF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -169,7 +232,7 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
     hipError_t e = hipGetDevice(&pl->device);
     if (e == hipSuccess) e = hipMalloc(&pl->d_tables, flat.size() * sizeof(float));
     if (e == hipSuccess) e = hipMemcpy(pl->d_tables, flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&pl->d_queue, 256);
+    if (e == hipSuccess) e = hipMalloc(&pl->d_queue, sizeof(uint32_t) * (kSeedAt + 2));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->ev_join, hipEventDisableTiming);
@@ -194,10 +257,11 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, pl->device) == hipSuccess) pl->cu = prop.multiProcessorCount;
     for (int c = 0; c < kClasses; c++)
-        for (int g = 0; g < 2; g++) {
-            int bpc = 0;
-            if (chain_occupancy_blocks(c, g == 1, &bpc) == BAMF_OK) pl->blocks_per_cu[c][g] = bpc;
-        }
+        for (int g = 0; g < 2; g++)
+            for (int sg = 0; sg < 2; sg++) {
+                int bpc = 0;
+                if (chain_occupancy_blocks(c, g == 1, sg == 1, &bpc) == BAMF_OK) pl->blocks_per_cu[c][g][sg] = bpc;
+            }
     *plan = pl;
     return BAMF_OK;
 }
@@ -205,6 +269,7 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
 int bamf_rjmcmc_plan_destroy(bamf_rjmcmc_plan* plan) {
     if (!plan) return BAMF_OK;
     (void)hipStreamSynchronize(plan->side);
+    if (plan->gexec) (void)hipDeviceSynchronize();  // a replay may still be in flight on the caller's stream
     plan_release(plan);
     return BAMF_OK;
 }
@@ -233,10 +298,11 @@ int bamf_rjmcmc_set_proposal(bamf_rjmcmc_plan* plan, const float* d_q, const flo
 
 int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed) {
     BAMF_CHECK_ARG(plan && claimed, "null argument");
-    uint32_t q[4] = {0, 0, 0, 0};
+    uint32_t q[kCountsAt + 2];
     BAMF_HIP_TRY(hipMemcpy(q, plan->d_queue, sizeof(q), hipMemcpyDeviceToHost));
     int64_t n = 0;
-    for (int c = 0; c < kClasses; c++) n += q[c] < q[2 + c] ? q[c] : q[2 + c];
+    const uint32_t* last = q + 2 * (plan->last_nseg - 1);  // the last segment's queues
+    for (int c = 0; c < kClasses; c++) n += last[c] < q[kCountsAt + c] ? last[c] : q[kCountsAt + c];
     *claimed = n;
     return BAMF_OK;
 }
@@ -334,6 +400,8 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     }
     a.recompute_every = 128;
     hipStream_t s = as_stream(stream);
+    const int total = p.n_burnin + p.n_trials;
+    const int nseg = std::max(1, std::min(plan->n_segments, total));
     if (plan->list_cap < batch->n_roi) {
         // grow the class lists (hipFree waits for work that may still use them)
         if (plan->d_lists) BAMF_HIP_TRY(hipFree(plan->d_lists));
@@ -341,34 +409,57 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
         plan->list_cap = 0;
         BAMF_HIP_TRY(hipMalloc(&plan->d_lists, sizeof(int32_t) * (size_t)kClasses * (size_t)batch->n_roi));
         plan->list_cap = batch->n_roi;
+        drop_graph(plan);
     }
-    int32_t* counts = reinterpret_cast<int32_t*>(plan->d_queue + 2);
-    int rc = launch_classify(batch->n_roi, batch->d_order, batch->d_xsize, batch->d_ysize, plan->d_lists,
-                             plan->d_lists + plan->list_cap, counts, plan->d_queue, s);
+    if (nseg > 1 && plan->seg_cap < batch->n_roi) {
+        if (plan->d_seg) BAMF_HIP_TRY(hipFree(plan->d_seg));
+        plan->d_seg = nullptr;
+        plan->seg_cap = 0;
+        BAMF_HIP_TRY(hipMalloc(&plan->d_seg, sizeof(SegState) * (size_t)batch->n_roi));
+        plan->seg_cap = batch->n_roi;
+        drop_graph(plan);
+    }
+    a.seg_state = nseg > 1 ? plan->d_seg : nullptr;
+    plan->last_nseg = nseg;
+    if (!plan->use_graph) return enqueue_run(plan, a, nseg, nseg > 1, s);
+    // graph replay: the launch sequence is captured once per argument set and
+    // replayed; the seed travels through device memory, written ahead of it
+    a.seed_lo = a.seed_hi = 0;
+    a.seed_dev = plan->d_queue + kSeedAt;
+    if (!plan->gexec || plan->graph_nseg != nseg || std::memcmp(&plan->graph_key, &a, sizeof(a)) != 0) {
+        drop_graph(plan);
+        BAMF_HIP_TRY(hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_run(plan, a, nseg, true, plan->cap);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(plan->cap, &g);
+        if (rc != BAMF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        BAMF_HIP_TRY(e);
+        const hipError_t ei = hipGraphInstantiate(&plan->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            plan->gexec = nullptr;
+            set_error("graph instantiation failed: %s", hipGetErrorString(ei));
+            return BAMF_E_HIP;
+        }
+        plan->graph_key = a;
+        plan->graph_nseg = nseg;
+    }
+    const int rc = launch_set_seed(plan->d_queue + kSeedAt, (uint32_t)seed, (uint32_t)(seed >> 32), s);
     if (rc != BAMF_OK) return rc;
-    const int cu = plan->cu > 0 ? plan->cu : 256;
-    const int need = (int)ceil_div(batch->n_roi, kWavesPerBlock);
-    // gSplit/gMerge are dispatched on P_Burnin's cumulative vector in both
-    // phases (LibRJMCMC.cpp:705-710, 737-742): reachable only when its
-    // entries 5 or 6 are nonzero
-    const bool gj = plan->params.p_burnin[5] != 0.0f || plan->params.p_burnin[6] != 0.0f;
-    // the large-ROI class goes on the caller's stream first; the small class
-    // runs beside it on the side stream and the caller's stream joins it
-    BAMF_HIP_TRY(hipEventRecord(plan->ev_fork, s));
-    BAMF_HIP_TRY(hipStreamWaitEvent(plan->side, plan->ev_fork, 0));
-    for (int c = kClasses - 1; c >= 0; c--) {
-        ChainArgs ac = a;
-        ac.list = plan->d_lists + (int64_t)c * plan->list_cap;
-        ac.list_count = counts + c;
-        ac.queue = plan->d_queue + c;
-        const int bpc = plan->blocks_per_cu[c][gj ? 1 : 0];
-        int grid = (bpc > 0 ? bpc : 1) * cu;
-        if (grid > need) grid = need;
-        rc = launch_chain(ac, c, gj, grid, c == kClasses - 1 ? s : plan->side);
-        if (rc != BAMF_OK) return rc;
-    }
-    BAMF_HIP_TRY(hipEventRecord(plan->ev_join, plan->side));
-    BAMF_HIP_TRY(hipStreamWaitEvent(s, plan->ev_join, 0));
+    BAMF_HIP_TRY(hipGraphLaunch(plan->gexec, s));
+    return BAMF_OK;
+}
+
+int bamf_rjmcmc_set_segments(bamf_rjmcmc_plan* plan, int n_segments, int use_graph) {
+    BAMF_CHECK_ARG(plan, "null argument");
+    BAMF_CHECK_ARG(n_segments >= 1 && n_segments <= kMaxSegments, "n_segments must be in 1..%d", kMaxSegments);
+    if (use_graph && !plan->cap) BAMF_HIP_TRY(hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking));
+    drop_graph(plan);
+    plan->n_segments = n_segments;
+    plan->use_graph = use_graph != 0;
     return BAMF_OK;
 }
 

@@ -70,6 +70,11 @@ class Plan:
               "bamf_rjmcmc_set_proposal")
         self._proposal = (d_q, d_mu)  # keep the buffers alive while the plan uses them
 
+    def set_segments(self, n_segments: int, graph: bool = False) -> None:
+        """Run every chain as n_segments consecutive launches (optionally replayed from a HIP graph)."""
+        check(lib().bamf_rjmcmc_set_segments(self.handle, int(n_segments), 1 if graph else 0),
+              "bamf_rjmcmc_set_segments")
+
     def __del__(self):
         try:
             if self.handle:

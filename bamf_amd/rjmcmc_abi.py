@@ -69,4 +69,5 @@ def declare(L: ctypes.CDLL) -> None:
     L.bamf_gauss_mle_host.argtypes = [c_void_p, c_int, c_int64, c_float, c_int] + [c_void_p] * 10
     L.bamf_find_roi_host.argtypes = [c_void_p, c_int, c_int, c_int, c_float, c_float, c_int, c_float, c_int64,
                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.bamf_rjmcmc_set_segments.argtypes = [c_void_p, c_int, c_int]
     L.bamf_rjmcmc_set_proposal.argtypes = [c_void_p, c_void_p, c_void_p, c_float, c_float, c_float]

@@ -241,6 +241,16 @@ int bamf_rjmcmc_plan_destroy(bamf_rjmcmc_plan* plan);
  * `stream` joins), so the call is asynchronous on `stream` as a whole. */
 int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_t seed, int mode,
                     const bamf_replay* replay, const bamf_rjmcmc_out* out, void* stream);
+/* Segmented runs (no counterpart in the reference, whose chain is one loop,
+ * LibRJMCMC.cpp:685-750): later bamf_rjmcmc_run calls on this plan split
+ * every chain into n_segments launches of consecutive steps (1..64), each
+ * chain's state kept in plan-owned device memory between them (about 7 KB
+ * per ROI); results equal the one-launch run bit for bit.  With use_graph
+ * the whole launch sequence is captured into a HIP graph on the first run
+ * and replayed by later runs with the same arguments (other than the seed);
+ * a run with other arguments captures anew.  Each launch then takes a
+ * bounded time, so other work on the device can interleave. */
+int bamf_rjmcmc_set_segments(bamf_rjmcmc_plan* plan, int n_segments, int use_graph);
 /* Learned birth proposal for BAMF_RJ_LEARNED runs of this plan (no
  * counterpart in the reference): d_q[pixel] is each ROI's birth-pixel
  * probability (summing to 1 over the ROI, the batch's pixel layout; see

@@ -1,0 +1,102 @@
+"""Segmented chains (bamf_rjmcmc_set_segments): a chain split into several
+launches, its state kept in device memory between them, and the launch
+sequence replayed from a HIP graph, must give exactly the one-launch run's
+outputs (states, counters, histograms, traces, step records, grids, chain
+records, log posteriors, work counts)."""
+import numpy as np
+import pytest
+
+from bamf_amd import _hip, synth
+from bamf_amd.rjmcmc import PLAIN, REPLAY, Plan, RoiSet, alloc_outputs, fetch, make_replay, run_batch
+from rj_common import BACK_PDF, OFF_PDF, SIG_PDF, params, run_oracle_chains
+
+pytestmark = pytest.mark.gpu
+
+KW = dict(hist_len=129, final_cap=128, trace_len=120, trace_thin=5, record_steps=True, grids=True,
+          chain_caps=(1300, 12000))
+
+
+def rois_uneven(n=160, seed=61):
+    return synth.make_rois(n, seed=seed, sides=(8, 12, 16, 24, 32), density=(0.5, 5.0), bg=(5.0, 30.0))
+
+
+def assert_same(a, b, n, n_trace):
+    """Bitwise equality of every written output (buffers are not cleared, so
+    final-state slots beyond N and trace slots beyond the chain are skipped)."""
+    for f in ("status", "counters", "hist", "logpost", "alg_flops", "pgrid", "pback"):
+        x, y = getattr(a, f), getattr(b, f)
+        assert x.tobytes() == y.tobytes(), f
+    assert a.trace_n[:, :n_trace].tobytes() == b.trace_n[:, :n_trace].tobytes()
+    assert a.trace_photons[:, :n_trace].tobytes() == b.trace_photons[:, :n_trace].tobytes()
+    cap = (a.final.shape[1] - 4) // 4
+    for r in range(n):
+        k = int(a.final[r, 0])
+        assert a.final[r, 0:4].tobytes() == b.final[r, 0:4].tobytes(), r  # N, BG, ABG, BBG
+        for j in range(4):  # I, X, Y, Signal of the first N slots
+            sl = slice(4 + j * cap, 4 + j * cap + k)
+            assert a.final[r, sl].tobytes() == b.final[r, sl].tobytes(), (r, j)
+    assert a.steps.tobytes() == b.steps.tobytes()
+    ca, cb = a.chain, b.chain
+    assert np.array_equal(ca["count"], cb["count"])
+    for r in range(n):
+        k, e1, e2 = (int(v) for v in ca["count"][r])
+        k = min(k, 1300)
+        for f in ("acc_rec", "prop_rec"):
+            assert ca[f][r, :k].tobytes() == cb[f][r, :k].tobytes(), (r, f)
+        assert ca["acc_em"][r, :min(e1, 12000)].tobytes() == cb["acc_em"][r, :min(e1, 12000)].tobytes(), r
+        assert ca["prop_em"][r, :min(e2, 12000)].tobytes() == cb["prop_em"][r, :min(e2, 12000)].tobytes(), r
+
+
+@pytest.mark.parametrize("nseg", [2, 7])
+def test_segments_equal_one_launch(nseg):
+    rois = rois_uneven()
+    n = len(rois["side"])
+    p = params(nb=600, nt=600, rho=0.0275)
+    rs = RoiSet.from_synth(rois, bg0=np.full(n, 20.0, np.float32))
+    one = run_batch(Plan(p, OFF_PDF, SIG_PDF, BACK_PDF), rs, seed=5, **KW)
+    assert np.all(one.status == 0)
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    plan.set_segments(nseg)
+    print(f"segments: {nseg}", flush=True)
+    seg = run_batch(plan, rs, seed=5, **KW)
+    assert_same(one, seg, n, 120)
+
+
+def test_graph_replays_with_new_seeds():
+    rois = rois_uneven(96, seed=62)
+    n = len(rois["side"])
+    p = params(nb=500, nt=500, rho=0.0275)
+    rs = RoiSet.from_synth(rois, bg0=np.full(n, 20.0, np.float32))
+    ref = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    plan.set_segments(4, graph=True)
+    total = p.n_burnin + p.n_trials
+    outs = alloc_outputs(n, total, KW["hist_len"], KW["final_cap"], KW["trace_len"], True, True, rs, p.grid_zoom,
+                         KW["chain_caps"])
+    for it, seed in enumerate((11, 12, 11)):
+        # the grids accumulate: clear them between replays, as for any run
+        for k in ("pgrid", "pback"):
+            outs[k].upload(np.zeros(outs[k].nbytes // 4, np.float32))
+        print(f"graph replay {it}, seed {seed}", flush=True)
+        run_batch(plan, rs, seed=seed, outs=outs, trace_thin=KW["trace_thin"], sync=False)
+        _hip.synchronize()
+        got = fetch(outs, n, total, KW["hist_len"], KW["final_cap"], KW["trace_len"])
+        want = run_batch(ref, rs, seed=seed, **KW)
+        assert np.all(want.status == 0)
+        assert_same(want, got, n, 100)
+
+
+def test_segments_in_replay_mode(oracle):
+    rois = synth.make_rois(12, seed=63, sides=(8, 16, 24), mean_n=3.0)
+    n = 12
+    p = params(nb=400, nt=400)
+    ores, tapes = run_oracle_chains(oracle, rois, p, seed=3)
+    rp = make_replay(tapes, np.stack([o["steps"] for o in ores]))
+    rs = RoiSet.from_synth(rois, bg0=np.full(n, 10.0, np.float32))
+    one = run_batch(Plan(p, OFF_PDF, SIG_PDF, BACK_PDF), rs, mode=REPLAY, replay=rp, **KW)
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    plan.set_segments(5)
+    print("replay segments: 5", flush=True)
+    seg = run_batch(plan, rs, mode=REPLAY, replay=rp, **KW)
+    assert np.all(one.status == 0)
+    assert_same(one, seg, n, 80)
