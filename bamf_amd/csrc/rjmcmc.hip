@@ -1740,16 +1740,22 @@ int launch_set_seed(uint32_t* d, uint32_t lo, uint32_t hi, hipStream_t s) {
 
 using ChainKernelFn = void (*)(ChainArgs);
 
-// Every instantiation: [class][gSplit/gMerge][segmented].
+// Every instantiation: [class][gSplit/gMerge][segmented]; the one-launch
+// kernel without group jumps also in each register budget of kTuneMinBlocks.
 template <int C>
-ChainKernelFn chain_kernel(bool gj, bool seg) {
+ChainKernelFn chain_kernel(bool gj, bool seg, int tune) {
     constexpr int P = kClassPPL[C], M = kClassMinBlocks[C];
-    return gj ? (seg ? k_chain<P, M, true, true> : k_chain<P, M, true, false>)
-              : (seg ? k_chain<P, M, false, true> : k_chain<P, M, false, false>);
+    if (gj) return seg ? k_chain<P, M, true, true> : k_chain<P, M, true, false>;
+    if (seg) return k_chain<P, M, false, true>;
+    static_assert(kTuneMinBlocks[kTuneDefault] == M, "the default budget is the class's");
+    return tune == 0 ? k_chain<P, kTuneMinBlocks[0], false, false>
+           : tune == 2 ? k_chain<P, kTuneMinBlocks[2], false, false>
+                       : k_chain<P, M, false, false>;
 }
 
-int launch_chain(const ChainArgs& a, int cls, bool group_jumps, bool segmented, int grid, hipStream_t s) {
-    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(group_jumps, segmented) : chain_kernel<1>(group_jumps, segmented);
+int launch_chain(const ChainArgs& a, int cls, bool group_jumps, bool segmented, int tune, int grid, hipStream_t s) {
+    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(group_jumps, segmented, tune)
+                                     : chain_kernel<1>(group_jumps, segmented, tune);
     hipLaunchKernelGGL(k, dim3(grid), dim3(64 * kWavesPerBlock), 0, s, a);
     BAMF_LAUNCH_CHECK("k_chain");
     return BAMF_OK;
@@ -1761,9 +1767,10 @@ int launch_erf(const float* x, float* y, int64_t n, hipStream_t s) {
     return BAMF_OK;
 }
 
-int chain_occupancy_blocks(int cls, bool group_jumps, bool segmented, int* blocks_per_cu) {
+int chain_occupancy_blocks(int cls, bool group_jumps, bool segmented, int tune, int* blocks_per_cu) {
     int n = 0;
-    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(group_jumps, segmented) : chain_kernel<1>(group_jumps, segmented);
+    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(group_jumps, segmented, tune)
+                                     : chain_kernel<1>(group_jumps, segmented, tune);
     BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(k),
                                                                64 * kWavesPerBlock, 0));
     *blocks_per_cu = n;

@@ -17,6 +17,12 @@ constexpr int kMaxSide = 32;
 constexpr int kClasses = 2;
 constexpr int kClassPPL[kClasses] = {4, 16};
 constexpr int kClassMinBlocks[kClasses] = {3, 3};  // work-groups per CU the register budget targets
+// Register budgets the autotuner chooses from for the one-launch plain
+// kernels (work-groups of kWavesPerBlock waves per CU = chains per SIMD):
+// 256, 168 and 128 VGPRs.
+constexpr int kTuneChoices = 3;
+constexpr int kTuneMinBlocks[kTuneChoices] = {2, 3, 4};
+constexpr int kTuneDefault = 1;
 
 // One prior table (Stat, LibRJMCMC.cpp:176-217) on the device.
 struct StatDev {

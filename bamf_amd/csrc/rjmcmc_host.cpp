@@ -12,8 +12,8 @@ namespace rj {
 int launch_classify(int n_roi, const int32_t* order, const int32_t* xs, const int32_t* ys, int32_t* list0,
                     int32_t* list1, int32_t* counts, uint32_t* queues, int nq, hipStream_t s);
 int launch_set_seed(uint32_t* d, uint32_t lo, uint32_t hi, hipStream_t s);
-int launch_chain(const ChainArgs& a, int cls, bool group_jumps, bool segmented, int grid, hipStream_t s);
-int chain_occupancy_blocks(int cls, bool group_jumps, bool segmented, int* blocks_per_cu);
+int launch_chain(const ChainArgs& a, int cls, bool group_jumps, bool segmented, int tune, int grid, hipStream_t s);
+int chain_occupancy_blocks(int cls, bool group_jumps, bool segmented, int tune, int* blocks_per_cu);
 int launch_erf(const float* x, float* y, int64_t n, hipStream_t s);
 }  // namespace rj
 }  // namespace bamf
@@ -111,6 +111,12 @@ struct bamf_rjmcmc_plan {
     int device = 0;
     int cu = 0;
     int blocks_per_cu[kClasses][2][2] = {};  // [class][gSplit/gMerge compiled in][segmented]
+    int blocks_per_cu_tune[kClasses][kTuneChoices] = {};  // one-launch plain kernels per register budget
+    // autotuned (or set) launch configuration of the one-launch plain kernels
+    int tune[kClasses] = {kTuneDefault, kTuneDefault};
+    bool concurrent = true;
+    int32_t* d_tune_status = nullptr;
+    int64_t tune_cap = 0;
     // segmented runs and their graph
     int n_segments = 1;
     bool use_graph = false;
@@ -137,6 +143,7 @@ void plan_release(bamf_rjmcmc_plan* pl) {
     if (pl->gexec) (void)hipGraphExecDestroy(pl->gexec);
     if (pl->cap) (void)hipStreamDestroy(pl->cap);
     if (pl->d_seg) (void)hipFree(pl->d_seg);
+    if (pl->d_tune_status) (void)hipFree(pl->d_tune_status);
     delete pl;
 }
 void drop_graph(bamf_rjmcmc_plan* pl) {
@@ -148,7 +155,8 @@ void drop_graph(bamf_rjmcmc_plan* pl) {
 // The launch sequence of one run: the classifier (which also resets the work
 // queues), then for each size class its segments in order on its own stream,
 // the small-ROI class on the side stream beside the caller's.
-int enqueue_run(bamf_rjmcmc_plan* plan, const ChainArgs& a, int nseg, bool segmented, hipStream_t s) {
+int enqueue_run(bamf_rjmcmc_plan* plan, const ChainArgs& a, int nseg, bool segmented, const int tune[kClasses],
+                bool concurrent, hipStream_t s) {
     const int total = a.p.n_burnin + a.p.n_trials;
     int32_t* counts = reinterpret_cast<int32_t*>(plan->d_queue + kCountsAt);
     int rc = launch_classify(a.n_roi, a.order, a.xs, a.ys, plan->d_lists, plan->d_lists + plan->list_cap, counts,
@@ -160,10 +168,14 @@ int enqueue_run(bamf_rjmcmc_plan* plan, const ChainArgs& a, int nseg, bool segme
     // phases (LibRJMCMC.cpp:705-710, 737-742): reachable only when its
     // entries 5 or 6 are nonzero
     const bool gj = a.p.p_burnin[5] != 0.0f || a.p.p_burnin[6] != 0.0f;
-    BAMF_HIP_TRY(hipEventRecord(plan->ev_fork, s));
-    BAMF_HIP_TRY(hipStreamWaitEvent(plan->side, plan->ev_fork, 0));
+    if (concurrent) {
+        BAMF_HIP_TRY(hipEventRecord(plan->ev_fork, s));
+        BAMF_HIP_TRY(hipStreamWaitEvent(plan->side, plan->ev_fork, 0));
+    }
     for (int c = kClasses - 1; c >= 0; c--) {
-        const int bpc = plan->blocks_per_cu[c][gj ? 1 : 0][segmented ? 1 : 0];
+        const bool tuned = !gj && !segmented;  // the register budget is tunable for these kernels only
+        const int t = tuned ? tune[c] : kTuneDefault;
+        const int bpc = tuned ? plan->blocks_per_cu_tune[c][t] : plan->blocks_per_cu[c][gj ? 1 : 0][segmented ? 1 : 0];
         int grid = (bpc > 0 ? bpc : 1) * cu;
         if (grid > need) grid = need;
         for (int k = 0; k < nseg; k++) {
@@ -173,12 +185,26 @@ int enqueue_run(bamf_rjmcmc_plan* plan, const ChainArgs& a, int nseg, bool segme
             ac.queue = plan->d_queue + 2 * k + c;
             ac.seg_begin = (int)((int64_t)total * k / nseg);
             ac.seg_end = k == nseg - 1 ? total : (int)((int64_t)total * (k + 1) / nseg);
-            rc = launch_chain(ac, c, gj, segmented, grid, c == kClasses - 1 ? s : plan->side);
+            rc = launch_chain(ac, c, gj, segmented, t, grid, (c == kClasses - 1 || !concurrent) ? s : plan->side);
             if (rc != BAMF_OK) return rc;
         }
     }
-    BAMF_HIP_TRY(hipEventRecord(plan->ev_join, plan->side));
-    BAMF_HIP_TRY(hipStreamWaitEvent(s, plan->ev_join, 0));
+    if (concurrent) {
+        BAMF_HIP_TRY(hipEventRecord(plan->ev_join, plan->side));
+        BAMF_HIP_TRY(hipStreamWaitEvent(s, plan->ev_join, 0));
+    }
+    return BAMF_OK;
+}
+
+// Class lists for n_roi ROIs (hipFree waits for work that may still use them).
+int ensure_lists(bamf_rjmcmc_plan* plan, int64_t n_roi) {
+    if (plan->list_cap >= n_roi) return BAMF_OK;
+    if (plan->d_lists) BAMF_HIP_TRY(hipFree(plan->d_lists));
+    plan->d_lists = nullptr;
+    plan->list_cap = 0;
+    BAMF_HIP_TRY(hipMalloc(&plan->d_lists, sizeof(int32_t) * (size_t)kClasses * (size_t)n_roi));
+    plan->list_cap = n_roi;
+    drop_graph(plan);
     return BAMF_OK;
 }
 
@@ -260,8 +286,14 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
         for (int g = 0; g < 2; g++)
             for (int sg = 0; sg < 2; sg++) {
                 int bpc = 0;
-                if (chain_occupancy_blocks(c, g == 1, sg == 1, &bpc) == BAMF_OK) pl->blocks_per_cu[c][g][sg] = bpc;
+                if (chain_occupancy_blocks(c, g == 1, sg == 1, kTuneDefault, &bpc) == BAMF_OK)
+                    pl->blocks_per_cu[c][g][sg] = bpc;
             }
+    for (int c = 0; c < kClasses; c++)
+        for (int t = 0; t < kTuneChoices; t++) {
+            int bpc = 0;
+            if (chain_occupancy_blocks(c, false, false, t, &bpc) == BAMF_OK) pl->blocks_per_cu_tune[c][t] = bpc;
+        }
     *plan = pl;
     return BAMF_OK;
 }
@@ -402,15 +434,8 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     hipStream_t s = as_stream(stream);
     const int total = p.n_burnin + p.n_trials;
     const int nseg = std::max(1, std::min(plan->n_segments, total));
-    if (plan->list_cap < batch->n_roi) {
-        // grow the class lists (hipFree waits for work that may still use them)
-        if (plan->d_lists) BAMF_HIP_TRY(hipFree(plan->d_lists));
-        plan->d_lists = nullptr;
-        plan->list_cap = 0;
-        BAMF_HIP_TRY(hipMalloc(&plan->d_lists, sizeof(int32_t) * (size_t)kClasses * (size_t)batch->n_roi));
-        plan->list_cap = batch->n_roi;
-        drop_graph(plan);
-    }
+    int rc0 = ensure_lists(plan, batch->n_roi);
+    if (rc0 != BAMF_OK) return rc0;
     if (nseg > 1 && plan->seg_cap < batch->n_roi) {
         if (plan->d_seg) BAMF_HIP_TRY(hipFree(plan->d_seg));
         plan->d_seg = nullptr;
@@ -421,7 +446,7 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     }
     a.seg_state = nseg > 1 ? plan->d_seg : nullptr;
     plan->last_nseg = nseg;
-    if (!plan->use_graph) return enqueue_run(plan, a, nseg, nseg > 1, s);
+    if (!plan->use_graph) return enqueue_run(plan, a, nseg, nseg > 1, plan->tune, plan->concurrent, s);
     // graph replay: the launch sequence is captured once per argument set and
     // replayed; the seed travels through device memory, written ahead of it
     a.seed_lo = a.seed_hi = 0;
@@ -429,7 +454,7 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     if (!plan->gexec || plan->graph_nseg != nseg || std::memcmp(&plan->graph_key, &a, sizeof(a)) != 0) {
         drop_graph(plan);
         BAMF_HIP_TRY(hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_run(plan, a, nseg, true, plan->cap);
+        const int rc = enqueue_run(plan, a, nseg, true, plan->tune, plan->concurrent, plan->cap);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(plan->cap, &g);
         if (rc != BAMF_OK) {
@@ -451,6 +476,110 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     if (rc != BAMF_OK) return rc;
     BAMF_HIP_TRY(hipGraphLaunch(plan->gexec, s));
     return BAMF_OK;
+}
+
+int bamf_rjmcmc_get_tuning(const bamf_rjmcmc_plan* plan, bamf_rjmcmc_tuning* t) {
+    BAMF_CHECK_ARG(plan && t, "null argument");
+    for (int c = 0; c < kClasses; c++) t->chains_per_simd[c] = kTuneMinBlocks[plan->tune[c]] * kWavesPerBlock / 4;
+    t->concurrent = plan->concurrent ? 1 : 0;
+    t->ms = 0.0f;
+    return BAMF_OK;
+}
+
+int bamf_rjmcmc_set_tuning(bamf_rjmcmc_plan* plan, const bamf_rjmcmc_tuning* t) {
+    BAMF_CHECK_ARG(plan && t, "null argument");
+    int idx[kClasses];
+    for (int c = 0; c < kClasses; c++) {
+        idx[c] = -1;
+        for (int k = 0; k < kTuneChoices; k++)
+            if (kTuneMinBlocks[k] * kWavesPerBlock / 4 == t->chains_per_simd[c]) idx[c] = k;
+        BAMF_CHECK_ARG(idx[c] >= 0, "chains_per_simd[%d] = %d is not one of the compiled budgets (2, 3, 4)", c,
+                       t->chains_per_simd[c]);
+    }
+    BAMF_CHECK_ARG(t->concurrent == 0 || t->concurrent == 1, "concurrent must be 0 or 1");
+    for (int c = 0; c < kClasses; c++) plan->tune[c] = idx[c];
+    plan->concurrent = t->concurrent != 0;
+    drop_graph(plan);
+    return BAMF_OK;
+}
+
+int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, int sample_jumps,
+                         bamf_rjmcmc_tuning* best, void* stream) {
+    BAMF_CHECK_ARG(plan && batch && best, "null argument");
+    BAMF_CHECK_ARG(sample_jumps >= 2, "sample_jumps must be >= 2");
+    BAMF_CHECK_ARG(batch->n_roi >= 1, "the sample batch is empty");
+    hipStream_t s = as_stream(stream);
+    if (plan->tune_cap < batch->n_roi) {
+        if (plan->d_tune_status) BAMF_HIP_TRY(hipFree(plan->d_tune_status));
+        plan->d_tune_status = nullptr;
+        plan->tune_cap = 0;
+        BAMF_HIP_TRY(hipMalloc(&plan->d_tune_status, sizeof(int32_t) * (size_t)batch->n_roi));
+        plan->tune_cap = batch->n_roi;
+    }
+    // the sample runs use this plan with short chains, one launch, no graph
+    const bamf_mcmc_params saved = plan->params;
+    const int saved_tune[kClasses] = {plan->tune[0], plan->tune[1]};
+    const bool saved_conc = plan->concurrent, saved_graph = plan->use_graph;
+    const int saved_nseg = plan->n_segments;
+    plan->params.n_burnin = sample_jumps / 2;
+    plan->params.n_trials = sample_jumps - sample_jumps / 2;
+    plan->use_graph = false;
+    plan->n_segments = 1;
+    bamf_rjmcmc_out out;
+    std::memset(&out, 0, sizeof(out));
+    out.d_status = plan->d_tune_status;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = BAMF_OK;
+    float best_ms = -1.0f;
+    int best_t[kClasses] = {kTuneDefault, kTuneDefault};
+    bool best_conc = true;
+    // group jumps and segments use the default budget: only the concurrency is tunable then
+    const bool gj = saved.p_burnin[5] != 0.0f || saved.p_burnin[6] != 0.0f;
+    const int nt = gj ? 1 : kTuneChoices;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        set_error("autotune: hipEventCreate failed");
+        rc = BAMF_E_HIP;
+    }
+    for (int conc = 1; conc >= 0 && rc == BAMF_OK; conc--)
+        for (int t1 = 0; t1 < nt && rc == BAMF_OK; t1++)
+            for (int t0 = 0; t0 < nt && rc == BAMF_OK; t0++) {
+                plan->tune[0] = gj ? kTuneDefault : t0;
+                plan->tune[1] = gj ? kTuneDefault : t1;
+                plan->concurrent = conc == 1;
+                float ms = 0.0f;
+                for (int rep = 0; rep < 2 && rc == BAMF_OK; rep++) {  // the first run loads the code objects
+                    if (hipEventRecord(e0, s) != hipSuccess) rc = BAMF_E_HIP;
+                    if (rc == BAMF_OK) rc = bamf_rjmcmc_run(plan, batch, 0x5eed, BAMF_RJ_PLAIN, nullptr, &out, s);
+                    if (rc == BAMF_OK && hipEventRecord(e1, s) != hipSuccess) rc = BAMF_E_HIP;
+                    if (rc == BAMF_OK && hipEventSynchronize(e1) != hipSuccess) rc = BAMF_E_HIP;
+                    if (rc == BAMF_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = BAMF_E_HIP;
+                }
+                if (rc == BAMF_OK && (best_ms < 0.0f || ms < best_ms)) {
+                    best_ms = ms;
+                    best_t[0] = plan->tune[0];
+                    best_t[1] = plan->tune[1];
+                    best_conc = plan->concurrent;
+                }
+            }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    plan->params = saved;
+    plan->use_graph = saved_graph;
+    plan->n_segments = saved_nseg;
+    if (rc != BAMF_OK) {
+        plan->tune[0] = saved_tune[0];
+        plan->tune[1] = saved_tune[1];
+        plan->concurrent = saved_conc;
+        if (rc == BAMF_E_HIP) set_error("autotune: a sample run failed on the device");
+        return rc;
+    }
+    plan->tune[0] = best_t[0];
+    plan->tune[1] = best_t[1];
+    plan->concurrent = best_conc;
+    drop_graph(plan);
+    rc = bamf_rjmcmc_get_tuning(plan, best);
+    best->ms = best_ms;
+    return rc;
 }
 
 int bamf_rjmcmc_set_segments(bamf_rjmcmc_plan* plan, int n_segments, int use_graph) {

@@ -13,7 +13,8 @@ import numpy as np
 
 from . import _hip
 from ._lib import BamfError, check, lib
-from .rjmcmc_abi import CHAIN_REC_DTYPE, LEARNED, PLAIN, REPLAY, STEP_DTYPE, McmcParams, Replay, RjOut, RoiBatch
+from .rjmcmc_abi import (CHAIN_REC_DTYPE, LEARNED, PLAIN, REPLAY, STEP_DTYPE, McmcParams, Replay, RjOut, RoiBatch,
+                         Tuning)
 
 COUNTER_NAMES = [f"{k}_{j}" for k in ("N", "A", "NFg", "AFg", "NBg", "ABg")
                  for j in ("jump", "split", "merge", "birth", "death")]
@@ -69,6 +70,24 @@ class Plan:
                                              ctypes.c_float(halfwidth), ctypes.c_float(beta)),
               "bamf_rjmcmc_set_proposal")
         self._proposal = (d_q, d_mu)  # keep the buffers alive while the plan uses them
+
+    def autotune(self, rois: "RoiSet", sample_jumps: int = 400, stream=None) -> dict:
+        """Times the launch settings on a short run of `rois`; the fastest stays in the plan."""
+        t = Tuning()
+        check(lib().bamf_rjmcmc_autotune(self.handle, ctypes.byref(rois.batch), int(sample_jumps), ctypes.byref(t),
+                                         stream.vp if stream else None), "bamf_rjmcmc_autotune")
+        return {"chains_per_simd": list(t.chains_per_simd), "concurrent": bool(t.concurrent), "ms": float(t.ms)}
+
+    def set_tuning(self, chains_per_simd=(3, 3), concurrent: bool = True) -> None:
+        t = Tuning()
+        t.chains_per_simd[0], t.chains_per_simd[1] = int(chains_per_simd[0]), int(chains_per_simd[1])
+        t.concurrent = 1 if concurrent else 0
+        check(lib().bamf_rjmcmc_set_tuning(self.handle, ctypes.byref(t)), "bamf_rjmcmc_set_tuning")
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        check(lib().bamf_rjmcmc_get_tuning(self.handle, ctypes.byref(t)), "bamf_rjmcmc_get_tuning")
+        return {"chains_per_simd": list(t.chains_per_simd), "concurrent": bool(t.concurrent)}
 
     def set_segments(self, n_segments: int, graph: bool = False) -> None:
         """Run every chain as n_segments consecutive launches (optionally replayed from a HIP graph)."""

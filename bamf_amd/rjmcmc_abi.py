@@ -7,6 +7,11 @@ from ctypes import POINTER, Structure, c_double, c_float, c_int, c_int16, c_int3
 import numpy as np
 
 
+class Tuning(Structure):
+    """bamf_rjmcmc_tuning."""
+    _fields_ = [("chains_per_simd", c_int32 * 2), ("concurrent", c_int32), ("ms", c_float)]
+
+
 class McmcParams(Structure):
     """bamf_mcmc_params: the MCMC struct fields of cRJMCMC.cpp:14-50."""
     _fields_ = [("psf_sigma", c_float), ("n_trials", c_int32), ("n_burnin", c_int32),
@@ -70,4 +75,7 @@ def declare(L: ctypes.CDLL) -> None:
     L.bamf_find_roi_host.argtypes = [c_void_p, c_int, c_int, c_int, c_float, c_float, c_int, c_float, c_int64,
                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.bamf_rjmcmc_set_segments.argtypes = [c_void_p, c_int, c_int]
+    L.bamf_rjmcmc_autotune.argtypes = [c_void_p, c_void_p, c_int, POINTER(Tuning), c_void_p]
+    L.bamf_rjmcmc_set_tuning.argtypes = [c_void_p, POINTER(Tuning)]
+    L.bamf_rjmcmc_get_tuning.argtypes = [c_void_p, POINTER(Tuning)]
     L.bamf_rjmcmc_set_proposal.argtypes = [c_void_p, c_void_p, c_void_p, c_float, c_float, c_float]

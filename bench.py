@@ -323,6 +323,7 @@ def main():
     ap.add_argument("--cpu-sample-fits", type=int, default=100000)
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
     ap.add_argument("--thin", type=int, default=5)
     ap.add_argument("--proposal", default="both", choices=["plain", "learned", "both"],
                     help="chain proposal(s) timed: the value is the plain chain's, learned is reported beside it")
@@ -360,6 +361,13 @@ def main():
     cap = 128
     outs = alloc_outputs(n, nb + nt, 129, cap, trace_len, False, False, rs, p.grid_zoom)
     stream = _hip.Stream()
+    tuning = {"chains_per_simd": [3, 3], "concurrent": True, "autotuned": False}
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        best = plan.autotune(rs, sample_jumps=400, stream=stream)
+        tuning = {"chains_per_simd": best["chains_per_simd"], "concurrent": best["concurrent"], "autotuned": True,
+                  "sample_ms": best["ms"], "tune_s": time.perf_counter() - t0}
+        print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
 
     def timed(mode, pre=None, pre_events=None):
         """Warm-up, then args.steps timed steps (HIP events on the launch stream)."""
@@ -406,7 +414,8 @@ def main():
         "dtype": "f32 (accept test f64)", "data": "synthetic (seeded forward-model ROIs)",
         "config": {"workload": args.workload, "n_roi": n, "roi_sides": sorted(set(int(s) for s in rois["side"])),
                    "jumps_per_chain": nb + nt, "burnin": nb, "ess_target": 200, "proposal": "plain",
-                   "parallelism": "1 GPU, one wavefront per chain, persistent work queues per ROI size class"},
+                   "parallelism": "1 GPU, one wavefront per chain, persistent work queues per ROI size class",
+                   "launch_tuning": tuning},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "k_chain", "alg_flop_per_launch": flops},

@@ -251,6 +251,25 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
  * a run with other arguments captures anew.  Each launch then takes a
  * bounded time, so other work on the device can interleave. */
 int bamf_rjmcmc_set_segments(bamf_rjmcmc_plan* plan, int n_segments, int use_graph);
+
+/* Launch configuration of the one-launch chain kernels (no counterpart in the
+ * reference).  chains_per_simd[c]: resident chains per SIMD that size class
+ * c's kernel is compiled for (2, 3 or 4: 256, 168 or 128 VGPRs, so more
+ * registers and fewer spills against more latency hiding); concurrent: the
+ * two classes run side by side (1) or one after the other (0).  Segmented
+ * runs and runs with gSplit/gMerge use 3 chains per SIMD. */
+typedef struct bamf_rjmcmc_tuning {
+    int32_t chains_per_simd[2];
+    int32_t concurrent;
+    float ms;  /* autotune: the chosen setting's time on the sample run */
+} bamf_rjmcmc_tuning;
+/* Times every setting on the caller's sample batch (its ROIs with chains of
+ * sample_jumps jumps, plain mode), keeps the fastest in the plan and returns
+ * it.  Synchronous. */
+int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, int sample_jumps,
+                         bamf_rjmcmc_tuning* best, void* stream);
+int bamf_rjmcmc_set_tuning(bamf_rjmcmc_plan* plan, const bamf_rjmcmc_tuning* t);
+int bamf_rjmcmc_get_tuning(const bamf_rjmcmc_plan* plan, bamf_rjmcmc_tuning* t);
 /* Learned birth proposal for BAMF_RJ_LEARNED runs of this plan (no
  * counterpart in the reference): d_q[pixel] is each ROI's birth-pixel
  * probability (summing to 1 over the ROI, the batch's pixel layout; see

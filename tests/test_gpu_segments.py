@@ -100,3 +100,28 @@ def test_segments_in_replay_mode(oracle):
     seg = run_batch(plan, rs, mode=REPLAY, replay=rp, **KW)
     assert np.all(one.status == 0)
     assert_same(one, seg, n, 80)
+
+
+def test_tuning_changes_no_result():
+    """Every register budget and the sequential class order give the same
+    chains, bit for bit; the autotuner returns one of the compiled settings."""
+    rois = rois_uneven(120, seed=64)
+    n = len(rois["side"])
+    p = params(nb=400, nt=400, rho=0.0275)
+    rs = RoiSet.from_synth(rois, bg0=np.full(n, 20.0, np.float32))
+    kw = dict(KW, trace_len=80)
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    assert plan.tuning() == {"chains_per_simd": [3, 3], "concurrent": True}
+    base = run_batch(plan, rs, seed=9, **kw)
+    assert np.all(base.status == 0)
+    for cps, conc in (((2, 2), True), ((4, 4), True), ((2, 4), False), ((4, 2), True)):
+        plan.set_tuning(cps, conc)
+        print(f"tuning {cps} concurrent={conc}", flush=True)
+        assert_same(base, run_batch(plan, rs, seed=9, **kw), n, 80)
+    best = plan.autotune(rs, sample_jumps=200)
+    print("autotune:", best, flush=True)
+    assert best["ms"] > 0 and all(c in (2, 3, 4) for c in best["chains_per_simd"])
+    assert plan.tuning()["chains_per_simd"] == best["chains_per_simd"]
+    assert_same(base, run_batch(plan, rs, seed=9, **kw), n, 80)
+    with pytest.raises(Exception):
+        plan.set_tuning((5, 3), True)
