@@ -183,13 +183,16 @@ __device__ __forceinline__ float normpdf(float x, float u, float s) {
 // Proposal kinds that need the likelihood ratio, with their accept counters.
 enum { EV_NONE = 0, EV_MOVE, EV_SPLIT, EV_MERGE, EV_BIRTH, EV_DEATH, EV_GSPLIT, EV_GMERGE };
 
-// GJ: the gSplit/gMerge jumps are compiled in (they are reachable only when
-// P_Burnin entries 5 or 6 are nonzero; without them the hot loop carries none
-// of their code).  SEG: the launch runs a segment of the chain (state loaded
-// from and saved to HBM, seed read from device memory when given); the
-// one-launch variant carries none of that.
-template <int PPL, bool GJ, bool SEG>
+// Feature bits of a kernel variant; the plain one-launch kernel (F = 0)
+// carries none of their code.  kGJ: the gSplit/gMerge jumps (reachable only
+// when P_Burnin entries 5 or 6 are nonzero).  kSEG: the launch runs a segment
+// of the chain (state loaded from and saved to HBM, seed read from device
+// memory when given).  kLRN: the learned proposals of BAMF_RJ_LEARNED.
+constexpr int kGJ = 1, kSEG = 2, kLRN = 4;
+
+template <int PPL, int F>
 struct Chain {
+    static constexpr bool GJ = (F & kGJ) != 0, SEG = (F & kSEG) != 0, LRN = (F & kLRN) != 0;
     const ChainArgs* a;  // the kernel argument block (in the kernarg segment)
     WaveLds<PPL>& L;
     const int lane;
@@ -522,6 +525,59 @@ struct Chain {
         wsync();
         return ysub < Y ? vy : 0.0f;
     }
+    // ---------------- conditional intensity fit (learned mode) ----------------
+    // R = M minus emitter (I, x, y) into mt[], leaving that emitter's unit
+    // stencil in place for cond_fit.
+    float c_vy;
+    __device__ __forceinline__ void cond_prep(float I, float x, float y) {
+        c_vy = stencil(1.0f, x, y);
+        const int s0 = sxa >> (6 - lgY), s1 = sxb >> (6 - lgY);
+#pragma unroll
+        for (int s = 0; s < PPL; s++) {
+            mt[s] = m[s];
+            if (s >= s0 && s <= s1) mt[s] = m[s] - I * (L.sten[px(s)] * c_vy);
+        }
+    }
+    __device__ __forceinline__ void cond_stencil(float x, float y) { c_vy = stencil(1.0f, x, y); }
+    // Gaussian approximation of the Poisson likelihood of D = R + I g in I
+    // (g the unit stencil in place): weighted least squares with weights 1/R,
+    // then one Fisher-scoring step with the information F = sum g^2 / (R + I g)
+    // at that start; sd = inflation / sqrt(F).  No weight inside the ROI: the
+    // signal prior's mean and sd.
+    __device__ __forceinline__ void cond_fit(float& mean, float& sd) {
+        const int s0 = sxa >> (6 - lgY), s1 = sxb >> (6 - lgY);
+        float a0 = 0.0f, b0 = 0.0f;
+#pragma unroll
+        for (int s = 0; s < PPL; s++)
+            if (s >= s0 && s <= s1) {
+                const float g = L.sten[px(s)] * c_vy, r = fmaxf(mt[s], 1e-3f), d = L.dpx[64 * s + lane];
+                const float w = __builtin_amdgcn_rcpf(r);
+                a0 = a0 + g * (d - r) * w;
+                b0 = b0 + g * g * w;
+            }
+        a0 = bcastf(wave_scan(a0), 63);
+        b0 = bcastf(wave_scan(b0), 63);
+        if (!(b0 > 1e-12f)) {
+            mean = a->c_mean;
+            sd = a->c_sd;
+            return;
+        }
+        const float m0 = fmaxf(a0 / b0, 1.0f);
+        float sc = 0.0f, fi = 0.0f;
+#pragma unroll
+        for (int s = 0; s < PPL; s++)
+            if (s >= s0 && s <= s1) {
+                const float g = L.sten[px(s)] * c_vy, r = fmaxf(mt[s], 1e-3f), d = L.dpx[64 * s + lane];
+                const float w = __builtin_amdgcn_rcpf(r + m0 * g);
+                sc = sc + g * (d * w - 1.0f);
+                fi = fi + g * g * w;
+            }
+        sc = bcastf(wave_scan(sc), 63);
+        fi = bcastf(wave_scan(fi), 63);
+        mean = m0 + sc / fi;
+        sd = a->c_infl * __builtin_amdgcn_rsqf(fi);
+    }
+
     // Full model of the current set: background plane + sCMOS term, then the
     // emitters in order (Model::Model, LibRJMCMC.cpp:42-59).
     __device__ __forceinline__ void rebuild_model() {
@@ -788,7 +844,7 @@ struct Chain {
             PrI = PrI * IPDFtest / IPDF;
             delta_push(0, nn, true);
             delta_push(1, nn, false);
-        } else if (Ub >= 0.5 && Ub < 0.75) {
+        } else if (Ub >= 0.5 && Ub < (LRN && a->mode == BAMF_RJ_LEARNED ? a->l_single_from : 0.75f)) {
             jt = 3;
             int np = closest_multi(nn, 2 * p.psf_sigma);
             if (lane == 0) L.nb[np] = nn;
@@ -820,9 +876,20 @@ struct Chain {
             // from the network's log-uniform window at the emitter's pixel
             // (an independence proposal; its own Metropolis-Hastings kernel,
             // chosen independently of the state), the position walks as usual.
-            const bool indep = a->mode == BAMF_RJ_LEARNED && urand() < a->l_beta;
-            float ti, qratio = 1.0f;
-            if (indep) {
+            const bool indep = LRN && a->mode == BAMF_RJ_LEARNED && urand() < a->l_beta;
+            const bool cond = LRN && indep && a->l_ikind == 1;
+            float ti, qratio = 1.0f, cm = 0.0f, cs = 1.0f;
+            if constexpr (LRN) {
+                if (cond) {
+                    // intensity from the Gaussian fit of this emitter's conditional
+                    // likelihood at its position, the rest of the model held fixed
+                    cond_prep(T.I[nn], T.X[nn], T.Y[nn]);
+                    cond_fit(cm, cs);
+                }
+            }
+            if (cond) {
+                ti = cm + cs * randn();
+            } else if (indep) {
                 const float w = a->l_halfwidth;
                 const float mu0 = map_mu(T.X[nn], T.Y[nn]);
                 ti = expf(mu0 + w * (2.0f * urand() - 1.0f));
@@ -832,7 +899,17 @@ struct Chain {
             }
             const float tx = T.X[nn] + xs * randn();
             const float ty = T.Y[nn] + xs * randn();
-            if (indep) {
+            if (cond) {
+                // q(I | x') / q(I' | x): the fit at the new position scores the
+                // way back (the rest of the model is the same in both directions)
+                float rm = 0.0f, rs = 1.0f;
+                if constexpr (LRN) {
+                    cond_stencil(tx, ty);
+                    cond_fit(rm, rs);
+                }
+                const float zf = (ti - cm) / cs, zr = (T.I[nn] - rm) / rs;
+                qratio = (cs / rs) * expf(0.5f * (zf * zf - zr * zr));
+            } else if (indep) {
                 // q(I | x') / q(I' | x) of the log-uniform windows: I' / I when
                 // the old intensity lies in the window at the new pixel, else 0
                 const float mu1 = map_mu(tx, ty);
@@ -1199,9 +1276,13 @@ struct Chain {
         const float ur = urand();
         float pixprob = 1.0f;
         int pix;
-        const bool learned = a->mode == BAMF_RJ_LEARNED;
-        if (learned) pix = map_pixel(ur, pixprob);
-        else pix = birth_pixel(ur);
+        const bool learned = LRN && a->mode == BAMF_RJ_LEARNED;
+        if constexpr (LRN) {
+            if (learned) pix = map_pixel(ur, pixprob);
+            else pix = birth_pixel(ur);
+        } else {
+            pix = birth_pixel(ur);
+        }
         r_pix = pix;
         if (a->mode == BAMF_RJ_REPLAY) pix = ref_pix;
         const float y = (float)(pix - Y * (pix / Y)) + urand() - 0.5f;
@@ -1654,13 +1735,13 @@ struct Chain {
 // the kernarg segment; the chain reads it through that segment pointer.
 // One instantiation per ROI size class (PPL pixel slots per lane), each
 // draining its own class list through an atomic queue.
-template <int PPL, int MINB, bool GJ, bool SEG>
+template <int PPL, int MINB, int F>
 __global__ __launch_bounds__(64 * kWavesPerBlock, MINB) void k_chain(ChainArgs) {
     __shared__ WaveLds<PPL> lds[kWavesPerBlock];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wavefront
     const int lane = threadIdx.x & 63;
     const ChainArgs* ap = (const ChainArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
-    Chain<PPL, GJ, SEG> ch(ap, lds[w], lane);
+    Chain<PPL, F> ch(ap, lds[w], lane);
     const int n = *ap->list_count;
     while (true) {
         int qi = 0;
@@ -1740,22 +1821,22 @@ int launch_set_seed(uint32_t* d, uint32_t lo, uint32_t hi, hipStream_t s) {
 
 using ChainKernelFn = void (*)(ChainArgs);
 
-// Every instantiation: [class][gSplit/gMerge][segmented]; the one-launch
-// kernel without group jumps also in each register budget of kTuneMinBlocks.
+// The instantiations per class: feature sets 0 (plain), kGJ, kSEG,
+// kGJ | kSEG, kLRN and all three (learned mode with group jumps or
+// segments); the plain kernel also in each register budget of kTuneMinBlocks.
 template <int C>
-ChainKernelFn chain_kernel(bool gj, bool seg, int tune) {
+ChainKernelFn chain_kernel(int f, int tune) {
     constexpr int P = kClassPPL[C], M = kClassMinBlocks[C];
-    if (gj) return seg ? k_chain<P, M, true, true> : k_chain<P, M, true, false>;
-    if (seg) return k_chain<P, M, false, true>;
     static_assert(kTuneMinBlocks[kTuneDefault] == M, "the default budget is the class's");
-    return tune == 0 ? k_chain<P, kTuneMinBlocks[0], false, false>
-           : tune == 2 ? k_chain<P, kTuneMinBlocks[2], false, false>
-                       : k_chain<P, M, false, false>;
+    if (f & kLRN) return f == kLRN ? k_chain<P, M, kLRN> : k_chain<P, M, kGJ | kSEG | kLRN>;
+    if (f == kGJ) return k_chain<P, M, kGJ>;
+    if (f == kSEG) return k_chain<P, M, kSEG>;
+    if (f == (kGJ | kSEG)) return k_chain<P, M, kGJ | kSEG>;
+    return tune == 0 ? k_chain<P, kTuneMinBlocks[0], 0> : tune == 2 ? k_chain<P, kTuneMinBlocks[2], 0> : k_chain<P, M, 0>;
 }
 
-int launch_chain(const ChainArgs& a, int cls, bool group_jumps, bool segmented, int tune, int grid, hipStream_t s) {
-    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(group_jumps, segmented, tune)
-                                     : chain_kernel<1>(group_jumps, segmented, tune);
+int launch_chain(const ChainArgs& a, int cls, int features, int tune, int grid, hipStream_t s) {
+    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(features, tune) : chain_kernel<1>(features, tune);
     hipLaunchKernelGGL(k, dim3(grid), dim3(64 * kWavesPerBlock), 0, s, a);
     BAMF_LAUNCH_CHECK("k_chain");
     return BAMF_OK;
@@ -1767,10 +1848,9 @@ int launch_erf(const float* x, float* y, int64_t n, hipStream_t s) {
     return BAMF_OK;
 }
 
-int chain_occupancy_blocks(int cls, bool group_jumps, bool segmented, int tune, int* blocks_per_cu) {
+int chain_occupancy_blocks(int cls, int features, int tune, int* blocks_per_cu) {
     int n = 0;
-    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(group_jumps, segmented, tune)
-                                     : chain_kernel<1>(group_jumps, segmented, tune);
+    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(features, tune) : chain_kernel<1>(features, tune);
     BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(k),
                                                                64 * kWavesPerBlock, 0));
     *blocks_per_cu = n;

@@ -96,6 +96,10 @@ struct ChainArgs {
     const float* prop_mu;
     float l_alpha, l_halfwidth, l_zinv;
     float l_beta;  // probability of the independence intensity proposal in a single-emitter move
+    int32_t l_ikind;  // that proposal: 0 the network's log-uniform window, 1 the conditional-likelihood fit
+    float c_infl;     // sd inflation of the conditional fit
+    float c_mean, c_sd;  // signal prior mean and sd (the fit's fallback)
+    float l_single_from;  // learned mode: single-emitter moves take Ub >= this (the reference: 0.75)
     // outputs
     int32_t* status;
     int32_t* counters;

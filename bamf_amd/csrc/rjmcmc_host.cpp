@@ -12,8 +12,9 @@ namespace rj {
 int launch_classify(int n_roi, const int32_t* order, const int32_t* xs, const int32_t* ys, int32_t* list0,
                     int32_t* list1, int32_t* counts, uint32_t* queues, int nq, hipStream_t s);
 int launch_set_seed(uint32_t* d, uint32_t lo, uint32_t hi, hipStream_t s);
-int launch_chain(const ChainArgs& a, int cls, bool group_jumps, bool segmented, int tune, int grid, hipStream_t s);
-int chain_occupancy_blocks(int cls, bool group_jumps, bool segmented, int tune, int* blocks_per_cu);
+// features: bit 0 gSplit/gMerge, bit 1 segmented, bit 2 learned (the kernel variants of rjmcmc.hip)
+int launch_chain(const ChainArgs& a, int cls, int features, int tune, int grid, hipStream_t s);
+int chain_occupancy_blocks(int cls, int features, int tune, int* blocks_per_cu);
 int launch_erf(const float* x, float* y, int64_t n, hipStream_t s);
 }  // namespace rj
 }  // namespace bamf
@@ -105,12 +106,16 @@ struct bamf_rjmcmc_plan {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     StatDev s_off, s_sig, s_back;
     double sig_integral = 0.0;  // trapezoid integral of the signal pdf table
+    double sig_mean = 0.0, sig_sd = 0.0;  // its mean and sd (trapezoid moments of the normalised table)
+    int l_ikind = 0;            // learned mode's independence intensity proposal
+    float c_infl = 1.2f;
+    float l_single_from = 0.75f;  // learned mode's share of single-emitter moves among moves: 1 - this
     const float* d_q = nullptr;   // learned proposal maps (caller-owned)
     const float* d_mu = nullptr;
     float l_alpha = 1.0f, l_halfwidth = 0.5f, l_beta = 0.0f;
     int device = 0;
     int cu = 0;
-    int blocks_per_cu[kClasses][2][2] = {};  // [class][gSplit/gMerge compiled in][segmented]
+    int blocks_per_cu[kClasses][8] = {};  // [class][features] at the default register budget
     int blocks_per_cu_tune[kClasses][kTuneChoices] = {};  // one-launch plain kernels per register budget
     // autotuned (or set) launch configuration of the one-launch plain kernels
     int tune[kClasses] = {kTuneDefault, kTuneDefault};
@@ -168,14 +173,15 @@ int enqueue_run(bamf_rjmcmc_plan* plan, const ChainArgs& a, int nseg, bool segme
     // phases (LibRJMCMC.cpp:705-710, 737-742): reachable only when its
     // entries 5 or 6 are nonzero
     const bool gj = a.p.p_burnin[5] != 0.0f || a.p.p_burnin[6] != 0.0f;
+    const int features = (gj ? 1 : 0) | (segmented ? 2 : 0) | (a.mode == BAMF_RJ_LEARNED ? 4 : 0);
     if (concurrent) {
         BAMF_HIP_TRY(hipEventRecord(plan->ev_fork, s));
         BAMF_HIP_TRY(hipStreamWaitEvent(plan->side, plan->ev_fork, 0));
     }
     for (int c = kClasses - 1; c >= 0; c--) {
-        const bool tuned = !gj && !segmented;  // the register budget is tunable for these kernels only
+        const bool tuned = features == 0;  // the register budget is tunable for the plain kernel only
         const int t = tuned ? tune[c] : kTuneDefault;
-        const int bpc = tuned ? plan->blocks_per_cu_tune[c][t] : plan->blocks_per_cu[c][gj ? 1 : 0][segmented ? 1 : 0];
+        const int bpc = tuned ? plan->blocks_per_cu_tune[c][t] : plan->blocks_per_cu[c][features];
         int grid = (bpc > 0 ? bpc : 1) * cu;
         if (grid > need) grid = need;
         for (int k = 0; k < nseg; k++) {
@@ -185,7 +191,7 @@ int enqueue_run(bamf_rjmcmc_plan* plan, const ChainArgs& a, int nseg, bool segme
             ac.queue = plan->d_queue + 2 * k + c;
             ac.seg_begin = (int)((int64_t)total * k / nseg);
             ac.seg_end = k == nseg - 1 ? total : (int)((int64_t)total * (k + 1) / nseg);
-            rc = launch_chain(ac, c, gj, segmented, t, grid, (c == kClasses - 1 || !concurrent) ? s : plan->side);
+            rc = launch_chain(ac, c, features, t, grid, (c == kClasses - 1 || !concurrent) ? s : plan->side);
             if (rc != BAMF_OK) return rc;
         }
     }
@@ -237,9 +243,17 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
     pl->off.build(h_off_pdf, off_len, params->dx);
     pl->sig.build(h_sig_pdf, sig_len, params->dx);
     pl->back.build(h_back_pdf, back_len, params->dx);
-    for (size_t i = 0; i + 1 < pl->sig.x.size(); i++)
-        pl->sig_integral += 0.5 * ((double)pl->sig.pdf[i] + (double)pl->sig.pdf[i + 1]) *
-                            ((double)pl->sig.x[i + 1] - (double)pl->sig.x[i]);
+    double m1 = 0.0, m2 = 0.0;
+    for (size_t i = 0; i + 1 < pl->sig.x.size(); i++) {
+        const double x0 = pl->sig.x[i], x1 = pl->sig.x[i + 1], f0 = pl->sig.pdf[i], f1 = pl->sig.pdf[i + 1];
+        pl->sig_integral += 0.5 * (f0 + f1) * (x1 - x0);
+        m1 += 0.5 * (f0 * x0 + f1 * x1) * (x1 - x0);
+        m2 += 0.5 * (f0 * x0 * x0 + f1 * x1 * x1) * (x1 - x0);
+    }
+    if (pl->sig_integral > 0.0) {
+        pl->sig_mean = m1 / pl->sig_integral;
+        pl->sig_sd = std::sqrt(std::max(m2 / pl->sig_integral - pl->sig_mean * pl->sig_mean, 1.0));
+    }
     std::vector<float> flat;
     auto put = [&](const std::vector<float>& v) {
         size_t at = flat.size();
@@ -283,16 +297,14 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, pl->device) == hipSuccess) pl->cu = prop.multiProcessorCount;
     for (int c = 0; c < kClasses; c++)
-        for (int g = 0; g < 2; g++)
-            for (int sg = 0; sg < 2; sg++) {
-                int bpc = 0;
-                if (chain_occupancy_blocks(c, g == 1, sg == 1, kTuneDefault, &bpc) == BAMF_OK)
-                    pl->blocks_per_cu[c][g][sg] = bpc;
-            }
+        for (int f = 0; f < 8; f++) {
+            int bpc = 0;
+            if (chain_occupancy_blocks(c, f, kTuneDefault, &bpc) == BAMF_OK) pl->blocks_per_cu[c][f] = bpc;
+        }
     for (int c = 0; c < kClasses; c++)
         for (int t = 0; t < kTuneChoices; t++) {
             int bpc = 0;
-            if (chain_occupancy_blocks(c, false, false, t, &bpc) == BAMF_OK) pl->blocks_per_cu_tune[c][t] = bpc;
+            if (chain_occupancy_blocks(c, 0, t, &bpc) == BAMF_OK) pl->blocks_per_cu_tune[c][t] = bpc;
         }
     *plan = pl;
     return BAMF_OK;
@@ -325,6 +337,18 @@ int bamf_rjmcmc_set_proposal(bamf_rjmcmc_plan* plan, const float* d_q, const flo
     plan->l_alpha = alpha;
     plan->l_halfwidth = halfwidth;
     plan->l_beta = beta;
+    return BAMF_OK;
+}
+
+int bamf_rjmcmc_set_intensity_proposal(bamf_rjmcmc_plan* plan, int kind, float inflation, float single_share) {
+    BAMF_CHECK_ARG(plan, "null argument");
+    BAMF_CHECK_ARG(kind == 0 || kind == 1, "kind must be 0 (network window) or 1 (conditional fit)");
+    BAMF_CHECK_ARG(inflation > 0.0f, "inflation must be > 0");
+    BAMF_CHECK_ARG(single_share >= 0.25f && single_share <= 0.5f, "single_share must be in [0.25, 0.5]");
+    plan->l_ikind = kind;
+    plan->c_infl = inflation;
+    plan->l_single_from = 1.0f - single_share;
+    drop_graph(plan);
     return BAMF_OK;
 }
 
@@ -429,6 +453,11 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
         a.l_halfwidth = plan->l_halfwidth;
         a.l_beta = plan->l_beta;
         a.l_zinv = (float)(1.0 / plan->sig_integral);
+        a.l_ikind = plan->l_ikind;
+        a.c_infl = plan->c_infl;
+        a.c_mean = (float)plan->sig_mean;
+        a.c_sd = (float)plan->sig_sd;
+        a.l_single_from = plan->l_single_from;
     }
     a.recompute_every = 128;
     hipStream_t s = as_stream(stream);

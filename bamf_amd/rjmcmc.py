@@ -89,6 +89,15 @@ class Plan:
         check(lib().bamf_rjmcmc_get_tuning(self.handle, ctypes.byref(t)), "bamf_rjmcmc_get_tuning")
         return {"chains_per_simd": list(t.chains_per_simd), "concurrent": bool(t.concurrent)}
 
+    def set_intensity_proposal(self, kind: str = "window", inflation: float = 1.2, single_share: float = 0.25) -> None:
+        """Learned mode's independence intensity proposal: "window" (the network's log-uniform
+        window) or "fit" (Gaussian fit of the emitter's conditional likelihood); single_share is
+        the share of moves that are single-emitter moves (0.25 as the reference, up to 0.5)."""
+        k = {"window": 0, "fit": 1}[kind]
+        check(lib().bamf_rjmcmc_set_intensity_proposal(self.handle, k, ctypes.c_float(inflation),
+                                                       ctypes.c_float(single_share)),
+              "bamf_rjmcmc_set_intensity_proposal")
+
     def set_segments(self, n_segments: int, graph: bool = False) -> None:
         """Run every chain as n_segments consecutive launches (optionally replayed from a HIP graph)."""
         check(lib().bamf_rjmcmc_set_segments(self.handle, int(n_segments), 1 if graph else 0),

@@ -433,6 +433,9 @@ def main():
         mm = proposal.MapMaker(prm)
         dq, dmu = _hip.DeviceBuffer(rs.total_pixels * 4), _hip.DeviceBuffer(rs.total_pixels * 4)
         plan.set_proposal(dq, dmu, alpha=0.2, halfwidth=0.35, beta=1.0)
+        # intensities of births and single-emitter moves from the conditional-likelihood fit,
+        # half of all moves single-emitter moves
+        plan.set_intensity_proposal("fit", 1.3, 0.5)
         em0, em1 = _hip.Event(), _hip.Event()
         em0.record(stream)
         mm.run(rs, dq, dmu, stream=stream)

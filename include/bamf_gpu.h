@@ -287,6 +287,17 @@ int bamf_rjmcmc_get_tuning(const bamf_rjmcmc_plan* plan, bamf_rjmcmc_tuning* t);
  * must stay valid while runs use them. */
 int bamf_rjmcmc_set_proposal(bamf_rjmcmc_plan* plan, const float* d_q, const float* d_mu, float alpha,
                              float halfwidth, float beta);
+/* The independence intensity proposal of BAMF_RJ_LEARNED single-emitter
+ * moves (taken with probability beta of bamf_rjmcmc_set_proposal; no
+ * counterpart in the reference): kind 0 draws log I uniformly in the
+ * network's window at the emitter's pixel; kind 1 draws I from a Gaussian
+ * fitted to the emitter's conditional likelihood (the rest of the model held
+ * fixed: weighted least squares, one Fisher-scoring step, sd multiplied by
+ * `inflation`), scored in the Metropolis-Hastings ratio by the same fit at
+ * the proposed position.  single_share: the share of moves that are
+ * single-emitter moves (the reference's 0.25, LibRJMCMC.cpp:882; up to 0.5,
+ * taken from the group random walk, jump type 3). */
+int bamf_rjmcmc_set_intensity_proposal(bamf_rjmcmc_plan* plan, int kind, float inflation, float single_share);
 /* The work-queue counters are reset by bamf_rjmcmc_run itself; this returns
  * how many chains the last run on the plan claimed (synchronous). */
 int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed);
