@@ -267,6 +267,39 @@ def fetch(d, n, total, hist_len, final_cap, trace_len) -> BatchResult:
     return res
 
 
+def find_map(outs: dict, n_roi: int, fb: int = 1, cap: int = 128, stream=None) -> dict:
+    """findMap (@RJ/findMap.m) for every ROI of a run whose outputs (from
+    alloc_outputs with chain_caps) are still on the device.  Returns
+    MostFrequent, the background terms and per-cluster X, Y, Photons means
+    and standard deviations (SMD.X .. SMD.I_SE)."""
+    if "chain_caps" not in outs:
+        raise BamfError("find_map needs the run's chain records (chain_caps)")
+    rc, ec = outs["chain_caps"]
+    B = _hip.DeviceBuffer
+    st, nm, bg, em = B(4 * n_roi), B(4 * n_roi), B(16 * n_roi), B(24 * n_roi * cap)
+    check(lib().bamf_find_map(outs["acc_rec"].vp, outs["acc_em"].vp, outs["rec_count"].vp, rc, ec, int(n_roi),
+                              int(fb), int(cap), st.vp, nm.vp, bg.vp, em.vp, stream.vp if stream else None),
+          "bamf_find_map")
+    _hip.synchronize()
+    return {"status": st.download(np.int32), "n": nm.download(np.int32),
+            "bg": bg.download(np.float32).reshape(n_roi, 4), "em": em.download(np.float32).reshape(n_roi, cap, 6)}
+
+
+def chain_summary(outs: dict, n_roi: int, cap: int = 128, stream=None) -> dict:
+    """The refinement summary of mcmc.m:144-166 (fixed-dimension chains): per
+    emitter index, means and standard deviations of X, Y and Photons."""
+    if "chain_caps" not in outs:
+        raise BamfError("chain_summary needs the run's chain records (chain_caps)")
+    rc, ec = outs["chain_caps"]
+    B = _hip.DeviceBuffer
+    st, nn, em = B(4 * n_roi), B(4 * n_roi), B(24 * n_roi * cap)
+    check(lib().bamf_chain_summary(outs["acc_rec"].vp, outs["acc_em"].vp, outs["rec_count"].vp, rc, ec, int(n_roi),
+                                   int(cap), st.vp, nn.vp, em.vp, stream.vp if stream else None), "bamf_chain_summary")
+    _hip.synchronize()
+    return {"status": st.download(np.int32), "n": nn.download(np.int32),
+            "em": em.download(np.float32).reshape(n_roi, cap, 6)}
+
+
 def make_replay(tapes: list[np.ndarray], ref_steps: np.ndarray) -> dict:
     """Device replay input from per-ROI raw draw tapes and the recorded step records."""
     lens = np.array([len(t) for t in tapes], np.int64)

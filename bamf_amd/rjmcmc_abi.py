@@ -75,6 +75,10 @@ def declare(L: ctypes.CDLL) -> None:
     L.bamf_find_roi_host.argtypes = [c_void_p, c_int, c_int, c_int, c_float, c_float, c_int, c_float, c_int64,
                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.bamf_rjmcmc_set_segments.argtypes = [c_void_p, c_int, c_int]
+    L.bamf_find_map.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int, c_int, c_int, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p]
+    L.bamf_chain_summary.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int, c_int, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]
     L.bamf_rjmcmc_set_intensity_proposal.argtypes = [c_void_p, c_int, c_float, c_float]
     L.bamf_rjmcmc_autotune.argtypes = [c_void_p, c_void_p, c_int, POINTER(Tuning), c_void_p]
     L.bamf_rjmcmc_set_tuning.argtypes = [c_void_p, POINTER(Tuning)]

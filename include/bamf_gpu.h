@@ -340,6 +340,40 @@ int bamf_net_train_step(bamf_net* net, const float* d_images, const float* d_occ
 int bamf_net_proposal_maps(bamf_net* net, const bamf_roi_batch* batch, float eps, float* d_q, float* d_mu,
                            void* stream);
 
+/* ------------------------------------------------------------------------
+ * Post-passes over the Chain records of a run (bamf_rjmcmc_out d_acc_rec,
+ * d_acc_em, d_rec_count with their capacities); one call for all ROIs.
+ * Chain(nn), nn = 1..NLoop, is record nn - 1 for nn < NLoop, and Chain(NLoop)
+ * the empty entry cRJMCMC leaves (cRJMCMC.cpp:159-174).
+ * ------------------------------------------------------------------------ */
+/* findMap(Chain, FB) (@RJ/findMap.m:1-112) per ROI: over nn = round(NLoop/2)..NLoop
+ * the most frequent number of emitters with Signal == fb (the first of equal
+ * maxima; the empty entry and N = 0 states count 0), then k-means of those
+ * emitters' positions over the states with that number, and the background
+ * terms over the same states.
+ *   d_n_map  [n_roi]           MostFrequent;
+ *   d_bg     [n_roi][4]        mean(BG), std(BG), mean(ABG), mean(BBG);
+ *   d_em     [n_roi][cap][6]   per cluster: X, Y, Photons means, then their
+ *                              standard deviations (SMD.X .. SMD.I_SE);
+ *   d_status [n_roi]           0, or 1 when records exceeded the capacities or
+ *                              MostFrequent > cap (clusters truncated to cap).
+ * k-means (kmeans(Points, MostFrequent), findMap.m:79) is seeded with the
+ * positions of the last state in the range with that number (the reference's
+ * kmeans seeds at random), then Lloyd iterations (at most 100) until no
+ * centre moves; an empty cluster keeps its centre. */
+int bamf_find_map(const bamf_chain_rec* d_acc_rec, const float* d_acc_em, const int64_t* d_rec_count,
+                  int32_t rec_cap, int32_t em_cap, int n_roi, int fb, int cap, int32_t* d_status, int32_t* d_n_map,
+                  float* d_bg, float* d_em, void* stream);
+/* The refinement summary of mcmc.m:144-166 (a fixed-dimension chain, P =
+ * [1 0 0 0 0 0 0 0], mcmc.m:103): over nn = round(NLoop/2)..NLoop, per
+ * emitter index the mean and standard deviation of X, Y and Photons
+ * (FoundMAP.X .. FoundMAP.I_SE).  d_n [n_roi]: emitters summarised;
+ * d_status 2 when the states of the range differ in N (the reference's
+ * cat(1, X, x) fails then), 1 when records exceeded the capacities. */
+int bamf_chain_summary(const bamf_chain_rec* d_acc_rec, const float* d_acc_em, const int64_t* d_rec_count,
+                       int32_t rec_cap, int32_t em_cap, int n_roi, int cap, int32_t* d_status, int32_t* d_n,
+                       float* d_em, void* stream);
+
 /* cRJMCMC on host arrays: the argument and output layout of the mex entry
  * (source/c/cRJMCMC.cpp:52-305) for one ROI, run on the device.
  *   mcmc                the MCMC struct (make_Params fields);
