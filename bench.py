@@ -296,8 +296,62 @@ def bench_gauss_mle(args):
             "cpu_baseline": cpu}
 
 
+def bench_find_map(args):
+    """findMap over the Chain records of the 256-ROI 16 x 16 workload (3000 + 3000 jumps)."""
+    from bamf_amd import _hip
+    from bamf_amd.rjmcmc import Plan, RoiSet, alloc_outputs, fetch, run_batch
+    from bamf_amd.rjmcmc_abi import CHAIN_REC_DTYPE
+    from bamf_amd._lib import check, lib
+
+    rois, nb, nt, rho, sig, off, back, bg0 = workload("plain_16")
+    p = params(nb, nt, rho)
+    n = len(rois["side"])
+    rs = RoiSet.from_synth(rois, bg0=np.full(n, bg0, np.float32))
+    caps = (nb + nt + 1, 16 * (nb + nt + 1))
+    outs = alloc_outputs(n, nb + nt, 64, 128, 0, False, False, rs, p.grid_zoom, caps)
+    run_batch(Plan(p, off, sig, back), rs, seed=SEED, outs=outs, sync=False)
+    res = fetch(outs, n, nb + nt, 64, 128, 0)
+    assert np.all(res.status == 0)
+    ch = res.chain
+    # algorithmic bytes: the second half's records and emitter entries, read once
+    nloop = ch["count"][:, 0]
+    alg_bytes = 0
+    for r in range(n):
+        lo = (int(nloop[r]) + 1) // 2
+        recs = ch["acc_rec"][r, lo - 1:int(nloop[r]) - 1]
+        alg_bytes += len(recs) * CHAIN_REC_DTYPE.itemsize + int(np.maximum(recs["n"], 1).sum()) * 16
+    cpu = None
+    if not args.no_cpu:
+        from oracle import findmap as fm
+
+        sample = min(n, 64)
+        t0 = time.perf_counter()
+        for r in range(sample):
+            fm.find_map(ch["acc_rec"][r], ch["acc_em"][r], int(nloop[r]), fb=1)
+        dt = time.perf_counter() - t0
+        cpu = {"value": sample / dt, "unit": "ROIs/s", "cores": 1, "kind": "port",
+               "sample": f"first {sample} ROIs, oracle findMap restatement (numpy), 1 core, {dt:.1f} s"}
+    B = _hip.DeviceBuffer
+    st, nm, bg, em = B(4 * n), B(4 * n), B(16 * n), B(24 * n * 128)
+    s = _hip.Stream()
+
+    def call():
+        check(lib().bamf_find_map(outs["acc_rec"].vp, outs["acc_em"].vp, outs["rec_count"].vp, caps[0], caps[1], n,
+                                  1, 128, st.vp, nm.vp, bg.vp, em.vp, s.vp), "bamf_find_map")
+
+    ms = _time_gpu(call, s, args.warmup, args.steps)
+    return {"metric": "findMap ROIs/s (Chain records of 256 ROIs of 16 x 16, 3000 + 3000 jumps)", "value": n / (ms / 1e3),
+            "unit": "ROIs/s", "ms_per_step": ms, "dtype": "f32 (k-means sums fixed point, statistics f64)",
+            "higher_is_better": True,
+            "config": {"workload": "find_map", "n_roi": n, "records": int(nloop.sum()),
+                       "mean_map_n": float(nm.download(np.int32).mean())},
+            "roofline": {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": None,
+                         "kernel": "k_find_map", "traffic": None, "alg_bytes_per_launch": float(alg_bytes)},
+            "cpu_baseline": cpu}
+
+
 def front_end_line(args):
-    res = bench_find_roi(args) if args.workload == "find_roi" else bench_gauss_mle(args)
+    res = {"find_roi": bench_find_roi, "gauss_mle": bench_gauss_mle, "find_map": bench_find_map}[args.workload](args)
     print(f"[bench] {args.workload}: {res['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     if res["roofline"]["bound"] == "hbm":
         r = res["roofline"]
@@ -317,7 +371,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batched_uneven",
-                    choices=["batched_uneven", "plain_16", "find_roi", "gauss_mle"])
+                    choices=["batched_uneven", "plain_16", "find_roi", "gauss_mle", "find_map"])
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--cpu-sample-frames", type=int, default=2000)
     ap.add_argument("--cpu-sample-fits", type=int, default=100000)
@@ -338,7 +392,7 @@ def main():
 
     if args.copy_mib:
         return copy_calibration(args.copy_mib)
-    if args.workload in ("find_roi", "gauss_mle"):
+    if args.workload in ("find_roi", "gauss_mle", "find_map"):
         return front_end_line(args)
     rois, nb, nt, rho, sig, off, back, bg0 = workload(args.workload)
     p = params(nb, nt, rho)
