@@ -166,78 +166,128 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ in, 
         }
 }
 
-// Weight gradient of a 3x3 layer: dW[cout][k] += sum over this block's
-// pixels of dZ[p][cout] im2col(A)[p][k], plus the bias gradient.  Block =
-// one wavefront per (M tile, N tile) pair of dW; blockIdx.y picks a pixel
-// chunk.  CIN = 1 uses K = 9 (one N tile, taps 0..8).
+// Weight and bias gradients of a 3 x 3 convolution, dW_tap = dZ^T A_tap with
+// the pixels as K: one work-group of four wavefronts per image (grid-strided
+// over the batch, each wavefront accumulating its output tiles in registers
+// across its images), the image's dZ transposed to [channel][pixel] and its
+// input zero-padded by one pixel in LDS, so every MFMA operand is an LDS read;
+// one atomic add per output element per work-group at the end.
+//   CIN = 32: 36 tiles (9 taps x 2 x 2 of 16 x 16), nine per wavefront;
+//   CIN = 1:  2 tiles (output channels x the 9 taps padded to 16).
 template <int CIN>
-__global__ __launch_bounds__(64) void k_conv3_wgrad(const uint16_t* __restrict__ dz, const uint16_t* __restrict__ in,
-                                                   float* __restrict__ dw, float* __restrict__ db, int64_t npix,
-                                                   int64_t chunk, Geo g) {
-    constexpr int KW = CIN == 1 ? 16 : kK;  // columns of dW handled (9 used for CIN = 1)
-    const int lane = threadIdx.x;
-    const int ntiles = KW / 16;
-    const int mt = blockIdx.x / ntiles, nt = blockIdx.x % ntiles;
+__global__ __launch_bounds__(256) void k_conv3_wgrad(const uint16_t* __restrict__ dz, const uint16_t* __restrict__ in,
+                                                    float* __restrict__ dw, float* __restrict__ db, int batch, Geo g) {
+    constexpr int TPW = CIN == 1 ? 1 : 9;  // tiles per wavefront
+    extern __shared__ uint16_t smem[];
+    const int P = g.H * g.W, Pp = (P + 31) & ~31, W2 = g.W + 2, HW2 = (g.H + 2) * W2;
+    uint16_t* dzT = smem;                                         // [32][Pp]
+    uint16_t* halo = dzT + 32 * Pp;                               // [CIN][HW2]
+    int32_t* hidx = reinterpret_cast<int32_t*>(halo + ((CIN * HW2 + 1) & ~1));  // [Pp]
+    __shared__ float bsum[32];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int row = lane & 15, grp = lane >> 4;
-    const int co = 16 * mt + row;          // A row: output channel
-    const int kk = 16 * nt + row;          // B column: im2col index
-    const int tap = CIN == 1 ? kk : kk / kC;
-    const int ci = CIN == 1 ? 0 : kk % kC;
-    const bool kvalid = CIN == 1 ? kk < 9 : true;
-    const int dh = tap / 3 - 1, dw_ = tap % 3 - 1;
-    const int hw = g.H * g.W;
-    const int64_t p0 = (int64_t)blockIdx.y * chunk;
-    const int64_t p1 = p0 + chunk < npix ? p0 + chunk : npix;
-    v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
-    float bsum = 0.f;
-    for (int64_t base = p0; base < p1; base += 32) {
-        uint16_t av[8], bv[8];
+    const bool active = CIN == 32 || wv < 2;
+    v4f acc[TPW];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int64_t p = base + 8 * grp + j;
-            uint16_t x = 0, y = 0;
-            if (p < p1) {
-                x = dz[p * kC + co];
-                if (kvalid) {
-                    const int b = (int)(p / hw);
-                    const int r = (int)(p - (int64_t)b * hw);
-                    const int h = r / g.W, w = r - (r / g.W) * g.W;
-                    int hv, wv;
-                    valid_hw(g, b, hv, wv);
-                    const int hh = h + dh, ww = w + dw_;
-                    if (hh >= 0 && hh < hv && ww >= 0 && ww < wv)
-                        y = CIN == 1 ? in[(int64_t)b * hw + hh * g.W + ww]
-                                     : in[((int64_t)b * hw + hh * g.W + ww) * kC + ci];
+    for (int i = 0; i < TPW; i++) acc[i] = v4f{0.f, 0.f, 0.f, 0.f};
+    if (tid < 32) bsum[tid] = 0.f;
+    __syncthreads();
+    float bpart[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // this thread's eight dZ channels
+    const int c8 = tid & 3;
+    for (int b = blockIdx.x; b < batch; b += gridDim.x) {
+        int hv, wvv;
+        valid_hw(g, b, hv, wvv);
+        const int64_t pb = (int64_t)b * P;
+        // stage dZ transposed (zeros past P), the padded input and the pixel -> halo index table
+        for (int idx = tid; idx < Pp * 4; idx += 256) {
+            const int p = idx >> 2;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (p < P) v = *reinterpret_cast<const uint4*>(dz + (pb + p) * kC + 8 * c8);
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const uint16_t h16 = (uint16_t)(w4[j >> 1] >> (16 * (j & 1)));
+                dzT[(8 * c8 + j) * Pp + p] = h16;
+                bpart[j] += bf2f(h16);
+            }
+        }
+        for (int idx = tid; idx < CIN * HW2; idx += 256) halo[idx] = 0;
+        __syncthreads();
+        for (int idx = tid; idx < Pp; idx += 256) {
+            const int h = idx / g.W, w = idx - h * g.W;
+            // padding pixels (dZ = 0): an interior cell, so every tap offset stays inside the halo
+            hidx[idx] = idx < P ? (h + 1) * W2 + (w + 1) : W2 + 1;
+        }
+        if constexpr (CIN == 1) {
+            for (int p = tid; p < P; p += 256) {
+                const int h = p / g.W, w = p - h * g.W;
+                if (h < hv && w < wvv) halo[(h + 1) * W2 + (w + 1)] = in[pb + p];
+            }
+        } else {
+            for (int idx = tid; idx < P * 4; idx += 256) {
+                const int p = idx >> 2, h = p / g.W, w = p - h * g.W;
+                if (h < hv && w < wvv) {
+                    const uint4 v = *reinterpret_cast<const uint4*>(in + (pb + p) * kC + 8 * c8);
+                    const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                    for (int j = 0; j < 8; j++)
+                        halo[(8 * c8 + j) * HW2 + (h + 1) * W2 + (w + 1)] = (uint16_t)(w4[j >> 1] >> (16 * (j & 1)));
                 }
             }
-            av[j] = x;
-            bv[j] = y;
-            bsum += bf2f(x);
         }
-        uint4 a, bb;
-        a.x = av[0] | ((uint32_t)av[1] << 16);
-        a.y = av[2] | ((uint32_t)av[3] << 16);
-        a.z = av[4] | ((uint32_t)av[5] << 16);
-        a.w = av[6] | ((uint32_t)av[7] << 16);
-        bb.x = bv[0] | ((uint32_t)bv[1] << 16);
-        bb.y = bv[2] | ((uint32_t)bv[3] << 16);
-        bb.z = bv[4] | ((uint32_t)bv[5] << 16);
-        bb.w = bv[6] | ((uint32_t)bv[7] << 16);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_v8bf(a), as_v8bf(bb), acc, 0, 0, 0);
-    }
-    // D rows 4 grp + i (output channels 16 mt + ...), column row (im2col 16 nt + row)
-    const int kw = CIN == 1 ? 9 : kK;
+        __syncthreads();
+        if (active) {
+            for (int k0 = 0; k0 < Pp; k0 += 32) {
+                const int p8 = k0 + 8 * grp;
+                int hx[8];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int o = 16 * mt + 4 * grp + i;
-        if (kk < kw) atomicAdd(&dw[(size_t)o * kw + kk], acc[i]);
+                for (int j = 0; j < 8; j++) hx[j] = hidx[p8 + j];
+#pragma unroll
+                for (int i = 0; i < TPW; i++) {
+                    const int t = wv * TPW + i;
+                    // CIN = 32: tile t = (tap, mt, nt); CIN = 1: tile = mt, the N columns are the taps
+                    const int tap = CIN == 1 ? row : t >> 2;
+                    const int mt = CIN == 1 ? t : (t >> 1) & 1, nt = CIN == 1 ? 0 : t & 1;
+                    const uint4 av = *reinterpret_cast<const uint4*>(dzT + (16 * mt + row) * Pp + p8);
+                    uint16_t bv[8];
+                    const bool tv = CIN == 32 || tap < 9;
+                    const int off = tv ? (tap / 3 - 1) * W2 + (tap % 3 - 1) : 0;
+                    const uint16_t* src = halo + (CIN == 1 ? 0 : (16 * nt + row) * HW2) + off;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) bv[j] = tv ? src[hx[j]] : (uint16_t)0;
+                    uint4 bb;
+                    bb.x = bv[0] | ((uint32_t)bv[1] << 16);
+                    bb.y = bv[2] | ((uint32_t)bv[3] << 16);
+                    bb.z = bv[4] | ((uint32_t)bv[5] << 16);
+                    bb.w = bv[6] | ((uint32_t)bv[7] << 16);
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_v8bf(av), as_v8bf(bb), acc[i], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();  // the buffers are restaged for the next image
     }
-    if (nt == 0) {
-        // bias: lanes of one output channel (row) across the four groups
-        bsum += __shfl_xor(bsum, 16);
-        bsum += __shfl_xor(bsum, 32);
-        if (grp == 0) atomicAdd(&db[co], bsum);
+    // D rows 4 grp + r (output channels), column row (input channel, or tap for CIN = 1)
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < TPW; i++) {
+            const int t = wv * TPW + i;
+            const int tap = CIN == 1 ? row : t >> 2;
+            const int mt = CIN == 1 ? t : (t >> 1) & 1, nt = CIN == 1 ? 0 : t & 1;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int o = 16 * mt + 4 * grp + r;
+                if (CIN == 1) {
+                    if (tap < 9) atomicAdd(&dw[(size_t)o * 9 + tap], acc[i][r]);
+                } else {
+                    atomicAdd(&dw[(size_t)o * kK + tap * kC + 16 * nt + row], acc[i][r]);
+                }
+            }
+        }
     }
+#pragma unroll
+    for (int j = 0; j < 8; j++) atomicAdd(&bsum[8 * c8 + j], bpart[j]);
+    __syncthreads();
+    if (tid < 32) atomicAdd(&db[tid], bsum[tid]);
 }
 
 // Input normalisation: one work-group per image: x = (sqrt(D + 3/8) - mean) / 2
@@ -278,52 +328,71 @@ __global__ __launch_bounds__(256) void k_head(const uint16_t* __restrict__ a4, c
                                               float* __restrict__ dwh, float* __restrict__ dbh,
                                               float* __restrict__ loss, float* __restrict__ outp, int64_t npix,
                                               Geo g) {
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int lane = threadIdx.x & 63;
+    __shared__ float red[4][2 * kC + 4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int hw = g.H * g.W;
-    bool ok = p < npix;
-    if (ok) {
-        const int b = (int)(p / hw);
-        const int r = (int)(p - (int64_t)b * hw);
-        int hv, wv;
-        valid_hw(g, b, hv, wv);
-        ok = (r / g.W) < hv && (r - (r / g.W) * g.W) < wv;
-    }
-    float a[kC];
-#pragma unroll
-    for (int c = 0; c < kC; c++) a[c] = ok ? bf2f(a4[p * kC + c]) : 0.f;
-    float h0 = bh[0], h1 = bh[1];
-#pragma unroll
-    for (int c = 0; c < kC; c++) {
-        h0 += wh[c] * a[c];
-        h1 += wh[kC + c] * a[c];
-    }
-    if (!t_occ) {
-        if (p < npix) {
-            outp[2 * p] = ok ? h0 : -INFINITY;
-            outp[2 * p + 1] = h1;
-        }
-        return;
-    }
-    float g0 = 0.f, g1 = 0.f, l0 = 0.f, l1 = 0.f;
-    if (ok) {
-        const float t = t_occ[p];
-        const float s = 1.f / (1.f + expf(-h0));
-        l0 = fmaxf(h0, 0.f) - h0 * t + log1pf(expf(-fabsf(h0)));
-        g0 = (s - t) * inv_n;
-        const float e = h1 - t_li[p];
-        l1 = t * e * e;
-        g1 = 2.f * t * e * inv_n;
-    }
     float gw0[kC], gw1[kC];
 #pragma unroll
-    for (int c = 0; c < kC; c++) {
-        const float d = (wh[c] * g0 + wh[kC + c] * g1);
-        if (p < npix) dz4[p * kC + c] = f2bf(a[c] > 0.f ? d : 0.f);
-        gw0[c] = a[c] * g0;
-        gw1[c] = a[c] * g1;
+    for (int c = 0; c < kC; c++) gw0[c] = gw1[c] = 0.f;
+    float s0 = 0.f, s1 = 0.f, q0 = 0.f, q1 = 0.f;
+    // grid-strided over pixels; a pixel's 32 activations and 32 gradients as
+    // four 16-byte loads and stores
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < npix; p += (int64_t)gridDim.x * 256) {
+        const int b = (int)(p / hw);
+        const int r = (int)(p - (int64_t)b * hw);
+        int hv, wv2;
+        valid_hw(g, b, hv, wv2);
+        const bool ok = (r / g.W) < hv && (r - (r / g.W) * g.W) < wv2;
+        float a[kC];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 v = ok ? *reinterpret_cast<const uint4*>(a4 + p * kC + 8 * q) : make_uint4(0u, 0u, 0u, 0u);
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 8; j++) a[8 * q + j] = bf2f((uint16_t)(w4[j >> 1] >> (16 * (j & 1))));
+        }
+        float h0 = bh[0], h1 = bh[1];
+#pragma unroll
+        for (int c = 0; c < kC; c++) {
+            h0 += wh[c] * a[c];
+            h1 += wh[kC + c] * a[c];
+        }
+        if (!t_occ) {
+            outp[2 * p] = ok ? h0 : -INFINITY;
+            outp[2 * p + 1] = h1;
+            continue;
+        }
+        float g0 = 0.f, g1 = 0.f;
+        if (ok) {
+            const float t = t_occ[p];
+            const float sg = 1.f / (1.f + expf(-h0));
+            q0 += (fmaxf(h0, 0.f) - h0 * t + log1pf(expf(-fabsf(h0)))) * inv_n;
+            g0 = (sg - t) * inv_n;
+            const float e = h1 - t_li[p];
+            q1 += t * e * e * inv_n;
+            g1 = 2.f * t * e * inv_n;
+        }
+        s0 += g0;
+        s1 += g1;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint32_t w4[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int c0 = 8 * q + 2 * j, c1 = c0 + 1;
+                const float d0 = wh[c0] * g0 + wh[kC + c0] * g1, d1 = wh[c1] * g0 + wh[kC + c1] * g1;
+                w4[j] = (uint32_t)f2bf(a[c0] > 0.f ? d0 : 0.f) | ((uint32_t)f2bf(a[c1] > 0.f ? d1 : 0.f) << 16);
+            }
+            *reinterpret_cast<uint4*>(dz4 + p * kC + 8 * q) = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        }
+#pragma unroll
+        for (int c = 0; c < kC; c++) {
+            gw0[c] += a[c] * g0;
+            gw1[c] += a[c] * g1;
+        }
     }
-    // wavefront sums, then one atomic per wave and entry
+    if (!t_occ) return;
+    // block sums (wavefront shuffles, then the four wavefronts through LDS), one atomic per block and entry
 #pragma unroll
     for (int c = 0; c < kC; c++) {
         float x = gw0[c], y = gw1[c];
@@ -332,11 +401,10 @@ __global__ __launch_bounds__(256) void k_head(const uint16_t* __restrict__ a4, c
             y += __shfl_xor(y, o);
         }
         if (lane == 0) {
-            atomicAdd(&dwh[c], x);
-            atomicAdd(&dwh[kC + c], y);
+            red[wv][c] = x;
+            red[wv][kC + c] = y;
         }
     }
-    float s0 = g0, s1 = g1, q0 = l0 * inv_n, q1 = l1 * inv_n;
     for (int o = 32; o > 0; o >>= 1) {
         s0 += __shfl_xor(s0, o);
         s1 += __shfl_xor(s1, o);
@@ -344,10 +412,18 @@ __global__ __launch_bounds__(256) void k_head(const uint16_t* __restrict__ a4, c
         q1 += __shfl_xor(q1, o);
     }
     if (lane == 0) {
-        atomicAdd(&dbh[0], s0);
-        atomicAdd(&dbh[1], s1);
-        atomicAdd(&loss[0], q0);
-        atomicAdd(&loss[1], q1);
+        red[wv][2 * kC + 0] = s0;
+        red[wv][2 * kC + 1] = s1;
+        red[wv][2 * kC + 2] = q0;
+        red[wv][2 * kC + 3] = q1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kC + 4) {
+        const int e = threadIdx.x;
+        const float v = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+        if (e < 2 * kC) atomicAdd(&dwh[e], v);
+        else if (e < 2 * kC + 2) atomicAdd(&dbh[e - 2 * kC], v);
+        else atomicAdd(&loss[e - 2 * kC - 2], v);
     }
 }
 
@@ -523,25 +599,35 @@ int loss_and_grad(bamf_net* n, int batch, const float* d_occ, const float* d_li,
     int rc = forward_convs(n, npix, g, s);
     if (rc) return rc;
     const float inv_n = 1.0f / (float)npix;
-    hipLaunchKernelGGL(k_head, dim3((unsigned)ceil_div(npix, 256)), dim3(256), 0, s, n->act[3], n->prm + n->o.wh,
-                       n->prm + n->o.bh, d_occ, d_li, inv_n, n->dz[0], n->grad + n->o.wh, n->grad + n->o.bh,
-                       n->loss, n->outp, npix, g);
+    // at most 256 work-groups: the head gradient's atomics are one per work-group and entry
+    const unsigned hg = (unsigned)(ceil_div(npix, 256) < 256 ? ceil_div(npix, 256) : 256);
+    hipLaunchKernelGGL(k_head, dim3(hg), dim3(256), 0, s, n->act[3], n->prm + n->o.wh, n->prm + n->o.bh, d_occ, d_li,
+                       inv_n, n->dz[0], n->grad + n->o.wh, n->grad + n->o.bh, n->loss, n->outp, npix, g);
     BAMF_LAUNCH_CHECK("k_head");
     // back through layers 4..2 (dz[cur] = dL/dZ_l), then layer 1's weights
-    const int64_t chunk = 4096;
-    const unsigned nchunk = (unsigned)ceil_div(npix, chunk);
+    const int P = g.H * g.W, Pp = (P + 31) & ~31, HW2 = (g.H + 2) * (g.W + 2);
+    const size_t lds32 = (size_t)32 * Pp * 2 + (size_t)((kC * HW2 + 1) & ~1) * 2 + (size_t)Pp * 4;
+    const size_t lds1 = (size_t)32 * Pp * 2 + (size_t)((HW2 + 1) & ~1) * 2 + (size_t)Pp * 4;
+    BAMF_CHECK_ARG(lds32 <= 160 * 1024, "gradients need an image of at most about 1,500 pixels");
+    if (lds32 > 64 * 1024)
+        BAMF_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3_wgrad<kC>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32));
+    if (lds1 > 64 * 1024)
+        BAMF_HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3_wgrad<1>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1));
+    const unsigned wg = (unsigned)(batch < 64 ? batch : 64);
     int cur = 0;
     for (int l = 3; l >= 1; l--) {
-        hipLaunchKernelGGL((k_conv3_wgrad<kC>), dim3(2 * (kK / 16), nchunk), dim3(64), 0, s, n->dz[cur],
-                           n->act[l - 1], n->grad + n->o.w[l], n->grad + n->o.b[l], npix, chunk, g);
+        hipLaunchKernelGGL((k_conv3_wgrad<kC>), dim3(wg), dim3(256), lds32, s, n->dz[cur], n->act[l - 1],
+                           n->grad + n->o.w[l], n->grad + n->o.b[l], batch, g);
         BAMF_LAUNCH_CHECK("k_conv3_wgrad<32>");
         hipLaunchKernelGGL((k_conv3<kC, 1>), dim3((unsigned)ceil_div(npix, 256)), dim3(256), 0, s, n->dz[cur],
                            n->wt + (size_t)(l - 1) * kC * kK, nullptr, n->act[l - 1], n->dz[cur ^ 1], npix, g);
         BAMF_LAUNCH_CHECK("k_conv3<32,grad>");
         cur ^= 1;
     }
-    hipLaunchKernelGGL((k_conv3_wgrad<1>), dim3(2, nchunk), dim3(64), 0, s, n->dz[cur], n->x0, n->grad + n->o.w[0],
-                       n->grad + n->o.b[0], npix, chunk, g);
+    hipLaunchKernelGGL((k_conv3_wgrad<1>), dim3(wg), dim3(256), lds1, s, n->dz[cur], n->x0, n->grad + n->o.w[0],
+                       n->grad + n->o.b[0], batch, g);
     BAMF_LAUNCH_CHECK("k_conv3_wgrad<1>");
     return BAMF_OK;
 }

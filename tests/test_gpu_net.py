@@ -13,12 +13,14 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_gradients_match_oracle():
+@pytest.mark.parametrize("batch", [8, 96])
+def test_gradients_match_oracle(batch):
+    """batch 96: the weight-gradient work-groups each take more than one image."""
     rng = np.random.default_rng(7)
-    img, occ, logi = proposal.training_batch(rng, 8, 16)
+    img, occ, logi = proposal.training_batch(rng, batch, 16)
     p = proposal.init_params(3)
     p += np.random.default_rng(4).normal(0, 0.02, p.size).astype(np.float32)
-    net = proposal.Network(p, 8, 16, 16)
+    net = proposal.Network(p, batch, 16, 16)
     g, loss = net.gradients(img, occ, logi)
     lo, go, h = onet.loss_and_grad(p, img, occ, logi, bf16_mode=True)
     out = net.forward(img)
