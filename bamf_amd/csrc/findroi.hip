@@ -7,19 +7,26 @@
 // semantics bit for bit regardless of how the work is tiled.
 //
 // gfx950 design
-//   rows pass   k_rows_iir   : 64 rows per work-group staged through LDS in
-//                              64-sample chunks (coalesced 256 B row segments),
-//                              the recursion for one or two sigmas per row;
-//   column pass k_cols_iir   : one work-item per column, adjacent lanes on
-//                              adjacent columns (coalesced), forward pass in
-//                              place, backward pass fused with the DoG subtract;
-//   local max   k_localmax   : 64x32 output tile, D and the row-max image held
-//                              in LDS with a kernelsz halo, bits set with sparse
-//                              atomicOr into a Res bitmap;
+//   rows pass   k_rows_dog   : 64 rows per work-group staged through LDS in
+//                              32-sample chunks (coalesced row segments), the
+//                              recursion for both sigmas per row; the forward
+//                              pass keeps only chunk-start states and the
+//                              backward pass recomputes each chunk's forward
+//                              outputs in registers (no HBM round trip);
+//   column pass k_cols_dog   : one work-item per column, adjacent lanes on
+//                              adjacent columns (coalesced), chunk-start states
+//                              in LDS, forward outputs recomputed per chunk in
+//                              registers, backward pass fused with the DoG
+//                              subtract;
+//   local max   k_localmax<K>: 64x64 output tile, D staged in LDS with a K
+//                              halo and overwritten by the row maxima; both
+//                              window maxima by doubling in registers; bits set
+//                              with sparse atomicOr into a Res bitmap;
 //   compaction  k_bits_*     : ordered stream compaction of the bitmap
 //                              (per-block popcount, scan, ordered emit) plus the
 //                              box corners of findROI.m:130-148.
 #include <cmath>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -65,7 +72,9 @@ __device__ inline float iir_step(float w0, float w1, float w2, float w3, const C
     return w0 * c.B + q;
 }
 
-// Row recursion (the reference's gaussX) for NS sigmas at once.  in may alias out0.
+// Row recursion (the reference's gaussX) for NS sigmas at once, writing the
+// forward outputs and re-reading them.  in may alias out0 (the per-kernel
+// entry point runs it in place with NS = 1).
 // The chunk is staged in LDS; sigma1's results overwrite the staged input in
 // place (each input sample is read before it is overwritten).
 template <int NS>
@@ -150,45 +159,324 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_rows_iir(const float* in, flo
     }
 }
 
-// Column recursion (the reference's gaussY).  DOG=false: in place on a.
-// DOG=true: a and b hold the row-filtered images for sigma1 and sigma2; both
-// are column-filtered and a receives G(sigma1) - G(sigma2) (kernel_subtract).
-template <bool DOG>
-__global__ __launch_bounds__(64) void k_cols_iir(float* a, float* b, int xsize, int ysize, Coeffs c0,
-                                                 Coeffs c1) {
+// Column recursion (the reference's gaussY) in place on a, one work-item per
+// column, adjacent lanes on adjacent columns.
+__global__ __launch_bounds__(64) void k_cols_iir(float* a, int xsize, int ysize, Coeffs c0) {
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int64_t z = blockIdx.y;
     if (x >= xsize) return;
-    const int64_t base = z * xsize * ysize + x;
     const int64_t st = xsize;
-    float* pa = a + base;
-    float* pb = b + base;
+    float* pa = a + z * xsize * ysize + x;
     // forward (cuda_FindROI.cu:60-70)
-    float a1, a2, a3, b1 = 0.f, b2 = 0.f, b3 = 0.f;
+    float a1, a2, a3;
     a1 = a2 = a3 = pa[0];
-    if (DOG) b1 = b2 = b3 = pb[0];
     for (int ii = 0; ii < ysize; ii++) {
         const float va = iir_step(pa[ii * st], a1, a2, a3, c0);
         pa[ii * st] = va;
         a3 = a2; a2 = a1; a1 = va;
-        if (DOG) {
-            const float vb = iir_step(pb[ii * st], b1, b2, b3, c1);
-            pb[ii * st] = vb;
-            b3 = b2; b2 = b1; b1 = vb;
-        }
     }
     // backward (cuda_FindROI.cu:72-82)
     a1 = a2 = a3 = pa[(int64_t)(ysize - 1) * st];
-    if (DOG) b1 = b2 = b3 = pb[(int64_t)(ysize - 1) * st];
     for (int ii = ysize - 1; ii >= 0; ii--) {
         const float va = iir_step(pa[ii * st], a1, a2, a3, c0);
         a3 = a2; a2 = a1; a1 = va;
-        if (DOG) {
-            const float vb = iir_step(pb[ii * st], b1, b2, b3, c1);
-            b3 = b2; b2 = b1; b1 = vb;
-            pa[ii * st] = va - vb;  // cuda_FindROI.cu:91
+        pa[ii * st] = va;
+    }
+}
+
+// Row recursion for both sigmas of the DoG pipeline (in must not alias the
+// outputs).  The forward pass keeps only the recursion state: the three last
+// forward outputs of each chunk, which are the next chunk's starting state, are
+// parked at their own positions in out0/out1 (the backward pass overwrites them
+// after use).  The backward pass walks the chunks from the end, re-reads a
+// chunk, recomputes its forward outputs in registers from the parked state (the
+// same operations on the same inputs, hence the same values) and runs the
+// backward recursion on them.  HBM traffic: the stack read twice and each
+// output written once, where writing and re-reading the forward outputs costs
+// four image passes more.
+// FULL: every block has kRowsPerBlock rows and xsize is a multiple of kChunk
+// (no guards, 16-byte row transfers); otherwise guarded scalar transfers.
+// Rows row_begin + blockIdx.x * kRowsPerBlock ... of nrows.
+template <bool FULL>
+__global__ __launch_bounds__(kRowsPerBlock) void k_rows_dog(const float* __restrict__ in, float* __restrict__ out0,
+                                                           float* __restrict__ out1, int xsize, int64_t nrows,
+                                                           int64_t row_begin, Coeffs c0, Coeffs c1) {
+    __shared__ float tA[kRowsPerBlock][kChunk + 1];
+    const int t = threadIdx.x;
+    const int64_t row0 = row_begin + (int64_t)blockIdx.x * kRowsPerBlock;
+    const int nr = FULL ? kRowsPerBlock : (int)min<int64_t>(kRowsPerBlock, nrows - row0);
+    const int nchunk = (xsize + kChunk - 1) / kChunk;
+    const int64_t rbase = (row0 + t) * xsize;  // this work-item's row (used when t < nr)
+    // Chunk transfers.  FULL: element i of work-item t is the float4 at row
+    // kRowStep4 * i + t / 8, columns 4 (t % 8) ..+3.  Otherwise element i is row
+    // kRowStep * i + t / kChunk, column t % kChunk, guarded.  Both address by a
+    // per-lane pointer stepped by whole rows, and the next chunk is fetched
+    // into registers while the current one is processed.
+    constexpr int kRowStep = kRowsPerBlock / kChunk, kPer = kChunk;
+    constexpr int kQuads = kChunk / 4, kRowStep4 = kRowsPerBlock / kQuads, kPer4 = kChunk / 4;
+    const int tr = t / kChunk, tc = t % kChunk;
+    const int tr4 = t / kQuads, tc4 = (t % kQuads) * 4;
+    const int64_t bstep = (int64_t)kRowStep * xsize, bstep4 = (int64_t)kRowStep4 * xsize;
+    const int64_t loff = FULL ? (row0 + tr4) * xsize + tc4 : (row0 + tr) * xsize + tc;
+    float pre[kPer];
+    auto fetch_any = [&](int x0, int nc) {
+        const float* p = in + loff + x0;
+        if (FULL) {
+#pragma unroll
+            for (int i = 0; i < kPer4; i++) {
+                const float4 v = *reinterpret_cast<const float4*>(p);
+                pre[4 * i] = v.x; pre[4 * i + 1] = v.y; pre[4 * i + 2] = v.z; pre[4 * i + 3] = v.w;
+                p += bstep4;
+            }
         } else {
-            pa[ii * st] = va;
+#pragma unroll
+            for (int i = 0; i < kPer; i++) {
+                if (kRowStep * i + tr < nr && tc < nc) pre[i] = *p;
+                p += bstep;
+            }
+        }
+    };
+    auto put_any = [&](int nc) {
+        if (FULL) {
+#pragma unroll
+            for (int i = 0; i < kPer4; i++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) tA[kRowStep4 * i + tr4][tc4 + q] = pre[4 * i + q];
+        } else {
+#pragma unroll
+            for (int i = 0; i < kPer; i++)
+                if (kRowStep * i + tr < nr && tc < nc) tA[kRowStep * i + tr][tc] = pre[i];
+        }
+    };
+    auto store_any = [&](float* out, int x0, int nc) {
+        float* p = out + loff + x0;
+        if (FULL) {
+#pragma unroll
+            for (int i = 0; i < kPer4; i++) {
+                const int r = kRowStep4 * i + tr4;
+                *reinterpret_cast<float4*>(p) = make_float4(tA[r][tc4], tA[r][tc4 + 1], tA[r][tc4 + 2], tA[r][tc4 + 3]);
+                p += bstep4;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < kPer; i++) {
+                if (kRowStep * i + tr < nr && tc < nc) *p = tA[kRowStep * i + tr][tc];
+                p += bstep;
+            }
+        }
+    };
+    float a1 = 0.f, a2 = 0.f, a3 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;
+    auto forward = [&](auto full, int nc) {
+#pragma unroll
+        for (int k = 0; k < kChunk; k++) {
+            if (decltype(full)::value || k < nc) {
+                const float w0 = tA[t][k];
+                const float va = iir_step(w0, a1, a2, a3, c0);
+                a3 = a2; a2 = a1; a1 = va;
+                const float vb = iir_step(w0, b1, b2, b3, c1);
+                b3 = b2; b2 = b1; b1 = vb;
+            }
+        }
+    };
+    // backward state, and the chunk's forward outputs for sigma2 (then its results)
+    float p1 = 0.f, p2 = 0.f, p3 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
+    float fb[kChunk];
+    auto backward = [&](auto full, int nc) {
+        float fa[kChunk];
+#pragma unroll
+        for (int k = 0; k < kChunk; k++) {
+            if (decltype(full)::value || k < nc) {
+                const float w0 = tA[t][k];
+                fa[k] = iir_step(w0, a1, a2, a3, c0);
+                a3 = a2; a2 = a1; a1 = fa[k];
+                fb[k] = iir_step(w0, b1, b2, b3, c1);
+                b3 = b2; b2 = b1; b1 = fb[k];
+            }
+        }
+#pragma unroll
+        for (int k = kChunk - 1; k >= 0; k--) {
+            if (decltype(full)::value || k < nc) {
+                const float va = iir_step(fa[k], p1, p2, p3, c0);
+                p3 = p2; p2 = p1; p1 = va;
+                tA[t][k] = va;
+                const float vb = iir_step(fb[k], q1, q2, q3, c1);
+                q3 = q2; q2 = q1; q1 = vb;
+                fb[k] = vb;
+            }
+        }
+    };
+    // starting state of the chunk the backward pass works on next
+    float s1 = 0.f, s2 = 0.f, s3 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f;
+    // ---- forward (cuda_FindROI.cu:23-33), states only ----
+    fetch_any(0, min(kChunk, xsize));
+    for (int ch = 0; ch < nchunk; ch++) {
+        const int x0 = ch * kChunk, nc = min(kChunk, xsize - x0);
+        put_any(nc);
+        __syncthreads();
+        if (ch + 1 < nchunk) fetch_any(x0 + kChunk, min(kChunk, xsize - x0 - kChunk));
+        if (t < nr) {
+            if (ch == 0) {
+                a1 = a2 = a3 = tA[t][0];
+                b1 = b2 = b3 = a1;
+            }
+            s1 = a1; s2 = a2; s3 = a3; u1 = b1; u2 = b2; u3 = b3;
+            if (FULL || nc == kChunk) forward(std::true_type{}, nc);
+            else forward(std::false_type{}, nc);
+            if (ch + 2 < nchunk) {  // park the state of chunk ch + 1 (a full chunk)
+                float* r0 = out0 + rbase + x0 + kChunk - 3;
+                float* r1 = out1 + rbase + x0 + kChunk - 3;
+                r0[0] = a3; r0[1] = a2; r0[2] = a1;
+                r1[0] = b3; r1[1] = b2; r1[2] = b1;
+            }
+        }
+        __syncthreads();
+    }
+    // ---- backward (cuda_FindROI.cu:35-45), from the last forward outputs;
+    // tA holds the last chunk's input and s/u its starting state ----
+    p1 = p2 = p3 = a1;
+    q1 = q2 = q3 = b1;
+    for (int ch = nchunk - 1; ch >= 0; ch--) {
+        const int x0 = ch * kChunk, nc = min(kChunk, xsize - x0);
+        if (ch != nchunk - 1) {
+            put_any(nc);
+            __syncthreads();
+        }
+        a1 = s1; a2 = s2; a3 = s3; b1 = u1; b2 = u2; b3 = u3;
+        // the next chunk's input and starting state travel together, so waiting
+        // for one never waits on the other's later loads
+        if (ch > 0) {
+            fetch_any(x0 - kChunk, kChunk);
+            if (t < nr && ch > 1) {
+                const float* r0 = out0 + rbase + x0 - kChunk - 3;
+                const float* r1 = out1 + rbase + x0 - kChunk - 3;
+                s3 = r0[0]; s2 = r0[1]; s1 = r0[2];
+                u3 = r1[0]; u2 = r1[1]; u1 = r1[2];
+            }
+        }
+        if (t < nr) {
+            if (ch == 0) {
+                a1 = a2 = a3 = tA[t][0];
+                b1 = b2 = b3 = a1;
+            }
+            if (FULL || nc == kChunk) backward(std::true_type{}, nc);
+            else backward(std::false_type{}, nc);
+        }
+        __syncthreads();
+        store_any(out0, x0, nc);
+        __syncthreads();
+        if (t < nr) {
+#pragma unroll
+            for (int k = 0; k < kChunk; k++)
+                if (FULL || k < nc) tA[t][k] = fb[k];
+        }
+        __syncthreads();
+        store_any(out1, x0, nc);
+        __syncthreads();
+    }
+}
+
+// Column recursion for both sigmas fused with the DoG subtract (cuda_FindROI.cu:
+// 56-91 and kernel_subtract): a receives G(sigma1) - G(sigma2), b is only read.
+// The forward pass keeps only the recursion state at the start of each chunk
+// of kColChunk rows, in ck (6 floats per column per chunk, lanes on adjacent
+// columns); the backward pass walks the chunks from the end, re-reads a chunk
+// of both images, recomputes its forward outputs in registers and runs the
+// backward recursion on them.  A work-group spans 256 adjacent columns, so each
+// row access covers 1 KiB of contiguous bytes.
+constexpr int kColChunk = 32, kColThreads = 256;
+__global__ __launch_bounds__(kColThreads) void k_cols_dog(float* a, const float* b, float* __restrict__ ck,
+                                                          int xsize, int ysize, Coeffs c0, Coeffs c1) {
+    const int x = blockIdx.x * kColThreads + threadIdx.x;
+    if (x >= xsize) return;
+    const int nchunk = (ysize + kColChunk - 1) / kColChunk;
+    const int64_t st = xsize;
+    float* const ac = a + (int64_t)blockIdx.y * xsize * ysize + x;  // this column
+    const float* const bc = b + (int64_t)blockIdx.y * xsize * ysize + x;
+    float* const cc = ck + (int64_t)blockIdx.y * nchunk * 6 * xsize + x;
+    // Every helper has a full form (no guards, kColChunk rows) and a guarded
+    // one for the ragged last chunk.
+    float wa[kColChunk], wb[kColChunk];
+    auto load = [&](auto full, int y0, int n) {  // every load is issued before the first use
+        const float* pa = ac + y0 * st;
+        const float* pb = bc + y0 * st;
+#pragma unroll
+        for (int k = 0; k < kColChunk; k++) {
+            if (decltype(full)::value || k < n) {
+                wa[k] = *pa;
+                wb[k] = *pb;
+            }
+            pa += st;
+            pb += st;
+        }
+    };
+    float a1, a2, a3, b1, b2, b3;
+    auto forward = [&](auto full, int n) {
+#pragma unroll
+        for (int k = 0; k < kColChunk; k++)
+            if (decltype(full)::value || k < n) {
+                const float va = iir_step(wa[k], a1, a2, a3, c0);
+                a3 = a2; a2 = a1; a1 = va;
+                const float vb = iir_step(wb[k], b1, b2, b3, c1);
+                b3 = b2; b2 = b1; b1 = vb;
+            }
+    };
+    float p1, p2, p3, q1, q2, q3;
+    auto backward = [&](auto full, int y0, int n) {
+#pragma unroll
+        for (int k = 0; k < kColChunk; k++)
+            if (decltype(full)::value || k < n) {
+                wa[k] = iir_step(wa[k], a1, a2, a3, c0);
+                a3 = a2; a2 = a1; a1 = wa[k];
+                wb[k] = iir_step(wb[k], b1, b2, b3, c1);
+                b3 = b2; b2 = b1; b1 = wb[k];
+            }
+        float* pa = ac + (y0 + kColChunk - 1) * st;
+#pragma unroll
+        for (int k = kColChunk - 1; k >= 0; k--) {
+            if (decltype(full)::value || k < n) {
+                const float va = iir_step(wa[k], p1, p2, p3, c0);
+                p3 = p2; p2 = p1; p1 = va;
+                const float vb = iir_step(wb[k], q1, q2, q3, c1);
+                q3 = q2; q2 = q1; q1 = vb;
+                *pa = va - vb;  // cuda_FindROI.cu:91
+            }
+            pa -= st;
+        }
+    };
+    a1 = a2 = a3 = ac[0];
+    b1 = b2 = b3 = bc[0];
+    for (int ch = 0; ch < nchunk; ch++) {
+        const int y0 = ch * kColChunk, n = min(kColChunk, ysize - y0);
+        if (ch > 0) {  // chunk 0 starts from its first row
+            float* c = cc + (int64_t)ch * 6 * st;
+            c[0] = a1; c[st] = a2; c[2 * st] = a3; c[3 * st] = b1; c[4 * st] = b2; c[5 * st] = b3;
+        }
+        if (n == kColChunk) {
+            load(std::true_type{}, y0, n);
+            forward(std::true_type{}, n);
+        } else {
+            load(std::false_type{}, y0, n);
+            forward(std::false_type{}, n);
+        }
+    }
+    // wa/wb hold the last chunk; the backward pass starts at the last forward outputs
+    p1 = p2 = p3 = a1;
+    q1 = q2 = q3 = b1;
+    for (int ch = nchunk - 1; ch >= 0; ch--) {
+        const int y0 = ch * kColChunk, n = min(kColChunk, ysize - y0);
+        if (ch > 0) {
+            const float* c = cc + (int64_t)ch * 6 * st;
+            a1 = c[0]; a2 = c[st]; a3 = c[2 * st]; b1 = c[3 * st]; b2 = c[4 * st]; b3 = c[5 * st];
+        } else {
+            a1 = a2 = a3 = ac[0];
+            b1 = b2 = b3 = bc[0];
+        }
+        if (n == kColChunk) {
+            if (ch != nchunk - 1) load(std::true_type{}, y0, n);
+            backward(std::true_type{}, y0, n);
+        } else {
+            if (ch != nchunk - 1) load(std::false_type{}, y0, n);
+            backward(std::false_type{}, y0, n);
         }
     }
 }
@@ -235,49 +523,115 @@ __global__ void k_max_y(float* a, const float* b, int xsize, int ysize, int k, f
     a[idx] = maxy_flag(maxval, in) ? 1.0f : 0.0f;
 }
 
-// Fused maxX + maxY on the DoG image, writing Res bits.  Window half-width K <= kMaxK.
-constexpr int kTX = 64, kTY = 32, kMaxK = 16;
+// Fused maxX + maxY on the DoG image, writing Res bits, for window half-width
+// K <= kMaxK (one instantiation per K).  A 64 x 64 output tile: the D tile with
+// its K halo is staged in LDS (-inf outside the frame, which never changes
+// fmaxf(maxval, .) and so reproduces the reference's clipped windows), the row
+// maxima (maxX) overwrite it in place, and both window maxima are taken in
+// registers by doubling.  fmaxf is exact, so any grouping of a window's
+// maximum gives the reference's value bit for bit.
+constexpr int kLmT = 64, kMaxK = 16;
+
+// out[j] = max(v[j .. j+2K]) for j < N.  After the pass of width w, v[i] holds
+// the maximum over [i, i+2w); the last step joins two overlapping windows of
+// length P (the largest power of two <= 2K+1).
+template <int N, int K>
+__device__ __forceinline__ void window_max(float (&v)[N + 2 * K], float (&out)[N]) {
+    constexpr int L = 2 * K + 1;
+    constexpr int P = L >= 32 ? 32 : L >= 16 ? 16 : L >= 8 ? 8 : L >= 4 ? 4 : L >= 2 ? 2 : 1;
+#pragma unroll
+    for (int w = 1; w < P; w *= 2) {
+#pragma unroll
+        for (int i = 0; i + 2 * w <= N + 2 * K; i++) v[i] = fmaxf(v[i], v[i + w]);
+    }
+#pragma unroll
+    for (int j = 0; j < N; j++) out[j] = fmaxf(v[j], v[j + L - P]);
+}
+
+template <int K>
 __global__ __launch_bounds__(256) void k_localmax(const float* __restrict__ dog, uint32_t* __restrict__ bits,
-                                                  int xsize, int ysize, int k, float minval) {
-    __shared__ float sD[kTY + 2 * kMaxK][kTX + 2 * kMaxK + 1];
-    __shared__ float sM[kTY + 2 * kMaxK][kTX + 1];
-    const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY;
-    const int64_t z = blockIdx.z;
-    const int64_t fbase = z * xsize * ysize;
-    const int t = threadIdx.x;
-    const int dw = kTX + 2 * k, dh = kTY + 2 * k;
-    // D tile: rows y0-k .. y0+kTY+k-1, cols x0-k .. x0+kTX+k-1 (outside the frame: -inf,
-    // which never changes fmaxf(maxval, .) so the clipped windows are reproduced).
-    for (int e = t; e < dw * dh; e += 256) {
-        const int r = e / dw, cc = e - r * dw;
-        const int y = y0 - k + r, x = x0 - k + cc;
-        float v = -INFINITY;
-        if (y >= 0 && y < ysize && x >= 0 && x < xsize) v = dog[fbase + (int64_t)y * xsize + x];
-        sD[r][cc] = v;
+                                                  int xsize, int ysize, float minval) {
+    constexpr int R = kLmT + 2 * K;      // staged rows (tile rows y0-K .. y0+63+K)
+    constexpr int S = kLmT + 2 * K + 1;  // row stride, odd: lanes on consecutive rows use distinct banks
+    constexpr int RS = 32 + 2 * K;       // row strip: 32 outputs and the halo
+    constexpr int CS = 16 + 2 * K;       // column strip: 16 outputs and the halo
+    __shared__ float sD[R * S];
+    const int x0 = blockIdx.x * kLmT, y0 = blockIdx.y * kLmT;
+    const int64_t fbase = (int64_t)blockIdx.z * xsize * ysize;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    {
+        // all of this wave's loads are issued before the first LDS store
+        constexpr int RPW = (R + 3) / 4;
+        float va[RPW], vb[RPW];
+#pragma unroll
+        for (int i = 0; i < RPW; i++) {
+            const int r = wv + 4 * i, y = y0 - K + r;
+            const bool yin = r < R && y >= 0 && y < ysize;
+            const int64_t rb = fbase + (int64_t)y * xsize;
+            const int xa = x0 - K + lane, xb = xa + kLmT;
+            va[i] = (yin && xa >= 0 && xa < xsize) ? dog[rb + xa] : -INFINITY;
+            vb[i] = (yin && lane < 2 * K && xb < xsize) ? dog[rb + xb] : -INFINITY;
+        }
+#pragma unroll
+        for (int i = 0; i < RPW; i++) {
+            const int r = wv + 4 * i;
+            if (r < R) {
+                sD[r * S + lane] = va[i];
+                if (lane < 2 * K) sD[r * S + kLmT + lane] = vb[i];
+            }
+        }
     }
     __syncthreads();
-    // row-max image (maxX) for the tile columns and all halo rows
-    for (int e = t; e < kTX * dh; e += 256) {
-        const int r = e / kTX, cc = e - r * kTX;
-        float maxval = minval;
-        for (int q = 0; q <= 2 * k; q++) maxval = fmaxf(maxval, sD[r][cc + q]);
-        sM[r][cc] = maxx_value(maxval, sD[r][cc + k]);
+    // maxX (cuda_FindROI.cu:105-116): work-item t < 2R takes row t % R, outputs
+    // 32 (t / R) .. +31; the results replace columns 0..63 of the row.
+    float mx[32];
+    const bool act = t < 2 * R;
+    const int seg = t / R, rr = t - seg * R, cs = seg * 32;
+    if (act) {
+        float v[RS], w[32], cv[32];
+#pragma unroll
+        for (int q = 0; q < RS; q++) v[q] = sD[rr * S + cs + q];
+#pragma unroll
+        for (int j = 0; j < 32; j++) cv[j] = v[j + K];
+        window_max<32, K>(v, w);
+#pragma unroll
+        for (int j = 0; j < 32; j++) mx[j] = maxx_value(fmaxf(minval, w[j]), cv[j]);
     }
     __syncthreads();
-    // column max of |maxX| (maxY) and the flag
-    for (int e = t; e < kTX * kTY; e += 256) {
-        const int r = e / kTX, cc = e - r * kTX;
-        const int y = y0 + r, x = x0 + cc;
-        if (y >= ysize || x >= xsize) continue;
-        const int ys = max(0, y - k), ye = min(ysize - 1, y + k);
-        float maxval = minval;
-        for (int yy = ys; yy <= ye; yy++) maxval = fmaxf(maxval, fabsf(sM[yy - y0 + k][cc]));
-        if (maxy_flag(maxval, sM[r + k][cc])) {
+    if (act) {
+#pragma unroll
+        for (int j = 0; j < 32; j++) sD[rr * S + cs + j] = mx[j];
+    }
+    __syncthreads();
+    // maxY (cuda_FindROI.cu:134-146): column lane, rows 16 wv .. +15; rows
+    // outside the frame leave the window (-inf).
+    const int x = x0 + lane, r0 = wv * 16;
+    if (x >= xsize) return;
+    float v[CS], w[16], cv[16];
+#pragma unroll
+    for (int q = 0; q < CS; q++) {
+        const int y = y0 - K + r0 + q;
+        const float m = sD[(r0 + q) * S + lane];
+        v[q] = (y >= 0 && y < ysize) ? fabsf(m) : -INFINITY;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) cv[j] = sD[(r0 + K + j) * S + lane];
+    window_max<16, K>(v, w);
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int y = y0 + r0 + j;
+        if (y < ysize && maxy_flag(fmaxf(minval, w[j]), cv[j])) {
             const int64_t lin = fbase + (int64_t)y * xsize + x;
             atomicOr(&bits[lin >> 5], 1u << (lin & 31));
         }
     }
 }
+
+using LocalMaxKernel = void (*)(const float*, uint32_t*, int, int, float);
+constexpr LocalMaxKernel kLocalMax[kMaxK + 1] = {
+    k_localmax<0>,  k_localmax<1>,  k_localmax<2>,  k_localmax<3>,  k_localmax<4>,  k_localmax<5>,
+    k_localmax<6>,  k_localmax<7>,  k_localmax<8>,  k_localmax<9>,  k_localmax<10>, k_localmax<11>,
+    k_localmax<12>, k_localmax<13>, k_localmax<14>, k_localmax<15>, k_localmax<16>};
 
 __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -415,13 +769,10 @@ Coeffs make_coeffs(float sigma) {
     return with_rcp(Coeffs{o[0], o[1], o[2], o[3], o[4], 0.0f});
 }
 
-int launch_rows(const float* in, float* o0, float* o1, int xsize, int64_t nrows, Coeffs c0, Coeffs c1,
-                int ns, hipStream_t s) {
+// The row recursion in place on d for one sigma (the reference's gaussX).
+int launch_rows_inplace(float* d, int xsize, int64_t nrows, Coeffs c, hipStream_t s) {
     dim3 grid((unsigned)ceil_div(nrows, kRowsPerBlock));
-    if (ns == 1)
-        hipLaunchKernelGGL(k_rows_iir<1>, grid, dim3(kRowsPerBlock), 0, s, in, o0, o1, xsize, nrows, c0, c1);
-    else
-        hipLaunchKernelGGL(k_rows_iir<2>, grid, dim3(kRowsPerBlock), 0, s, in, o0, o1, xsize, nrows, c0, c1);
+    hipLaunchKernelGGL(k_rows_iir<1>, grid, dim3(kRowsPerBlock), 0, s, d, d, nullptr, xsize, nrows, c, c);
     BAMF_LAUNCH_CHECK("k_rows_iir");
     return BAMF_OK;
 }
@@ -453,7 +804,7 @@ int bamf_find_roi_gauss_x(float* d, int xsize, int ysize, int nframes, float b0,
     if (nframes == 0) return BAMF_OK;
     BAMF_CHECK_ARG(d, "null device pointer");
     const Coeffs c = with_rcp(Coeffs{b0, b1, b2, b3, B, 0.0f});
-    return launch_rows(d, d, nullptr, xsize, (int64_t)ysize * nframes, c, c, 1, as_stream(stream));
+    return launch_rows_inplace(d, xsize, (int64_t)ysize * nframes, c, as_stream(stream));
 }
 
 int bamf_find_roi_gauss_y(float* d, int xsize, int ysize, int nframes, float b0, float b1, float b2,
@@ -464,7 +815,7 @@ int bamf_find_roi_gauss_y(float* d, int xsize, int ysize, int nframes, float b0,
     BAMF_CHECK_ARG(nframes <= 65535, "nframes must be <= 65535 per call");
     const Coeffs c = with_rcp(Coeffs{b0, b1, b2, b3, B, 0.0f});
     dim3 grid((unsigned)ceil_div(xsize, 64), (unsigned)nframes);
-    hipLaunchKernelGGL(k_cols_iir<false>, grid, dim3(64), 0, as_stream(stream), d, d, xsize, ysize, c, c);
+    hipLaunchKernelGGL(k_cols_iir, grid, dim3(64), 0, as_stream(stream), d, xsize, ysize, c);
     BAMF_LAUNCH_CHECK("k_cols_iir");
     return BAMF_OK;
 }
@@ -507,10 +858,16 @@ static int64_t nwords_of(int64_t n) { return (n + 31) / 32; }
 static int64_t nblocks_of(int64_t nwords) { return (nwords + kWordsPerBlock - 1) / kWordsPerBlock; }
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// Column-pass checkpoints: 6 floats per column per kColChunk rows.
+static size_t col_ck_bytes(int xsize, int ysize, int nframes) {
+    return (size_t)xsize * nframes * ceil_div(ysize, kColChunk) * 6 * sizeof(float);
+}
+
 size_t bamf_find_roi_workspace_bytes(int xsize, int ysize, int nframes) {
     const int64_t n = (int64_t)xsize * ysize * nframes;
     const int64_t nw = nwords_of(n);
-    return align256((size_t)n * 4) * 2 + align256((size_t)nw * 4) + align256((size_t)nblocks_of(nw) * 8);
+    return align256((size_t)n * 4) * 2 + align256((size_t)nw * 4) + align256((size_t)nblocks_of(nw) * 8) +
+           align256(col_ck_bytes(xsize, ysize, nframes));
 }
 
 int bamf_find_roi(const float* d_stack, int xsize, int ysize, int nframes, float sigma1, float sigma2,
@@ -533,6 +890,8 @@ int bamf_find_roi(const float* d_stack, int xsize, int ysize, int nframes, float
     float* Bf = reinterpret_cast<float*>(ws + align256((size_t)n * 4));
     uint32_t* wbits = reinterpret_cast<uint32_t*>(ws + 2 * align256((size_t)n * 4));
     int64_t* bcounts = reinterpret_cast<int64_t*>(ws + 2 * align256((size_t)n * 4) + align256((size_t)nw * 4));
+    float* colck = reinterpret_cast<float*>(ws + 2 * align256((size_t)n * 4) + align256((size_t)nw * 4) +
+                                            align256((size_t)nb * 8));
     uint32_t* bits = d_res_bits ? d_res_bits : wbits;
     if (n == 0) {
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(d_count), 2, 0u);
@@ -540,20 +899,32 @@ int bamf_find_roi(const float* d_stack, int xsize, int ysize, int nframes, float
         return BAMF_OK;
     }
     const Coeffs c1 = make_coeffs(sigma1), c2 = make_coeffs(sigma2);
-    int rc = launch_rows(d_stack, A, Bf, xsize, (int64_t)ysize * nframes, c1, c2, 2, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_cols_iir<true>, dim3((unsigned)ceil_div(xsize, 64), (unsigned)nframes), dim3(64), 0, s,
-                       A, Bf, xsize, ysize, c1, c2);
-    BAMF_LAUNCH_CHECK("k_cols_iir");
+    const int64_t nrows = (int64_t)ysize * nframes;
+    // whole blocks of a chunk-multiple row length unguarded, the rest guarded
+    const int64_t nfull = xsize % kChunk == 0 ? nrows / kRowsPerBlock : 0;
+    if (nfull > 0) {
+        hipLaunchKernelGGL(k_rows_dog<true>, dim3((unsigned)nfull), dim3(kRowsPerBlock), 0, s, d_stack, A, Bf, xsize,
+                           nrows, (int64_t)0, c1, c2);
+        BAMF_LAUNCH_CHECK("k_rows_dog");
+    }
+    if (nrows > nfull * kRowsPerBlock) {
+        hipLaunchKernelGGL(k_rows_dog<false>, dim3((unsigned)ceil_div(nrows - nfull * kRowsPerBlock, kRowsPerBlock)),
+                           dim3(kRowsPerBlock), 0, s, d_stack, A, Bf, xsize, nrows, nfull * kRowsPerBlock, c1, c2);
+        BAMF_LAUNCH_CHECK("k_rows_dog");
+    }
+    hipLaunchKernelGGL(k_cols_dog, dim3((unsigned)ceil_div(xsize, kColThreads), (unsigned)nframes),
+                       dim3(kColThreads), 0, s, A, Bf, colck, xsize, ysize, c1, c2);
+    BAMF_LAUNCH_CHECK("k_cols_dog");
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<int64_t>(ceil_div(nw, 256), 4096)), dim3(256), 0, s,
                        bits, nw, 0u);
     BAMF_LAUNCH_CHECK("k_fill_u32");
     if (boxsize <= kMaxK) {
-        dim3 grid((unsigned)ceil_div(xsize, kTX), (unsigned)ceil_div(ysize, kTY), (unsigned)nframes);
-        hipLaunchKernelGGL(k_localmax, grid, dim3(256), 0, s, A, bits, xsize, ysize, boxsize, minval);
+        dim3 grid((unsigned)ceil_div(xsize, kLmT), (unsigned)ceil_div(ysize, kLmT), (unsigned)nframes);
+        hipLaunchKernelGGL(kLocalMax[boxsize], grid, dim3(256), 0, s, A, bits, xsize, ysize, minval);
         BAMF_LAUNCH_CHECK("k_localmax");
     } else {
         // wide windows: the reference's two passes, then pack the 0/1 map
+        int rc;
         if ((rc = bamf_find_roi_max_x(A, Bf, xsize, ysize, nframes, boxsize, minval, stream))) return rc;
         if ((rc = bamf_find_roi_max_y(A, Bf, xsize, ysize, nframes, boxsize, minval, stream))) return rc;
         hipLaunchKernelGGL(k_pack_bits, dim3((unsigned)ceil_div(nw, 256)), dim3(256), 0, s, A, bits, n);

@@ -71,6 +71,32 @@ def test_find_roi_pipeline_bit_exact(oracle, nframes, size, box, seed):
     assert np.array_equal(dev.res_bits().astype(np.float32), res_o)
 
 
+@pytest.mark.parametrize("nframes,ysize,xsize,box", [
+    (2, 48, 200, 5),    # rectangular, partial last row chunk
+    (3, 37, 29, 0),     # one partial chunk per row, 1 x 1 window
+    (2, 33, 65, 1),
+    (2, 70, 70, 16),    # widest window of the fused local max
+    (1, 64, 96, 17),    # wide window: the reference's two max passes
+    (1, 400, 50, 4),    # column checkpoints in 64-column work-groups
+    (1, 1100, 40, 2),   # more column chunks than the checkpointed column pass holds
+])
+def test_find_roi_pipeline_shapes(oracle, nframes, ysize, xsize, box):
+    side = max(ysize, xsize)
+    frames = np.ascontiguousarray(synth.make_frames(nframes, side, seed=ysize + xsize)[:, :ysize, :xsize])
+    mv = default_minval()
+    print(f"nframes={nframes} ysize={ysize} xsize={xsize} box={box}", flush=True)
+    res_o, lin_o, row_o, col_o, fr_o = oracle.find_roi(frames.ravel(), xsize, ysize, nframes, 1.3, 2.6, box, mv)
+    d = _hip.DeviceBuffer.from_array(frames)
+    dev = FindRoiDevice(xsize, ysize, nframes, capacity=xsize * ysize * nframes)
+    dev.run(d, 1.3, 2.6, box, mv)
+    _hip.synchronize()
+    n, lin, row, col, fr = dev.results()
+    assert n == len(lin_o) and n > 0
+    assert np.array_equal(lin, lin_o)
+    assert np.array_equal(row, row_o) and np.array_equal(col, col_o) and np.array_equal(fr, fr_o)
+    assert np.array_equal(dev.res_bits().astype(np.float32), res_o)
+
+
 def test_find_roi_capacity_and_empty():
     frames = np.full((2, 64, 64), 10.0, np.float32)  # flat: nothing above minval
     d = _hip.DeviceBuffer.from_array(frames)
