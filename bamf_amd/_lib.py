@@ -58,6 +58,12 @@ def _declare(L: ctypes.CDLL) -> None:
                                 c_int64, c_void_p, c_void_p]
     L.bamf_find_roi_crop.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int64, c_int,
                                      c_void_p, c_void_p]
+    L.bamf_subregion_count.argtypes = [c_int, c_int, c_int, c_int, c_int]
+    L.bamf_subregion_count.restype = c_int64
+    L.bamf_subregion_pixels.argtypes = [c_int, c_int, c_int, c_int, c_int]
+    L.bamf_subregion_pixels.restype = c_int64
+    L.bamf_make_subregions.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int] + [c_void_p] * 7 + \
+        [c_int, c_void_p, c_void_p, c_void_p]
     if hasattr(L, "bamf_rjmcmc_plan_create"):
         from . import rjmcmc_abi
 

@@ -758,6 +758,13 @@ struct Chain {
         const Em& c = cur();
         const int z = a->p.grid_zoom;
         const int64_t go = a->grid_off[roi];
+        // the grid's column stride and kept window: the whole ROI grid, or its
+        // core inside a stitched image (counts exact: integer-valued floats)
+        int ld = Y * z, xl = 0, xh = X * z, yl = 0, yh = Y * z;
+        if (a->grid_place) {
+            const int32_t* gp = a->grid_place + 5 * (int64_t)roi;
+            ld = gp[0]; xl = gp[1]; xh = gp[2]; yl = gp[3]; yh = gp[4];
+        }
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int n = lane + 64 * h;
@@ -765,9 +772,9 @@ struct Chain {
                 const int x = kround((float)z * (c.X[n] + a->p.drift_x));
                 const int y = kround((float)z * (c.Y[n] + a->p.drift_y));
                 const float i = c.I[n];
-                if (x >= 0 && x < X * z && y >= 0 && y < Y * z && i > a->p.icutoff) {
+                if (x >= xl && x < xh && y >= yl && y < yh && i > a->p.icutoff) {
                     float* g = c.S[n] == 1 ? a->pgrid : a->pback;
-                    atomicAdd(&g[go + (int64_t)x * Y * z + y], 1.0f);
+                    atomicAdd(&g[go + (int64_t)x * ld + y], 1.0f);
                 }
             }
         }

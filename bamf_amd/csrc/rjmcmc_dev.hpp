@@ -116,6 +116,7 @@ struct ChainArgs {
     float* pgrid;
     float* pback;
     const int64_t* grid_off;
+    const int32_t* grid_place;  // [n_roi][5]: column stride, x_lo, x_hi, y_lo, y_hi, or null
     bamf_chain_rec* acc_rec;
     bamf_chain_rec* prop_rec;
     float* acc_em;

@@ -434,6 +434,7 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     a.pgrid = out->d_pgrid;
     a.pback = out->d_pback;
     a.grid_off = out->d_grid_offset;
+    a.grid_place = out->d_grid_place;
     a.alg_flops = out->d_alg_flops;
     BAMF_CHECK_ARG((!out->d_acc_rec && !out->d_prop_rec && !out->d_acc_em && !out->d_prop_em) ||
                        (out->d_acc_rec && out->d_prop_rec && out->d_acc_em && out->d_prop_em && out->d_rec_count &&

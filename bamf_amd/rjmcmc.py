@@ -130,8 +130,11 @@ class BatchResult:
 class RoiSet:
     """A device-resident batch of ROIs (bamf_roi_batch)."""
 
-    def __init__(self, data: np.ndarray, offsets: np.ndarray, xsize, ysize, rho=None, bg0=None, keys=None,
-                 order=None, scmos=None, n0=None, em0=None):
+    def __init__(self, data: np.ndarray | None, offsets: np.ndarray, xsize, ysize, rho=None, bg0=None, keys=None,
+                 order=None, scmos=None, n0=None, em0=None, device=None):
+        """device = (d_data, d_offsets, d_xsize, d_ysize) DeviceBuffers already
+        holding the batch (bamf_make_subregions output); data is then None and
+        offsets/xsize/ysize are host copies of what they hold."""
         self.n_roi = len(xsize)
         self.xsize = np.ascontiguousarray(xsize, np.int32)
         self.ysize = np.ascontiguousarray(ysize, np.int32)
@@ -140,10 +143,13 @@ class RoiSet:
         if np.any(self.xsize < 1) or np.any(self.xsize > 32) or np.any(self.ysize < 1) or np.any(self.ysize > 32):
             raise BamfError("ROI sides must be in [1, 32]")
         up = _hip.DeviceBuffer.from_array
-        self.d_data = up(np.ascontiguousarray(data, np.float32))
-        self.d_off = up(self.offsets)
-        self.d_xs = up(self.xsize)
-        self.d_ys = up(self.ysize)
+        if device is not None:
+            self.d_data, self.d_off, self.d_xs, self.d_ys = device
+        else:
+            self.d_data = up(np.ascontiguousarray(data, np.float32))
+            self.d_off = up(self.offsets)
+            self.d_xs = up(self.xsize)
+            self.d_ys = up(self.ysize)
         self.bufs = {}
         b = RoiBatch()
         b.n_roi = self.n_roi
@@ -177,11 +183,12 @@ def lpt_order(xsize: np.ndarray, ysize: np.ndarray, expected_n: np.ndarray | Non
 
 def run_batch(plan: Plan, rois: RoiSet, seed: int = 0, mode: int = PLAIN, replay=None, *, hist_len: int = 64,
               final_cap: int = 128, trace_len: int = 0, trace_thin: int = 1, record_steps: bool = False,
-              grids: bool = False, chain_caps: tuple | None = None, stream=None, sync: bool = True,
+              grids: bool | dict = False, chain_caps: tuple | None = None, stream=None, sync: bool = True,
               outs=None) -> BatchResult | dict:
     """Runs every chain of the batch; returns host copies of the outputs.
     chain_caps = (records, emitter entries) per ROI turns the Chain /
-    ProposedChain records on."""
+    ProposedChain records on.  grids: True for one zoomed grid per ROI, or a
+    placement (bamf_amd.tiles.Tiling.placement) for one stitched image."""
     n = rois.n_roi
     p = plan.params
     total = p.n_burnin + p.n_trials
@@ -217,7 +224,15 @@ def alloc_outputs(n, total, hist_len, final_cap, trace_len, record_steps, grids,
     if record_steps:
         d["steps"] = B(STEP_DTYPE.itemsize * n * total)
         o.d_steps = d["steps"].ptr
-    if grids:
+    if isinstance(grids, dict):
+        # the tiles' grids placed in one stitched image (bamf_amd.tiles.Tiling.placement)
+        npx = int(grids["pixels"])
+        d["pgrid"] = B.zeros(4 * npx)
+        d["pback"] = B.zeros(4 * npx)
+        o.d_pgrid, o.d_pback = d["pgrid"].ptr, d["pback"].ptr
+        o.d_grid_offset, o.d_grid_place = grids["offset"].ptr, grids["place"].ptr
+        d["grid_placement"] = grids
+    elif grids:
         sizes = rois.xsize.astype(np.int64) * rois.ysize * zoom * zoom
         goff = np.zeros(n, np.int64)
         goff[1:] = np.cumsum(sizes)[:-1]

@@ -45,7 +45,7 @@ class RjOut(Structure):
                 ("d_n_hist", c_void_p), ("hist_len", c_int32), ("d_logpost", c_void_p),
                 ("d_trace_n", c_void_p), ("d_trace_photons", c_void_p), ("trace_len", c_int32),
                 ("trace_thin", c_int32), ("d_steps", c_void_p), ("d_pgrid", c_void_p), ("d_pback", c_void_p),
-                ("d_grid_offset", c_void_p),
+                ("d_grid_offset", c_void_p), ("d_grid_place", c_void_p),
                 ("d_acc_rec", c_void_p), ("d_prop_rec", c_void_p), ("d_acc_em", c_void_p), ("d_prop_em", c_void_p),
                 ("d_rec_count", c_void_p), ("rec_cap", c_int32), ("em_cap", c_int32),
                 ("d_alg_flops", c_void_p)]

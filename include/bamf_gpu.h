@@ -94,6 +94,35 @@ int bamf_find_roi_crop(const float* d_stack, int xsize, int ysize,
                        float* d_roi_stack, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Frame tiling and the stitched posterior image.
+ * bamf_make_subregions replaces @RJ/makeSubRegions.m:42-102 (called at
+ * RJ.m:192,227,249): frames in MATLAB layout (Data(y, x, f) at
+ * y + ypix (x + xpix f)) are cut into Xsub x Ysub tiles of box + 2 overlap
+ * pixels (less at the frame edges), numbered frame-major then column-major
+ * over positions (Ind, makeSubRegions.m:90).  Outputs (any may be NULL):
+ *   d_pixels, d_pix_offset, d_xsize, d_ysize  the chain batch (bamf_roi_batch):
+ *                  tile b compact, column-major, at d_pix_offset[b];
+ *                  bamf_subregion_pixels() floats in all;
+ *   d_frame, d_roi_ind   ROI_Struct.FrameNum and ROI_Struct.ROI_Ind (1-based);
+ *   d_roi_data     ROI_Data: (box+2overlap)^2 floats per tile, zero padded;
+ *   d_grid_offset, d_grid_place (with zoom)  the tiles' placement in one
+ *                  stitched (zoom ypix) x (zoom xpix) column-major image, for
+ *                  bamf_rjmcmc_out: the chains then add their counts straight
+ *                  into it, which is removeOverlapping.m:29-54 followed by
+ *                  stitchROIs.m:28-65 with ImSize(3) = 1 (RJ.m:231-234), with
+ *                  the per-position grids of analyzeROIs.m:153 summed over
+ *                  frames (all counts are integers, so the sum is exact).
+ * Tilings the MATLAB cannot index (a tile reaching past the frame) are rejected.
+ * bamf_subregion_count / _pixels return -1 for them.
+ * ------------------------------------------------------------------------ */
+int64_t bamf_subregion_count(int ypix, int xpix, int nframes, int box, int overlap);
+int64_t bamf_subregion_pixels(int ypix, int xpix, int nframes, int box, int overlap);
+int bamf_make_subregions(const float* d_data, int ypix, int xpix, int nframes, int box, int overlap,
+                         float* d_pixels, int64_t* d_pix_offset, int32_t* d_xsize, int32_t* d_ysize,
+                         int32_t* d_frame, int32_t* d_roi_ind, float* d_roi_data, int zoom,
+                         int64_t* d_grid_offset, int32_t* d_grid_place, void* stream);
+
+/* ------------------------------------------------------------------------
  * Reversible-jump MCMC chains, one per ROI.
  * Replaces the cRJMCMC mex entry (source/c/cRJMCMC.cpp:52, driving
  * RJMCMC::start_Chain, source/c/LibRJMCMC.cpp:685-750) for a whole batch of
@@ -204,6 +233,13 @@ typedef struct bamf_rjmcmc_out {
     float* d_pgrid;          /* zoomed posterior images (update_grids, LibRJMCMC.cpp:1629-1648) */
     float* d_pback;
     const int64_t* d_grid_offset; /* [n_roi] offset of ROI r's (ysize*zoom) x (xsize*zoom) grid */
+    /* [n_roi][5] or NULL: where ROI r's grid lands in a larger image (the stitched
+     * posterior image of removeOverlapping.m + stitchROIs.m): the image's row count
+     * (the grid's column stride), then the core window x_lo, x_hi, y_lo, y_hi in grid
+     * coordinates; counts outside the window are dropped, as removeOverlapping.m:38-52
+     * zeroes them.  bamf_make_subregions writes it.  NULL: the ROI's own
+     * (ysize*zoom) x (xsize*zoom) grid, whole. */
+    const int32_t* d_grid_place;
     /* Chain records (all four arrays or none): ROI r's records at
      * [r * rec_cap ..), its emitter entries (x + DriftX, y + DriftY, I, Signal; one
      * (-100, -100, 0, -1) entry for an empty state) at [r * em_cap ..) in units
