@@ -45,29 +45,31 @@ __host__ __device__ inline int64_t tile_prefix(int i, int box, int ov) {
     return i <= 1 ? 0 : (int64_t)(box + ov) + (int64_t)(i - 2) * (box + 2 * ov);
 }
 
-// One work-group per tile: metadata by thread 0, pixels by all threads
-// (rows fastest, matching both layouts, so the frame reads are coalesced).
-__global__ __launch_bounds__(256) void k_tiles(const float* __restrict__ data, TileGeo g, float* __restrict__ pixels,
-                                               int64_t* __restrict__ pix_offset, int32_t* __restrict__ xsize,
-                                               int32_t* __restrict__ ysize, int32_t* __restrict__ frame,
-                                               int32_t* __restrict__ roi_ind, float* __restrict__ roi_data,
-                                               int zoom, int64_t* __restrict__ grid_offset,
-                                               int32_t* __restrict__ grid_place) {
-    const int64_t b = blockIdx.x;  // 0-based Ind - 1
-    const int per_frame = g.xsub * g.ysub;
-    const int f = (int)(b / per_frame);
-    const int pos = (int)(b - (int64_t)f * per_frame);  // Ysub (ix-1) + (iy-1)
-    const int ix = pos / g.ysub + 1, iy = pos - (ix - 1) * g.ysub + 1;
-    const int w = tile_size(ix, g.xsub, g.xpix, g.box, g.ov), h = tile_size(iy, g.ysub, g.ypix, g.box, g.ov);
-    const int sx = tile_start(ix, g.box, g.ov), sy = tile_start(iy, g.box, g.ov);
-    const int64_t off = (int64_t)f * g.sw * g.sh + tile_prefix(ix, g.box, g.ov) * g.sh +
-                        (int64_t)w * tile_prefix(iy, g.box, g.ov);
-    if (threadIdx.x == 0) {
-        if (pix_offset) pix_offset[b] = off;
+// One work-group per tile column (frame f, position ix: the Ysub tiles that
+// share a column range, contiguous in the compact batch, tile iy's W x H(iy)
+// block after tile iy-1's): metadata by the first Ysub threads, then the
+// pixels, one stacked row per work-item, several loads in flight per thread.
+constexpr int kTileThreads = 256, kTileUnroll = 4;
+__global__ __launch_bounds__(kTileThreads) void k_tiles(const float* __restrict__ data, TileGeo g,
+                                                        float* __restrict__ pixels, int64_t* __restrict__ pix_offset,
+                                                        int32_t* __restrict__ xsize, int32_t* __restrict__ ysize,
+                                                        int32_t* __restrict__ frame, int32_t* __restrict__ roi_ind,
+                                                        float* __restrict__ roi_data, int zoom,
+                                                        int64_t* __restrict__ grid_offset,
+                                                        int32_t* __restrict__ grid_place) {
+    const int f = blockIdx.x / g.xsub;
+    const int ix = blockIdx.x - f * g.xsub + 1;
+    const int w = tile_size(ix, g.xsub, g.xpix, g.box, g.ov), sx = tile_start(ix, g.box, g.ov);
+    const int64_t col0 = (int64_t)f * g.sw * g.sh + tile_prefix(ix, g.box, g.ov) * g.sh;  // compact offset
+    const int64_t b0 = ((int64_t)f * g.xsub + (ix - 1)) * g.ysub;                          // first tile (0-based)
+    for (int iy = threadIdx.x + 1; iy <= g.ysub; iy += blockDim.x) {
+        const int64_t b = b0 + iy - 1;
+        const int h = tile_size(iy, g.ysub, g.ypix, g.box, g.ov), sy = tile_start(iy, g.box, g.ov);
+        if (pix_offset) pix_offset[b] = col0 + (int64_t)w * tile_prefix(iy, g.box, g.ov);
         if (xsize) xsize[b] = w;
         if (ysize) ysize[b] = h;
         if (frame) frame[b] = f + 1;
-        if (roi_ind) roi_ind[b] = pos + 1;
+        if (roi_ind) roi_ind[b] = g.ysub * (ix - 1) + iy;
         if (grid_offset) {
             const int64_t ld = (int64_t)zoom * g.ypix;
             grid_offset[b] = (int64_t)zoom * (sx - 1) * ld + (int64_t)zoom * (sy - 1);
@@ -80,19 +82,46 @@ __global__ __launch_bounds__(256) void k_tiles(const float* __restrict__ data, T
             gp[4] = iy != g.ysub ? hz - zo : hz;
         }
     }
-    const float* src = data + ((int64_t)f * g.xpix + (sx - 1)) * g.ypix + (sy - 1);
-    const int side = g.box + 2 * g.ov;
+    const float* src = data + ((int64_t)f * g.xpix + (sx - 1)) * g.ypix;  // column sx of frame f, row 1
     if (pixels) {
-        for (int e = threadIdx.x; e < w * h; e += blockDim.x) {
-            const int x = e / h, y = e - x * h;
-            pixels[off + e] = src[(int64_t)x * g.ypix + y];
+        // Stacked row R of the column (the tiles' rows one after another, R <
+        // sum of the heights): its tile iy has a closed form (tiles 2 .. Ysub-1
+        // all have height hm), so each work-item resolves its row once and then
+        // walks the w columns: frame reads coalesced across rows, compact
+        // writes contiguous within each tile's column.
+        const int h1 = tile_size(1, g.ysub, g.ypix, g.box, g.ov), hm = g.box + 2 * g.ov;
+        for (int R = threadIdx.x; R < (int)g.sh; R += kTileThreads) {
+            int iy, y;
+            if (R < h1 || g.ysub == 1) {
+                iy = 1; y = R;
+            } else {
+                iy = min((R - h1) / hm + 2, g.ysub);
+                y = R - h1 - (iy - 2) * hm;
+            }
+            const int h = tile_size(iy, g.ysub, g.ypix, g.box, g.ov);
+            const float* sp = src + tile_start(iy, g.box, g.ov) - 1 + y;
+            float* dp = pixels + col0 + (int64_t)w * tile_prefix(iy, g.box, g.ov) + y;
+            int x = 0;
+            for (; x + kTileUnroll <= w; x += kTileUnroll) {
+                float v[kTileUnroll];
+#pragma unroll
+                for (int u = 0; u < kTileUnroll; u++) v[u] = sp[(int64_t)(x + u) * g.ypix];
+#pragma unroll
+                for (int u = 0; u < kTileUnroll; u++) dp[(int64_t)(x + u) * h] = v[u];
+            }
+            for (; x < w; x++) dp[(int64_t)x * h] = sp[(int64_t)x * g.ypix];
         }
     }
     if (roi_data) {  // ROI_Data(1:ROIey, 1:ROIex, Ind), zero elsewhere (makeSubRegions.m:45,95)
-        float* dst = roi_data + b * side * side;
-        for (int e = threadIdx.x; e < side * side; e += blockDim.x) {
-            const int x = e / side, y = e - x * side;
-            dst[e] = (x < w && y < h) ? src[(int64_t)x * g.ypix + y] : 0.0f;
+        const int side = g.box + 2 * g.ov;
+        const int64_t ne = (int64_t)g.ysub * side * side;
+        float* dst = roi_data + b0 * side * side;
+        for (int64_t e = threadIdx.x; e < ne; e += kTileThreads) {
+            const int t = (int)(e / (side * side));
+            const int r = (int)(e - (int64_t)t * side * side);
+            const int x = r / side, y = r - x * side;
+            const int iy = t + 1, h = tile_size(iy, g.ysub, g.ypix, g.box, g.ov);
+            dst[e] = (x < w && y < h) ? src[(int64_t)x * g.ypix + tile_start(iy, g.box, g.ov) - 1 + y] : 0.0f;
         }
     }
 }
@@ -149,8 +178,9 @@ int bamf_make_subregions(const float* d_data, int ypix, int xpix, int nframes, i
     const int64_t n = (int64_t)g.xsub * g.ysub * nframes;
     if (n == 0) return BAMF_OK;
     BAMF_CHECK_ARG(d_data || !(d_pixels || d_roi_data), "null frames");
-    BAMF_CHECK_ARG(n <= 0x7fffffff, "too many tiles");
-    hipLaunchKernelGGL(k_tiles, dim3((unsigned)n), dim3(256), 0, as_stream(stream), d_data, g, d_pixels,
+    BAMF_CHECK_ARG((int64_t)g.xsub * nframes <= 0x7fffffff, "too many tile columns");
+    hipLaunchKernelGGL(k_tiles, dim3((unsigned)((int64_t)g.xsub * nframes)), dim3(kTileThreads), 0,
+                       as_stream(stream), d_data, g, d_pixels,
                        d_pix_offset, d_xsize, d_ysize, d_frame, d_roi_ind, d_roi_data, zoom, d_grid_offset,
                        d_grid_place);
     BAMF_LAUNCH_CHECK("k_tiles");

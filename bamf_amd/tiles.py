@@ -34,18 +34,21 @@ class Tiling:
         also places each tile's zoom x zoom grid in the stitched image."""
         B = _hip.DeviceBuffer
         n = max(self.n, 1)
-        self.d_pixels = B(4 * max(self.npix, 1))
-        self.d_off, self.d_xs, self.d_ys = B(8 * n), B(4 * n), B(4 * n)
-        self.d_frame, self.d_ind = B(4 * n), B(4 * n)
-        self.d_roi_data = B(4 * self.side * self.side * n) if roi_data else None
+        if getattr(self, "d_pixels", None) is None:  # outputs allocated once, reused by later runs
+            self.d_pixels = B(4 * max(self.npix, 1))
+            self.d_off, self.d_xs, self.d_ys = B(8 * n), B(4 * n), B(4 * n)
+            self.d_frame, self.d_ind = B(4 * n), B(4 * n)
+            self.d_roi_data = self.d_goff = self.d_place = None
+        if roi_data and self.d_roi_data is None:
+            self.d_roi_data = B(4 * self.side * self.side * n)
         self.zoom = int(zoom)
-        self.d_goff = B(8 * n) if zoom else None
-        self.d_place = B(4 * 5 * n) if zoom else None
-        vp = lambda b: b.vp if b is not None else None  # noqa: E731
+        if zoom and self.d_goff is None:
+            self.d_goff, self.d_place = B(8 * n), B(4 * 5 * n)
+        vp = lambda b, on=True: b.vp if (b is not None and on) else None  # noqa: E731
         check(lib().bamf_make_subregions(d_data.vp, self.ypix, self.xpix, self.nframes, self.box, self.overlap,
                                          self.d_pixels.vp, self.d_off.vp, self.d_xs.vp, self.d_ys.vp,
-                                         self.d_frame.vp, self.d_ind.vp, vp(self.d_roi_data), self.zoom,
-                                         vp(self.d_goff), vp(self.d_place), stream.vp if stream else None),
+                                         self.d_frame.vp, self.d_ind.vp, vp(self.d_roi_data, roi_data), self.zoom,
+                                         vp(self.d_goff, zoom), vp(self.d_place, zoom), stream.vp if stream else None),
               "bamf_make_subregions")
         return self
 

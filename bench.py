@@ -350,8 +350,51 @@ def bench_find_map(args):
             "cpu_baseline": cpu}
 
 
+def _cpu_tiles(fr):
+    from oracle import tiles as ot
+
+    data, box, ov = _FE["args"]
+    for f in fr:
+        ot.make_subregions(data[:, :, f:f + 1], box, ov)
+    return len(fr)
+
+
+def bench_tiles(args):
+    """Frame tiling (makeSubRegions.m, box 16, overlap 3) of 2000 frames of
+    256 x 256 into the chain batch, with each tile's placement in the stitched
+    posterior image at zoom 20."""
+    from bamf_amd import _hip
+    from bamf_amd.tiles import Tiling
+
+    nf, size, box, ov, zoom = 2000, 256, 16, 3, 20
+    rng = np.random.default_rng(SEED + 6)
+    data = rng.poisson(10.0, size=(size, size, nf)).astype(np.float32)  # (y, x, f)
+    cpu = None
+    if not args.no_cpu:
+        sample = nf  # the whole workload: about 3 s on one core
+        _FE["args"] = (data[:, :, :sample], box, ov)
+        dt = _cpu_pool(_cpu_tiles, [list(range(sample))], 1)
+        print(f"[bench] cpu baseline: {sample} frames in {dt:.1f} s", file=sys.stderr, flush=True)
+        cpu = {"value": sample * size * size / dt, "unit": "pixels/s", "cores": 1, "kind": "port",
+               "sample": f"all {sample} frames, oracle numpy makeSubRegions restatement, 1 core, {dt:.1f} s wall"}
+    d = _hip.DeviceBuffer.from_array(data.ravel(order="F"))
+    t = Tiling(size, size, nf, box, ov)
+    s = _hip.Stream()
+    ms = _time_gpu(lambda: t.run(d, zoom=zoom, stream=s), s, args.warmup, args.steps)
+    px = nf * size * size
+    alg_bytes = 4.0 * px + 4.0 * t.npix + t.n * (8 + 4 * 4 + 8 + 20)  # frames in, tiles and metadata out
+    return {"metric": "frame tiling pixels/s (2000 x 256 x 256, box 16, overlap 3)", "value": px / (ms / 1e3),
+            "unit": "pixels/s", "ms_per_step": ms, "dtype": "f32", "higher_is_better": True,
+            "config": {"workload": "tiles", "frames": nf, "size": size, "box": box, "overlap": ov, "tiles": t.n,
+                       "tile_pixels": t.npix},
+            "roofline": {"bound": "hbm", "achieved": None, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": None,
+                         "kernel": "k_tiles", "traffic": None, "alg_bytes_per_launch": alg_bytes},
+            "cpu_baseline": cpu}
+
+
 def front_end_line(args):
-    res = {"find_roi": bench_find_roi, "gauss_mle": bench_gauss_mle, "find_map": bench_find_map}[args.workload](args)
+    res = {"find_roi": bench_find_roi, "gauss_mle": bench_gauss_mle, "find_map": bench_find_map,
+           "tiles": bench_tiles}[args.workload](args)
     print(f"[bench] {args.workload}: {res['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     if res["roofline"]["bound"] == "hbm":
         r = res["roofline"]
@@ -371,7 +414,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batched_uneven",
-                    choices=["batched_uneven", "plain_16", "find_roi", "gauss_mle", "find_map"])
+                    choices=["batched_uneven", "plain_16", "find_roi", "gauss_mle", "find_map", "tiles"])
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--cpu-sample-frames", type=int, default=2000)
     ap.add_argument("--cpu-sample-fits", type=int, default=100000)
@@ -392,7 +435,7 @@ def main():
 
     if args.copy_mib:
         return copy_calibration(args.copy_mib)
-    if args.workload in ("find_roi", "gauss_mle", "find_map"):
+    if args.workload in ("find_roi", "gauss_mle", "find_map", "tiles"):
         return front_end_line(args)
     rois, nb, nt, rho, sig, off, back, bg0 = workload(args.workload)
     p = params(nb, nt, rho)
