@@ -1,6 +1,8 @@
-// Single-emitter Newton MLE of theta = {x, y, N, bg} with CRLB and
-// log-likelihood (fit type 'Basic').  Same inputs, outputs and iteration as
-// kernel_MLEFit_XYNB_ (source/cuda/cuda_gaussMLEv2/cuda_gaussMLEv2.cu:16-141).
+// Single-emitter Newton MLE of theta = {x, y, N, bg} (fit type 'Basic', NV =
+// 4) or {x, y, N, bg, sigma} (fit type 'Sigma', NV = 5) with CRLB and
+// log-likelihood.  Same inputs, outputs and iteration as kernel_MLEFit_XYNB_
+// (source/cuda/cuda_gaussMLEv2/cuda_gaussMLEv2.cu:16-141) and
+// kernel_MLEFit_XYNBS_ (cuda_gaussMLEv2.cu:144-276).
 //
 // gfx950 design: one work-item per fit, the box staged through LDS so that
 // global loads are contiguous, the pixel-integrated Gaussian evaluated in
@@ -16,13 +18,13 @@ namespace bamf {
 namespace {
 
 constexpr float kPi = 3.141592f;  // definitions.h:13
-constexpr int kNV = 4;
 constexpr int kBlock = 64;
 
-// 4x4 partial inverse: in-place LU without pivoting, diagonal of the inverse
-// (same algorithm as kernel_MatInvN, MatInvLib.h:15-75).
-__device__ inline void mat_inv4_diag(float* m, float* diag) {
-    constexpr int sz = kNV;
+// NV x NV partial inverse: in-place LU without pivoting, diagonal of the
+// inverse (same algorithm as kernel_MatInvN, MatInvLib.h:15-75).
+template <int NV>
+__device__ inline void mat_inv_diag(float* m, float* diag) {
+    constexpr int sz = NV;
     float minv[sz * sz];
     for (int jj = 0; jj < sz; jj++) {
         for (int ii = 1; ii <= jj; ii++) {
@@ -58,9 +60,14 @@ __device__ inline void mat_inv4_diag(float* m, float* diag) {
 // derivatives (GPUgaussLib.cuh:14-24, 67-87) for pixel index i.  The
 // reference's derivative is ((-N/sqrt(2pi))/sigma*(a-b))*PSF_other, so the
 // axis factor d1 times the other axis's PSF reproduces it exactly.
+// With SG, also the sigma-derivative factors of GPUgaussLib.cuh:90-111: the
+// reference's dS = ((-N/sqrt(2pi))/S/S*(a(i-x+1) - b(i-x)))*PSF_other = s1 *
+// PSF_other, and its second derivative -2/S*dS - s2 * PSF_other with
+// s2 = (N/sqrt(2pi))/S^5*(a(i-x+1)^3 - b(i-x)^3).
 struct AxisTerms {
-    float psf, d1, d2;
+    float psf, d1, d2, s1, s2;
 };
+template <bool SG>
 __device__ inline AxisTerms axis_terms(int i, float x, float sigma, float sqrt_norm, float n,
                                        float c1, float c2) {
     AxisTerms t;
@@ -72,10 +79,18 @@ __device__ inline AxisTerms axis_terms(int i, float x, float sigma, float sqrt_n
     float b = __expf(-0.5f * (tb * tb));
     t.d1 = -n / c1 / sigma * (a - b);
     t.d2 = -n / c1 / c2 * ((i + 1.0f - x) * a - (i - x) * b);
+    if constexpr (SG) {
+        const float p1 = i - x + 1.0f, p0 = i - x;
+        t.s1 = -n / c1 / sigma / sigma * (a * p1 - b * p0);
+        const float s2 = sigma * sigma;
+        t.s2 = n / c1 / (s2 * s2 * sigma) * (a * (p1 * p1 * p1) - b * (p0 * p0 * p0));
+    } else {
+        t.s1 = t.s2 = 0.f;
+    }
     return t;
 }
 
-template <int SZ>
+template <int SZ, int NV>
 __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ data, float sigma,
                                                      int iterations, float* __restrict__ params,
                                                      float* __restrict__ crlb, float* __restrict__ ll,
@@ -99,7 +114,8 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
     for (int p = 0; p < NP; p++) d[p] = box[p];
 
     // --- initial values: centre of mass (GPUgaussLib.cuh:207-227) ---
-    float theta[kNV];
+    constexpr bool SG = NV == 5;
+    float theta[NV];
     {
         float sx = 0.f, sy = 0.f, ss = 0.f;
 #pragma unroll
@@ -152,19 +168,27 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
             }
         theta[3] = fmin_;
         theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
+        if constexpr (SG) theta[4] = sigma;
     }
 
-    const float sqrt_norm = sqrtf(1.0f / 2.0f / sigma / sigma);
     const float c1 = sqrtf(2.0f * kPi);
-    const float c2 = sigma * sigma * sigma;
+    float sg = sigma, sqrt_norm = sqrtf(1.0f / 2.0f / sigma / sigma), c2 = sigma * sigma * sigma;
     for (int it = 0; it < iterations; it++) {
+        if constexpr (SG) {  // the PSF width is theta[4]
+            sg = theta[4];
+            sqrt_norm = sqrtf(1.0f / 2.0f / sg / sg);
+            c2 = sg * sg * sg;
+        }
         AxisTerms ax[SZ], ay[SZ];
 #pragma unroll
         for (int i = 0; i < SZ; i++) {
-            ax[i] = axis_terms(i, theta[0], sigma, sqrt_norm, theta[2], c1, c2);
-            ay[i] = axis_terms(i, theta[1], sigma, sqrt_norm, theta[2], c1, c2);
+            ax[i] = axis_terms<SG>(i, theta[0], sg, sqrt_norm, theta[2], c1, c2);
+            ay[i] = axis_terms<SG>(i, theta[1], sg, sqrt_norm, theta[2], c1, c2);
         }
-        float num[kNV] = {0.f, 0.f, 0.f, 0.f}, den[kNV] = {0.f, 0.f, 0.f, 0.f};
+        const float m2s = -2.0f / sg;
+        float num[NV], den[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) num[k] = den[k] = 0.f;
 #pragma unroll
         for (int ii = 0; ii < SZ; ii++)
 #pragma unroll
@@ -191,26 +215,40 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
                 den[2] += -(u2 * u2) * df;
                 num[3] += cf;
                 den[3] += -df;
+                if constexpr (SG) {
+                    const float dsx = ax[ii].s1 * py, dsy = ay[jj].s1 * px;
+                    const float u4 = dsx + dsy;
+                    const float v4 = (m2s * dsx - ax[ii].s2 * py) + (m2s * dsy - ay[jj].s2 * px);
+                    num[4] += u4 * cf;
+                    den[4] += v4 * cf - (u4 * u4) * df;
+                }
             }
         theta[0] -= fminf(fmaxf(num[0] / den[0], -1e0f), 1e0f);
         theta[1] -= fminf(fmaxf(num[1] / den[1], -1e0f), 1e0f);
         theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
         theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
+        if constexpr (SG) theta[4] -= fminf(fmaxf(num[4] / den[4], -theta[4]), theta[4]);
         theta[2] = fmaxf(theta[2], 1.0f);
         theta[3] = fmaxf(theta[3], 0.01f);
+        if constexpr (SG) theta[4] = fminf(fmaxf(theta[4], 0.5f), SZ / 2.0f);  // cuda_gaussMLEv2.cu:237-238
+    }
+    if constexpr (SG) {
+        sg = theta[4];
+        sqrt_norm = sqrtf(1.0f / 2.0f / sg / sg);
+        c2 = sg * sg * sg;
     }
 
-    // --- Fisher information, CRLB and log-likelihood (cuda_gaussMLEv2.cu:105-138) ---
-    float M[kNV * kNV];
+    // --- Fisher information, CRLB and log-likelihood (cuda_gaussMLEv2.cu:105-138, 243-266) ---
+    float M[NV * NV];
 #pragma unroll
-    for (int k = 0; k < kNV * kNV; k++) M[k] = 0.f;
+    for (int k = 0; k < NV * NV; k++) M[k] = 0.f;
     float div = 0.f;
     {
         AxisTerms ax[SZ], ay[SZ];
 #pragma unroll
         for (int i = 0; i < SZ; i++) {
-            ax[i] = axis_terms(i, theta[0], sigma, sqrt_norm, theta[2], c1, c2);
-            ay[i] = axis_terms(i, theta[1], sigma, sqrt_norm, theta[2], c1, c2);
+            ax[i] = axis_terms<SG>(i, theta[0], sg, sqrt_norm, theta[2], c1, c2);
+            ay[i] = axis_terms<SG>(i, theta[1], sg, sqrt_norm, theta[2], c1, c2);
         }
 #pragma unroll
         for (int ii = 0; ii < SZ; ii++)
@@ -219,27 +257,32 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
                 const float px = ax[ii].psf, py = ay[jj].psf;
                 const float model = theta[3] + theta[2] * px * py;
                 const float dat = d[SZ * jj + ii];
-                float du[kNV] = {ax[ii].d1 * py, ay[jj].d1 * px, px * py, 1.0f};
+                float du[NV];
+                du[0] = ax[ii].d1 * py;
+                du[1] = ay[jj].d1 * px;
+                du[2] = px * py;
+                du[3] = 1.0f;
+                if constexpr (SG) du[4] = ax[ii].s1 * py + ay[jj].s1 * px;
                 const float rm = __builtin_amdgcn_rcpf(model);
 #pragma unroll
-                for (int k = 0; k < kNV; k++)
+                for (int k = 0; k < NV; k++)
 #pragma unroll
-                    for (int l = k; l < kNV; l++) M[k * kNV + l] += du[l] * du[k] * rm;
+                    for (int l = k; l < NV; l++) M[k * NV + l] += du[l] * du[k] * rm;
                 if (model > 0) {
                     if (dat > 0) div += dat * logf(model) - model - dat * logf(dat) + dat;
                     else div += -model;
                 }
             }
 #pragma unroll
-        for (int k = 0; k < kNV; k++)
+        for (int k = 0; k < NV; k++)
 #pragma unroll
-            for (int l = k + 1; l < kNV; l++) M[l * kNV + k] = M[k * kNV + l];
+            for (int l = k + 1; l < NV; l++) M[l * NV + k] = M[k * NV + l];
     }
-    float diag[kNV];
-    mat_inv4_diag(M, diag);
+    float diag[NV];
+    mat_inv_diag<NV>(M, diag);
     const int64_t i = first + tx;
 #pragma unroll
-    for (int k = 0; k < kNV; k++) {
+    for (int k = 0; k < NV; k++) {
         params[(int64_t)nfits * k + i] = theta[k];
         crlb[(int64_t)nfits * k + i] = diag[k];
     }
@@ -247,14 +290,16 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
 }
 
 // Any box size: same algorithm reading the box from global memory.
+template <int NV>
 __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict__ data, float sigma,
                                                          int sz, int iterations, float* __restrict__ params,
                                                          float* __restrict__ crlb, float* __restrict__ ll,
                                                          int nfits) {
+    constexpr bool SG = NV == 5;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= nfits) return;
     const float* d = data + i * sz * sz;
-    float theta[kNV];
+    float theta[NV];
     {
         float sx = 0.f, sy = 0.f, ss = 0.f;
         for (int ii = 0; ii < sz; ii++)
@@ -287,16 +332,23 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
             }
         theta[3] = fmin_;
         theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
+        if constexpr (SG) theta[4] = sigma;
     }
-    const float sqrt_norm = sqrtf(1.0f / 2.0f / sigma / sigma);
     const float c1 = sqrtf(2.0f * kPi);
-    const float c2 = sigma * sigma * sigma;
+    float sg = sigma, sqrt_norm = sqrtf(1.0f / 2.0f / sigma / sigma), c2 = sigma * sigma * sigma;
     for (int it = 0; it < iterations; it++) {
-        float num[kNV] = {0.f, 0.f, 0.f, 0.f}, den[kNV] = {0.f, 0.f, 0.f, 0.f};
+        if constexpr (SG) {
+            sg = theta[4];
+            sqrt_norm = sqrtf(1.0f / 2.0f / sg / sg);
+            c2 = sg * sg * sg;
+        }
+        const float m2s = -2.0f / sg;
+        float num[NV], den[NV];
+        for (int k = 0; k < NV; k++) num[k] = den[k] = 0.f;
         for (int ii = 0; ii < sz; ii++) {
-            AxisTerms ax = axis_terms(ii, theta[0], sigma, sqrt_norm, theta[2], c1, c2);
+            AxisTerms ax = axis_terms<SG>(ii, theta[0], sg, sqrt_norm, theta[2], c1, c2);
             for (int jj = 0; jj < sz; jj++) {
-                AxisTerms ay = axis_terms(jj, theta[1], sigma, sqrt_norm, theta[2], c1, c2);
+                AxisTerms ay = axis_terms<SG>(jj, theta[1], sg, sqrt_norm, theta[2], c1, c2);
                 const float px = ax.psf, py = ay.psf;
                 const float model = theta[3] + theta[2] * px * py;
                 const float dat = d[sz * jj + ii];
@@ -313,51 +365,67 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
                 num[1] += u1 * cf; den[1] += v1 * cf - (u1 * u1) * df;
                 num[2] += u2 * cf; den[2] += -(u2 * u2) * df;
                 num[3] += cf;      den[3] += -df;
+                if constexpr (SG) {
+                    const float dsx = ax.s1 * py, dsy = ay.s1 * px;
+                    const float u4 = dsx + dsy;
+                    const float v4 = (m2s * dsx - ax.s2 * py) + (m2s * dsy - ay.s2 * px);
+                    num[4] += u4 * cf;
+                    den[4] += v4 * cf - (u4 * u4) * df;
+                }
             }
         }
         theta[0] -= fminf(fmaxf(num[0] / den[0], -1e0f), 1e0f);
         theta[1] -= fminf(fmaxf(num[1] / den[1], -1e0f), 1e0f);
         theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
         theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
+        if constexpr (SG) theta[4] -= fminf(fmaxf(num[4] / den[4], -theta[4]), theta[4]);
         theta[2] = fmaxf(theta[2], 1.0f);
         theta[3] = fmaxf(theta[3], 0.01f);
+        if constexpr (SG) theta[4] = fminf(fmaxf(theta[4], 0.5f), sz / 2.0f);
     }
-    float M[kNV * kNV];
-    for (int k = 0; k < kNV * kNV; k++) M[k] = 0.f;
+    if constexpr (SG) {
+        sg = theta[4];
+        sqrt_norm = sqrtf(1.0f / 2.0f / sg / sg);
+        c2 = sg * sg * sg;
+    }
+    float M[NV * NV];
+    for (int k = 0; k < NV * NV; k++) M[k] = 0.f;
     float div = 0.f;
     for (int ii = 0; ii < sz; ii++) {
-        AxisTerms ax = axis_terms(ii, theta[0], sigma, sqrt_norm, theta[2], c1, c2);
+        AxisTerms ax = axis_terms<SG>(ii, theta[0], sg, sqrt_norm, theta[2], c1, c2);
         for (int jj = 0; jj < sz; jj++) {
-            AxisTerms ay = axis_terms(jj, theta[1], sigma, sqrt_norm, theta[2], c1, c2);
+            AxisTerms ay = axis_terms<SG>(jj, theta[1], sg, sqrt_norm, theta[2], c1, c2);
             const float px = ax.psf, py = ay.psf;
             const float model = theta[3] + theta[2] * px * py;
             const float dat = d[sz * jj + ii];
-            float du[kNV] = {ax.d1 * py, ay.d1 * px, px * py, 1.0f};
-            for (int k = 0; k < kNV; k++)
-                for (int l = k; l < kNV; l++) M[k * kNV + l] += du[l] * du[k] / model;
+            float du[NV];
+            du[0] = ax.d1 * py;
+            du[1] = ay.d1 * px;
+            du[2] = px * py;
+            du[3] = 1.0f;
+            if constexpr (SG) du[4] = ax.s1 * py + ay.s1 * px;
+            for (int k = 0; k < NV; k++)
+                for (int l = k; l < NV; l++) M[k * NV + l] += du[l] * du[k] / model;
             if (model > 0) {
                 if (dat > 0) div += dat * logf(model) - model - dat * logf(dat) + dat;
                 else div += -model;
             }
         }
     }
-    for (int k = 0; k < kNV; k++)
-        for (int l = k + 1; l < kNV; l++) M[l * kNV + k] = M[k * kNV + l];
-    float diag[kNV];
-    mat_inv4_diag(M, diag);
-    for (int k = 0; k < kNV; k++) {
+    for (int k = 0; k < NV; k++)
+        for (int l = k + 1; l < NV; l++) M[l * NV + k] = M[k * NV + l];
+    float diag[NV];
+    mat_inv_diag<NV>(M, diag);
+    for (int k = 0; k < NV; k++) {
         params[(int64_t)nfits * k + i] = theta[k];
         crlb[(int64_t)nfits * k + i] = diag[k];
     }
     ll[i] = div;
 }
 
-}  // namespace
-}  // namespace bamf
-
-extern "C" int bamf_gauss_mle_xynb(const float* d_data, float psf_sigma, int sz, int iterations,
-                                   float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream) {
-    using namespace bamf;
+template <int NV>
+int launch_mle(const float* d_data, float psf_sigma, int sz, int iterations, float* d_params, float* d_crlb,
+               float* d_ll, int nfits, void* stream) {
     BAMF_CHECK_ARG(nfits >= 0, "nfits must be >= 0 (got %d)", nfits);
     BAMF_CHECK_ARG(sz >= 1 && sz <= 64, "sz must be in [1, 64] (got %d)", sz);
     BAMF_CHECK_ARG(iterations >= 0, "iterations must be >= 0");
@@ -367,14 +435,27 @@ extern "C" int bamf_gauss_mle_xynb(const float* d_data, float psf_sigma, int sz,
     dim3 grid((unsigned)ceil_div(nfits, kBlock)), block(kBlock);
     hipStream_t s = as_stream(stream);
     switch (sz) {
-    case 5: hipLaunchKernelGGL(k_mle_xynb<5>, grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
-    case 6: hipLaunchKernelGGL(k_mle_xynb<6>, grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
-    case 7: hipLaunchKernelGGL(k_mle_xynb<7>, grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
-    case 8: hipLaunchKernelGGL(k_mle_xynb<8>, grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
-    case 9: hipLaunchKernelGGL(k_mle_xynb<9>, grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
+    case 5: hipLaunchKernelGGL((k_mle_xynb<5, NV>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
+    case 6: hipLaunchKernelGGL((k_mle_xynb<6, NV>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
+    case 7: hipLaunchKernelGGL((k_mle_xynb<7, NV>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
+    case 8: hipLaunchKernelGGL((k_mle_xynb<8, NV>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
+    case 9: hipLaunchKernelGGL((k_mle_xynb<9, NV>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
     default:
-        hipLaunchKernelGGL(k_mle_xynb_any, grid, block, 0, s, d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits);
+        hipLaunchKernelGGL(k_mle_xynb_any<NV>, grid, block, 0, s, d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits);
     }
     BAMF_LAUNCH_CHECK("k_mle_xynb");
     return BAMF_OK;
+}
+
+}  // namespace
+}  // namespace bamf
+
+extern "C" int bamf_gauss_mle_xynb(const float* d_data, float psf_sigma, int sz, int iterations,
+                                   float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream) {
+    return bamf::launch_mle<4>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, stream);
+}
+
+extern "C" int bamf_gauss_mle_xynbs(const float* d_data, float psf_sigma, int sz, int iterations,
+                                    float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream) {
+    return bamf::launch_mle<5>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, stream);
 }

@@ -1,10 +1,11 @@
-"""Host mirror of @RJ/gaussMLE.m for fit type 'Basic' on a CCD camera.
+"""Host mirror of @RJ/gaussMLE.m for fit types 'Basic' and 'Sigma' on a CCD camera.
 
 ``gauss_mle(data, 'Basic', 'CCD', sigma)`` returns ``(Results, Statistics)``
-with the fields gaussMLE.m:183-210 builds (Y, X, Photons, BG, *_SE,
-LogLikelihood; FitTime, FitPerSecond, NKernelsCalls, NFitsPerCall).  The fit
-runs in one call of bamf_gauss_mle_xynb on the device; there is no host
-fallback.
+with the fields gaussMLE.m:183-210 builds (Y, X, Photons, BG, *_SE, and Sigma,
+Sigma_SE for 'Sigma'; LogLikelihood; FitTime, FitPerSecond, NKernelsCalls,
+NFitsPerCall).  The fit runs in one call of bamf_gauss_mle_xynb ('Basic',
+kernel _XYNB_) or bamf_gauss_mle_xynbs ('Sigma', kernel _XYNBS_) on the
+device; there is no host fallback.
 """
 from __future__ import annotations
 
@@ -16,34 +17,45 @@ from . import _hip
 from ._lib import BamfError, check, lib
 
 
+_KERNELS = {"Basic": ("bamf_gauss_mle_xynb", 4), "Sigma": ("bamf_gauss_mle_xynbs", 5)}
+
+
 def gauss_mle_device(d_data, sigma: float, sz: int, iterations: int, nfits: int, d_params, d_crlb, d_ll,
-                     stream=None) -> None:
-    """Direct C-ABI call on device buffers (kernel_MLEFit_XYNB_ argument layout)."""
-    check(lib().bamf_gauss_mle_xynb(d_data.vp, float(sigma), int(sz), int(iterations), d_params.vp, d_crlb.vp,
-                                    d_ll.vp, int(nfits), stream.vp if stream else None), "bamf_gauss_mle_xynb")
+                     stream=None, fit_type: str = "Basic") -> None:
+    """Direct C-ABI call on device buffers (kernel_MLEFit_XYNB_ / _XYNBS_ argument layout)."""
+    if fit_type not in _KERNELS:
+        raise BamfError(f"fit type {fit_type!r}: 'Basic' and 'Sigma' are implemented")
+    name = _KERNELS[fit_type][0]
+    check(getattr(lib(), name)(d_data.vp, float(sigma), int(sz), int(iterations), d_params.vp, d_crlb.vp,
+                               d_ll.vp, int(nfits), stream.vp if stream else None), name)
 
 
-def fit_boxes(boxes: np.ndarray, sigma: float, iterations: int = 20):
+def fit_boxes(boxes: np.ndarray, sigma: float, iterations: int = 20, fit_type: str = "Basic"):
     """boxes: (N, sz, sz) float32 with box[i, jj, ii] = data[sz*jj + ii].
-    Returns (params (4, N), crlb (4, N), ll (N,)) in the kernel's parameter-major layout."""
+    Returns (params (NP, N), crlb (NP, N), ll (N,)) in the kernel's parameter-major
+    layout, NP = 4 ('Basic') or 5 ('Sigma')."""
     boxes = np.ascontiguousarray(boxes, dtype=np.float32)
     n, sz = boxes.shape[0], boxes.shape[1]
     if boxes.shape[1] != boxes.shape[2]:
         raise BamfError("boxes must be square")
+    if fit_type not in _KERNELS:
+        raise BamfError(f"fit type {fit_type!r}: 'Basic' and 'Sigma' are implemented")
+    npar = _KERNELS[fit_type][1]
     d_data = _hip.DeviceBuffer.from_array(boxes)
-    d_p = _hip.DeviceBuffer(4 * n * 4)
-    d_c = _hip.DeviceBuffer(4 * n * 4)
+    d_p = _hip.DeviceBuffer(4 * npar * n)
+    d_c = _hip.DeviceBuffer(4 * npar * n)
     d_l = _hip.DeviceBuffer(n * 4)
-    gauss_mle_device(d_data, sigma, sz, iterations, n, d_p, d_c, d_l)
+    gauss_mle_device(d_data, sigma, sz, iterations, n, d_p, d_c, d_l, fit_type=fit_type)
     _hip.synchronize()
-    return (d_p.download(np.float32).reshape(4, n), d_c.download(np.float32).reshape(4, n),
+    return (d_p.download(np.float32).reshape(npar, n), d_c.download(np.float32).reshape(npar, n),
             d_l.download(np.float32))
 
 
 def gauss_mle(data: np.ndarray, fit_type: str = "Basic", camera_type: str = "CCD", sigma=1.3, iterations: int = 20):
     """data: MATLAB-layout stack (SZ, SZ, N) as a Fortran-ordered array, or (N, SZ, SZ) C-ordered boxes."""
-    if fit_type != "Basic" or camera_type != "CCD":
-        raise BamfError(f"fit type {fit_type!r} / camera {camera_type!r}: only 'Basic' on 'CCD' is implemented")
+    if fit_type not in _KERNELS or camera_type != "CCD":
+        raise BamfError(f"fit type {fit_type!r} / camera {camera_type!r}: 'Basic' and 'Sigma' on 'CCD' are "
+                        "implemented")
     data = np.asarray(data, dtype=np.float32)
     if data.ndim == 2:
         data = data[:, :, None] if data.flags.f_contiguous else data[None]
@@ -52,7 +64,7 @@ def gauss_mle(data: np.ndarray, fit_type: str = "Basic", camera_type: str = "CCD
     else:
         boxes = np.ascontiguousarray(data)
     t0 = time.perf_counter()
-    p, c, ll = fit_boxes(boxes, float(np.mean(sigma)), iterations)
+    p, c, ll = fit_boxes(boxes, float(np.mean(sigma)), iterations, fit_type)
     dt = time.perf_counter() - t0
     n = boxes.shape[0]
     results = {
@@ -60,6 +72,9 @@ def gauss_mle(data: np.ndarray, fit_type: str = "Basic", camera_type: str = "CCD
         "Y_SE": np.sqrt(c[0]), "X_SE": np.sqrt(c[1]), "Photons_SE": np.sqrt(c[2]), "BG_SE": np.sqrt(c[3]),
         "LogLikelihood": ll,
     }
+    if fit_type == "Sigma":  # gaussMLE.m:193-195
+        results["Sigma"] = p[4]
+        results["Sigma_SE"] = np.sqrt(c[4])
     stats = {"FitTime": dt, "FitPerSecond": n / dt if dt > 0 else float("inf"), "NKernelsCalls": 1,
              "NFitsPerCall": n}
     return results, stats

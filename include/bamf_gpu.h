@@ -46,6 +46,12 @@ int bamf_device_info(int device, int* cu_count, int* clock_khz, char* name, int 
  * ------------------------------------------------------------------------ */
 int bamf_gauss_mle_xynb(const float* d_data, float psf_sigma, int sz, int iterations,
                         float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream);
+/* gaussMLE fit type 'Sigma' (theta = {x, y, N, bg, sigma}): replaces
+ * kernel_MLEFit_XYNBS_ (cuda_gaussMLEv2.cu:144-145), bound by gaussMLE.m:98-100.
+ * Same layouts with five parameters; psf_sigma is the starting width, and the
+ * fitted width is kept in [0.5, sz/2] (cuda_gaussMLEv2.cu:237-238). */
+int bamf_gauss_mle_xynbs(const float* d_data, float psf_sigma, int sz, int iterations,
+                         float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream);
 
 /* ------------------------------------------------------------------------
  * FindROI.  A frame is `ysize` runs of `xsize` contiguous floats (the

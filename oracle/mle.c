@@ -1,8 +1,10 @@
 /*
- * Oracle: single-emitter Newton MLE of (x, y, I, bg) with CRLB and
- * log-likelihood.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+ * Oracle: single-emitter Newton MLE of (x, y, I, bg) and of (x, y, I, bg,
+ * sigma) with CRLB and log-likelihood.  TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
- * Restates kernel_MLEFit_XYNB_ (source/cuda/cuda_gaussMLEv2/cuda_gaussMLEv2.cu:16-141)
+ * Restates kernel_MLEFit_XYNB_ (source/cuda/cuda_gaussMLEv2/cuda_gaussMLEv2.cu:16-141),
+ * kernel_MLEFit_XYNBS_ (cuda_gaussMLEv2.cu:144-276) with its sigma derivatives
+ * kernel_DerivativeIntGauss1DSigma / 2DSigma (GPUgaussLib.cuh:90-137)
  * and its helpers kernel_IntGauss1D / kernel_DerivativeIntGauss1D
  * (GPUgaussLib.cuh:14-24, 67-87), kernel_CenterofMass2D (GPUgaussLib.cuh:207-227),
  * kernel_GaussFMaxMin2D (GPUgaussLib.cuh:230-258) and kernel_MatInvN
@@ -194,6 +196,127 @@ void orc_mle_xynb(const float* data, float psf_sigma, int sz, int iterations,
         float theta[NV], diag[NV], l;
         mle_one(data + (size_t)sz * sz * i, psf_sigma, sz, iterations, theta, diag, &l);
         for (int k = 0; k < NV; k++) {
+            params[(size_t)nfits * k + i] = theta[k];
+            crlb[(size_t)nfits * k + i] = diag[k];
+        }
+        ll[i] = l;
+    }
+}
+
+/* GPUgaussLib.cuh:90-111.  pow(v, 2.0f) is written v*v, pow(v, 3) v*v*v. */
+static void deriv_int_gauss_1d_sigma(int ii, float x, float sx, float n, float psfy, float* dudt, float* d2udt2) {
+    float ta = (ii + 1.0f - x) / sx;
+    float tb = (ii - x) / sx;
+    float ax = expf(-1.0f / 2.0f * (ta * ta));
+    float bx = expf(-1.0f / 2.0f * (tb * tb));
+    *dudt = -n / sqrtf(2.0f * kPi) / sx / sx * (ax * (ii - x + 1.0f) - bx * (ii - x)) * psfy;
+    if (d2udt2) {
+        const float p1 = ii - x + 1.0f, p0 = ii - x;
+        *d2udt2 = -2.0f / sx * dudt[0] -
+                  n / sqrtf(2.0f * kPi) / powf(sx, 5.0f) * (ax * (p1 * p1 * p1) - bx * (p0 * p0 * p0)) * psfy;
+    }
+}
+
+/* GPUgaussLib.cuh:114-137 */
+static void deriv_int_gauss_2d_sigma(int ii, int jj, float x, float y, float s, float n, float psfx, float psfy,
+                                     float* dudt, float* d2udt2) {
+    float dsx, dsy, ddsx, ddsy;
+    deriv_int_gauss_1d_sigma(ii, x, s, n, psfy, &dsx, &ddsx);
+    deriv_int_gauss_1d_sigma(jj, y, s, n, psfx, &dsy, &ddsy);
+    *dudt = dsx + dsy;
+    if (d2udt2) *d2udt2 = ddsx + ddsy;
+}
+
+#define NVS 5
+/* cuda_gaussMLEv2.cu:144-276, one fit per loop iteration instead of per thread. */
+static void mle_one_s(const float* s_data, float sigma, int sz, int iterations, float* theta, float* diagout,
+                      float* llout) {
+    float m[NVS * NVS], diag[NVS], minv[NVS * NVS];
+    float dudt[NVS], d2udt2[NVS], num[NVS], den[NVS];
+    float nmax, model, cf, df, data, psfx, psfy, div;
+    memset(m, 0, sizeof(m));
+    memset(minv, 0, sizeof(minv));
+    orc_center_of_mass_2d(sz, s_data, &theta[0], &theta[1]);
+    orc_gauss_fmaxmin_2d(sz, sigma, s_data, &nmax, &theta[3]);
+    theta[2] = fmaxf(0.0f, (nmax - theta[3]) * 2 * 2 * kPi * sigma * sigma);
+    theta[4] = sigma;
+
+    for (int kk = 0; kk < iterations; kk++) {
+        memset(num, 0, sizeof(num));
+        memset(den, 0, sizeof(den));
+        for (int ii = 0; ii < sz; ii++)
+            for (int jj = 0; jj < sz; jj++) {
+                psfx = orc_int_gauss_1d(ii, theta[0], theta[4]);
+                psfy = orc_int_gauss_1d(jj, theta[1], theta[4]);
+                model = theta[3] + theta[2] * psfx * psfy;
+                data = s_data[sz * jj + ii];
+                deriv_int_gauss_1d(ii, theta[0], theta[4], theta[2], psfy, &dudt[0], &d2udt2[0]);
+                deriv_int_gauss_1d(jj, theta[1], theta[4], theta[2], psfx, &dudt[1], &d2udt2[1]);
+                deriv_int_gauss_2d_sigma(ii, jj, theta[0], theta[1], theta[4], theta[2], psfx, psfy, &dudt[4],
+                                         &d2udt2[4]);
+                dudt[2] = psfx * psfy;
+                d2udt2[2] = 0.0f;
+                dudt[3] = 1.0f;
+                d2udt2[3] = 0.0f;
+                cf = 0.0f;
+                df = 0.0f;
+                if (model > 10e-3f) cf = data / model - 1;
+                if (model > 10e-3f) df = data / (model * model);
+                cf = fminf(cf, 10e4f);
+                df = fminf(df, 10e4f);
+                for (int ll = 0; ll < NVS; ll++) {
+                    num[ll] += dudt[ll] * cf;
+                    den[ll] += d2udt2[ll] * cf - (dudt[ll] * dudt[ll]) * df;
+                }
+            }
+        /* cuda_gaussMLEv2.cu:229-240 */
+        theta[0] -= fminf(fmaxf(num[0] / den[0], -1e0f), 1e0f);
+        theta[1] -= fminf(fmaxf(num[1] / den[1], -1e0f), 1e0f);
+        theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
+        theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
+        theta[4] -= fminf(fmaxf(num[4] / den[4], -theta[4]), theta[4]);
+        theta[2] = fmaxf(theta[2], 1.0f);
+        theta[3] = fmaxf(theta[3], 0.01f);
+        theta[4] = fmaxf(theta[4], 0.5f);
+        theta[4] = fminf(theta[4], sz / 2.0f);
+    }
+
+    /* cuda_gaussMLEv2.cu:243-266: Fisher matrix and log-likelihood. */
+    div = 0.0f;
+    for (int ii = 0; ii < sz; ii++)
+        for (int jj = 0; jj < sz; jj++) {
+            psfx = orc_int_gauss_1d(ii, theta[0], theta[4]);
+            psfy = orc_int_gauss_1d(jj, theta[1], theta[4]);
+            model = theta[3] + theta[2] * psfx * psfy;
+            data = s_data[sz * jj + ii];
+            deriv_int_gauss_1d(ii, theta[0], theta[4], theta[2], psfy, &dudt[0], NULL);
+            deriv_int_gauss_1d(jj, theta[1], theta[4], theta[2], psfx, &dudt[1], NULL);
+            deriv_int_gauss_2d_sigma(ii, jj, theta[0], theta[1], theta[4], theta[2], psfx, psfy, &dudt[4], NULL);
+            dudt[2] = psfx * psfy;
+            dudt[3] = 1.0f;
+            for (int k = 0; k < NVS; k++)
+                for (int l = k; l < NVS; l++) {
+                    m[k * NVS + l] += dudt[l] * dudt[k] / model;
+                    m[l * NVS + k] = m[k * NVS + l];
+                }
+            if (model > 0) {
+                if (data > 0)
+                    div += data * logf(model) - model - data * logf(data) + data;
+                else
+                    div += -model;
+            }
+        }
+    orc_mat_inv_n(m, minv, diag, NVS);
+    for (int k = 0; k < NVS; k++) diagout[k] = diag[k];
+    *llout = div;
+}
+
+void orc_mle_xynbs(const float* data, float psf_sigma, int sz, int iterations,
+                   float* params, float* crlb, float* ll, int nfits) {
+    for (int i = 0; i < nfits; i++) {
+        float theta[NVS], diag[NVS], l;
+        mle_one_s(data + (size_t)sz * sz * i, psf_sigma, sz, iterations, theta, diag, &l);
+        for (int k = 0; k < NVS; k++) {
             params[(size_t)nfits * k + i] = theta[k];
             crlb[(size_t)nfits * k + i] = diag[k];
         }

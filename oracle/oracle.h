@@ -33,6 +33,9 @@ uint32_t orc_chain_raw(uint64_t seed, uint32_t roi, uint32_t step, uint32_t j);
 /* ---- gaussMLE XYNB (cuda_gaussMLEv2.cu:16-141) --------------------------- */
 void orc_mle_xynb(const float* data, float psf_sigma, int sz, int iterations,
                   float* params, float* crlb, float* ll, int nfits);
+/* kernel_MLEFit_XYNBS_ (cuda_gaussMLEv2.cu:144-276): theta = {x, y, N, bg, sigma}. */
+void orc_mle_xynbs(const float* data, float psf_sigma, int sz, int iterations,
+                   float* params, float* crlb, float* ll, int nfits);
 /* Helpers exposed for unit tests. */
 float orc_int_gauss_1d(int ii, float x, float sigma);
 void  orc_center_of_mass_2d(int sz, const float* data, float* x, float* y);

@@ -56,3 +56,28 @@ def test_gauss_mle_matlab_mirror():
     np.testing.assert_allclose(res["X"], t["y"], atol=2e-3)
     np.testing.assert_allclose(res["Photons"], t["I"], rtol=2e-3)
     assert stats["NFitsPerCall"] == 256
+
+
+@pytest.mark.parametrize("sz,n,seed,true_sigma", [(7, 2000, 31, 1.3), (9, 500, 32, 1.6), (11, 200, 33, 1.1)])
+def test_mle_sigma_matches_oracle(oracle, sz, n, seed, true_sigma):
+    """bamf_gauss_mle_xynbs (kernel_MLEFit_XYNBS_) against the oracle: the
+    same bar as the 'Basic' fit for (x, y, I, bg, sigma)."""
+    boxes, _ = synth.make_mle_boxes(n, sz, seed=seed, sigma=true_sigma)
+    pg, cg, lg = fit_boxes(boxes, 1.3, 20, "Sigma")
+    po, co, lo = oracle.mle_xynbs(boxes, 1.3, 20)
+    print(f"sigma fit sz={sz} n={n} seed={seed}", flush=True)
+    for k in range(5):
+        r = rel(pg[k], po[k], 1.0)
+        assert np.quantile(r, 0.999) < 1e-4 and r.max() < 1e-3, (k, np.quantile(r, 0.999), r.max())
+    assert np.quantile(rel(cg, co, 1e-6), 0.999) < 1e-3
+    assert np.quantile(rel(lg, lo, 1.0), 0.999) < 1e-3
+    assert np.all(pg[4] >= 0.5) and np.all(pg[4] <= sz / 2)
+
+
+def test_gauss_mle_sigma_mirror():
+    boxes, t = synth.make_mle_boxes(400, 9, seed=34, noise=False, sigma=1.5, photons=(20.0, 100.0))
+    res, stats = gauss_mle(boxes, "Sigma", "CCD", 1.3, iterations=100)
+    np.testing.assert_allclose(res["Sigma"], 1.5, rtol=2e-3)
+    np.testing.assert_allclose(res["Photons"], t["I"], rtol=3e-3)
+    assert res["Sigma_SE"].shape == (400,) and np.all(np.isfinite(res["Sigma_SE"]))
+    assert stats["NFitsPerCall"] == 400

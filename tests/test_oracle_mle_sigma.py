@@ -1,0 +1,96 @@
+"""Pins the oracle's kernel_MLEFit_XYNBS_ restatement (oracle/mle.c,
+theta = {x, y, I, bg, sigma}) without the reference, by checks independent of
+its analytic sigma derivatives:
+  - noiseless boxes drawn with a PSF width other than the starting sigma: the
+    fit returns the generating (x, y, I, bg, sigma);
+  - CRLB against the inverse of a Fisher matrix from finite differences of the
+    float64 model (a dropped or mis-signed term of the sigma derivative moves
+    the sigma row);
+  - noisy boxes: the fit is the float64 Poisson maximum-likelihood point over
+    all five parameters;
+  - the sigma clamps of cuda_gaussMLEv2.cu:237-238 ([0.5, sz/2]) hold.
+"""
+import numpy as np
+import pytest
+from scipy.optimize import minimize
+from scipy.special import erf
+
+from bamf_amd import synth
+
+SIG0 = 1.3
+
+
+def model64(theta, sz):
+    x, y, n, bg, s = theta
+    e = np.arange(sz + 1, dtype=np.float64)
+    c = 1 / (s * np.sqrt(2))
+    gx = 0.5 * np.diff(erf((e - x) * c))
+    gy = 0.5 * np.diff(erf((e - y) * c))
+    return bg + n * np.outer(gy, gx)  # [jj, ii]
+
+
+@pytest.mark.parametrize("true_sigma", [1.1, 1.6])
+def test_noiseless_recovers_sigma(oracle, true_sigma):
+    boxes, t = synth.make_mle_boxes(48, 9, seed=21, noise=False, sigma=true_sigma, photons=(20.0, 100.0))
+    p, c, ll = oracle.mle_xynbs(boxes, SIG0, 100)  # the diagonal Newton step converges slowly in (I, sigma)
+    np.testing.assert_allclose(p[0], t["x"], atol=3e-3)
+    np.testing.assert_allclose(p[1], t["y"], atol=3e-3)
+    np.testing.assert_allclose(p[2], t["I"], rtol=3e-3)
+    np.testing.assert_allclose(p[3], t["bg"], rtol=3e-3, atol=2e-2)
+    np.testing.assert_allclose(p[4], true_sigma, rtol=2e-3)
+    assert np.all(np.abs(ll) < 1e-3 * boxes.reshape(48, -1).sum(1))
+
+
+def test_crlb_against_finite_difference_fisher(oracle):
+    sz, ts = 9, 1.4
+    boxes, t = synth.make_mle_boxes(12, sz, seed=22, noise=False, sigma=ts, photons=(20.0, 100.0))
+    p, c, _ = oracle.mle_xynbs(boxes, SIG0, 100)
+    for i in range(12):
+        th = np.array([t["x"][i], t["y"][i], t["I"][i], t["bg"][i], ts])
+        mu = model64(th, sz)
+        J = np.zeros((sz * sz, 5))
+        for k in range(5):
+            h = 1e-5 * max(1.0, abs(th[k]))
+            tp, tm = th.copy(), th.copy()
+            tp[k] += h
+            tm[k] -= h
+            J[:, k] = ((model64(tp, sz) - model64(tm, sz)) / (2 * h)).ravel()
+        F = J.T @ (J / mu.ravel()[:, None])
+        np.testing.assert_allclose(c[:, i], np.diag(np.linalg.inv(F)), rtol=1e-2)
+
+
+@pytest.mark.parametrize("seed", [23, 24])
+def test_noisy_fit_is_poisson_mle(oracle, seed):
+    sz = 9
+    boxes, t = synth.make_mle_boxes(10, sz, seed=seed, noise=True, sigma=1.3, photons=(50.0, 40.0))
+    p, _, _ = oracle.mle_xynbs(boxes, SIG0, 100)
+    for i in range(10):
+        d = boxes[i].astype(np.float64)
+
+        def nll(th):
+            if th[4] <= 0.05:
+                return 1e30
+            mu = model64(th, sz)
+            if np.any(mu <= 0):
+                return 1e30
+            return float(np.sum(mu - d * np.log(mu)))
+
+        th0 = np.array([t["x"][i], t["y"][i], t["I"][i], t["bg"][i], 1.3])
+        r = minimize(nll, th0, method="Nelder-Mead",
+                     options={"xatol": 1e-8, "fatol": 1e-11, "maxiter": 40000, "maxfev": 40000})
+        if not (0.5 < r.x[4] < sz / 2):
+            continue  # the clamps bind; that case has its own test
+        assert abs(p[0, i] - r.x[0]) < 3e-3 and abs(p[1, i] - r.x[1]) < 3e-3
+        assert abs(p[2, i] - r.x[2]) < 3e-3 * r.x[2]
+        assert abs(p[3, i] - r.x[3]) < 1e-2 * max(r.x[3], 1.0)
+        assert abs(p[4, i] - r.x[4]) < 3e-3 * r.x[4]
+
+
+def test_sigma_clamps(oracle):
+    # a flat box (no emitter) drives sigma to a clamp; every result stays in [0.5, sz/2]
+    sz = 7
+    rng = np.random.default_rng(25)
+    boxes = rng.poisson(10.0, size=(32, sz, sz)).astype(np.float32)
+    p, _, _ = oracle.mle_xynbs(boxes, SIG0, 20)
+    assert np.all(p[4] >= 0.5) and np.all(p[4] <= sz / 2)
+    assert np.all(p[2] >= 1.0) and np.all(p[3] >= 0.01)
