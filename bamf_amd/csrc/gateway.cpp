@@ -2,6 +2,7 @@
 // MATLAB-level entry points, each over the device entry points:
 //   bamf_crjmcmc        cRJMCMC (source/c/cRJMCMC.cpp:52-305), one ROI per call;
 //   bamf_gauss_mle_host gaussMLE(Data, 'Basic', 'CCD', Sigma) (@RJ/gaussMLE.m:64-211);
+//   bamf_gauss_mle_fit_host the same for FitType 'Basic' or 'Sigma', raw outputs;
 //   bamf_find_roi_host  findROI(Image, Sigma1, Sigma2, Boxsize, Minval) (@RJ/findROI.m:1-163).
 // The MATLAB mex functions under csrc/mex only map mxArrays onto these.
 #include <algorithm>
@@ -157,11 +158,12 @@ int bamf_crjmcmc(const bamf_mcmc_params* mcmc, const float* data, int xsize, int
     return BAMF_OK;
 }
 
-int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, float* y, float* x,
-                        float* photons, float* bg, float* y_se, float* x_se, float* photons_se, float* bg_se,
-                        float* ll, double* fit_seconds) {
-    BAMF_CHECK_ARG(data && y && x && photons && bg && ll, "null argument");
+int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, int fit_type,
+                            float* params, float* crlb, float* ll, double* fit_seconds) {
+    BAMF_CHECK_ARG(data && params && crlb && ll, "null argument");
     BAMF_CHECK_ARG(sz >= 1 && nfits >= 0 && iterations >= 0, "bad size");
+    BAMF_CHECK_ARG(fit_type == BAMF_FIT_BASIC || fit_type == BAMF_FIT_SIGMA, "fit_type must be BASIC or SIGMA");
+    const int np = fit_type == BAMF_FIT_SIGMA ? 5 : 4;
     if (nfits == 0) {
         if (fit_seconds) *fit_seconds = 0;
         return BAMF_OK;
@@ -173,27 +175,25 @@ int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, i
     const int64_t chunk = std::min<int64_t>(nfits, int64_t(1) << 24);
     DevBufs b;
     float* d_data = b.alloc<float>((size_t)chunk * sz * sz);
-    float* d_p = b.alloc<float>(4 * (size_t)chunk);
-    float* d_c = b.alloc<float>(4 * (size_t)chunk);
+    float* d_p = b.alloc<float>(np * (size_t)chunk);
+    float* d_c = b.alloc<float>(np * (size_t)chunk);
     float* d_l = b.alloc<float>((size_t)chunk);
-    std::vector<float> hp(4 * (size_t)chunk), hc(4 * (size_t)chunk);
-    float* out[4] = {y, x, photons, bg};
-    float* se[4] = {y_se, x_se, photons_se, bg_se};
+    std::vector<float> hp(np * (size_t)chunk), hc(np * (size_t)chunk);
     for (int64_t f0 = 0; f0 < nfits && b.err == hipSuccess; f0 += chunk) {
         const int n = (int)std::min<int64_t>(chunk, nfits - f0);
         b.err = hipMemcpy(d_data, data + (size_t)f0 * sz * sz, (size_t)n * sz * sz * 4, hipMemcpyHostToDevice);
         if (b.err != hipSuccess) break;
-        const int rc = bamf_gauss_mle_xynb(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr);
+        const int rc = fit_type == BAMF_FIT_SIGMA
+                           ? bamf_gauss_mle_xynbs(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr)
+                           : bamf_gauss_mle_xynb(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr);
         if (rc != BAMF_OK) return rc;
-        b.err = hipMemcpy(hp.data(), d_p, 4 * (size_t)n * 4, hipMemcpyDeviceToHost);
-        if (b.err == hipSuccess) b.err = hipMemcpy(hc.data(), d_c, 4 * (size_t)n * 4, hipMemcpyDeviceToHost);
+        b.err = hipMemcpy(hp.data(), d_p, np * (size_t)n * 4, hipMemcpyDeviceToHost);
+        if (b.err == hipSuccess) b.err = hipMemcpy(hc.data(), d_c, np * (size_t)n * 4, hipMemcpyDeviceToHost);
         if (b.err == hipSuccess) b.err = hipMemcpy(ll + f0, d_l, (size_t)n * 4, hipMemcpyDeviceToHost);
         if (b.err != hipSuccess) break;
-        // gaussMLE.m:183-190: parameter k = 0..3 is Y, X, Photons, BG; SE = sqrt(CRLB)
-        for (int k = 0; k < 4; k++) {
-            std::memcpy(out[k] + f0, hp.data() + (size_t)k * n, (size_t)n * 4);
-            if (se[k])
-                for (int i = 0; i < n; i++) se[k][f0 + i] = std::sqrt(hc[(size_t)k * n + i]);
+        for (int k = 0; k < np; k++) {  // parameter-major over all nfits
+            std::memcpy(params + (size_t)k * nfits + f0, hp.data() + (size_t)k * n, (size_t)n * 4);
+            std::memcpy(crlb + (size_t)k * nfits + f0, hc.data() + (size_t)k * n, (size_t)n * 4);
         }
     }
     if (b.err != hipSuccess) {
@@ -201,6 +201,26 @@ int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, i
         return BAMF_E_HIP;
     }
     if (fit_seconds) fit_seconds[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return BAMF_OK;
+}
+
+int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, float* y, float* x,
+                        float* photons, float* bg, float* y_se, float* x_se, float* photons_se, float* bg_se,
+                        float* ll, double* fit_seconds) {
+    BAMF_CHECK_ARG(data && y && x && photons && bg && ll, "null argument");
+    BAMF_CHECK_ARG(sz >= 1 && nfits >= 0 && iterations >= 0, "bad size");
+    std::vector<float> p(4 * (size_t)nfits), c(4 * (size_t)nfits);
+    const int rc = bamf_gauss_mle_fit_host(data, sz, nfits, sigma, iterations, BAMF_FIT_BASIC, p.data(), c.data(),
+                                           ll, fit_seconds);
+    if (rc != BAMF_OK) return rc;
+    // gaussMLE.m:183-190: parameter k = 0..3 is Y, X, Photons, BG; SE = sqrt(CRLB)
+    float* out[4] = {y, x, photons, bg};
+    float* se[4] = {y_se, x_se, photons_se, bg_se};
+    for (int k = 0; k < 4; k++) {
+        std::memcpy(out[k], p.data() + (size_t)k * nfits, (size_t)nfits * 4);
+        if (se[k])
+            for (int64_t i = 0; i < nfits; i++) se[k][i] = std::sqrt(c[(size_t)k * nfits + i]);
+    }
     return BAMF_OK;
 }
 

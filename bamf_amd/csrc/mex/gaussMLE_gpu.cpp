@@ -1,10 +1,11 @@
 // MATLAB mex entry with the argument list and outputs of gaussMLE for the
-// fit the hot path uses (@RJ/gaussMLE.m:64-211, FitType 'Basic', CameraType
-// 'CCD'), fitting on the GPU through bamf_gauss_mle_host.  Build inside
-// MATLAB (mex.h is needed; not compiled by bamf_amd/build.py):
+// fits of FitType 'Basic' and 'Sigma' with CameraType 'CCD'
+// (@RJ/gaussMLE.m:64-211), fitting on the GPU through bamf_gauss_mle_fit_host.
+// Build inside MATLAB (mex.h is needed; not compiled by bamf_amd/build.py):
 //   mex -R2018a gaussMLE_gpu.cpp -I<repo>/include -L<repo>/bamf_amd -lbamf_gpu
 //
 // [Results, Statistics] = gaussMLE_gpu(Data, FitType, CameraType, Sigma)
+#include <cmath>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -17,8 +18,11 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     char fit[32] = {0}, cam[32] = {0};
     mxGetString(prhs[1], fit, sizeof(fit));
     mxGetString(prhs[2], cam, sizeof(cam));
-    if (std::string(fit) != "Basic" || std::string(cam) != "CCD")
-        mexErrMsgIdAndTxt("bamf:args", "only FitType 'Basic' with CameraType 'CCD' is implemented");
+    const std::string ft(fit);
+    if ((ft != "Basic" && ft != "Sigma") || std::string(cam) != "CCD")
+        mexErrMsgIdAndTxt("bamf:args", "FitType 'Basic' or 'Sigma' with CameraType 'CCD' is implemented");
+    const int fit_type = ft == "Sigma" ? BAMF_FIT_SIGMA : BAMF_FIT_BASIC;
+    const int np = fit_type == BAMF_FIT_SIGMA ? 5 : 4;  // gaussMLE.m:93-101
     const mxArray* D = prhs[0];
     if (!mxIsSingle(D)) mexErrMsgIdAndTxt("bamf:args", "Data must be single (SZ, SZ, N)");
     const mwSize nd = mxGetNumberOfDimensions(D);
@@ -32,21 +36,34 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     const size_t ns = mxGetNumberOfElements(prhs[3]);
     for (size_t i = 0; i < ns; i++) sum += sg[i];
     const float sigma = (float)(sum / (double)(ns ? ns : 1));
-    std::vector<float> out(9 * (size_t)n);
+    std::vector<float> p(np * (size_t)n), c(np * (size_t)n), ll((size_t)n);
     double secs = 0;
-    float* o = out.data();
-    const int rc = bamf_gauss_mle_host((const float*)mxGetData(D), sz, n, sigma, 20, o, o + n, o + 2 * n, o + 3 * n,
-                                       o + 4 * n, o + 5 * n, o + 6 * n, o + 7 * n, o + 8 * n, &secs);
+    const int rc = bamf_gauss_mle_fit_host((const float*)mxGetData(D), sz, n, sigma, 20, fit_type, p.data(),
+                                           c.data(), ll.data(), &secs);
     if (rc != BAMF_OK) mexErrMsgIdAndTxt("bamf:gpu", "%s", bamf_last_error());
-    // Results fields in gaussMLE.m:183-193 order
-    static const char* names[9] = {"Y", "X", "Photons", "BG", "Y_SE", "X_SE", "Photons_SE", "BG_SE", "LogLikelihood"};
-    plhs[0] = mxCreateStructMatrix(1, 1, 9, names);
-    for (int f = 0; f < 9; f++) {
+    // Results fields in gaussMLE.m:183-196 order: parameters, their SEs, the
+    // fit type's extra pair, LogLikelihood
+    static const char* names_b[9] = {"Y", "X", "Photons", "BG", "Y_SE", "X_SE", "Photons_SE", "BG_SE",
+                                     "LogLikelihood"};
+    static const char* names_s[11] = {"Y", "X", "Photons", "BG", "Y_SE", "X_SE", "Photons_SE", "BG_SE",
+                                      "Sigma", "Sigma_SE", "LogLikelihood"};
+    const int nf = np == 5 ? 11 : 9;
+    plhs[0] = mxCreateStructMatrix(1, 1, nf, np == 5 ? names_s : names_b);
+    auto column = [&](const float* src, bool root) {
         mxArray* v = mxCreateNumericMatrix((mwSize)n, 1, mxSINGLE_CLASS, mxREAL);
         float* d = (float*)mxGetData(v);
-        for (int64_t i = 0; i < n; i++) d[i] = o[(size_t)f * n + i];
-        mxSetFieldByNumber(plhs[0], 0, f, v);
+        for (int64_t i = 0; i < n; i++) d[i] = root ? std::sqrt(src[i]) : src[i];
+        return v;
+    };
+    for (int k = 0; k < 4; k++) {
+        mxSetFieldByNumber(plhs[0], 0, k, column(p.data() + (size_t)k * n, false));
+        mxSetFieldByNumber(plhs[0], 0, 4 + k, column(c.data() + (size_t)k * n, true));
     }
+    if (np == 5) {
+        mxSetFieldByNumber(plhs[0], 0, 8, column(p.data() + 4 * (size_t)n, false));
+        mxSetFieldByNumber(plhs[0], 0, 9, column(c.data() + 4 * (size_t)n, true));
+    }
+    mxSetFieldByNumber(plhs[0], 0, nf - 1, column(ll.data(), false));
     if (nlhs > 1) {
         static const char* sn[4] = {"FitTime", "FitPerSecond", "NKernelsCalls", "NFitsPerCall"};
         plhs[1] = mxCreateStructMatrix(1, 1, 4, sn);

@@ -451,6 +451,14 @@ int bamf_crjmcmc(const bamf_mcmc_params* mcmc, const float* data, int xsize, int
 int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, float* y, float* x,
                         float* photons, float* bg, float* y_se, float* x_se, float* photons_se, float* bg_se,
                         float* ll, double* fit_seconds);
+/* gaussMLE(Data, FitType, 'CCD', Sigma) for FitType 'Basic' (BAMF_FIT_BASIC,
+ * kernel _XYNB_) or 'Sigma' (BAMF_FIT_SIGMA, kernel _XYNBS_), host arrays in and
+ * out: params and crlb are NP x nfits parameter-major (NP = 4 or 5: Y, X,
+ * Photons, BG[, Sigma] as gaussMLE.m:183-195 maps them; SE = sqrt(crlb)). */
+#define BAMF_FIT_BASIC 0
+#define BAMF_FIT_SIGMA 1
+int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, int fit_type,
+                            float* params, float* crlb, float* ll, double* fit_seconds);
 
 /* findROI(Image, Sigma1, Sigma2, Boxsize, Minval) on host arrays
  * (@RJ/findROI.m:1-163):
