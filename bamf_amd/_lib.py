@@ -44,6 +44,8 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bamf_device_info.argtypes = [c_int, ip, ip, c_char_p, c_int]
     L.bamf_gauss_mle_xynb.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]
     L.bamf_gauss_mle_xynbs.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]
+    L.bamf_gauss_mle_xynbsxsy.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int,
+                                          c_void_p]
     L.bamf_gauss_mle_fit_host.argtypes = [c_void_p, c_int, c_int64, c_float, c_int, c_int, c_void_p, c_void_p,
                                           c_void_p, c_void_p]
     L.bamf_gauss_coeffs.argtypes = [c_float, fp]

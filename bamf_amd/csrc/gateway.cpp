@@ -2,7 +2,7 @@
 // MATLAB-level entry points, each over the device entry points:
 //   bamf_crjmcmc        cRJMCMC (source/c/cRJMCMC.cpp:52-305), one ROI per call;
 //   bamf_gauss_mle_host gaussMLE(Data, 'Basic', 'CCD', Sigma) (@RJ/gaussMLE.m:64-211);
-//   bamf_gauss_mle_fit_host the same for FitType 'Basic' or 'Sigma', raw outputs;
+//   bamf_gauss_mle_fit_host the same for FitType 'Basic', 'Sigma' or 'SigmaYX', raw outputs;
 //   bamf_find_roi_host  findROI(Image, Sigma1, Sigma2, Boxsize, Minval) (@RJ/findROI.m:1-163).
 // The MATLAB mex functions under csrc/mex only map mxArrays onto these.
 #include <algorithm>
@@ -162,8 +162,9 @@ int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigm
                             float* params, float* crlb, float* ll, double* fit_seconds) {
     BAMF_CHECK_ARG(data && params && crlb && ll, "null argument");
     BAMF_CHECK_ARG(sz >= 1 && nfits >= 0 && iterations >= 0, "bad size");
-    BAMF_CHECK_ARG(fit_type == BAMF_FIT_BASIC || fit_type == BAMF_FIT_SIGMA, "fit_type must be BASIC or SIGMA");
-    const int np = fit_type == BAMF_FIT_SIGMA ? 5 : 4;
+    BAMF_CHECK_ARG(fit_type == BAMF_FIT_BASIC || fit_type == BAMF_FIT_SIGMA || fit_type == BAMF_FIT_SIGMAYX,
+                   "fit_type must be BASIC, SIGMA or SIGMAYX");
+    const int np = fit_type == BAMF_FIT_SIGMAYX ? 6 : fit_type == BAMF_FIT_SIGMA ? 5 : 4;
     if (nfits == 0) {
         if (fit_seconds) *fit_seconds = 0;
         return BAMF_OK;
@@ -183,7 +184,9 @@ int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigm
         const int n = (int)std::min<int64_t>(chunk, nfits - f0);
         b.err = hipMemcpy(d_data, data + (size_t)f0 * sz * sz, (size_t)n * sz * sz * 4, hipMemcpyHostToDevice);
         if (b.err != hipSuccess) break;
-        const int rc = fit_type == BAMF_FIT_SIGMA
+        const int rc = fit_type == BAMF_FIT_SIGMAYX
+                           ? bamf_gauss_mle_xynbsxsy(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr)
+                       : fit_type == BAMF_FIT_SIGMA
                            ? bamf_gauss_mle_xynbs(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr)
                            : bamf_gauss_mle_xynb(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr);
         if (rc != BAMF_OK) return rc;

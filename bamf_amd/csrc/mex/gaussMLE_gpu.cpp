@@ -1,5 +1,5 @@
 // MATLAB mex entry with the argument list and outputs of gaussMLE for the
-// fits of FitType 'Basic' and 'Sigma' with CameraType 'CCD'
+// fits of FitType 'Basic', 'Sigma' and 'SigmaYX' with CameraType 'CCD'
 // (@RJ/gaussMLE.m:64-211), fitting on the GPU through bamf_gauss_mle_fit_host.
 // Build inside MATLAB (mex.h is needed; not compiled by bamf_amd/build.py):
 //   mex -R2018a gaussMLE_gpu.cpp -I<repo>/include -L<repo>/bamf_amd -lbamf_gpu
@@ -19,10 +19,10 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     mxGetString(prhs[1], fit, sizeof(fit));
     mxGetString(prhs[2], cam, sizeof(cam));
     const std::string ft(fit);
-    if ((ft != "Basic" && ft != "Sigma") || std::string(cam) != "CCD")
-        mexErrMsgIdAndTxt("bamf:args", "FitType 'Basic' or 'Sigma' with CameraType 'CCD' is implemented");
-    const int fit_type = ft == "Sigma" ? BAMF_FIT_SIGMA : BAMF_FIT_BASIC;
-    const int np = fit_type == BAMF_FIT_SIGMA ? 5 : 4;  // gaussMLE.m:93-101
+    if ((ft != "Basic" && ft != "Sigma" && ft != "SigmaYX") || std::string(cam) != "CCD")
+        mexErrMsgIdAndTxt("bamf:args", "FitType 'Basic', 'Sigma' or 'SigmaYX' with CameraType 'CCD' is implemented");
+    const int fit_type = ft == "SigmaYX" ? BAMF_FIT_SIGMAYX : ft == "Sigma" ? BAMF_FIT_SIGMA : BAMF_FIT_BASIC;
+    const int np = fit_type == BAMF_FIT_SIGMAYX ? 6 : fit_type == BAMF_FIT_SIGMA ? 5 : 4;  // gaussMLE.m:93-109
     const mxArray* D = prhs[0];
     if (!mxIsSingle(D)) mexErrMsgIdAndTxt("bamf:args", "Data must be single (SZ, SZ, N)");
     const mwSize nd = mxGetNumberOfDimensions(D);
@@ -47,8 +47,10 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
                                      "LogLikelihood"};
     static const char* names_s[11] = {"Y", "X", "Photons", "BG", "Y_SE", "X_SE", "Photons_SE", "BG_SE",
                                       "Sigma", "Sigma_SE", "LogLikelihood"};
-    const int nf = np == 5 ? 11 : 9;
-    plhs[0] = mxCreateStructMatrix(1, 1, nf, np == 5 ? names_s : names_b);
+    static const char* names_yx[13] = {"Y", "X", "Photons", "BG", "Y_SE", "X_SE", "Photons_SE", "BG_SE",
+                                       "SigmaY", "SigmaX", "SigmaY_SE", "SigmaX_SE", "LogLikelihood"};
+    const int nf = np == 6 ? 13 : np == 5 ? 11 : 9;
+    plhs[0] = mxCreateStructMatrix(1, 1, nf, np == 6 ? names_yx : np == 5 ? names_s : names_b);
     auto column = [&](const float* src, bool root) {
         mxArray* v = mxCreateNumericMatrix((mwSize)n, 1, mxSINGLE_CLASS, mxREAL);
         float* d = (float*)mxGetData(v);
@@ -62,6 +64,12 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     if (np == 5) {
         mxSetFieldByNumber(plhs[0], 0, 8, column(p.data() + 4 * (size_t)n, false));
         mxSetFieldByNumber(plhs[0], 0, 9, column(c.data() + 4 * (size_t)n, true));
+    }
+    if (np == 6) {  // gaussMLE.m:199-202
+        mxSetFieldByNumber(plhs[0], 0, 8, column(p.data() + 4 * (size_t)n, false));
+        mxSetFieldByNumber(plhs[0], 0, 9, column(p.data() + 5 * (size_t)n, false));
+        mxSetFieldByNumber(plhs[0], 0, 10, column(c.data() + 4 * (size_t)n, true));
+        mxSetFieldByNumber(plhs[0], 0, 11, column(c.data() + 5 * (size_t)n, true));
     }
     mxSetFieldByNumber(plhs[0], 0, nf - 1, column(ll.data(), false));
     if (nlhs > 1) {

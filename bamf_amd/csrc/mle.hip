@@ -1,8 +1,9 @@
 // Single-emitter Newton MLE of theta = {x, y, N, bg} (fit type 'Basic', NV =
-// 4) or {x, y, N, bg, sigma} (fit type 'Sigma', NV = 5) with CRLB and
-// log-likelihood.  Same inputs, outputs and iteration as kernel_MLEFit_XYNB_
-// (source/cuda/cuda_gaussMLEv2/cuda_gaussMLEv2.cu:16-141) and
-// kernel_MLEFit_XYNBS_ (cuda_gaussMLEv2.cu:144-276).
+// 4), {x, y, N, bg, sigma} ('Sigma', NV = 5) or {x, y, N, bg, sigma_ii,
+// sigma_jj} ('SigmaYX', NV = 6) with CRLB and log-likelihood.  Same inputs,
+// outputs and iteration as kernel_MLEFit_XYNB_
+// (source/cuda/cuda_gaussMLEv2/cuda_gaussMLEv2.cu:16-141), kernel_MLEFit_XYNBS_
+// (:144-276) and kernel_MLEFit_XYNBSXSY_ (:414-556).
 //
 // gfx950 design: one work-item per fit, the box staged through LDS so that
 // global loads are contiguous, the pixel-integrated Gaussian evaluated in
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
     for (int p = 0; p < NP; p++) d[p] = box[p];
 
     // --- initial values: centre of mass (GPUgaussLib.cuh:207-227) ---
-    constexpr bool SG = NV == 5;
+    constexpr bool SG = NV >= 5, XY = NV == 6;  // fitted width(s): one, or one per axis
     float theta[NV];
     {
         float sx = 0.f, sy = 0.f, ss = 0.f;
@@ -169,23 +170,35 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
         theta[3] = fmin_;
         theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
         if constexpr (SG) theta[4] = sigma;
+        if constexpr (XY) theta[5] = sigma;
     }
 
     const float c1 = sqrtf(2.0f * kPi);
-    float sg = sigma, sqrt_norm = sqrtf(1.0f / 2.0f / sigma / sigma), c2 = sigma * sigma * sigma;
-    for (int it = 0; it < iterations; it++) {
-        if constexpr (SG) {  // the PSF width is theta[4]
-            sg = theta[4];
-            sqrt_norm = sqrtf(1.0f / 2.0f / sg / sg);
-            c2 = sg * sg * sg;
+    // widths along ii (x) and jj (y): sigma, theta[4] for both, or theta[4] and theta[5]
+    float sgx = sigma, snx = sqrtf(1.0f / 2.0f / sigma / sigma), c2x = sigma * sigma * sigma;
+    float sgy = sgx, sny = snx, c2y = c2x;
+    auto set_widths = [&]() {
+        if constexpr (SG) {
+            sgx = theta[4];
+            snx = sqrtf(1.0f / 2.0f / sgx / sgx);
+            c2x = sgx * sgx * sgx;
+            sgy = sgx, sny = snx, c2y = c2x;
         }
+        if constexpr (XY) {
+            sgy = theta[5];
+            sny = sqrtf(1.0f / 2.0f / sgy / sgy);
+            c2y = sgy * sgy * sgy;
+        }
+    };
+    for (int it = 0; it < iterations; it++) {
+        set_widths();
         AxisTerms ax[SZ], ay[SZ];
 #pragma unroll
         for (int i = 0; i < SZ; i++) {
-            ax[i] = axis_terms<SG>(i, theta[0], sg, sqrt_norm, theta[2], c1, c2);
-            ay[i] = axis_terms<SG>(i, theta[1], sg, sqrt_norm, theta[2], c1, c2);
+            ax[i] = axis_terms<SG>(i, theta[0], sgx, snx, theta[2], c1, c2x);
+            ay[i] = axis_terms<SG>(i, theta[1], sgy, sny, theta[2], c1, c2y);
         }
-        const float m2s = -2.0f / sg;
+        const float m2sx = -2.0f / sgx, m2sy = -2.0f / sgy;
         float num[NV], den[NV];
 #pragma unroll
         for (int k = 0; k < NV; k++) num[k] = den[k] = 0.f;
@@ -217,10 +230,17 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
                 den[3] += -df;
                 if constexpr (SG) {
                     const float dsx = ax[ii].s1 * py, dsy = ay[jj].s1 * px;
-                    const float u4 = dsx + dsy;
-                    const float v4 = (m2s * dsx - ax[ii].s2 * py) + (m2s * dsy - ay[jj].s2 * px);
-                    num[4] += u4 * cf;
-                    den[4] += v4 * cf - (u4 * u4) * df;
+                    const float vx = m2sx * dsx - ax[ii].s2 * py, vy = m2sy * dsy - ay[jj].s2 * px;
+                    if constexpr (XY) {
+                        num[4] += dsx * cf;
+                        den[4] += vx * cf - (dsx * dsx) * df;
+                        num[5] += dsy * cf;
+                        den[5] += vy * cf - (dsy * dsy) * df;
+                    } else {
+                        const float u4 = dsx + dsy, v4 = vx + vy;
+                        num[4] += u4 * cf;
+                        den[4] += v4 * cf - (u4 * u4) * df;
+                    }
                 }
             }
         theta[0] -= fminf(fmaxf(num[0] / den[0], -1e0f), 1e0f);
@@ -228,17 +248,15 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
         theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
         theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
         if constexpr (SG) theta[4] -= fminf(fmaxf(num[4] / den[4], -theta[4]), theta[4]);
+        if constexpr (XY) theta[5] -= fminf(fmaxf(num[5] / den[5], -theta[5]), theta[5]);
         theta[2] = fmaxf(theta[2], 1.0f);
         theta[3] = fmaxf(theta[3], 0.01f);
         if constexpr (SG) theta[4] = fminf(fmaxf(theta[4], 0.5f), SZ / 2.0f);  // cuda_gaussMLEv2.cu:237-238
+        if constexpr (XY) theta[5] = fminf(fmaxf(theta[5], 0.5f), SZ / 2.0f);  // cuda_gaussMLEv2.cu:512-513
     }
-    if constexpr (SG) {
-        sg = theta[4];
-        sqrt_norm = sqrtf(1.0f / 2.0f / sg / sg);
-        c2 = sg * sg * sg;
-    }
+    set_widths();
 
-    // --- Fisher information, CRLB and log-likelihood (cuda_gaussMLEv2.cu:105-138, 243-266) ---
+    // --- Fisher information, CRLB and log-likelihood (cuda_gaussMLEv2.cu:105-138, 243-266, 518-546) ---
     float M[NV * NV];
 #pragma unroll
     for (int k = 0; k < NV * NV; k++) M[k] = 0.f;
@@ -247,8 +265,8 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
         AxisTerms ax[SZ], ay[SZ];
 #pragma unroll
         for (int i = 0; i < SZ; i++) {
-            ax[i] = axis_terms<SG>(i, theta[0], sg, sqrt_norm, theta[2], c1, c2);
-            ay[i] = axis_terms<SG>(i, theta[1], sg, sqrt_norm, theta[2], c1, c2);
+            ax[i] = axis_terms<SG>(i, theta[0], sgx, snx, theta[2], c1, c2x);
+            ay[i] = axis_terms<SG>(i, theta[1], sgy, sny, theta[2], c1, c2y);
         }
 #pragma unroll
         for (int ii = 0; ii < SZ; ii++)
@@ -262,7 +280,12 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
                 du[1] = ay[jj].d1 * px;
                 du[2] = px * py;
                 du[3] = 1.0f;
-                if constexpr (SG) du[4] = ax[ii].s1 * py + ay[jj].s1 * px;
+                if constexpr (XY) {
+                    du[4] = ax[ii].s1 * py;
+                    du[5] = ay[jj].s1 * px;
+                } else if constexpr (SG) {
+                    du[4] = ax[ii].s1 * py + ay[jj].s1 * px;
+                }
                 const float rm = __builtin_amdgcn_rcpf(model);
 #pragma unroll
                 for (int k = 0; k < NV; k++)
@@ -295,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
                                                          int sz, int iterations, float* __restrict__ params,
                                                          float* __restrict__ crlb, float* __restrict__ ll,
                                                          int nfits) {
-    constexpr bool SG = NV == 5;
+    constexpr bool SG = NV >= 5, XY = NV == 6;  // fitted width(s): one, or one per axis
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= nfits) return;
     const float* d = data + i * sz * sz;
@@ -333,22 +356,34 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
         theta[3] = fmin_;
         theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
         if constexpr (SG) theta[4] = sigma;
+        if constexpr (XY) theta[5] = sigma;
     }
     const float c1 = sqrtf(2.0f * kPi);
-    float sg = sigma, sqrt_norm = sqrtf(1.0f / 2.0f / sigma / sigma), c2 = sigma * sigma * sigma;
-    for (int it = 0; it < iterations; it++) {
+    // widths along ii (x) and jj (y): sigma, theta[4] for both, or theta[4] and theta[5]
+    float sgx = sigma, snx = sqrtf(1.0f / 2.0f / sigma / sigma), c2x = sigma * sigma * sigma;
+    float sgy = sgx, sny = snx, c2y = c2x;
+    auto set_widths = [&]() {
         if constexpr (SG) {
-            sg = theta[4];
-            sqrt_norm = sqrtf(1.0f / 2.0f / sg / sg);
-            c2 = sg * sg * sg;
+            sgx = theta[4];
+            snx = sqrtf(1.0f / 2.0f / sgx / sgx);
+            c2x = sgx * sgx * sgx;
+            sgy = sgx, sny = snx, c2y = c2x;
         }
-        const float m2s = -2.0f / sg;
+        if constexpr (XY) {
+            sgy = theta[5];
+            sny = sqrtf(1.0f / 2.0f / sgy / sgy);
+            c2y = sgy * sgy * sgy;
+        }
+    };
+    for (int it = 0; it < iterations; it++) {
+        set_widths();
+        const float m2sx = -2.0f / sgx, m2sy = -2.0f / sgy;
         float num[NV], den[NV];
         for (int k = 0; k < NV; k++) num[k] = den[k] = 0.f;
         for (int ii = 0; ii < sz; ii++) {
-            AxisTerms ax = axis_terms<SG>(ii, theta[0], sg, sqrt_norm, theta[2], c1, c2);
+            AxisTerms ax = axis_terms<SG>(ii, theta[0], sgx, snx, theta[2], c1, c2x);
             for (int jj = 0; jj < sz; jj++) {
-                AxisTerms ay = axis_terms<SG>(jj, theta[1], sg, sqrt_norm, theta[2], c1, c2);
+                AxisTerms ay = axis_terms<SG>(jj, theta[1], sgy, sny, theta[2], c1, c2y);
                 const float px = ax.psf, py = ay.psf;
                 const float model = theta[3] + theta[2] * px * py;
                 const float dat = d[sz * jj + ii];
@@ -367,10 +402,17 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
                 num[3] += cf;      den[3] += -df;
                 if constexpr (SG) {
                     const float dsx = ax.s1 * py, dsy = ay.s1 * px;
-                    const float u4 = dsx + dsy;
-                    const float v4 = (m2s * dsx - ax.s2 * py) + (m2s * dsy - ay.s2 * px);
-                    num[4] += u4 * cf;
-                    den[4] += v4 * cf - (u4 * u4) * df;
+                    const float vx = m2sx * dsx - ax.s2 * py, vy = m2sy * dsy - ay.s2 * px;
+                    if constexpr (XY) {
+                        num[4] += dsx * cf;
+                        den[4] += vx * cf - (dsx * dsx) * df;
+                        num[5] += dsy * cf;
+                        den[5] += vy * cf - (dsy * dsy) * df;
+                    } else {
+                        const float u4 = dsx + dsy, v4 = vx + vy;
+                        num[4] += u4 * cf;
+                        den[4] += v4 * cf - (u4 * u4) * df;
+                    }
                 }
             }
         }
@@ -379,22 +421,20 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
         theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
         theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
         if constexpr (SG) theta[4] -= fminf(fmaxf(num[4] / den[4], -theta[4]), theta[4]);
+        if constexpr (XY) theta[5] -= fminf(fmaxf(num[5] / den[5], -theta[5]), theta[5]);
         theta[2] = fmaxf(theta[2], 1.0f);
         theta[3] = fmaxf(theta[3], 0.01f);
         if constexpr (SG) theta[4] = fminf(fmaxf(theta[4], 0.5f), sz / 2.0f);
+        if constexpr (XY) theta[5] = fminf(fmaxf(theta[5], 0.5f), sz / 2.0f);
     }
-    if constexpr (SG) {
-        sg = theta[4];
-        sqrt_norm = sqrtf(1.0f / 2.0f / sg / sg);
-        c2 = sg * sg * sg;
-    }
+    set_widths();
     float M[NV * NV];
     for (int k = 0; k < NV * NV; k++) M[k] = 0.f;
     float div = 0.f;
     for (int ii = 0; ii < sz; ii++) {
-        AxisTerms ax = axis_terms<SG>(ii, theta[0], sg, sqrt_norm, theta[2], c1, c2);
+        AxisTerms ax = axis_terms<SG>(ii, theta[0], sgx, snx, theta[2], c1, c2x);
         for (int jj = 0; jj < sz; jj++) {
-            AxisTerms ay = axis_terms<SG>(jj, theta[1], sg, sqrt_norm, theta[2], c1, c2);
+            AxisTerms ay = axis_terms<SG>(jj, theta[1], sgy, sny, theta[2], c1, c2y);
             const float px = ax.psf, py = ay.psf;
             const float model = theta[3] + theta[2] * px * py;
             const float dat = d[sz * jj + ii];
@@ -403,7 +443,12 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
             du[1] = ay.d1 * px;
             du[2] = px * py;
             du[3] = 1.0f;
-            if constexpr (SG) du[4] = ax.s1 * py + ay.s1 * px;
+            if constexpr (XY) {
+                du[4] = ax.s1 * py;
+                du[5] = ay.s1 * px;
+            } else if constexpr (SG) {
+                du[4] = ax.s1 * py + ay.s1 * px;
+            }
             for (int k = 0; k < NV; k++)
                 for (int l = k; l < NV; l++) M[k * NV + l] += du[l] * du[k] / model;
             if (model > 0) {
@@ -458,4 +503,9 @@ extern "C" int bamf_gauss_mle_xynb(const float* d_data, float psf_sigma, int sz,
 extern "C" int bamf_gauss_mle_xynbs(const float* d_data, float psf_sigma, int sz, int iterations,
                                     float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream) {
     return bamf::launch_mle<5>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, stream);
+}
+
+extern "C" int bamf_gauss_mle_xynbsxsy(const float* d_data, float psf_sigma, int sz, int iterations,
+                                       float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream) {
+    return bamf::launch_mle<6>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, stream);
 }

@@ -1,11 +1,12 @@
-"""Host mirror of @RJ/gaussMLE.m for fit types 'Basic' and 'Sigma' on a CCD camera.
+"""Host mirror of @RJ/gaussMLE.m for fit types 'Basic', 'Sigma' and 'SigmaYX' on a CCD camera.
 
 ``gauss_mle(data, 'Basic', 'CCD', sigma)`` returns ``(Results, Statistics)``
 with the fields gaussMLE.m:183-210 builds (Y, X, Photons, BG, *_SE, and Sigma,
-Sigma_SE for 'Sigma'; LogLikelihood; FitTime, FitPerSecond, NKernelsCalls,
-NFitsPerCall).  The fit runs in one call of bamf_gauss_mle_xynb ('Basic',
-kernel _XYNB_) or bamf_gauss_mle_xynbs ('Sigma', kernel _XYNBS_) on the
-device; there is no host fallback.
+Sigma_SE for 'Sigma', SigmaY, SigmaX and their SEs for 'SigmaYX';
+LogLikelihood; FitTime, FitPerSecond, NKernelsCalls, NFitsPerCall).  The fit
+runs in one call of bamf_gauss_mle_xynb ('Basic', kernel _XYNB_),
+bamf_gauss_mle_xynbs ('Sigma', _XYNBS_) or bamf_gauss_mle_xynbsxsy
+('SigmaYX', _XYNBSXSY_) on the device; there is no host fallback.
 """
 from __future__ import annotations
 
@@ -17,14 +18,15 @@ from . import _hip
 from ._lib import BamfError, check, lib
 
 
-_KERNELS = {"Basic": ("bamf_gauss_mle_xynb", 4), "Sigma": ("bamf_gauss_mle_xynbs", 5)}
+_KERNELS = {"Basic": ("bamf_gauss_mle_xynb", 4), "Sigma": ("bamf_gauss_mle_xynbs", 5),
+            "SigmaYX": ("bamf_gauss_mle_xynbsxsy", 6)}
 
 
 def gauss_mle_device(d_data, sigma: float, sz: int, iterations: int, nfits: int, d_params, d_crlb, d_ll,
                      stream=None, fit_type: str = "Basic") -> None:
     """Direct C-ABI call on device buffers (kernel_MLEFit_XYNB_ / _XYNBS_ argument layout)."""
     if fit_type not in _KERNELS:
-        raise BamfError(f"fit type {fit_type!r}: 'Basic' and 'Sigma' are implemented")
+        raise BamfError(f"fit type {fit_type!r}: 'Basic', 'Sigma' and 'SigmaYX' are implemented")
     name = _KERNELS[fit_type][0]
     check(getattr(lib(), name)(d_data.vp, float(sigma), int(sz), int(iterations), d_params.vp, d_crlb.vp,
                                d_ll.vp, int(nfits), stream.vp if stream else None), name)
@@ -39,7 +41,7 @@ def fit_boxes(boxes: np.ndarray, sigma: float, iterations: int = 20, fit_type: s
     if boxes.shape[1] != boxes.shape[2]:
         raise BamfError("boxes must be square")
     if fit_type not in _KERNELS:
-        raise BamfError(f"fit type {fit_type!r}: 'Basic' and 'Sigma' are implemented")
+        raise BamfError(f"fit type {fit_type!r}: 'Basic', 'Sigma' and 'SigmaYX' are implemented")
     npar = _KERNELS[fit_type][1]
     d_data = _hip.DeviceBuffer.from_array(boxes)
     d_p = _hip.DeviceBuffer(4 * npar * n)
@@ -54,8 +56,8 @@ def fit_boxes(boxes: np.ndarray, sigma: float, iterations: int = 20, fit_type: s
 def gauss_mle(data: np.ndarray, fit_type: str = "Basic", camera_type: str = "CCD", sigma=1.3, iterations: int = 20):
     """data: MATLAB-layout stack (SZ, SZ, N) as a Fortran-ordered array, or (N, SZ, SZ) C-ordered boxes."""
     if fit_type not in _KERNELS or camera_type != "CCD":
-        raise BamfError(f"fit type {fit_type!r} / camera {camera_type!r}: 'Basic' and 'Sigma' on 'CCD' are "
-                        "implemented")
+        raise BamfError(f"fit type {fit_type!r} / camera {camera_type!r}: 'Basic', 'Sigma' and 'SigmaYX' on "
+                        "'CCD' are implemented")
     data = np.asarray(data, dtype=np.float32)
     if data.ndim == 2:
         data = data[:, :, None] if data.flags.f_contiguous else data[None]
@@ -75,6 +77,9 @@ def gauss_mle(data: np.ndarray, fit_type: str = "Basic", camera_type: str = "CCD
     if fit_type == "Sigma":  # gaussMLE.m:193-195
         results["Sigma"] = p[4]
         results["Sigma_SE"] = np.sqrt(c[4])
+    if fit_type == "SigmaYX":  # gaussMLE.m:199-202
+        results["SigmaY"], results["SigmaX"] = p[4], p[5]
+        results["SigmaY_SE"], results["SigmaX_SE"] = np.sqrt(c[4]), np.sqrt(c[5])
     stats = {"FitTime": dt, "FitPerSecond": n / dt if dt > 0 else float("inf"), "NKernelsCalls": 1,
              "NFitsPerCall": n}
     return results, stats

@@ -52,6 +52,12 @@ int bamf_gauss_mle_xynb(const float* d_data, float psf_sigma, int sz, int iterat
  * fitted width is kept in [0.5, sz/2] (cuda_gaussMLEv2.cu:237-238). */
 int bamf_gauss_mle_xynbs(const float* d_data, float psf_sigma, int sz, int iterations,
                          float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream);
+/* gaussMLE fit type 'SigmaYX' (theta = {x, y, N, bg, sigma along ii, sigma
+ * along jj}): replaces kernel_MLEFit_XYNBSXSY_ (cuda_gaussMLEv2.cu:414-415),
+ * bound by gaussMLE.m:106-109.  Six parameters; both widths start at psf_sigma
+ * and are kept in [0.5, sz/2] (cuda_gaussMLEv2.cu:510-513). */
+int bamf_gauss_mle_xynbsxsy(const float* d_data, float psf_sigma, int sz, int iterations,
+                            float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream);
 
 /* ------------------------------------------------------------------------
  * FindROI.  A frame is `ysize` runs of `xsize` contiguous floats (the
@@ -452,11 +458,13 @@ int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, i
                         float* photons, float* bg, float* y_se, float* x_se, float* photons_se, float* bg_se,
                         float* ll, double* fit_seconds);
 /* gaussMLE(Data, FitType, 'CCD', Sigma) for FitType 'Basic' (BAMF_FIT_BASIC,
- * kernel _XYNB_) or 'Sigma' (BAMF_FIT_SIGMA, kernel _XYNBS_), host arrays in and
- * out: params and crlb are NP x nfits parameter-major (NP = 4 or 5: Y, X,
- * Photons, BG[, Sigma] as gaussMLE.m:183-195 maps them; SE = sqrt(crlb)). */
+ * kernel _XYNB_), 'Sigma' (BAMF_FIT_SIGMA, _XYNBS_) or 'SigmaYX'
+ * (BAMF_FIT_SIGMAYX, _XYNBSXSY_), host arrays in and out: params and crlb are
+ * NP x nfits parameter-major (NP = 4, 5 or 6: Y, X, Photons, BG[, Sigma |
+ * SigmaY, SigmaX] as gaussMLE.m:183-202 maps them; SE = sqrt(crlb)). */
 #define BAMF_FIT_BASIC 0
 #define BAMF_FIT_SIGMA 1
+#define BAMF_FIT_SIGMAYX 2
 int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, int fit_type,
                             float* params, float* crlb, float* ll, double* fit_seconds);
 

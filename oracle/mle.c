@@ -3,7 +3,8 @@
  * sigma) with CRLB and log-likelihood.  TEST INFRASTRUCTURE ONLY (see oracle.h).
  *
  * Restates kernel_MLEFit_XYNB_ (source/cuda/cuda_gaussMLEv2/cuda_gaussMLEv2.cu:16-141),
- * kernel_MLEFit_XYNBS_ (cuda_gaussMLEv2.cu:144-276) with its sigma derivatives
+ * kernel_MLEFit_XYNBS_ (cuda_gaussMLEv2.cu:144-276), kernel_MLEFit_XYNBSXSY_
+ * (cuda_gaussMLEv2.cu:414-556: one width per axis) with their sigma derivatives
  * kernel_DerivativeIntGauss1DSigma / 2DSigma (GPUgaussLib.cuh:90-137)
  * and its helpers kernel_IntGauss1D / kernel_DerivativeIntGauss1D
  * (GPUgaussLib.cuh:14-24, 67-87), kernel_CenterofMass2D (GPUgaussLib.cuh:207-227),
@@ -317,6 +318,106 @@ void orc_mle_xynbs(const float* data, float psf_sigma, int sz, int iterations,
         float theta[NVS], diag[NVS], l;
         mle_one_s(data + (size_t)sz * sz * i, psf_sigma, sz, iterations, theta, diag, &l);
         for (int k = 0; k < NVS; k++) {
+            params[(size_t)nfits * k + i] = theta[k];
+            crlb[(size_t)nfits * k + i] = diag[k];
+        }
+        ll[i] = l;
+    }
+}
+
+#define NVXY 6
+/* cuda_gaussMLEv2.cu:414-556: theta = {x, y, N, bg, sigma along ii, sigma along jj}. */
+static void mle_one_sxsy(const float* s_data, float sigma, int sz, int iterations, float* theta, float* diagout,
+                         float* llout) {
+    float m[NVXY * NVXY], diag[NVXY], minv[NVXY * NVXY];
+    float dudt[NVXY], d2udt2[NVXY], num[NVXY], den[NVXY];
+    float nmax, model, cf, df, data, psfx, psfy, div;
+    memset(m, 0, sizeof(m));
+    memset(minv, 0, sizeof(minv));
+    orc_center_of_mass_2d(sz, s_data, &theta[0], &theta[1]);
+    orc_gauss_fmaxmin_2d(sz, sigma, s_data, &nmax, &theta[3]);
+    theta[2] = fmaxf(0.0f, (nmax - theta[3]) * 2 * 2 * kPi * sigma * sigma);
+    theta[4] = sigma;
+    theta[5] = sigma;
+    for (int kk = 0; kk < iterations; kk++) {
+        memset(num, 0, sizeof(num));
+        memset(den, 0, sizeof(den));
+        for (int ii = 0; ii < sz; ii++)
+            for (int jj = 0; jj < sz; jj++) {
+                psfx = orc_int_gauss_1d(ii, theta[0], theta[4]);
+                psfy = orc_int_gauss_1d(jj, theta[1], theta[5]);
+                model = theta[3] + theta[2] * psfx * psfy;
+                data = s_data[sz * jj + ii];
+                deriv_int_gauss_1d(ii, theta[0], theta[4], theta[2], psfy, &dudt[0], &d2udt2[0]);
+                deriv_int_gauss_1d(jj, theta[1], theta[5], theta[2], psfx, &dudt[1], &d2udt2[1]);
+                deriv_int_gauss_1d_sigma(ii, theta[0], theta[4], theta[2], psfy, &dudt[4], &d2udt2[4]);
+                deriv_int_gauss_1d_sigma(jj, theta[1], theta[5], theta[2], psfx, &dudt[5], &d2udt2[5]);
+                dudt[2] = psfx * psfy;
+                d2udt2[2] = 0.0f;
+                dudt[3] = 1.0f;
+                d2udt2[3] = 0.0f;
+                cf = 0.0f;
+                df = 0.0f;
+                if (model > 10e-3f) cf = data / model - 1;
+                if (model > 10e-3f) df = data / (model * model);
+                cf = fminf(cf, 10e4f);
+                df = fminf(df, 10e4f);
+                for (int ll = 0; ll < NVXY; ll++) {
+                    num[ll] += dudt[ll] * cf;
+                    den[ll] += d2udt2[ll] * cf - (dudt[ll] * dudt[ll]) * df;
+                }
+            }
+        /* cuda_gaussMLEv2.cu:500-513 */
+        theta[0] -= fminf(fmaxf(num[0] / den[0], -1e0f), 1e0f);
+        theta[1] -= fminf(fmaxf(num[1] / den[1], -1e0f), 1e0f);
+        theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
+        theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
+        theta[4] -= fminf(fmaxf(num[4] / den[4], -theta[4]), theta[4]);
+        theta[5] -= fminf(fmaxf(num[5] / den[5], -theta[5]), theta[5]);
+        theta[2] = fmaxf(theta[2], 1.0f);
+        theta[3] = fmaxf(theta[3], 0.01f);
+        theta[4] = fmaxf(theta[4], 0.5f);
+        theta[4] = fminf(theta[4], sz / 2.0f);
+        theta[5] = fmaxf(theta[5], 0.5f);
+        theta[5] = fminf(theta[5], sz / 2.0f);
+    }
+    /* cuda_gaussMLEv2.cu:518-546 */
+    div = 0.0f;
+    for (int ii = 0; ii < sz; ii++)
+        for (int jj = 0; jj < sz; jj++) {
+            psfx = orc_int_gauss_1d(ii, theta[0], theta[4]);
+            psfy = orc_int_gauss_1d(jj, theta[1], theta[5]);
+            model = theta[3] + theta[2] * psfx * psfy;
+            data = s_data[sz * jj + ii];
+            deriv_int_gauss_1d(ii, theta[0], theta[4], theta[2], psfy, &dudt[0], NULL);
+            deriv_int_gauss_1d(jj, theta[1], theta[5], theta[2], psfx, &dudt[1], NULL);
+            deriv_int_gauss_1d_sigma(ii, theta[0], theta[4], theta[2], psfy, &dudt[4], NULL);
+            deriv_int_gauss_1d_sigma(jj, theta[1], theta[5], theta[2], psfx, &dudt[5], NULL);
+            dudt[2] = psfx * psfy;
+            dudt[3] = 1.0f;
+            for (int k = 0; k < NVXY; k++)
+                for (int l = k; l < NVXY; l++) {
+                    m[k * NVXY + l] += dudt[l] * dudt[k] / model;
+                    m[l * NVXY + k] = m[k * NVXY + l];
+                }
+            if (model > 0) {
+                if (data > 0)
+                    div += data * logf(model) - model - data * logf(data) + data;
+                else
+                    div += -model;
+            }
+        }
+    orc_mat_inv_n(m, minv, diag, NVXY);
+    for (int k = 0; k < NVXY; k++) diagout[k] = diag[k];
+    *llout = div;
+}
+
+void orc_mle_xynbsxsy(const float* data, float psf_sigma, int sz, int iterations,
+                      float* params, float* crlb, float* ll, int nfits) {
+    for (int i = 0; i < nfits; i++) {
+        float theta[NVXY], diag[NVXY], l;
+        mle_one_sxsy(data + (size_t)sz * sz * i, psf_sigma, sz, iterations, theta, diag, &l);
+        for (int k = 0; k < NVXY; k++) {
             params[(size_t)nfits * k + i] = theta[k];
             crlb[(size_t)nfits * k + i] = diag[k];
         }

@@ -36,6 +36,9 @@ void orc_mle_xynb(const float* data, float psf_sigma, int sz, int iterations,
 /* kernel_MLEFit_XYNBS_ (cuda_gaussMLEv2.cu:144-276): theta = {x, y, N, bg, sigma}. */
 void orc_mle_xynbs(const float* data, float psf_sigma, int sz, int iterations,
                    float* params, float* crlb, float* ll, int nfits);
+/* kernel_MLEFit_XYNBSXSY_ (cuda_gaussMLEv2.cu:414-556): theta = {x, y, N, bg, sigma_ii, sigma_jj}. */
+void orc_mle_xynbsxsy(const float* data, float psf_sigma, int sz, int iterations,
+                      float* params, float* crlb, float* ll, int nfits);
 /* Helpers exposed for unit tests. */
 float orc_int_gauss_1d(int ii, float x, float sigma);
 void  orc_center_of_mass_2d(int sz, const float* data, float* x, float* y);

@@ -75,6 +75,7 @@ def _declare(L):
     L.orc_chain_raw.restype = c_uint32
     L.orc_mle_xynb.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
     L.orc_mle_xynbs.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
+    L.orc_mle_xynbsxsy.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
     L.orc_int_gauss_1d.argtypes = [c_int, c_float, c_float]
     L.orc_int_gauss_1d.restype = c_float
     L.orc_center_of_mass_2d.argtypes = [c_int, c_void_p, POINTER(c_float), POINTER(c_float)]
@@ -139,6 +140,18 @@ def mle_xynbs(boxes: np.ndarray, sigma: float, iterations: int = 20):
     lib().orc_mle_xynbs(boxes.ctypes.data, float(sigma), sz, iterations, p.ctypes.data, c.ctypes.data,
                         ll.ctypes.data, n)
     return p.reshape(5, n), c.reshape(5, n), ll
+
+
+def mle_xynbsxsy(boxes: np.ndarray, sigma: float, iterations: int = 20):
+    """kernel_MLEFit_XYNBSXSY_ restated.  Returns params (6,N), crlb (6,N), ll (N,)."""
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32)
+    n, sz = boxes.shape[0], boxes.shape[1]
+    p = np.zeros(6 * n, np.float32)
+    c = np.zeros(6 * n, np.float32)
+    ll = np.zeros(n, np.float32)
+    lib().orc_mle_xynbsxsy(boxes.ctypes.data, float(sigma), sz, iterations, p.ctypes.data, c.ctypes.data,
+                           ll.ctypes.data, n)
+    return p.reshape(6, n), c.reshape(6, n), ll
 
 
 def gauss_coeffs(sigma: float):

@@ -81,3 +81,37 @@ def test_gauss_mle_sigma_mirror():
     np.testing.assert_allclose(res["Photons"], t["I"], rtol=3e-3)
     assert res["Sigma_SE"].shape == (400,) and np.all(np.isfinite(res["Sigma_SE"]))
     assert stats["NFitsPerCall"] == 400
+
+
+def elliptic_boxes(n, sz, seed, sx, sy):
+    from scipy.special import erf
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(sz / 2 - 1, sz / 2 + 1, n) - 0.5
+    y = rng.uniform(sz / 2 - 1, sz / 2 + 1, n) - 0.5
+    I = rng.gamma(5.0, 200.0, n)
+    b = rng.uniform(5, 30, n)
+    e = np.arange(sz + 1, dtype=np.float64)
+    gx = 0.5 * np.diff(erf((e[None, :] - x[:, None]) / (sx * np.sqrt(2))), axis=1)
+    gy = 0.5 * np.diff(erf((e[None, :] - y[:, None]) / (sy * np.sqrt(2))), axis=1)
+    mu = b[:, None, None] + I[:, None, None] * gy[:, :, None] * gx[:, None, :]
+    return rng.poisson(mu).astype(np.float32)
+
+
+@pytest.mark.parametrize("sz,n,seed", [(7, 2000, 35), (9, 500, 36), (11, 200, 37)])
+def test_mle_sigmayx_matches_oracle(oracle, sz, n, seed):
+    """bamf_gauss_mle_xynbsxsy (kernel_MLEFit_XYNBSXSY_) against the oracle."""
+    boxes = elliptic_boxes(n, sz, seed, 1.2, 1.6)
+    pg, cg, lg = fit_boxes(boxes, 1.3, 20, "SigmaYX")
+    po, co, lo = oracle.mle_xynbsxsy(boxes, 1.3, 20)
+    print(f"sigma-yx fit sz={sz} n={n} seed={seed}", flush=True)
+    for k in range(6):
+        r = rel(pg[k], po[k], 1.0)
+        assert np.quantile(r, 0.999) < 1e-4 and r.max() < 1e-3, (k, np.quantile(r, 0.999), r.max())
+    # a fit whose emitter leaves the box with both widths clamped has a singular
+    # Fisher matrix: NaN in the reference's arithmetic, so non-finite on both sides
+    fin = np.isfinite(co).all(0)
+    assert np.array_equal(fin, np.isfinite(cg).all(0)) and fin.mean() > 0.99
+    assert np.quantile(rel(cg[:, fin], co[:, fin], 1e-6), 0.999) < 1e-3
+    assert np.quantile(rel(lg, lo, 1.0), 0.999) < 1e-3
+    res, _ = gauss_mle(boxes, "SigmaYX", "CCD", 1.3)
+    assert np.array_equal(res["SigmaY"], pg[4]) and np.array_equal(res["SigmaX"], pg[5])

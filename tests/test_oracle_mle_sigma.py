@@ -94,3 +94,75 @@ def test_sigma_clamps(oracle):
     p, _, _ = oracle.mle_xynbs(boxes, SIG0, 20)
     assert np.all(p[4] >= 0.5) and np.all(p[4] <= sz / 2)
     assert np.all(p[2] >= 1.0) and np.all(p[3] >= 0.01)
+
+
+# ---- kernel_MLEFit_XYNBSXSY_: one width per axis (theta[4] along ii, theta[5] along jj) ----
+
+def model64_xy(theta, sz):
+    x, y, n, bg, sx, sy = theta
+    e = np.arange(sz + 1, dtype=np.float64)
+    gx = 0.5 * np.diff(erf((e - x) / (sx * np.sqrt(2))))
+    gy = 0.5 * np.diff(erf((e - y) / (sy * np.sqrt(2))))
+    return bg + n * np.outer(gy, gx)  # [jj, ii]
+
+
+def elliptic_boxes(n, sz, seed, sx, sy, noise):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(sz / 2 - 1, sz / 2 + 1, n) - 0.5
+    y = rng.uniform(sz / 2 - 1, sz / 2 + 1, n) - 0.5
+    I = rng.uniform(800, 3000, n)
+    b = rng.uniform(5, 20, n)
+    mu = np.stack([model64_xy((x[i], y[i], I[i], b[i], sx, sy), sz) for i in range(n)])
+    boxes = rng.poisson(mu).astype(np.float32) if noise else mu.astype(np.float32)
+    return boxes, {"x": x, "y": y, "I": I, "bg": b}
+
+
+def test_xy_noiseless_recovers_both_widths(oracle):
+    boxes, t = elliptic_boxes(32, 9, 41, 1.1, 1.7, noise=False)
+    p, c, ll = oracle.mle_xynbsxsy(boxes, SIG0, 150)
+    np.testing.assert_allclose(p[0], t["x"], atol=3e-3)
+    np.testing.assert_allclose(p[1], t["y"], atol=3e-3)
+    np.testing.assert_allclose(p[2], t["I"], rtol=3e-3)
+    np.testing.assert_allclose(p[4], 1.1, rtol=2e-3)
+    np.testing.assert_allclose(p[5], 1.7, rtol=2e-3)
+
+
+def test_xy_crlb_against_finite_difference_fisher(oracle):
+    sz = 9
+    boxes, t = elliptic_boxes(8, sz, 42, 1.2, 1.5, noise=False)
+    p, c, _ = oracle.mle_xynbsxsy(boxes, SIG0, 150)
+    for i in range(8):
+        th = np.array([t["x"][i], t["y"][i], t["I"][i], t["bg"][i], 1.2, 1.5])
+        mu = model64_xy(th, sz)
+        J = np.zeros((sz * sz, 6))
+        for k in range(6):
+            h = 1e-5 * max(1.0, abs(th[k]))
+            tp, tm = th.copy(), th.copy()
+            tp[k] += h
+            tm[k] -= h
+            J[:, k] = ((model64_xy(tp, sz) - model64_xy(tm, sz)) / (2 * h)).ravel()
+        F = J.T @ (J / mu.ravel()[:, None])
+        np.testing.assert_allclose(c[:, i], np.diag(np.linalg.inv(F)), rtol=1e-2)
+
+
+def test_xy_noisy_fit_is_poisson_mle(oracle):
+    sz = 9
+    boxes, t = elliptic_boxes(8, sz, 43, 1.2, 1.5, noise=True)
+    p, _, _ = oracle.mle_xynbsxsy(boxes, SIG0, 150)
+    for i in range(8):
+        d = boxes[i].astype(np.float64)
+
+        def nll(th):
+            if th[4] <= 0.05 or th[5] <= 0.05:
+                return 1e30
+            mu = model64_xy(th, sz)
+            if np.any(mu <= 0):
+                return 1e30
+            return float(np.sum(mu - d * np.log(mu)))
+
+        th0 = np.array([t["x"][i], t["y"][i], t["I"][i], t["bg"][i], 1.2, 1.5])
+        r = minimize(nll, th0, method="Nelder-Mead",
+                     options={"xatol": 1e-8, "fatol": 1e-11, "maxiter": 60000, "maxfev": 60000})
+        assert np.all(np.abs(p[[0, 1], i] - r.x[[0, 1]]) < 3e-3)
+        assert abs(p[2, i] - r.x[2]) < 3e-3 * r.x[2]
+        assert np.all(np.abs(p[[4, 5], i] - r.x[[4, 5]]) < 3e-3 * r.x[[4, 5]])
