@@ -47,7 +47,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bamf_gauss_mle_xynbsxsy.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int,
                                           c_void_p]
     L.bamf_gauss_mle_fit_host.argtypes = [c_void_p, c_int, c_int64, c_float, c_int, c_int, c_void_p, c_void_p,
-                                          c_void_p, c_void_p]
+                                          c_void_p, c_void_p, c_void_p]
+    L.bamf_gauss_mle_xynbz.argtypes = [c_void_p] + [c_float] * 8 + [c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                                                     c_int, c_void_p]
     L.bamf_gauss_coeffs.argtypes = [c_float, fp]
     L.bamf_gauss_coeffs.restype = None
     coef = [c_float] * 5

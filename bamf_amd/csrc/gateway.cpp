@@ -2,7 +2,7 @@
 // MATLAB-level entry points, each over the device entry points:
 //   bamf_crjmcmc        cRJMCMC (source/c/cRJMCMC.cpp:52-305), one ROI per call;
 //   bamf_gauss_mle_host gaussMLE(Data, 'Basic', 'CCD', Sigma) (@RJ/gaussMLE.m:64-211);
-//   bamf_gauss_mle_fit_host the same for FitType 'Basic', 'Sigma' or 'SigmaYX', raw outputs;
+//   bamf_gauss_mle_fit_host the same for FitType 'Basic', 'Sigma', 'SigmaYX' or 'Z', raw outputs;
 //   bamf_find_roi_host  findROI(Image, Sigma1, Sigma2, Boxsize, Minval) (@RJ/findROI.m:1-163).
 // The MATLAB mex functions under csrc/mex only map mxArrays onto these.
 #include <algorithm>
@@ -159,12 +159,12 @@ int bamf_crjmcmc(const bamf_mcmc_params* mcmc, const float* data, int xsize, int
 }
 
 int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, int fit_type,
-                            float* params, float* crlb, float* ll, double* fit_seconds) {
+                            const float* zcal, float* params, float* crlb, float* ll, double* fit_seconds) {
     BAMF_CHECK_ARG(data && params && crlb && ll, "null argument");
     BAMF_CHECK_ARG(sz >= 1 && nfits >= 0 && iterations >= 0, "bad size");
-    BAMF_CHECK_ARG(fit_type == BAMF_FIT_BASIC || fit_type == BAMF_FIT_SIGMA || fit_type == BAMF_FIT_SIGMAYX,
-                   "fit_type must be BASIC, SIGMA or SIGMAYX");
-    const int np = fit_type == BAMF_FIT_SIGMAYX ? 6 : fit_type == BAMF_FIT_SIGMA ? 5 : 4;
+    BAMF_CHECK_ARG(fit_type >= BAMF_FIT_BASIC && fit_type <= BAMF_FIT_Z, "fit_type must be BASIC, SIGMA, SIGMAYX or Z");
+    BAMF_CHECK_ARG(fit_type != BAMF_FIT_Z || zcal, "fit type Z needs its calibration");
+    const int np = fit_type == BAMF_FIT_SIGMAYX ? 6 : fit_type == BAMF_FIT_BASIC ? 4 : 5;
     if (nfits == 0) {
         if (fit_seconds) *fit_seconds = 0;
         return BAMF_OK;
@@ -184,11 +184,18 @@ int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigm
         const int n = (int)std::min<int64_t>(chunk, nfits - f0);
         b.err = hipMemcpy(d_data, data + (size_t)f0 * sz * sz, (size_t)n * sz * sz * 4, hipMemcpyHostToDevice);
         if (b.err != hipSuccess) break;
-        const int rc = fit_type == BAMF_FIT_SIGMAYX
-                           ? bamf_gauss_mle_xynbsxsy(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr)
-                       : fit_type == BAMF_FIT_SIGMA
-                           ? bamf_gauss_mle_xynbs(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr)
-                           : bamf_gauss_mle_xynb(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr);
+        int rc;
+        switch (fit_type) {
+        case BAMF_FIT_SIGMA: rc = bamf_gauss_mle_xynbs(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr); break;
+        case BAMF_FIT_SIGMAYX:
+            rc = bamf_gauss_mle_xynbsxsy(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr);
+            break;
+        case BAMF_FIT_Z:
+            rc = bamf_gauss_mle_xynbz(d_data, sigma, zcal[0], zcal[1], zcal[2], zcal[3], zcal[4], zcal[5], zcal[6], sz,
+                                      iterations, d_p, d_c, d_l, n, nullptr);
+            break;
+        default: rc = bamf_gauss_mle_xynb(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr);
+        }
         if (rc != BAMF_OK) return rc;
         b.err = hipMemcpy(hp.data(), d_p, np * (size_t)n * 4, hipMemcpyDeviceToHost);
         if (b.err == hipSuccess) b.err = hipMemcpy(hc.data(), d_c, np * (size_t)n * 4, hipMemcpyDeviceToHost);
@@ -213,8 +220,8 @@ int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, i
     BAMF_CHECK_ARG(data && y && x && photons && bg && ll, "null argument");
     BAMF_CHECK_ARG(sz >= 1 && nfits >= 0 && iterations >= 0, "bad size");
     std::vector<float> p(4 * (size_t)nfits), c(4 * (size_t)nfits);
-    const int rc = bamf_gauss_mle_fit_host(data, sz, nfits, sigma, iterations, BAMF_FIT_BASIC, p.data(), c.data(),
-                                           ll, fit_seconds);
+    const int rc = bamf_gauss_mle_fit_host(data, sz, nfits, sigma, iterations, BAMF_FIT_BASIC, nullptr, p.data(),
+                                           c.data(), ll, fit_seconds);
     if (rc != BAMF_OK) return rc;
     // gaussMLE.m:183-190: parameter k = 0..3 is Y, X, Photons, BG; SE = sqrt(CRLB)
     float* out[4] = {y, x, photons, bg};

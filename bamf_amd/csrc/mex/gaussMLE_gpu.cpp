@@ -1,10 +1,10 @@
 // MATLAB mex entry with the argument list and outputs of gaussMLE for the
-// fits of FitType 'Basic', 'Sigma' and 'SigmaYX' with CameraType 'CCD'
+// fits of FitType 'Basic', 'Sigma', 'SigmaYX' and 'Z' with CameraType 'CCD'
 // (@RJ/gaussMLE.m:64-211), fitting on the GPU through bamf_gauss_mle_fit_host.
 // Build inside MATLAB (mex.h is needed; not compiled by bamf_amd/build.py):
 //   mex -R2018a gaussMLE_gpu.cpp -I<repo>/include -L<repo>/bamf_amd -lbamf_gpu
 //
-// [Results, Statistics] = gaussMLE_gpu(Data, FitType, CameraType, Sigma)
+// [Results, Statistics] = gaussMLE_gpu(Data, FitType, CameraType, Sigma[, ZStruct])
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -19,10 +19,13 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     mxGetString(prhs[1], fit, sizeof(fit));
     mxGetString(prhs[2], cam, sizeof(cam));
     const std::string ft(fit);
-    if ((ft != "Basic" && ft != "Sigma" && ft != "SigmaYX") || std::string(cam) != "CCD")
-        mexErrMsgIdAndTxt("bamf:args", "FitType 'Basic', 'Sigma' or 'SigmaYX' with CameraType 'CCD' is implemented");
-    const int fit_type = ft == "SigmaYX" ? BAMF_FIT_SIGMAYX : ft == "Sigma" ? BAMF_FIT_SIGMA : BAMF_FIT_BASIC;
-    const int np = fit_type == BAMF_FIT_SIGMAYX ? 6 : fit_type == BAMF_FIT_SIGMA ? 5 : 4;  // gaussMLE.m:93-109
+    if ((ft != "Basic" && ft != "Sigma" && ft != "SigmaYX" && ft != "Z") || std::string(cam) != "CCD")
+        mexErrMsgIdAndTxt("bamf:args", "FitType 'Basic', 'Sigma', 'SigmaYX' or 'Z' with CameraType 'CCD' is implemented");
+    const int fit_type = ft == "SigmaYX" ? BAMF_FIT_SIGMAYX
+                         : ft == "Sigma" ? BAMF_FIT_SIGMA
+                         : ft == "Z"     ? BAMF_FIT_Z
+                                         : BAMF_FIT_BASIC;
+    const int np = fit_type == BAMF_FIT_SIGMAYX ? 6 : fit_type == BAMF_FIT_BASIC ? 4 : 5;  // gaussMLE.m:93-109
     const mxArray* D = prhs[0];
     if (!mxIsSingle(D)) mexErrMsgIdAndTxt("bamf:args", "Data must be single (SZ, SZ, N)");
     const mwSize nd = mxGetNumberOfDimensions(D);
@@ -30,15 +33,29 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     const int sz = (int)dims[0];
     if (nd < 2 || dims[1] != dims[0]) mexErrMsgIdAndTxt("bamf:args", "Data must be SZ x SZ x N");
     const int64_t n = nd >= 3 ? (int64_t)dims[2] : 1;
-    // gaussMLE.m:154 passes mean(Sigma)
+    // gaussMLE.m:154 passes mean(Sigma); 'Z' passes Sigma(1), the ZStruct
+    // fields and Sigma(2) (gaussMLE.m:155-158)
     const double* sg = mxGetPr(prhs[3]);
     double sum = 0;
     const size_t ns = mxGetNumberOfElements(prhs[3]);
     for (size_t i = 0; i < ns; i++) sum += sg[i];
-    const float sigma = (float)(sum / (double)(ns ? ns : 1));
+    float sigma = (float)(sum / (double)(ns ? ns : 1));
+    float zcal[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (fit_type == BAMF_FIT_Z) {
+        if (nrhs < 5 || !mxIsStruct(prhs[4]) || ns < 2)
+            mexErrMsgIdAndTxt("bamf:args", "FitType 'Z' needs Sigma = [SigmaX SigmaY] and ZStruct");
+        static const char* zf[6] = {"Ax", "Ay", "Bx", "By", "Gamma", "D"};
+        for (int k = 0; k < 6; k++) {
+            const mxArray* v = mxGetField(prhs[4], 0, zf[k]);
+            if (!v) mexErrMsgIdAndTxt("bamf:args", "ZStruct.%s is missing", zf[k]);
+            zcal[k] = (float)mxGetScalar(v);
+        }
+        sigma = (float)sg[0];
+        zcal[6] = (float)sg[1];
+    }
     std::vector<float> p(np * (size_t)n), c(np * (size_t)n), ll((size_t)n);
     double secs = 0;
-    const int rc = bamf_gauss_mle_fit_host((const float*)mxGetData(D), sz, n, sigma, 20, fit_type, p.data(),
+    const int rc = bamf_gauss_mle_fit_host((const float*)mxGetData(D), sz, n, sigma, 20, fit_type, zcal, p.data(),
                                            c.data(), ll.data(), &secs);
     if (rc != BAMF_OK) mexErrMsgIdAndTxt("bamf:gpu", "%s", bamf_last_error());
     // Results fields in gaussMLE.m:183-196 order: parameters, their SEs, the
@@ -47,10 +64,13 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
                                      "LogLikelihood"};
     static const char* names_s[11] = {"Y", "X", "Photons", "BG", "Y_SE", "X_SE", "Photons_SE", "BG_SE",
                                       "Sigma", "Sigma_SE", "LogLikelihood"};
+    static const char* names_z[11] = {"Y", "X", "Photons", "BG", "Y_SE", "X_SE", "Photons_SE", "BG_SE",
+                                      "Z", "Z_SE", "LogLikelihood"};
     static const char* names_yx[13] = {"Y", "X", "Photons", "BG", "Y_SE", "X_SE", "Photons_SE", "BG_SE",
                                        "SigmaY", "SigmaX", "SigmaY_SE", "SigmaX_SE", "LogLikelihood"};
     const int nf = np == 6 ? 13 : np == 5 ? 11 : 9;
-    plhs[0] = mxCreateStructMatrix(1, 1, nf, np == 6 ? names_yx : np == 5 ? names_s : names_b);
+    plhs[0] = mxCreateStructMatrix(1, 1, nf,
+                                   np == 6 ? names_yx : fit_type == BAMF_FIT_Z ? names_z : np == 5 ? names_s : names_b);
     auto column = [&](const float* src, bool root) {
         mxArray* v = mxCreateNumericMatrix((mwSize)n, 1, mxSINGLE_CLASS, mxREAL);
         float* d = (float*)mxGetData(v);

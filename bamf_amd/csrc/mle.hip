@@ -1,9 +1,10 @@
 // Single-emitter Newton MLE of theta = {x, y, N, bg} (fit type 'Basic', NV =
-// 4), {x, y, N, bg, sigma} ('Sigma', NV = 5) or {x, y, N, bg, sigma_ii,
-// sigma_jj} ('SigmaYX', NV = 6) with CRLB and log-likelihood.  Same inputs,
-// outputs and iteration as kernel_MLEFit_XYNB_
+// 4), {x, y, N, bg, sigma} ('Sigma'), {x, y, N, bg, sigma_ii, sigma_jj}
+// ('SigmaYX') or {x, y, N, bg, z} ('Z', astigmatic widths) with CRLB and
+// log-likelihood.  Same inputs, outputs and iteration as kernel_MLEFit_XYNB_
 // (source/cuda/cuda_gaussMLEv2/cuda_gaussMLEv2.cu:16-141), kernel_MLEFit_XYNBS_
-// (:144-276) and kernel_MLEFit_XYNBSXSY_ (:414-556).
+// (:144-276), kernel_MLEFit_XYNBZ_ (:280-411) and kernel_MLEFit_XYNBSXSY_
+// (:414-556).
 //
 // gfx950 design: one work-item per fit, the box staged through LDS so that
 // global loads are contiguous, the pixel-integrated Gaussian evaluated in
@@ -20,6 +21,29 @@ namespace {
 
 constexpr float kPi = 3.141592f;  // definitions.h:13
 constexpr int kBlock = 64;
+
+// Fit types (gaussMLE.m:93-109) and their parameter counts.
+constexpr int kFtBasic = 0, kFtSigma = 1, kFtSigmaYX = 2, kFtZ = 3;
+constexpr int nv_of(int ft) { return ft == kFtSigmaYX ? 6 : ft == kFtBasic ? 4 : 5; }
+
+// Astigmatism calibration of fit type 'Z' (kernel_MLEFit_XYNBZ_'s PSFSigma_y,
+// Ax, Ay, Bx, By, gamma, d; PSFSigma_x is the kernels' sigma argument).
+struct ZCal {
+    float sy0, ax, ay, bx, by, gamma, d;
+};
+// GPUgaussLib.cuh:26-64: alpha(z) = 1 + (z/d)^2 + A (z/d)^3 + B (z/d)^4 and its derivatives.
+__device__ inline float alpha_z(float z, float a, float b, float d) {
+    const float t = z / d;
+    return 1.0f + (t * t) + a * (t * t * t) + b * (t * t * t * t);
+}
+__device__ inline float dalphadz(float z, float a, float b, float d) {
+    const float d2 = d * d;
+    return (2.0f * z / d2 + 3.0f * a * (z * z) / (d2 * d) + 4.0f * b * (z * z * z) / (d2 * d2));
+}
+__device__ inline float d2alphadz2(float z, float a, float b, float d) {
+    const float d2 = d * d;
+    return (2.0f / d2 + 6.0f * a * z / (d2 * d) + 12.0f * b * (z * z) / (d2 * d2));
+}
 
 // NV x NV partial inverse: in-place LU without pivoting, diagonal of the
 // inverse (same algorithm as kernel_MatInvN, MatInvLib.h:15-75).
@@ -91,11 +115,12 @@ __device__ inline AxisTerms axis_terms(int i, float x, float sigma, float sqrt_n
     return t;
 }
 
-template <int SZ, int NV>
+template <int SZ, int FT>
 __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ data, float sigma,
                                                      int iterations, float* __restrict__ params,
                                                      float* __restrict__ crlb, float* __restrict__ ll,
-                                                     int nfits) {
+                                                     int nfits, ZCal zc) {
+    constexpr int NV = nv_of(FT);
     constexpr int NP = SZ * SZ;
     // +1 word per box keeps the per-work-item stride odd (no LDS bank conflicts).
     __shared__ float s_box[kBlock * (NP + 1)];
@@ -115,7 +140,9 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
     for (int p = 0; p < NP; p++) d[p] = box[p];
 
     // --- initial values: centre of mass (GPUgaussLib.cuh:207-227) ---
-    constexpr bool SG = NV >= 5, XY = NV == 6;  // fitted width(s): one, or one per axis
+    // SG: the axis widths vary (fitted, or functions of z); XY: one fitted width
+    // per axis; ZF: widths from the fitted z through the astigmatism calibration
+    constexpr bool SG = FT != kFtBasic, XY = FT == kFtSigmaYX, ZF = FT == kFtZ;
     float theta[NV];
     {
         float sx = 0.f, sy = 0.f, ss = 0.f;
@@ -168,16 +195,43 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
                 fmin_ = fminf(fmin_, f);
             }
         theta[3] = fmin_;
-        theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
-        if constexpr (SG) theta[4] = sigma;
-        if constexpr (XY) theta[5] = sigma;
+        if constexpr (ZF) {  // cuda_gaussMLEv2.cu:331-332
+            theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * kPi * sigma * zc.sy0 * sqrtf(2.0f));
+            theta[4] = 0;
+        } else {
+            theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
+            if constexpr (SG) theta[4] = sigma;
+            if constexpr (XY) theta[5] = sigma;
+        }
     }
 
     const float c1 = sqrtf(2.0f * kPi);
     // widths along ii (x) and jj (y): sigma, theta[4] for both, or theta[4] and theta[5]
     float sgx = sigma, snx = sqrtf(1.0f / 2.0f / sigma / sigma), c2x = sigma * sigma * sigma;
     float sgy = sgx, sny = snx, c2y = c2x;
+    float dszx = 0.f, dszy = 0.f, ddszx = 0.f, ddszy = 0.f;  // 'Z': dS/dz and d2S/dz2 per axis
     auto set_widths = [&]() {
+        if constexpr (ZF) {  // GPUgaussLib.cuh:155-202
+            const float z = theta[4];
+            const float alx = alpha_z(z - zc.gamma, zc.ax, zc.bx, zc.d);
+            const float aly = alpha_z(z + zc.gamma, zc.ay, zc.by, zc.d);
+            const float rax = sqrtf(alx), ray = sqrtf(aly);
+            sgx = sigma * rax;
+            sgy = zc.sy0 * ray;
+            snx = sqrtf(1.0f / 2.0f / sgx / sgx);
+            sny = sqrtf(1.0f / 2.0f / sgy / sgy);
+            c2x = sgx * sgx * sgx;
+            c2y = sgy * sgy * sgy;
+            const float dsax = sigma / 2.0f / rax, dsay = zc.sy0 / 2.0f / ray;
+            const float dax = dalphadz(z - zc.gamma, zc.ax, zc.bx, zc.d);
+            const float day = dalphadz(z + zc.gamma, zc.ay, zc.by, zc.d);
+            dszx = dsax * dax;
+            dszy = dsay * day;
+            const float d2x = -sigma / 4.0f / (alx * rax), d2y = -zc.sy0 / 4.0f / (aly * ray);
+            ddszx = d2x * (dax * dax) + dsax * d2alphadz2(z - zc.gamma, zc.ax, zc.bx, zc.d);
+            ddszy = d2y * (day * day) + dsay * d2alphadz2(z + zc.gamma, zc.ay, zc.by, zc.d);
+            return;
+        }
         if constexpr (SG) {
             sgx = theta[4];
             snx = sqrtf(1.0f / 2.0f / sgx / sgx);
@@ -231,7 +285,12 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
                 if constexpr (SG) {
                     const float dsx = ax[ii].s1 * py, dsy = ay[jj].s1 * px;
                     const float vx = m2sx * dsx - ax[ii].s2 * py, vy = m2sy * dsy - ay[jj].s2 * px;
-                    if constexpr (XY) {
+                    if constexpr (ZF) {  // GPUgaussLib.cuh:178,198-199
+                        const float u4 = dsx * dszx + dsy * dszy;
+                        const float v4 = vx * (dszx * dszx) + dsx * ddszx + vy * (dszy * dszy) + dsy * ddszy;
+                        num[4] += u4 * cf;
+                        den[4] += v4 * cf - (u4 * u4) * df;
+                    } else if constexpr (XY) {
                         num[4] += dsx * cf;
                         den[4] += vx * cf - (dsx * dsx) * df;
                         num[5] += dsy * cf;
@@ -247,11 +306,12 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
         theta[1] -= fminf(fmaxf(num[1] / den[1], -1e0f), 1e0f);
         theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
         theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
-        if constexpr (SG) theta[4] -= fminf(fmaxf(num[4] / den[4], -theta[4]), theta[4]);
+        if constexpr (ZF) theta[4] -= fminf(fmaxf(num[4] / den[4], -1e-1f), 1e-1f);  // cuda_gaussMLEv2.cu:369
+        else if constexpr (SG) theta[4] -= fminf(fmaxf(num[4] / den[4], -theta[4]), theta[4]);
         if constexpr (XY) theta[5] -= fminf(fmaxf(num[5] / den[5], -theta[5]), theta[5]);
         theta[2] = fmaxf(theta[2], 1.0f);
         theta[3] = fmaxf(theta[3], 0.01f);
-        if constexpr (SG) theta[4] = fminf(fmaxf(theta[4], 0.5f), SZ / 2.0f);  // cuda_gaussMLEv2.cu:237-238
+        if constexpr (SG && !ZF) theta[4] = fminf(fmaxf(theta[4], 0.5f), SZ / 2.0f);  // cuda_gaussMLEv2.cu:237-238
         if constexpr (XY) theta[5] = fminf(fmaxf(theta[5], 0.5f), SZ / 2.0f);  // cuda_gaussMLEv2.cu:512-513
     }
     set_widths();
@@ -280,7 +340,9 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
                 du[1] = ay[jj].d1 * px;
                 du[2] = px * py;
                 du[3] = 1.0f;
-                if constexpr (XY) {
+                if constexpr (ZF) {
+                    du[4] = ax[ii].s1 * py * dszx + ay[jj].s1 * px * dszy;
+                } else if constexpr (XY) {
                     du[4] = ax[ii].s1 * py;
                     du[5] = ay[jj].s1 * px;
                 } else if constexpr (SG) {
@@ -313,12 +375,15 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
 }
 
 // Any box size: same algorithm reading the box from global memory.
-template <int NV>
+template <int FT>
 __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict__ data, float sigma,
                                                          int sz, int iterations, float* __restrict__ params,
                                                          float* __restrict__ crlb, float* __restrict__ ll,
-                                                         int nfits) {
-    constexpr bool SG = NV >= 5, XY = NV == 6;  // fitted width(s): one, or one per axis
+                                                         int nfits, ZCal zc) {
+    constexpr int NV = nv_of(FT);
+    // SG: the axis widths vary (fitted, or functions of z); XY: one fitted width
+    // per axis; ZF: widths from the fitted z through the astigmatism calibration
+    constexpr bool SG = FT != kFtBasic, XY = FT == kFtSigmaYX, ZF = FT == kFtZ;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= nfits) return;
     const float* d = data + i * sz * sz;
@@ -354,15 +419,42 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
                 fmin_ = fminf(fmin_, f);
             }
         theta[3] = fmin_;
-        theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
-        if constexpr (SG) theta[4] = sigma;
-        if constexpr (XY) theta[5] = sigma;
+        if constexpr (ZF) {  // cuda_gaussMLEv2.cu:331-332
+            theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * kPi * sigma * zc.sy0 * sqrtf(2.0f));
+            theta[4] = 0;
+        } else {
+            theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
+            if constexpr (SG) theta[4] = sigma;
+            if constexpr (XY) theta[5] = sigma;
+        }
     }
     const float c1 = sqrtf(2.0f * kPi);
     // widths along ii (x) and jj (y): sigma, theta[4] for both, or theta[4] and theta[5]
     float sgx = sigma, snx = sqrtf(1.0f / 2.0f / sigma / sigma), c2x = sigma * sigma * sigma;
     float sgy = sgx, sny = snx, c2y = c2x;
+    float dszx = 0.f, dszy = 0.f, ddszx = 0.f, ddszy = 0.f;  // 'Z': dS/dz and d2S/dz2 per axis
     auto set_widths = [&]() {
+        if constexpr (ZF) {  // GPUgaussLib.cuh:155-202
+            const float z = theta[4];
+            const float alx = alpha_z(z - zc.gamma, zc.ax, zc.bx, zc.d);
+            const float aly = alpha_z(z + zc.gamma, zc.ay, zc.by, zc.d);
+            const float rax = sqrtf(alx), ray = sqrtf(aly);
+            sgx = sigma * rax;
+            sgy = zc.sy0 * ray;
+            snx = sqrtf(1.0f / 2.0f / sgx / sgx);
+            sny = sqrtf(1.0f / 2.0f / sgy / sgy);
+            c2x = sgx * sgx * sgx;
+            c2y = sgy * sgy * sgy;
+            const float dsax = sigma / 2.0f / rax, dsay = zc.sy0 / 2.0f / ray;
+            const float dax = dalphadz(z - zc.gamma, zc.ax, zc.bx, zc.d);
+            const float day = dalphadz(z + zc.gamma, zc.ay, zc.by, zc.d);
+            dszx = dsax * dax;
+            dszy = dsay * day;
+            const float d2x = -sigma / 4.0f / (alx * rax), d2y = -zc.sy0 / 4.0f / (aly * ray);
+            ddszx = d2x * (dax * dax) + dsax * d2alphadz2(z - zc.gamma, zc.ax, zc.bx, zc.d);
+            ddszy = d2y * (day * day) + dsay * d2alphadz2(z + zc.gamma, zc.ay, zc.by, zc.d);
+            return;
+        }
         if constexpr (SG) {
             sgx = theta[4];
             snx = sqrtf(1.0f / 2.0f / sgx / sgx);
@@ -403,7 +495,12 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
                 if constexpr (SG) {
                     const float dsx = ax.s1 * py, dsy = ay.s1 * px;
                     const float vx = m2sx * dsx - ax.s2 * py, vy = m2sy * dsy - ay.s2 * px;
-                    if constexpr (XY) {
+                    if constexpr (ZF) {  // GPUgaussLib.cuh:178,198-199
+                        const float u4 = dsx * dszx + dsy * dszy;
+                        const float v4 = vx * (dszx * dszx) + dsx * ddszx + vy * (dszy * dszy) + dsy * ddszy;
+                        num[4] += u4 * cf;
+                        den[4] += v4 * cf - (u4 * u4) * df;
+                    } else if constexpr (XY) {
                         num[4] += dsx * cf;
                         den[4] += vx * cf - (dsx * dsx) * df;
                         num[5] += dsy * cf;
@@ -420,11 +517,12 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
         theta[1] -= fminf(fmaxf(num[1] / den[1], -1e0f), 1e0f);
         theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
         theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
-        if constexpr (SG) theta[4] -= fminf(fmaxf(num[4] / den[4], -theta[4]), theta[4]);
+        if constexpr (ZF) theta[4] -= fminf(fmaxf(num[4] / den[4], -1e-1f), 1e-1f);  // cuda_gaussMLEv2.cu:369
+        else if constexpr (SG) theta[4] -= fminf(fmaxf(num[4] / den[4], -theta[4]), theta[4]);
         if constexpr (XY) theta[5] -= fminf(fmaxf(num[5] / den[5], -theta[5]), theta[5]);
         theta[2] = fmaxf(theta[2], 1.0f);
         theta[3] = fmaxf(theta[3], 0.01f);
-        if constexpr (SG) theta[4] = fminf(fmaxf(theta[4], 0.5f), sz / 2.0f);
+        if constexpr (SG && !ZF) theta[4] = fminf(fmaxf(theta[4], 0.5f), sz / 2.0f);
         if constexpr (XY) theta[5] = fminf(fmaxf(theta[5], 0.5f), sz / 2.0f);
     }
     set_widths();
@@ -443,7 +541,9 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
             du[1] = ay.d1 * px;
             du[2] = px * py;
             du[3] = 1.0f;
-            if constexpr (XY) {
+            if constexpr (ZF) {
+                du[4] = ax.s1 * py * dszx + ay.s1 * px * dszy;
+            } else if constexpr (XY) {
                 du[4] = ax.s1 * py;
                 du[5] = ay.s1 * px;
             } else if constexpr (SG) {
@@ -468,9 +568,9 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
     ll[i] = div;
 }
 
-template <int NV>
+template <int FT>
 int launch_mle(const float* d_data, float psf_sigma, int sz, int iterations, float* d_params, float* d_crlb,
-               float* d_ll, int nfits, void* stream) {
+               float* d_ll, int nfits, void* stream, ZCal zc = {}) {
     BAMF_CHECK_ARG(nfits >= 0, "nfits must be >= 0 (got %d)", nfits);
     BAMF_CHECK_ARG(sz >= 1 && sz <= 64, "sz must be in [1, 64] (got %d)", sz);
     BAMF_CHECK_ARG(iterations >= 0, "iterations must be >= 0");
@@ -480,13 +580,13 @@ int launch_mle(const float* d_data, float psf_sigma, int sz, int iterations, flo
     dim3 grid((unsigned)ceil_div(nfits, kBlock)), block(kBlock);
     hipStream_t s = as_stream(stream);
     switch (sz) {
-    case 5: hipLaunchKernelGGL((k_mle_xynb<5, NV>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
-    case 6: hipLaunchKernelGGL((k_mle_xynb<6, NV>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
-    case 7: hipLaunchKernelGGL((k_mle_xynb<7, NV>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
-    case 8: hipLaunchKernelGGL((k_mle_xynb<8, NV>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
-    case 9: hipLaunchKernelGGL((k_mle_xynb<9, NV>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits); break;
+    case 5: hipLaunchKernelGGL((k_mle_xynb<5, FT>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc); break;
+    case 6: hipLaunchKernelGGL((k_mle_xynb<6, FT>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc); break;
+    case 7: hipLaunchKernelGGL((k_mle_xynb<7, FT>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc); break;
+    case 8: hipLaunchKernelGGL((k_mle_xynb<8, FT>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc); break;
+    case 9: hipLaunchKernelGGL((k_mle_xynb<9, FT>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc); break;
     default:
-        hipLaunchKernelGGL(k_mle_xynb_any<NV>, grid, block, 0, s, d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits);
+        hipLaunchKernelGGL(k_mle_xynb_any<FT>, grid, block, 0, s, d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, zc);
     }
     BAMF_LAUNCH_CHECK("k_mle_xynb");
     return BAMF_OK;
@@ -497,15 +597,24 @@ int launch_mle(const float* d_data, float psf_sigma, int sz, int iterations, flo
 
 extern "C" int bamf_gauss_mle_xynb(const float* d_data, float psf_sigma, int sz, int iterations,
                                    float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream) {
-    return bamf::launch_mle<4>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, stream);
+    return bamf::launch_mle<bamf::kFtBasic>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, stream);
 }
 
 extern "C" int bamf_gauss_mle_xynbs(const float* d_data, float psf_sigma, int sz, int iterations,
                                     float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream) {
-    return bamf::launch_mle<5>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, stream);
+    return bamf::launch_mle<bamf::kFtSigma>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, stream);
 }
 
 extern "C" int bamf_gauss_mle_xynbsxsy(const float* d_data, float psf_sigma, int sz, int iterations,
                                        float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream) {
-    return bamf::launch_mle<6>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, stream);
+    return bamf::launch_mle<bamf::kFtSigmaYX>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits,
+                                               stream);
+}
+
+extern "C" int bamf_gauss_mle_xynbz(const float* d_data, float psf_sigma_x, float ax, float ay, float bx, float by,
+                                    float gamma, float d, float psf_sigma_y, int sz, int iterations, float* d_params,
+                                    float* d_crlb, float* d_ll, int nfits, void* stream) {
+    BAMF_CHECK_ARG(psf_sigma_y > 0.f && d != 0.f, "psf_sigma_y must be > 0 and d nonzero");
+    return bamf::launch_mle<bamf::kFtZ>(d_data, psf_sigma_x, sz, iterations, d_params, d_crlb, d_ll, nfits, stream,
+                                        bamf::ZCal{psf_sigma_y, ax, ay, bx, by, gamma, d});
 }

@@ -1,12 +1,14 @@
-"""Host mirror of @RJ/gaussMLE.m for fit types 'Basic', 'Sigma' and 'SigmaYX' on a CCD camera.
+"""Host mirror of @RJ/gaussMLE.m for fit types 'Basic', 'Sigma', 'SigmaYX' and 'Z' on a CCD camera.
 
 ``gauss_mle(data, 'Basic', 'CCD', sigma)`` returns ``(Results, Statistics)``
 with the fields gaussMLE.m:183-210 builds (Y, X, Photons, BG, *_SE, and Sigma,
-Sigma_SE for 'Sigma', SigmaY, SigmaX and their SEs for 'SigmaYX';
+Sigma_SE for 'Sigma', SigmaY, SigmaX and their SEs for 'SigmaYX', Z and Z_SE
+for 'Z';
 LogLikelihood; FitTime, FitPerSecond, NKernelsCalls, NFitsPerCall).  The fit
 runs in one call of bamf_gauss_mle_xynb ('Basic', kernel _XYNB_),
-bamf_gauss_mle_xynbs ('Sigma', _XYNBS_) or bamf_gauss_mle_xynbsxsy
-('SigmaYX', _XYNBSXSY_) on the device; there is no host fallback.
+bamf_gauss_mle_xynbs ('Sigma', _XYNBS_), bamf_gauss_mle_xynbsxsy
+('SigmaYX', _XYNBSXSY_) or bamf_gauss_mle_xynbz ('Z', _XYNBZ_) on the
+device; there is no host fallback.
 """
 from __future__ import annotations
 
@@ -19,45 +21,57 @@ from ._lib import BamfError, check, lib
 
 
 _KERNELS = {"Basic": ("bamf_gauss_mle_xynb", 4), "Sigma": ("bamf_gauss_mle_xynbs", 5),
-            "SigmaYX": ("bamf_gauss_mle_xynbsxsy", 6)}
+            "SigmaYX": ("bamf_gauss_mle_xynbsxsy", 6), "Z": ("bamf_gauss_mle_xynbz", 5)}
+_ZFIELDS = ("Ax", "Ay", "Bx", "By", "Gamma", "D")
 
 
-def gauss_mle_device(d_data, sigma: float, sz: int, iterations: int, nfits: int, d_params, d_crlb, d_ll,
-                     stream=None, fit_type: str = "Basic") -> None:
-    """Direct C-ABI call on device buffers (kernel_MLEFit_XYNB_ / _XYNBS_ argument layout)."""
+def gauss_mle_device(d_data, sigma, sz: int, iterations: int, nfits: int, d_params, d_crlb, d_ll,
+                     stream=None, fit_type: str = "Basic", zstruct: dict | None = None) -> None:
+    """Direct C-ABI call on device buffers (the kernel_MLEFit_* argument layouts).
+    For 'Z', sigma = (SigmaX, SigmaY) and zstruct holds Ax, Ay, Bx, By, Gamma, D."""
     if fit_type not in _KERNELS:
-        raise BamfError(f"fit type {fit_type!r}: 'Basic', 'Sigma' and 'SigmaYX' are implemented")
+        raise BamfError(f"fit type {fit_type!r}: 'Basic', 'Sigma', 'SigmaYX' and 'Z' are implemented")
     name = _KERNELS[fit_type][0]
-    check(getattr(lib(), name)(d_data.vp, float(sigma), int(sz), int(iterations), d_params.vp, d_crlb.vp,
-                               d_ll.vp, int(nfits), stream.vp if stream else None), name)
+    sv = stream.vp if stream else None
+    if fit_type == "Z":
+        if zstruct is None:
+            raise BamfError("fit type 'Z' needs the ZStruct calibration")
+        sx, sy = (float(v) for v in np.ravel(sigma)[:2])
+        cal = [float(zstruct[k]) for k in _ZFIELDS]
+        check(getattr(lib(), name)(d_data.vp, sx, *cal, sy, int(sz), int(iterations), d_params.vp, d_crlb.vp,
+                                   d_ll.vp, int(nfits), sv), name)
+        return
+    check(getattr(lib(), name)(d_data.vp, float(np.mean(sigma)), int(sz), int(iterations), d_params.vp, d_crlb.vp,
+                               d_ll.vp, int(nfits), sv), name)
 
 
-def fit_boxes(boxes: np.ndarray, sigma: float, iterations: int = 20, fit_type: str = "Basic"):
+def fit_boxes(boxes: np.ndarray, sigma, iterations: int = 20, fit_type: str = "Basic", zstruct: dict | None = None):
     """boxes: (N, sz, sz) float32 with box[i, jj, ii] = data[sz*jj + ii].
     Returns (params (NP, N), crlb (NP, N), ll (N,)) in the kernel's parameter-major
-    layout, NP = 4 ('Basic') or 5 ('Sigma')."""
+    layout, NP = 4 ('Basic'), 5 ('Sigma', 'Z') or 6 ('SigmaYX')."""
     boxes = np.ascontiguousarray(boxes, dtype=np.float32)
     n, sz = boxes.shape[0], boxes.shape[1]
     if boxes.shape[1] != boxes.shape[2]:
         raise BamfError("boxes must be square")
     if fit_type not in _KERNELS:
-        raise BamfError(f"fit type {fit_type!r}: 'Basic', 'Sigma' and 'SigmaYX' are implemented")
+        raise BamfError(f"fit type {fit_type!r}: 'Basic', 'Sigma', 'SigmaYX' and 'Z' are implemented")
     npar = _KERNELS[fit_type][1]
     d_data = _hip.DeviceBuffer.from_array(boxes)
     d_p = _hip.DeviceBuffer(4 * npar * n)
     d_c = _hip.DeviceBuffer(4 * npar * n)
     d_l = _hip.DeviceBuffer(n * 4)
-    gauss_mle_device(d_data, sigma, sz, iterations, n, d_p, d_c, d_l, fit_type=fit_type)
+    gauss_mle_device(d_data, sigma, sz, iterations, n, d_p, d_c, d_l, fit_type=fit_type, zstruct=zstruct)
     _hip.synchronize()
     return (d_p.download(np.float32).reshape(npar, n), d_c.download(np.float32).reshape(npar, n),
             d_l.download(np.float32))
 
 
-def gauss_mle(data: np.ndarray, fit_type: str = "Basic", camera_type: str = "CCD", sigma=1.3, iterations: int = 20):
+def gauss_mle(data: np.ndarray, fit_type: str = "Basic", camera_type: str = "CCD", sigma=1.3, zstruct=None,
+              iterations: int = 20):
     """data: MATLAB-layout stack (SZ, SZ, N) as a Fortran-ordered array, or (N, SZ, SZ) C-ordered boxes."""
     if fit_type not in _KERNELS or camera_type != "CCD":
-        raise BamfError(f"fit type {fit_type!r} / camera {camera_type!r}: 'Basic', 'Sigma' and 'SigmaYX' on "
-                        "'CCD' are implemented")
+        raise BamfError(f"fit type {fit_type!r} / camera {camera_type!r}: 'Basic', 'Sigma', 'SigmaYX' and 'Z' "
+                        "on 'CCD' are implemented")
     data = np.asarray(data, dtype=np.float32)
     if data.ndim == 2:
         data = data[:, :, None] if data.flags.f_contiguous else data[None]
@@ -66,7 +80,7 @@ def gauss_mle(data: np.ndarray, fit_type: str = "Basic", camera_type: str = "CCD
     else:
         boxes = np.ascontiguousarray(data)
     t0 = time.perf_counter()
-    p, c, ll = fit_boxes(boxes, float(np.mean(sigma)), iterations, fit_type)
+    p, c, ll = fit_boxes(boxes, sigma, iterations, fit_type, zstruct)
     dt = time.perf_counter() - t0
     n = boxes.shape[0]
     results = {
@@ -77,6 +91,9 @@ def gauss_mle(data: np.ndarray, fit_type: str = "Basic", camera_type: str = "CCD
     if fit_type == "Sigma":  # gaussMLE.m:193-195
         results["Sigma"] = p[4]
         results["Sigma_SE"] = np.sqrt(c[4])
+    if fit_type == "Z":  # gaussMLE.m:196-198
+        results["Z"] = p[4]
+        results["Z_SE"] = np.sqrt(c[4])
     if fit_type == "SigmaYX":  # gaussMLE.m:199-202
         results["SigmaY"], results["SigmaX"] = p[4], p[5]
         results["SigmaY_SE"], results["SigmaX_SE"] = np.sqrt(c[4]), np.sqrt(c[5])

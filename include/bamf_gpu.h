@@ -58,6 +58,16 @@ int bamf_gauss_mle_xynbs(const float* d_data, float psf_sigma, int sz, int itera
  * and are kept in [0.5, sz/2] (cuda_gaussMLEv2.cu:510-513). */
 int bamf_gauss_mle_xynbsxsy(const float* d_data, float psf_sigma, int sz, int iterations,
                             float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream);
+/* gaussMLE fit type 'Z' (theta = {x, y, N, bg, z}, astigmatic PSF): replaces
+ * kernel_MLEFit_XYNBZ_ (cuda_gaussMLEv2.cu:280-282), called by gaussMLE.m:155-158
+ * with Sigma(1), ZStruct.Ax, Ay, Bx, By, Gamma, D, Sigma(2).  The widths are
+ * psf_sigma_x sqrt(alpha(z - gamma; ax, bx, d)) along the box rows and
+ * psf_sigma_y sqrt(alpha(z + gamma; ay, by, d)) along its columns, alpha(t) =
+ * 1 + (t/d)^2 + A (t/d)^3 + B (t/d)^4; z starts at 0 and moves at most 0.1 per
+ * iteration. */
+int bamf_gauss_mle_xynbz(const float* d_data, float psf_sigma_x, float ax, float ay, float bx, float by,
+                         float gamma, float d, float psf_sigma_y, int sz, int iterations, float* d_params,
+                         float* d_crlb, float* d_ll, int nfits, void* stream);
 
 /* ------------------------------------------------------------------------
  * FindROI.  A frame is `ysize` runs of `xsize` contiguous floats (the
@@ -457,16 +467,19 @@ int bamf_crjmcmc(const bamf_mcmc_params* mcmc, const float* data, int xsize, int
 int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, float* y, float* x,
                         float* photons, float* bg, float* y_se, float* x_se, float* photons_se, float* bg_se,
                         float* ll, double* fit_seconds);
-/* gaussMLE(Data, FitType, 'CCD', Sigma) for FitType 'Basic' (BAMF_FIT_BASIC,
- * kernel _XYNB_), 'Sigma' (BAMF_FIT_SIGMA, _XYNBS_) or 'SigmaYX'
- * (BAMF_FIT_SIGMAYX, _XYNBSXSY_), host arrays in and out: params and crlb are
- * NP x nfits parameter-major (NP = 4, 5 or 6: Y, X, Photons, BG[, Sigma |
- * SigmaY, SigmaX] as gaussMLE.m:183-202 maps them; SE = sqrt(crlb)). */
+/* gaussMLE(Data, FitType, 'CCD', Sigma[, ZStruct]) for FitType 'Basic'
+ * (BAMF_FIT_BASIC, kernel _XYNB_), 'Sigma' (BAMF_FIT_SIGMA, _XYNBS_), 'SigmaYX'
+ * (BAMF_FIT_SIGMAYX, _XYNBSXSY_) or 'Z' (BAMF_FIT_Z, _XYNBZ_), host arrays in
+ * and out: params and crlb are NP x nfits parameter-major (NP = 4, 5 or 6: Y,
+ * X, Photons, BG[, Sigma | SigmaY, SigmaX | Z] as gaussMLE.m:183-202 maps them;
+ * SE = sqrt(crlb)).  sigma is mean(Sigma), or Sigma(1) for 'Z'; zcal (only for
+ * 'Z', else may be NULL) = {Ax, Ay, Bx, By, Gamma, D, Sigma(2)}. */
 #define BAMF_FIT_BASIC 0
 #define BAMF_FIT_SIGMA 1
 #define BAMF_FIT_SIGMAYX 2
+#define BAMF_FIT_Z 3
 int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, int fit_type,
-                            float* params, float* crlb, float* ll, double* fit_seconds);
+                            const float* zcal, float* params, float* crlb, float* ll, double* fit_seconds);
 
 /* findROI(Image, Sigma1, Sigma2, Boxsize, Minval) on host arrays
  * (@RJ/findROI.m:1-163):

@@ -4,7 +4,10 @@
  *
  * Restates kernel_MLEFit_XYNB_ (source/cuda/cuda_gaussMLEv2/cuda_gaussMLEv2.cu:16-141),
  * kernel_MLEFit_XYNBS_ (cuda_gaussMLEv2.cu:144-276), kernel_MLEFit_XYNBSXSY_
- * (cuda_gaussMLEv2.cu:414-556: one width per axis) with their sigma derivatives
+ * (cuda_gaussMLEv2.cu:414-556: one width per axis), kernel_MLEFit_XYNBZ_
+ * (cuda_gaussMLEv2.cu:280-411: astigmatic z, with kernel_alpha / dalphadz /
+ * d2alphadz2 and kernel_DerivativeIntGauss2Dz, GPUgaussLib.cuh:26-64, 140-205)
+ * with their sigma derivatives
  * kernel_DerivativeIntGauss1DSigma / 2DSigma (GPUgaussLib.cuh:90-137)
  * and its helpers kernel_IntGauss1D / kernel_DerivativeIntGauss1D
  * (GPUgaussLib.cuh:14-24, 67-87), kernel_CenterofMass2D (GPUgaussLib.cuh:207-227),
@@ -418,6 +421,138 @@ void orc_mle_xynbsxsy(const float* data, float psf_sigma, int sz, int iterations
         float theta[NVXY], diag[NVXY], l;
         mle_one_sxsy(data + (size_t)sz * sz * i, psf_sigma, sz, iterations, theta, diag, &l);
         for (int k = 0; k < NVXY; k++) {
+            params[(size_t)nfits * k + i] = theta[k];
+            crlb[(size_t)nfits * k + i] = diag[k];
+        }
+        ll[i] = l;
+    }
+}
+
+/* GPUgaussLib.cuh:26-64.  pow(v, k) for integer k is written as products. */
+static float alpha_z(float z, float a, float b, float d) {
+    const float t = z / d;
+    return 1.0f + (t * t) + a * (t * t * t) + b * (t * t * t * t);
+}
+static float dalphadz(float z, float a, float b, float d) {
+    return (2.0f * z / (d * d) + 3.0f * a * (z * z) / (d * d * d) + 4.0f * b * (z * z * z) / powf(d, 4.0f));
+}
+static float d2alphadz2(float z, float a, float b, float d) {
+    return (2.0f / (d * d) + 6.0f * a * z / (d * d * d) + 12.0f * b * (z * z) / powf(d, 4.0f));
+}
+
+typedef struct { float sx0, ax, ay, bx, by, gamma, d, sy0; } ZCal;
+
+/* GPUgaussLib.cuh:140-205 */
+static void deriv_int_gauss_2d_z(int ii, int jj, const float* theta, const ZCal* c, float* ppsfx, float* ppsfy,
+                                 float* dudt, float* d2udt2) {
+    float dsx, dsy, ddsx, ddsy, ddx, ddy;
+    const float z = theta[4];
+    const float alphax = alpha_z(z - c->gamma, c->ax, c->bx, c->d);
+    const float alphay = alpha_z(z + c->gamma, c->ay, c->by, c->d);
+    const float sx = c->sx0 * sqrtf(alphax);
+    const float sy = c->sy0 * sqrtf(alphay);
+    const float psfx = orc_int_gauss_1d(ii, theta[0], sx);
+    const float psfy = orc_int_gauss_1d(jj, theta[1], sy);
+    *ppsfx = psfx;
+    *ppsfy = psfy;
+    deriv_int_gauss_1d(ii, theta[0], sx, theta[2], psfy, &dudt[0], &ddx);
+    deriv_int_gauss_1d(jj, theta[1], sy, theta[2], psfx, &dudt[1], &ddy);
+    deriv_int_gauss_1d_sigma(ii, theta[0], sx, theta[2], psfy, &dsx, &ddsx);
+    deriv_int_gauss_1d_sigma(jj, theta[1], sy, theta[2], psfx, &dsy, &ddsy);
+    const float dsdalpha_x = c->sx0 / 2.0f / sqrtf(alphax);
+    const float dsdalpha_y = c->sy0 / 2.0f / sqrtf(alphay);
+    const float dsdzx = dsdalpha_x * dalphadz(z - c->gamma, c->ax, c->bx, c->d);
+    const float dsdzy = dsdalpha_y * dalphadz(z + c->gamma, c->ay, c->by, c->d);
+    dudt[4] = dsx * dsdzx + dsy * dsdzy;
+    if (d2udt2) {
+        d2udt2[0] = ddx;
+        d2udt2[1] = ddy;
+        const float d2x = -c->sx0 / 4.0f / powf(alphax, 1.5f);
+        const float d2y = -c->sy0 / 4.0f / powf(alphay, 1.5f);
+        const float dax = dalphadz(z - c->gamma, c->ax, c->bx, c->d);
+        const float day = dalphadz(z + c->gamma, c->ay, c->by, c->d);
+        const float ddsddzx = d2x * (dax * dax) + dsdalpha_x * d2alphadz2(z - c->gamma, c->ax, c->bx, c->d);
+        const float ddsddzy = d2y * (day * day) + dsdalpha_y * d2alphadz2(z + c->gamma, c->ay, c->by, c->d);
+        d2udt2[4] = ddsx * (dsdzx * dsdzx) + dsx * ddsddzx + ddsy * (dsdzy * dsdzy) + dsy * ddsddzy;
+    }
+}
+
+/* cuda_gaussMLEv2.cu:280-411: theta = {x, y, N, bg, z}. */
+static void mle_one_z(const float* s_data, const ZCal* c, int sz, int iterations, float* theta, float* diagout,
+                      float* llout) {
+    float m[NVS * NVS], diag[NVS], minv[NVS * NVS];
+    float dudt[NVS], d2udt2[NVS], num[NVS], den[NVS];
+    float nmax, model, cf, df, data, psfx, psfy, div;
+    memset(m, 0, sizeof(m));
+    memset(minv, 0, sizeof(minv));
+    orc_center_of_mass_2d(sz, s_data, &theta[0], &theta[1]);
+    orc_gauss_fmaxmin_2d(sz, c->sx0, s_data, &nmax, &theta[3]);
+    theta[2] = fmaxf(0.0f, (nmax - theta[3]) * 2 * kPi * c->sx0 * c->sy0 * sqrtf(2.0f));
+    theta[4] = 0;
+    for (int kk = 0; kk < iterations; kk++) {
+        memset(num, 0, sizeof(num));
+        memset(den, 0, sizeof(den));
+        for (int ii = 0; ii < sz; ii++)
+            for (int jj = 0; jj < sz; jj++) {
+                deriv_int_gauss_2d_z(ii, jj, theta, c, &psfx, &psfy, dudt, d2udt2);
+                model = theta[3] + theta[2] * psfx * psfy;
+                data = s_data[sz * jj + ii];
+                dudt[2] = psfx * psfy;
+                d2udt2[2] = 0.0f;
+                dudt[3] = 1.0f;
+                d2udt2[3] = 0.0f;
+                cf = 0.0f;
+                df = 0.0f;
+                if (model > 10e-3f) cf = data / model - 1;
+                if (model > 10e-3f) df = data / (model * model);
+                cf = fminf(cf, 10e4f);
+                df = fminf(df, 10e4f);
+                for (int ll = 0; ll < NVS; ll++) {
+                    num[ll] += dudt[ll] * cf;
+                    den[ll] += d2udt2[ll] * cf - (dudt[ll] * dudt[ll]) * df;
+                }
+            }
+        /* cuda_gaussMLEv2.cu:365-373 */
+        theta[0] -= fminf(fmaxf(num[0] / den[0], -1e0f), 1e0f);
+        theta[1] -= fminf(fmaxf(num[1] / den[1], -1e0f), 1e0f);
+        theta[2] -= fminf(fmaxf(num[2] / den[2], -theta[2]), theta[2]);
+        theta[3] -= fminf(fmaxf(num[3] / den[3], -theta[3]), theta[3]);
+        theta[4] -= fminf(fmaxf(num[4] / den[4], -1e-1f), 1e-1f);
+        theta[2] = fmaxf(theta[2], 1.0f);
+        theta[3] = fmaxf(theta[3], 0.01f);
+    }
+    div = 0.0f;
+    for (int ii = 0; ii < sz; ii++)
+        for (int jj = 0; jj < sz; jj++) {
+            deriv_int_gauss_2d_z(ii, jj, theta, c, &psfx, &psfy, dudt, NULL);
+            model = theta[3] + theta[2] * psfx * psfy;
+            data = s_data[sz * jj + ii];
+            dudt[2] = psfx * psfy;
+            dudt[3] = 1.0f;
+            for (int k = 0; k < NVS; k++)
+                for (int l = k; l < NVS; l++) {
+                    m[k * NVS + l] += dudt[l] * dudt[k] / model;
+                    m[l * NVS + k] = m[k * NVS + l];
+                }
+            if (model > 0) {
+                if (data > 0)
+                    div += data * logf(model) - model - data * logf(data) + data;
+                else
+                    div += -model;
+            }
+        }
+    orc_mat_inv_n(m, minv, diag, NVS);
+    for (int k = 0; k < NVS; k++) diagout[k] = diag[k];
+    *llout = div;
+}
+
+void orc_mle_xynbz(const float* data, float sx0, float ax, float ay, float bx, float by, float gamma, float d,
+                   float sy0, int sz, int iterations, float* params, float* crlb, float* ll, int nfits) {
+    const ZCal c = {sx0, ax, ay, bx, by, gamma, d, sy0};
+    for (int i = 0; i < nfits; i++) {
+        float theta[NVS], diag[NVS], l;
+        mle_one_z(data + (size_t)sz * sz * i, &c, sz, iterations, theta, diag, &l);
+        for (int k = 0; k < NVS; k++) {
             params[(size_t)nfits * k + i] = theta[k];
             crlb[(size_t)nfits * k + i] = diag[k];
         }

@@ -39,6 +39,10 @@ void orc_mle_xynbs(const float* data, float psf_sigma, int sz, int iterations,
 /* kernel_MLEFit_XYNBSXSY_ (cuda_gaussMLEv2.cu:414-556): theta = {x, y, N, bg, sigma_ii, sigma_jj}. */
 void orc_mle_xynbsxsy(const float* data, float psf_sigma, int sz, int iterations,
                       float* params, float* crlb, float* ll, int nfits);
+/* kernel_MLEFit_XYNBZ_ (cuda_gaussMLEv2.cu:280-411): theta = {x, y, N, bg, z}, widths
+ * sx0 sqrt(alpha(z - gamma; ax, bx, d)) along ii and sy0 sqrt(alpha(z + gamma; ay, by, d)) along jj. */
+void orc_mle_xynbz(const float* data, float sx0, float ax, float ay, float bx, float by, float gamma, float d,
+                   float sy0, int sz, int iterations, float* params, float* crlb, float* ll, int nfits);
 /* Helpers exposed for unit tests. */
 float orc_int_gauss_1d(int ii, float x, float sigma);
 void  orc_center_of_mass_2d(int sz, const float* data, float* x, float* y);

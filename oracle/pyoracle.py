@@ -76,6 +76,7 @@ def _declare(L):
     L.orc_mle_xynb.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
     L.orc_mle_xynbs.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
     L.orc_mle_xynbsxsy.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
+    L.orc_mle_xynbz.argtypes = [c_void_p] + [c_float] * 8 + [c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
     L.orc_int_gauss_1d.argtypes = [c_int, c_float, c_float]
     L.orc_int_gauss_1d.restype = c_float
     L.orc_center_of_mass_2d.argtypes = [c_int, c_void_p, POINTER(c_float), POINTER(c_float)]
@@ -152,6 +153,19 @@ def mle_xynbsxsy(boxes: np.ndarray, sigma: float, iterations: int = 20):
     lib().orc_mle_xynbsxsy(boxes.ctypes.data, float(sigma), sz, iterations, p.ctypes.data, c.ctypes.data,
                            ll.ctypes.data, n)
     return p.reshape(6, n), c.reshape(6, n), ll
+
+
+def mle_xynbz(boxes: np.ndarray, cal, iterations: int = 20):
+    """kernel_MLEFit_XYNBZ_ restated; cal = (sx0, ax, ay, bx, by, gamma, d, sy0).
+    Returns params (5,N), crlb (5,N), ll (N,)."""
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32)
+    n, sz = boxes.shape[0], boxes.shape[1]
+    p = np.zeros(5 * n, np.float32)
+    c = np.zeros(5 * n, np.float32)
+    ll = np.zeros(n, np.float32)
+    lib().orc_mle_xynbz(boxes.ctypes.data, *[float(v) for v in cal], sz, iterations, p.ctypes.data, c.ctypes.data,
+                        ll.ctypes.data, n)
+    return p.reshape(5, n), c.reshape(5, n), ll
 
 
 def gauss_coeffs(sigma: float):

@@ -129,14 +129,16 @@ def test_gauss_mle_fit_gateway_sigma_matches_device_entry():
     p, c, ll = fit_boxes(boxes, 1.3, 20, "Sigma")
     hp, hc, hl = np.zeros(5 * n, np.float32), np.zeros(5 * n, np.float32), np.zeros(n, np.float32)
     secs = np.zeros(1, np.float64)
-    rc = lib().bamf_gauss_mle_fit_host(ptr(boxes), 9, n, ctypes.c_float(1.3), 20, 1, ptr(hp), ptr(hc), ptr(hl),
-                                       ptr(secs))
+    rc = lib().bamf_gauss_mle_fit_host(ptr(boxes), 9, n, ctypes.c_float(1.3), 20, 1, None, ptr(hp), ptr(hc),
+                                       ptr(hl), ptr(secs))
     assert rc == 0, lib().bamf_last_error()
     assert np.array_equal(hp.reshape(5, n).view(np.uint32), p.view(np.uint32))
     assert np.array_equal(hc.reshape(5, n).view(np.uint32), c.view(np.uint32))
     assert np.array_equal(hl.view(np.uint32), ll.view(np.uint32))
-    assert lib().bamf_gauss_mle_fit_host(ptr(boxes), 9, n, ctypes.c_float(1.3), 20, 7, ptr(hp), ptr(hc), ptr(hl),
-                                         None) != 0
+    assert lib().bamf_gauss_mle_fit_host(ptr(boxes), 9, n, ctypes.c_float(1.3), 20, 7, None, ptr(hp), ptr(hc),
+                                         ptr(hl), None) != 0
+    assert lib().bamf_gauss_mle_fit_host(ptr(boxes), 9, n, ctypes.c_float(1.3), 20, 3, None, ptr(hp), ptr(hc),
+                                         ptr(hl), None) != 0  # 'Z' without its calibration
 
 
 def test_find_roi_gateway_matches_device_entry():

@@ -115,3 +115,49 @@ def test_mle_sigmayx_matches_oracle(oracle, sz, n, seed):
     assert np.quantile(rel(lg, lo, 1.0), 0.999) < 1e-3
     res, _ = gauss_mle(boxes, "SigmaYX", "CCD", 1.3)
     assert np.array_equal(res["SigmaY"], pg[4]) and np.array_equal(res["SigmaX"], pg[5])
+
+
+ZSTRUCT = {"Ax": 0.05, "Ay": -0.05, "Bx": 0.02, "By": 0.03, "Gamma": 0.35, "D": 0.6}
+ZSIG = (1.1, 1.25)
+
+
+def z_boxes(n, sz, seed):
+    from scipy.special import erf
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(sz / 2 - 1, sz / 2 + 1, n) - 0.5
+    y = rng.uniform(sz / 2 - 1, sz / 2 + 1, n) - 0.5
+    I = rng.gamma(5.0, 300.0, n)
+    b = rng.uniform(5, 20, n)
+    z = rng.uniform(-0.3, 0.3, n)
+    d = ZSTRUCT["D"]
+
+    def alpha(t, a, bb):
+        u = t / d
+        return 1 + u ** 2 + a * u ** 3 + bb * u ** 4
+    sx = ZSIG[0] * np.sqrt(alpha(z - ZSTRUCT["Gamma"], ZSTRUCT["Ax"], ZSTRUCT["Bx"]))
+    sy = ZSIG[1] * np.sqrt(alpha(z + ZSTRUCT["Gamma"], ZSTRUCT["Ay"], ZSTRUCT["By"]))
+    e = np.arange(sz + 1, dtype=np.float64)
+    gx = 0.5 * np.diff(erf((e[None, :] - x[:, None]) / (sx[:, None] * np.sqrt(2))), axis=1)
+    gy = 0.5 * np.diff(erf((e[None, :] - y[:, None]) / (sy[:, None] * np.sqrt(2))), axis=1)
+    mu = b[:, None, None] + I[:, None, None] * gy[:, :, None] * gx[:, None, :]
+    return rng.poisson(mu).astype(np.float32)
+
+
+@pytest.mark.parametrize("sz,n,seed", [(7, 2000, 38), (9, 500, 39), (11, 200, 40)])
+def test_mle_z_matches_oracle(oracle, sz, n, seed):
+    """bamf_gauss_mle_xynbz (kernel_MLEFit_XYNBZ_) against the oracle."""
+    boxes = z_boxes(n, sz, seed)
+    pg, cg, lg = fit_boxes(boxes, ZSIG, 20, "Z", ZSTRUCT)
+    cal = (ZSIG[0], ZSTRUCT["Ax"], ZSTRUCT["Ay"], ZSTRUCT["Bx"], ZSTRUCT["By"], ZSTRUCT["Gamma"], ZSTRUCT["D"],
+           ZSIG[1])
+    po, co, lo = oracle.mle_xynbz(boxes, cal, 20)
+    print(f"z fit sz={sz} n={n} seed={seed}", flush=True)
+    for k in range(5):
+        r = rel(pg[k], po[k], 1.0)
+        assert np.quantile(r, 0.999) < 1e-4 and r.max() < 1e-3, (k, np.quantile(r, 0.999), r.max())
+    fin = np.isfinite(co).all(0)
+    assert np.array_equal(fin, np.isfinite(cg).all(0)) and fin.mean() > 0.99
+    assert np.quantile(rel(cg[:, fin], co[:, fin], 1e-6), 0.999) < 1e-3
+    assert np.quantile(rel(lg, lo, 1.0), 0.999) < 1e-3
+    res, _ = gauss_mle(boxes, "Z", "CCD", ZSIG, ZSTRUCT)
+    assert np.array_equal(res["Z"], pg[4]) and res["Z_SE"].shape == (n,)

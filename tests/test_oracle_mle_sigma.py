@@ -166,3 +166,83 @@ def test_xy_noisy_fit_is_poisson_mle(oracle):
         assert np.all(np.abs(p[[0, 1], i] - r.x[[0, 1]]) < 3e-3)
         assert abs(p[2, i] - r.x[2]) < 3e-3 * r.x[2]
         assert np.all(np.abs(p[[4, 5], i] - r.x[[4, 5]]) < 3e-3 * r.x[[4, 5]])
+
+
+# ---- kernel_MLEFit_XYNBZ_: astigmatic z (widths sx0 sqrt(alpha(z - gamma)), sy0 sqrt(alpha(z + gamma))) ----
+
+ZCAL = (1.1, 0.05, -0.05, 0.02, 0.03, 0.35, 0.6, 1.25)  # sx0, ax, ay, bx, by, gamma, d, sy0 (um-like units)
+
+
+def widths_z(z, cal=ZCAL):
+    sx0, ax, ay, bx, by, g, d, sy0 = cal
+
+    def alpha(t, a, b):
+        u = t / d
+        return 1 + u ** 2 + a * u ** 3 + b * u ** 4
+    return sx0 * np.sqrt(alpha(z - g, ax, bx)), sy0 * np.sqrt(alpha(z + g, ay, by))
+
+
+def model64_z(theta, sz):
+    x, y, n, bg, z = theta
+    sx, sy = widths_z(z)
+    return model64_xy((x, y, n, bg, sx, sy), sz)
+
+
+def z_boxes(n, sz, seed, zs, noise):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(sz / 2 - 1, sz / 2 + 1, n) - 0.5
+    y = rng.uniform(sz / 2 - 1, sz / 2 + 1, n) - 0.5
+    I = rng.uniform(1500, 4000, n)
+    b = rng.uniform(5, 15, n)
+    z = np.asarray(zs, np.float64) if np.ndim(zs) else np.full(n, zs)
+    mu = np.stack([model64_z((x[i], y[i], I[i], b[i], z[i]), sz) for i in range(n)])
+    boxes = rng.poisson(mu).astype(np.float32) if noise else mu.astype(np.float32)
+    return boxes, {"x": x, "y": y, "I": I, "bg": b, "z": z}
+
+
+def test_z_noiseless_recovers_z(oracle):
+    boxes, t = z_boxes(24, 11, 51, np.linspace(-0.3, 0.3, 24), noise=False)
+    p, c, ll = oracle.mle_xynbz(boxes, ZCAL, 200)
+    np.testing.assert_allclose(p[0], t["x"], atol=3e-3)
+    np.testing.assert_allclose(p[1], t["y"], atol=3e-3)
+    np.testing.assert_allclose(p[2], t["I"], rtol=3e-3)
+    np.testing.assert_allclose(p[4], t["z"], atol=3e-3)
+
+
+def test_z_crlb_against_finite_difference_fisher(oracle):
+    sz = 11
+    boxes, t = z_boxes(8, sz, 52, np.linspace(-0.25, 0.25, 8), noise=False)
+    p, c, _ = oracle.mle_xynbz(boxes, ZCAL, 200)
+    for i in range(8):
+        th = np.array([t["x"][i], t["y"][i], t["I"][i], t["bg"][i], t["z"][i]])
+        mu = model64_z(th, sz)
+        J = np.zeros((sz * sz, 5))
+        for k in range(5):
+            h = 1e-5 * max(1.0, abs(th[k]))
+            tp, tm = th.copy(), th.copy()
+            tp[k] += h
+            tm[k] -= h
+            J[:, k] = ((model64_z(tp, sz) - model64_z(tm, sz)) / (2 * h)).ravel()
+        F = J.T @ (J / mu.ravel()[:, None])
+        np.testing.assert_allclose(c[:, i], np.diag(np.linalg.inv(F)), rtol=1e-2)
+
+
+def test_z_noisy_fit_is_poisson_mle(oracle):
+    sz = 11
+    boxes, t = z_boxes(8, sz, 53, np.linspace(-0.2, 0.2, 8), noise=True)
+    p, _, _ = oracle.mle_xynbz(boxes, ZCAL, 200)
+    for i in range(8):
+        d = boxes[i].astype(np.float64)
+
+        def nll(th):
+            mu = model64_z(th, sz)
+            if np.any(mu <= 0) or not np.all(np.isfinite(mu)):
+                return 1e30
+            return float(np.sum(mu - d * np.log(mu)))
+
+        th0 = np.array([t["x"][i], t["y"][i], t["I"][i], t["bg"][i], t["z"][i]])
+        r = minimize(nll, th0, method="Nelder-Mead",
+                     options={"xatol": 1e-8, "fatol": 1e-11, "maxiter": 40000, "maxfev": 40000})
+        assert np.all(np.abs(p[[0, 1], i] - r.x[[0, 1]]) < 3e-3)
+        assert abs(p[2, i] - r.x[2]) < 3e-3 * r.x[2]
+        assert abs(p[4, i] - r.x[4]) < 3e-3
