@@ -50,6 +50,13 @@ def _declare(L: ctypes.CDLL) -> None:
                                           c_void_p, c_void_p, c_void_p]
     L.bamf_gauss_mle_xynbz.argtypes = [c_void_p] + [c_float] * 8 + [c_int, c_int, c_void_p, c_void_p, c_void_p,
                                                                      c_int, c_void_p]
+    sc = [c_void_p, c_void_p, c_void_p, c_float, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p]
+    for name in ("bamf_gauss_mle_scmos_xynb", "bamf_gauss_mle_scmos_xynbs", "bamf_gauss_mle_scmos_xynbsxsy"):
+        getattr(L, name).argtypes = sc
+    L.bamf_gauss_mle_scmos_xynbz.argtypes = [c_void_p] * 4 + [c_float] * 8 + [c_int, c_int, c_int, c_void_p,
+                                                                               c_void_p, c_void_p, c_int, c_void_p]
+    L.bamf_gauss_mle_scmos_fit_host.argtypes = [c_void_p, c_int, c_int64, c_void_p, c_void_p, c_int, c_float, c_int,
+                                                c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.bamf_gauss_coeffs.argtypes = [c_float, fp]
     L.bamf_gauss_coeffs.restype = None
     coef = [c_float] * 5

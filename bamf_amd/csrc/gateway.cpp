@@ -158,43 +158,76 @@ int bamf_crjmcmc(const bamf_mcmc_params* mcmc, const float* data, int xsize, int
     return BAMF_OK;
 }
 
-int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, int fit_type,
-                            const float* zcal, float* params, float* crlb, float* ll, double* fit_seconds) {
+}  // extern "C"
+
+namespace {
+
+// gaussMLE.m:113-180 on host arrays: fits in calls of at most 2^24 (the
+// reference splits by free device memory the same way; fits are independent).
+// coords/gain non-null: the sCMOS kernels.
+int mle_fit_host(const float* data, int sz, int64_t nfits, const float* coords, const float* gain, int mapsz,
+                 float sigma, int iterations, int fit_type, const float* zcal, const float* z0, float* params,
+                 float* crlb, float* ll, double* fit_seconds) {
     BAMF_CHECK_ARG(data && params && crlb && ll, "null argument");
     BAMF_CHECK_ARG(sz >= 1 && nfits >= 0 && iterations >= 0, "bad size");
     BAMF_CHECK_ARG(fit_type >= BAMF_FIT_BASIC && fit_type <= BAMF_FIT_Z, "fit_type must be BASIC, SIGMA, SIGMAYX or Z");
     BAMF_CHECK_ARG(fit_type != BAMF_FIT_Z || zcal, "fit type Z needs its calibration");
+    const bool sc = coords != nullptr;
+    BAMF_CHECK_ARG(!sc || (gain && mapsz >= sz && (fit_type != BAMF_FIT_Z || z0)), "sCMOS fits need the gain map");
     const int np = fit_type == BAMF_FIT_SIGMAYX ? 6 : fit_type == BAMF_FIT_BASIC ? 4 : 5;
     if (nfits == 0) {
         if (fit_seconds) *fit_seconds = 0;
         return BAMF_OK;
     }
     const auto t0 = std::chrono::steady_clock::now();
-    // Calls of at most 2^24 fits each: gaussMLE.m:123-131 splits by free
-    // device memory the same way; fits are independent, so the split changes
-    // nothing in the results.
     const int64_t chunk = std::min<int64_t>(nfits, int64_t(1) << 24);
     DevBufs b;
     float* d_data = b.alloc<float>((size_t)chunk * sz * sz);
     float* d_p = b.alloc<float>(np * (size_t)chunk);
     float* d_c = b.alloc<float>(np * (size_t)chunk);
     float* d_l = b.alloc<float>((size_t)chunk);
+    float* d_xy = sc ? b.alloc<float>(2 * (size_t)chunk) : nullptr;
+    float* d_z0 = sc && fit_type == BAMF_FIT_Z ? b.alloc<float>((size_t)chunk) : nullptr;
+    const float* d_gain = sc ? b.upload(gain, (size_t)mapsz * mapsz) : nullptr;
     std::vector<float> hp(np * (size_t)chunk), hc(np * (size_t)chunk);
     for (int64_t f0 = 0; f0 < nfits && b.err == hipSuccess; f0 += chunk) {
         const int n = (int)std::min<int64_t>(chunk, nfits - f0);
         b.err = hipMemcpy(d_data, data + (size_t)f0 * sz * sz, (size_t)n * sz * sz * 4, hipMemcpyHostToDevice);
+        if (b.err == hipSuccess && d_xy) b.err = hipMemcpy(d_xy, coords + 2 * f0, 2 * (size_t)n * 4, hipMemcpyHostToDevice);
+        if (b.err == hipSuccess && d_z0) b.err = hipMemcpy(d_z0, z0 + f0, (size_t)n * 4, hipMemcpyHostToDevice);
         if (b.err != hipSuccess) break;
         int rc;
-        switch (fit_type) {
-        case BAMF_FIT_SIGMA: rc = bamf_gauss_mle_xynbs(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr); break;
-        case BAMF_FIT_SIGMAYX:
-            rc = bamf_gauss_mle_xynbsxsy(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr);
-            break;
-        case BAMF_FIT_Z:
-            rc = bamf_gauss_mle_xynbz(d_data, sigma, zcal[0], zcal[1], zcal[2], zcal[3], zcal[4], zcal[5], zcal[6], sz,
-                                      iterations, d_p, d_c, d_l, n, nullptr);
-            break;
-        default: rc = bamf_gauss_mle_xynb(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr);
+        if (sc) {
+            switch (fit_type) {
+            case BAMF_FIT_SIGMA:
+                rc = bamf_gauss_mle_scmos_xynbs(d_data, d_xy, d_gain, sigma, sz, mapsz, iterations, d_p, d_c, d_l, n,
+                                                nullptr);
+                break;
+            case BAMF_FIT_SIGMAYX:
+                rc = bamf_gauss_mle_scmos_xynbsxsy(d_data, d_xy, d_gain, sigma, sz, mapsz, iterations, d_p, d_c, d_l,
+                                                   n, nullptr);
+                break;
+            case BAMF_FIT_Z:
+                rc = bamf_gauss_mle_scmos_xynbz(d_data, d_xy, d_gain, d_z0, sigma, zcal[0], zcal[1], zcal[2], zcal[3],
+                                                zcal[4], zcal[5], zcal[6], sz, mapsz, iterations, d_p, d_c, d_l, n,
+                                                nullptr);
+                break;
+            default:
+                rc = bamf_gauss_mle_scmos_xynb(d_data, d_xy, d_gain, sigma, sz, mapsz, iterations, d_p, d_c, d_l, n,
+                                               nullptr);
+            }
+        } else {
+            switch (fit_type) {
+            case BAMF_FIT_SIGMA: rc = bamf_gauss_mle_xynbs(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr); break;
+            case BAMF_FIT_SIGMAYX:
+                rc = bamf_gauss_mle_xynbsxsy(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr);
+                break;
+            case BAMF_FIT_Z:
+                rc = bamf_gauss_mle_xynbz(d_data, sigma, zcal[0], zcal[1], zcal[2], zcal[3], zcal[4], zcal[5],
+                                          zcal[6], sz, iterations, d_p, d_c, d_l, n, nullptr);
+                break;
+            default: rc = bamf_gauss_mle_xynb(d_data, sigma, sz, iterations, d_p, d_c, d_l, n, nullptr);
+            }
         }
         if (rc != BAMF_OK) return rc;
         b.err = hipMemcpy(hp.data(), d_p, np * (size_t)n * 4, hipMemcpyDeviceToHost);
@@ -212,6 +245,25 @@ int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigm
     }
     if (fit_seconds) fit_seconds[0] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return BAMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, int fit_type,
+                            const float* zcal, float* params, float* crlb, float* ll, double* fit_seconds) {
+    return mle_fit_host(data, sz, nfits, nullptr, nullptr, 0, sigma, iterations, fit_type, zcal, nullptr, params,
+                        crlb, ll, fit_seconds);
+}
+
+int bamf_gauss_mle_scmos_fit_host(const float* data, int sz, int64_t nfits, const float* coords,
+                                  const float* gain_ratio, int mapsz, float sigma, int iterations, int fit_type,
+                                  const float* zcal, const float* z0, float* params, float* crlb, float* ll,
+                                  double* fit_seconds) {
+    BAMF_CHECK_ARG(coords && gain_ratio, "sCMOS fits need box corners and the gain map");
+    return mle_fit_host(data, sz, nfits, coords, gain_ratio, mapsz, sigma, iterations, fit_type, zcal, z0, params,
+                        crlb, ll, fit_seconds);
 }
 
 int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, float* y, float* x,

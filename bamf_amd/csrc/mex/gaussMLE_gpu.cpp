@@ -1,10 +1,13 @@
 // MATLAB mex entry with the argument list and outputs of gaussMLE for the
-// fits of FitType 'Basic', 'Sigma', 'SigmaYX' and 'Z' with CameraType 'CCD'
+// fits of FitType 'Basic', 'Sigma', 'SigmaYX' and 'Z' with CameraType 'CCD' or 'SCMOS'
 // (@RJ/gaussMLE.m:64-211), fitting on the GPU through bamf_gauss_mle_fit_host.
 // Build inside MATLAB (mex.h is needed; not compiled by bamf_amd/build.py):
 //   mex -R2018a gaussMLE_gpu.cpp -I<repo>/include -L<repo>/bamf_amd -lbamf_gpu
 //
-// [Results, Statistics] = gaussMLE_gpu(Data, FitType, CameraType, Sigma[, ZStruct])
+// [Results, Statistics] = gaussMLE_gpu(Data, FitType, 'CCD', Sigma[, ZStruct])
+// [Results, Statistics] = gaussMLE_gpu(Data, FitType, 'SCMOS', Sigma, VarianceImage, BoxCorners[, ZStruct])
+// (BoxCorners as the kernels read d_Coords: the corner pair of box b at
+// entries 2b, 2b+1, i.e. a 2 x N array.)
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -19,8 +22,9 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     mxGetString(prhs[1], fit, sizeof(fit));
     mxGetString(prhs[2], cam, sizeof(cam));
     const std::string ft(fit);
-    if ((ft != "Basic" && ft != "Sigma" && ft != "SigmaYX" && ft != "Z") || std::string(cam) != "CCD")
-        mexErrMsgIdAndTxt("bamf:args", "FitType 'Basic', 'Sigma', 'SigmaYX' or 'Z' with CameraType 'CCD' is implemented");
+    const bool scmos = std::string(cam) == "SCMOS";
+    if ((ft != "Basic" && ft != "Sigma" && ft != "SigmaYX" && ft != "Z") || (!scmos && std::string(cam) != "CCD"))
+        mexErrMsgIdAndTxt("bamf:args", "FitType 'Basic', 'Sigma', 'SigmaYX' or 'Z' with CameraType 'CCD' or 'SCMOS'");
     const int fit_type = ft == "SigmaYX" ? BAMF_FIT_SIGMAYX
                          : ft == "Sigma" ? BAMF_FIT_SIGMA
                          : ft == "Z"     ? BAMF_FIT_Z
@@ -41,12 +45,14 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     for (size_t i = 0; i < ns; i++) sum += sg[i];
     float sigma = (float)(sum / (double)(ns ? ns : 1));
     float zcal[7] = {0, 0, 0, 0, 0, 0, 0};
+    // gaussMLE.m:69-84: VarianceImage and BoxCorners come before ZStruct for sCMOS
+    const int zarg = scmos ? 6 : 4;
     if (fit_type == BAMF_FIT_Z) {
-        if (nrhs < 5 || !mxIsStruct(prhs[4]) || ns < 2)
+        if (nrhs <= zarg || !mxIsStruct(prhs[zarg]) || ns < 2)
             mexErrMsgIdAndTxt("bamf:args", "FitType 'Z' needs Sigma = [SigmaX SigmaY] and ZStruct");
         static const char* zf[6] = {"Ax", "Ay", "Bx", "By", "Gamma", "D"};
         for (int k = 0; k < 6; k++) {
-            const mxArray* v = mxGetField(prhs[4], 0, zf[k]);
+            const mxArray* v = mxGetField(prhs[zarg], 0, zf[k]);
             if (!v) mexErrMsgIdAndTxt("bamf:args", "ZStruct.%s is missing", zf[k]);
             zcal[k] = (float)mxGetScalar(v);
         }
@@ -55,8 +61,19 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     }
     std::vector<float> p(np * (size_t)n), c(np * (size_t)n), ll((size_t)n);
     double secs = 0;
-    const int rc = bamf_gauss_mle_fit_host((const float*)mxGetData(D), sz, n, sigma, 20, fit_type, zcal, p.data(),
-                                           c.data(), ll.data(), &secs);
+    int rc;
+    if (scmos) {
+        if (nrhs < 6 || !mxIsSingle(prhs[4]) || !mxIsSingle(prhs[5]) ||
+            mxGetNumberOfElements(prhs[5]) < 2 * (size_t)n || mxGetM(prhs[4]) != mxGetN(prhs[4]))
+            mexErrMsgIdAndTxt("bamf:args", "SCMOS needs single VarianceImage (square) and BoxCorners (2 x N)");
+        std::vector<float> z0((size_t)n, 0.0f);  // gaussMLE.m:167
+        rc = bamf_gauss_mle_scmos_fit_host((const float*)mxGetData(D), sz, n, (const float*)mxGetData(prhs[5]),
+                                           (const float*)mxGetData(prhs[4]), (int)mxGetM(prhs[4]), sigma, 20, fit_type,
+                                           zcal, z0.data(), p.data(), c.data(), ll.data(), &secs);
+    } else {
+        rc = bamf_gauss_mle_fit_host((const float*)mxGetData(D), sz, n, sigma, 20, fit_type, zcal, p.data(), c.data(),
+                                     ll.data(), &secs);
+    }
     if (rc != BAMF_OK) mexErrMsgIdAndTxt("bamf:gpu", "%s", bamf_last_error());
     // Results fields in gaussMLE.m:183-196 order: parameters, their SEs, the
     // fit type's extra pair, LogLikelihood

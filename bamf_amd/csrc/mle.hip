@@ -26,6 +26,21 @@ constexpr int kBlock = 64;
 constexpr int kFtBasic = 0, kFtSigma = 1, kFtSigmaYX = 2, kFtZ = 3;
 constexpr int nv_of(int ft) { return ft == kFtSigmaYX ? 6 : ft == kFtBasic ? 4 : 5; }
 
+// sCMOS inputs (kernel_MLEFit_SCMOS*, cuda_gaussMLEv2.cu:559-1124): per fit the
+// box corner (x, y) in the square map of variance/gain^2 ratios, the map's
+// side, and for 'Z' the starting z of each fit.
+struct ScIn {
+    const float* coords;
+    const float* gain;
+    int mapsz;
+    const float* z0;
+};
+// The map index of pixel (ii, jj), formed in float as the reference forms
+// GRind = (Coords[1] + jj) * Mapsz + Coords[0] + ii (cuda_gaussMLEv2.cu:620).
+__device__ inline float gain_at(const ScIn& sc, float cx, float cy, int ii, int jj) {
+    return sc.gain[(int)((cy + jj) * sc.mapsz + cx + ii)];
+}
+
 // Astigmatism calibration of fit type 'Z' (kernel_MLEFit_XYNBZ_'s PSFSigma_y,
 // Ax, Ay, Bx, By, gamma, d; PSFSigma_x is the kernels' sigma argument).
 struct ZCal {
@@ -115,11 +130,11 @@ __device__ inline AxisTerms axis_terms(int i, float x, float sigma, float sqrt_n
     return t;
 }
 
-template <int SZ, int FT>
+template <int SZ, int FT, bool SC>
 __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ data, float sigma,
                                                      int iterations, float* __restrict__ params,
                                                      float* __restrict__ crlb, float* __restrict__ ll,
-                                                     int nfits, ZCal zc) {
+                                                     int nfits, ZCal zc, ScIn sc) {
     constexpr int NV = nv_of(FT);
     constexpr int NP = SZ * SZ;
     // +1 word per box keeps the per-work-item stride odd (no LDS bank conflicts).
@@ -134,10 +149,20 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
     }
     __syncthreads();
     if (tx >= nb) return;
+    const int64_t i0 = first + tx;
     const float* box = s_box + tx * (NP + 1);
     float d[NP];
 #pragma unroll
     for (int p = 0; p < NP; p++) d[p] = box[p];
+    // sCMOS: the box's variance/gain^2 ratios, pixel (ii, jj) at g[SZ*jj + ii]
+    float g[SC ? NP : 1];
+    if constexpr (SC) {
+        const float cx = sc.coords[2 * i0], cy = sc.coords[2 * i0 + 1];
+#pragma unroll
+        for (int jj = 0; jj < SZ; jj++)
+#pragma unroll
+            for (int ii = 0; ii < SZ; ii++) g[SZ * jj + ii] = gain_at(sc, cx, cy, ii, jj);
+    }
 
     // --- initial values: centre of mass (GPUgaussLib.cuh:207-227) ---
     // SG: the axis widths vary (fitted, or functions of z); XY: one fitted width
@@ -197,7 +222,8 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
         theta[3] = fmin_;
         if constexpr (ZF) {  // cuda_gaussMLEv2.cu:331-332
             theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * kPi * sigma * zc.sy0 * sqrtf(2.0f));
-            theta[4] = 0;
+            if constexpr (SC) theta[4] = sc.z0[i0];  // z_initial (cuda_gaussMLEv2.cu:896)
+            else theta[4] = 0;
         } else {
             theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
             if constexpr (SG) theta[4] = sigma;
@@ -268,9 +294,16 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
                 const float u2 = px * py;
                 float cf = 0.f, df = 0.f;
                 if (model > 10e-3f) {
-                    const float r = __builtin_amdgcn_rcpf(model);  // 1 ulp, for two divisions
-                    cf = dat * r - 1;
-                    df = dat * r * r;
+                    if constexpr (SC) {  // cuda_gaussMLEv2.cu:631-632
+                        const float gv = g[SZ * jj + ii];
+                        const float r = __builtin_amdgcn_rcpf(model + gv);
+                        cf = (dat - model) * r;
+                        df = (dat + gv) * r * r;
+                    } else {
+                        const float r = __builtin_amdgcn_rcpf(model);  // 1 ulp, for two divisions
+                        cf = dat * r - 1;
+                        df = dat * r * r;
+                    }
                 }
                 cf = fminf(cf, 10e4f);
                 df = fminf(df, 10e4f);
@@ -321,6 +354,9 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
 #pragma unroll
     for (int k = 0; k < NV * NV; k++) M[k] = 0.f;
     float div = 0.f;
+    // sCMOS: the reference forms the pixel's ratio only in the Newton loop, so its
+    // CRLB and log-likelihood loop uses the last pixel's (ii = jj = sz-1) for all
+    [[maybe_unused]] const float gl = SC ? g[SC ? NP - 1 : 0] : 0.f;
     {
         AxisTerms ax[SZ], ay[SZ];
 #pragma unroll
@@ -348,12 +384,17 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
                 } else if constexpr (SG) {
                     du[4] = ax[ii].s1 * py + ay[jj].s1 * px;
                 }
-                const float rm = __builtin_amdgcn_rcpf(model);
+                const float rm = __builtin_amdgcn_rcpf(SC ? model + gl : model);
 #pragma unroll
                 for (int k = 0; k < NV; k++)
 #pragma unroll
                     for (int l = k; l < NV; l++) M[k * NV + l] += du[l] * du[k] * rm;
-                if (model > 0) {
+                if constexpr (SC) {  // cuda_gaussMLEv2.cu:676-679
+                    if ((model + gl) > 0) {
+                        if ((dat + gl) > 0) div += (dat + gl) * logf(model + gl) - model - (dat + gl) * logf(dat + gl) + dat;
+                        else div += -model - gl;
+                    }
+                } else if (model > 0) {
                     if (dat > 0) div += dat * logf(model) - model - dat * logf(dat) + dat;
                     else div += -model;
                 }
@@ -375,17 +416,19 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb(const float* __restrict__ d
 }
 
 // Any box size: same algorithm reading the box from global memory.
-template <int FT>
+template <int FT, bool SC>
 __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict__ data, float sigma,
                                                          int sz, int iterations, float* __restrict__ params,
                                                          float* __restrict__ crlb, float* __restrict__ ll,
-                                                         int nfits, ZCal zc) {
+                                                         int nfits, ZCal zc, ScIn sc) {
     constexpr int NV = nv_of(FT);
     // SG: the axis widths vary (fitted, or functions of z); XY: one fitted width
     // per axis; ZF: widths from the fitted z through the astigmatism calibration
     constexpr bool SG = FT != kFtBasic, XY = FT == kFtSigmaYX, ZF = FT == kFtZ;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= nfits) return;
+    const int64_t i0 = i;
+    const float cx = SC ? sc.coords[2 * i] : 0.f, cy = SC ? sc.coords[2 * i + 1] : 0.f;
     const float* d = data + i * sz * sz;
     float theta[NV];
     {
@@ -421,7 +464,8 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
         theta[3] = fmin_;
         if constexpr (ZF) {  // cuda_gaussMLEv2.cu:331-332
             theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * kPi * sigma * zc.sy0 * sqrtf(2.0f));
-            theta[4] = 0;
+            if constexpr (SC) theta[4] = sc.z0[i0];  // z_initial (cuda_gaussMLEv2.cu:896)
+            else theta[4] = 0;
         } else {
             theta[2] = fmaxf(0.0f, (fmax_ - theta[3]) * 2 * 2 * kPi * sigma * sigma);
             if constexpr (SG) theta[4] = sigma;
@@ -482,9 +526,16 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
                 const float u0 = ax.d1 * py, v0 = ax.d2 * py, u1 = ay.d1 * px, v1 = ay.d2 * px, u2 = px * py;
                 float cf = 0.f, df = 0.f;
                 if (model > 10e-3f) {
-                    const float r = __builtin_amdgcn_rcpf(model);  // 1 ulp, for two divisions
-                    cf = dat * r - 1;
-                    df = dat * r * r;
+                    if constexpr (SC) {  // cuda_gaussMLEv2.cu:631-632
+                        const float gv = gain_at(sc, cx, cy, ii, jj);
+                        const float r = __builtin_amdgcn_rcpf(model + gv);
+                        cf = (dat - model) * r;
+                        df = (dat + gv) * r * r;
+                    } else {
+                        const float r = __builtin_amdgcn_rcpf(model);  // 1 ulp, for two divisions
+                        cf = dat * r - 1;
+                        df = dat * r * r;
+                    }
                 }
                 cf = fminf(cf, 10e4f);
                 df = fminf(df, 10e4f);
@@ -529,6 +580,9 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
     float M[NV * NV];
     for (int k = 0; k < NV * NV; k++) M[k] = 0.f;
     float div = 0.f;
+    // sCMOS: the reference forms the pixel's ratio only in the Newton loop, so its
+    // CRLB and log-likelihood loop uses the last pixel's (ii = jj = sz-1) for all
+    [[maybe_unused]] const float gl = SC ? gain_at(sc, cx, cy, sz - 1, sz - 1) : 0.f;
     for (int ii = 0; ii < sz; ii++) {
         AxisTerms ax = axis_terms<SG>(ii, theta[0], sgx, snx, theta[2], c1, c2x);
         for (int jj = 0; jj < sz; jj++) {
@@ -550,8 +604,13 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
                 du[4] = ax.s1 * py + ay.s1 * px;
             }
             for (int k = 0; k < NV; k++)
-                for (int l = k; l < NV; l++) M[k * NV + l] += du[l] * du[k] / model;
-            if (model > 0) {
+                for (int l = k; l < NV; l++) M[k * NV + l] += du[l] * du[k] / (SC ? model + gl : model);
+            if constexpr (SC) {  // cuda_gaussMLEv2.cu:676-679
+                if ((model + gl) > 0) {
+                    if ((dat + gl) > 0) div += (dat + gl) * logf(model + gl) - model - (dat + gl) * logf(dat + gl) + dat;
+                    else div += -model - gl;
+                }
+            } else if (model > 0) {
                 if (dat > 0) div += dat * logf(model) - model - dat * logf(dat) + dat;
                 else div += -model;
             }
@@ -568,25 +627,29 @@ __global__ __launch_bounds__(kBlock) void k_mle_xynb_any(const float* __restrict
     ll[i] = div;
 }
 
-template <int FT>
+template <int FT, bool SC = false>
 int launch_mle(const float* d_data, float psf_sigma, int sz, int iterations, float* d_params, float* d_crlb,
-               float* d_ll, int nfits, void* stream, ZCal zc = {}) {
+               float* d_ll, int nfits, void* stream, ZCal zc = {}, ScIn sc = {}) {
     BAMF_CHECK_ARG(nfits >= 0, "nfits must be >= 0 (got %d)", nfits);
     BAMF_CHECK_ARG(sz >= 1 && sz <= 64, "sz must be in [1, 64] (got %d)", sz);
     BAMF_CHECK_ARG(iterations >= 0, "iterations must be >= 0");
     BAMF_CHECK_ARG(psf_sigma > 0.f, "psf_sigma must be > 0");
     if (nfits == 0) return BAMF_OK;
     BAMF_CHECK_ARG(d_data && d_params && d_crlb && d_ll, "null device pointer");
+    if (SC) {
+        BAMF_CHECK_ARG(sc.coords && sc.gain && sc.mapsz >= sz, "sCMOS fits need box corners and a gain map of side >= sz");
+        BAMF_CHECK_ARG(FT != kFtZ || sc.z0, "the sCMOS 'Z' fit needs its starting z");
+    }
     dim3 grid((unsigned)ceil_div(nfits, kBlock)), block(kBlock);
     hipStream_t s = as_stream(stream);
     switch (sz) {
-    case 5: hipLaunchKernelGGL((k_mle_xynb<5, FT>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc); break;
-    case 6: hipLaunchKernelGGL((k_mle_xynb<6, FT>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc); break;
-    case 7: hipLaunchKernelGGL((k_mle_xynb<7, FT>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc); break;
-    case 8: hipLaunchKernelGGL((k_mle_xynb<8, FT>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc); break;
-    case 9: hipLaunchKernelGGL((k_mle_xynb<9, FT>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc); break;
+    case 5: hipLaunchKernelGGL((k_mle_xynb<5, FT, SC>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc, sc); break;
+    case 6: hipLaunchKernelGGL((k_mle_xynb<6, FT, SC>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc, sc); break;
+    case 7: hipLaunchKernelGGL((k_mle_xynb<7, FT, SC>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc, sc); break;
+    case 8: hipLaunchKernelGGL((k_mle_xynb<8, FT, SC>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc, sc); break;
+    case 9: hipLaunchKernelGGL((k_mle_xynb<9, FT, SC>), grid, block, 0, s, d_data, psf_sigma, iterations, d_params, d_crlb, d_ll, nfits, zc, sc); break;
     default:
-        hipLaunchKernelGGL(k_mle_xynb_any<FT>, grid, block, 0, s, d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, zc);
+        hipLaunchKernelGGL((k_mle_xynb_any<FT, SC>), grid, block, 0, s, d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits, zc, sc);
     }
     BAMF_LAUNCH_CHECK("k_mle_xynb");
     return BAMF_OK;
@@ -617,4 +680,37 @@ extern "C" int bamf_gauss_mle_xynbz(const float* d_data, float psf_sigma_x, floa
     BAMF_CHECK_ARG(psf_sigma_y > 0.f && d != 0.f, "psf_sigma_y must be > 0 and d nonzero");
     return bamf::launch_mle<bamf::kFtZ>(d_data, psf_sigma_x, sz, iterations, d_params, d_crlb, d_ll, nfits, stream,
                                         bamf::ZCal{psf_sigma_y, ax, ay, bx, by, gamma, d});
+}
+
+// The sCMOS kernels (cuda_gaussMLEv2.cu:559-1124), same argument lists.
+extern "C" int bamf_gauss_mle_scmos_xynb(const float* d_data, const float* d_coords, const float* d_gain_ratio,
+                                         float psf_sigma, int sz, int mapsz, int iterations, float* d_params,
+                                         float* d_crlb, float* d_ll, int nfits, void* stream) {
+    return bamf::launch_mle<bamf::kFtBasic, true>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits,
+                                                  stream, {}, bamf::ScIn{d_coords, d_gain_ratio, mapsz, nullptr});
+}
+
+extern "C" int bamf_gauss_mle_scmos_xynbs(const float* d_data, const float* d_coords, const float* d_gain_ratio,
+                                          float psf_sigma, int sz, int mapsz, int iterations, float* d_params,
+                                          float* d_crlb, float* d_ll, int nfits, void* stream) {
+    return bamf::launch_mle<bamf::kFtSigma, true>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll, nfits,
+                                                  stream, {}, bamf::ScIn{d_coords, d_gain_ratio, mapsz, nullptr});
+}
+
+extern "C" int bamf_gauss_mle_scmos_xynbsxsy(const float* d_data, const float* d_coords, const float* d_gain_ratio,
+                                             float psf_sigma, int sz, int mapsz, int iterations, float* d_params,
+                                             float* d_crlb, float* d_ll, int nfits, void* stream) {
+    return bamf::launch_mle<bamf::kFtSigmaYX, true>(d_data, psf_sigma, sz, iterations, d_params, d_crlb, d_ll,
+                                                    nfits, stream, {},
+                                                    bamf::ScIn{d_coords, d_gain_ratio, mapsz, nullptr});
+}
+
+extern "C" int bamf_gauss_mle_scmos_xynbz(const float* d_data, const float* d_coords, const float* d_gain_ratio,
+                                          const float* d_x0, float psf_sigma_x, float ax, float ay, float bx, float by,
+                                          float gamma, float d, float psf_sigma_y, int sz, int mapsz, int iterations,
+                                          float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream) {
+    BAMF_CHECK_ARG(psf_sigma_y > 0.f && d != 0.f, "psf_sigma_y must be > 0 and d nonzero");
+    return bamf::launch_mle<bamf::kFtZ, true>(d_data, psf_sigma_x, sz, iterations, d_params, d_crlb, d_ll, nfits,
+                                              stream, bamf::ZCal{psf_sigma_y, ax, ay, bx, by, gamma, d},
+                                              bamf::ScIn{d_coords, d_gain_ratio, mapsz, d_x0});
 }

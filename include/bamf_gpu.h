@@ -481,6 +481,37 @@ int bamf_gauss_mle_host(const float* data, int sz, int64_t nfits, float sigma, i
 int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigma, int iterations, int fit_type,
                             const float* zcal, float* params, float* crlb, float* ll, double* fit_seconds);
 
+/* The sCMOS kernels kernel_MLEFit_SCMOSXYNB_ / _SCMOSXYNBS_ / _SCMOSXYNBSXSY_ /
+ * _SCMOSXYNBZ_ (cuda_gaussMLEv2.cu:559-561, 697-699, 980-982, 839-842), called by
+ * gaussMLE.m:162-173 for CameraType 'SCMOS', with their argument lists: d_coords
+ * holds the box corner of fit b at [2b], [2b+1] (read as the reference reads
+ * d_Coords), d_gain_ratio the mapsz x mapsz map of variance/gain^2 (the
+ * VarianceImage argument), pixel (ii, jj) of fit b reads
+ * d_gain_ratio[(int)((d_coords[2b+1] + jj) * mapsz + d_coords[2b] + ii)]; d_x0
+ * one starting z per fit.  The model adds the ratio g to model and data: cf =
+ * (data - model)/(model + g), df = (data + g)/(model + g)^2; like the reference,
+ * the CRLB and log-likelihood use the ratio of the box's last pixel (ii = jj =
+ * sz-1) for every pixel (the reference forms the ratio only in its Newton loop). */
+int bamf_gauss_mle_scmos_xynb(const float* d_data, const float* d_coords, const float* d_gain_ratio,
+                              float psf_sigma, int sz, int mapsz, int iterations, float* d_params,
+                              float* d_crlb, float* d_ll, int nfits, void* stream);
+int bamf_gauss_mle_scmos_xynbs(const float* d_data, const float* d_coords, const float* d_gain_ratio,
+                               float psf_sigma, int sz, int mapsz, int iterations, float* d_params,
+                               float* d_crlb, float* d_ll, int nfits, void* stream);
+int bamf_gauss_mle_scmos_xynbsxsy(const float* d_data, const float* d_coords, const float* d_gain_ratio,
+                                  float psf_sigma, int sz, int mapsz, int iterations, float* d_params,
+                                  float* d_crlb, float* d_ll, int nfits, void* stream);
+int bamf_gauss_mle_scmos_xynbz(const float* d_data, const float* d_coords, const float* d_gain_ratio,
+                               const float* d_x0, float psf_sigma_x, float ax, float ay, float bx, float by,
+                               float gamma, float d, float psf_sigma_y, int sz, int mapsz, int iterations,
+                               float* d_params, float* d_crlb, float* d_ll, int nfits, void* stream);
+/* bamf_gauss_mle_fit_host for CameraType 'SCMOS': coords (2 per fit), the gain
+ * map (mapsz^2) and, for 'Z', z0 (one per fit; gaussMLE.m:167 passes zeros). */
+int bamf_gauss_mle_scmos_fit_host(const float* data, int sz, int64_t nfits, const float* coords,
+                                  const float* gain_ratio, int mapsz, float sigma, int iterations, int fit_type,
+                                  const float* zcal, const float* z0, float* params, float* crlb, float* ll,
+                                  double* fit_seconds);
+
 /* findROI(Image, Sigma1, Sigma2, Boxsize, Minval) on host arrays
  * (@RJ/findROI.m:1-163):
  *   image       single (ysize, xsize, nframes), column-major (MATLAB memory);

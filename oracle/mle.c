@@ -117,12 +117,16 @@ void orc_mat_inv_n(float* m, float* minv, float* diag, int sz) {
 }
 
 /* cuda_gaussMLEv2.cu:16-141, one fit per loop iteration instead of per thread. */
-static void mle_one(const float* s_data, float sigma, int sz, int iterations,
+static void mle_one(const float* s_data, const float* gr, float sigma, int sz, int iterations,
                     float* theta, float* diagout, float* llout) {
     float m[NV * NV], diag[NV], minv[NV * NV];
     float dudt[NV], d2udt2[NV], num[NV], den[NV];
     const float maxjump[NV] = {1e0f, 1e0f, 1e2f, 2e0f};
     float nmax, model, cf, df, data, psfx, psfy, div;
+    /* sCMOS: the pixel's variance/gain^2 (gr != NULL).  Like the reference, the
+     * Fisher and log-likelihood loop below reuses the value of the last pixel of
+     * the last Newton iteration (cuda_gaussMLEv2.cu:620-621 set it only there). */
+    float gainr = 0.0f;
     (void)maxjump;
     memset(m, 0, sizeof(m));
     memset(minv, 0, sizeof(minv));
@@ -147,8 +151,15 @@ static void mle_one(const float* s_data, float sigma, int sz, int iterations,
                 d2udt2[3] = 0.0f;
                 cf = 0.0f;
                 df = 0.0f;
-                if (model > 10e-3f) cf = data / model - 1;
-                if (model > 10e-3f) df = data / (model * model);
+                if (gr) {
+                    /* cuda_gaussMLEv2.cu:620-632 */
+                    gainr = gr[sz * jj + ii];
+                    if (model > 10e-3f) cf = (data - model) / (model + gainr);
+                    if (model > 10e-3f) df = (data + gainr) / ((model + gainr) * (model + gainr));
+                } else {
+                    if (model > 10e-3f) cf = data / model - 1;
+                    if (model > 10e-3f) df = data / (model * model);
+                }
                 cf = fminf(cf, 10e4f);
                 df = fminf(df, 10e4f);
                 for (int ll = 0; ll < NV; ll++) {
@@ -179,10 +190,17 @@ static void mle_one(const float* s_data, float sigma, int sz, int iterations,
             dudt[3] = 1.0f;
             for (int k = 0; k < NV; k++)
                 for (int l = k; l < NV; l++) {
-                    m[k * NV + l] += dudt[l] * dudt[k] / model;
+                    m[k * NV + l] += dudt[l] * dudt[k] / (gr ? model + gainr : model);
                     m[l * NV + k] = m[k * NV + l];
                 }
-            if (model > 0) {
+            if (gr) { /* cuda_gaussMLEv2.cu:676-679 */
+                if ((model + gainr) > 0) {
+                    if ((data + gainr) > 0)
+                        div += (data + gainr) * logf(model + gainr) - model - (data + gainr) * logf(data + gainr) + data;
+                    else
+                        div += -model - gainr;
+                }
+            } else if (model > 0) {
                 if (data > 0)
                     div += data * logf(model) - model - data * logf(data) + data;
                 else
@@ -198,7 +216,7 @@ void orc_mle_xynb(const float* data, float psf_sigma, int sz, int iterations,
                   float* params, float* crlb, float* ll, int nfits) {
     for (int i = 0; i < nfits; i++) {
         float theta[NV], diag[NV], l;
-        mle_one(data + (size_t)sz * sz * i, psf_sigma, sz, iterations, theta, diag, &l);
+        mle_one(data + (size_t)sz * sz * i, NULL, psf_sigma, sz, iterations, theta, diag, &l);
         for (int k = 0; k < NV; k++) {
             params[(size_t)nfits * k + i] = theta[k];
             crlb[(size_t)nfits * k + i] = diag[k];
@@ -233,11 +251,15 @@ static void deriv_int_gauss_2d_sigma(int ii, int jj, float x, float y, float s, 
 
 #define NVS 5
 /* cuda_gaussMLEv2.cu:144-276, one fit per loop iteration instead of per thread. */
-static void mle_one_s(const float* s_data, float sigma, int sz, int iterations, float* theta, float* diagout,
-                      float* llout) {
+static void mle_one_s(const float* s_data, const float* gr, float sigma, int sz, int iterations, float* theta,
+                      float* diagout, float* llout) {
     float m[NVS * NVS], diag[NVS], minv[NVS * NVS];
     float dudt[NVS], d2udt2[NVS], num[NVS], den[NVS];
     float nmax, model, cf, df, data, psfx, psfy, div;
+    /* sCMOS: the pixel's variance/gain^2 (gr != NULL).  Like the reference, the
+     * Fisher and log-likelihood loop below reuses the value of the last pixel of
+     * the last Newton iteration (cuda_gaussMLEv2.cu:620-621 set it only there). */
+    float gainr = 0.0f;
     memset(m, 0, sizeof(m));
     memset(minv, 0, sizeof(minv));
     orc_center_of_mass_2d(sz, s_data, &theta[0], &theta[1]);
@@ -264,8 +286,15 @@ static void mle_one_s(const float* s_data, float sigma, int sz, int iterations, 
                 d2udt2[3] = 0.0f;
                 cf = 0.0f;
                 df = 0.0f;
-                if (model > 10e-3f) cf = data / model - 1;
-                if (model > 10e-3f) df = data / (model * model);
+                if (gr) {
+                    /* cuda_gaussMLEv2.cu:620-632 */
+                    gainr = gr[sz * jj + ii];
+                    if (model > 10e-3f) cf = (data - model) / (model + gainr);
+                    if (model > 10e-3f) df = (data + gainr) / ((model + gainr) * (model + gainr));
+                } else {
+                    if (model > 10e-3f) cf = data / model - 1;
+                    if (model > 10e-3f) df = data / (model * model);
+                }
                 cf = fminf(cf, 10e4f);
                 df = fminf(df, 10e4f);
                 for (int ll = 0; ll < NVS; ll++) {
@@ -300,10 +329,17 @@ static void mle_one_s(const float* s_data, float sigma, int sz, int iterations, 
             dudt[3] = 1.0f;
             for (int k = 0; k < NVS; k++)
                 for (int l = k; l < NVS; l++) {
-                    m[k * NVS + l] += dudt[l] * dudt[k] / model;
+                    m[k * NVS + l] += dudt[l] * dudt[k] / (gr ? model + gainr : model);
                     m[l * NVS + k] = m[k * NVS + l];
                 }
-            if (model > 0) {
+            if (gr) { /* cuda_gaussMLEv2.cu:676-679 */
+                if ((model + gainr) > 0) {
+                    if ((data + gainr) > 0)
+                        div += (data + gainr) * logf(model + gainr) - model - (data + gainr) * logf(data + gainr) + data;
+                    else
+                        div += -model - gainr;
+                }
+            } else if (model > 0) {
                 if (data > 0)
                     div += data * logf(model) - model - data * logf(data) + data;
                 else
@@ -319,7 +355,7 @@ void orc_mle_xynbs(const float* data, float psf_sigma, int sz, int iterations,
                    float* params, float* crlb, float* ll, int nfits) {
     for (int i = 0; i < nfits; i++) {
         float theta[NVS], diag[NVS], l;
-        mle_one_s(data + (size_t)sz * sz * i, psf_sigma, sz, iterations, theta, diag, &l);
+        mle_one_s(data + (size_t)sz * sz * i, NULL, psf_sigma, sz, iterations, theta, diag, &l);
         for (int k = 0; k < NVS; k++) {
             params[(size_t)nfits * k + i] = theta[k];
             crlb[(size_t)nfits * k + i] = diag[k];
@@ -330,11 +366,15 @@ void orc_mle_xynbs(const float* data, float psf_sigma, int sz, int iterations,
 
 #define NVXY 6
 /* cuda_gaussMLEv2.cu:414-556: theta = {x, y, N, bg, sigma along ii, sigma along jj}. */
-static void mle_one_sxsy(const float* s_data, float sigma, int sz, int iterations, float* theta, float* diagout,
-                         float* llout) {
+static void mle_one_sxsy(const float* s_data, const float* gr, float sigma, int sz, int iterations, float* theta,
+                         float* diagout, float* llout) {
     float m[NVXY * NVXY], diag[NVXY], minv[NVXY * NVXY];
     float dudt[NVXY], d2udt2[NVXY], num[NVXY], den[NVXY];
     float nmax, model, cf, df, data, psfx, psfy, div;
+    /* sCMOS: the pixel's variance/gain^2 (gr != NULL).  Like the reference, the
+     * Fisher and log-likelihood loop below reuses the value of the last pixel of
+     * the last Newton iteration (cuda_gaussMLEv2.cu:620-621 set it only there). */
+    float gainr = 0.0f;
     memset(m, 0, sizeof(m));
     memset(minv, 0, sizeof(minv));
     orc_center_of_mass_2d(sz, s_data, &theta[0], &theta[1]);
@@ -361,8 +401,15 @@ static void mle_one_sxsy(const float* s_data, float sigma, int sz, int iteration
                 d2udt2[3] = 0.0f;
                 cf = 0.0f;
                 df = 0.0f;
-                if (model > 10e-3f) cf = data / model - 1;
-                if (model > 10e-3f) df = data / (model * model);
+                if (gr) {
+                    /* cuda_gaussMLEv2.cu:620-632 */
+                    gainr = gr[sz * jj + ii];
+                    if (model > 10e-3f) cf = (data - model) / (model + gainr);
+                    if (model > 10e-3f) df = (data + gainr) / ((model + gainr) * (model + gainr));
+                } else {
+                    if (model > 10e-3f) cf = data / model - 1;
+                    if (model > 10e-3f) df = data / (model * model);
+                }
                 cf = fminf(cf, 10e4f);
                 df = fminf(df, 10e4f);
                 for (int ll = 0; ll < NVXY; ll++) {
@@ -400,10 +447,17 @@ static void mle_one_sxsy(const float* s_data, float sigma, int sz, int iteration
             dudt[3] = 1.0f;
             for (int k = 0; k < NVXY; k++)
                 for (int l = k; l < NVXY; l++) {
-                    m[k * NVXY + l] += dudt[l] * dudt[k] / model;
+                    m[k * NVXY + l] += dudt[l] * dudt[k] / (gr ? model + gainr : model);
                     m[l * NVXY + k] = m[k * NVXY + l];
                 }
-            if (model > 0) {
+            if (gr) { /* cuda_gaussMLEv2.cu:676-679 */
+                if ((model + gainr) > 0) {
+                    if ((data + gainr) > 0)
+                        div += (data + gainr) * logf(model + gainr) - model - (data + gainr) * logf(data + gainr) + data;
+                    else
+                        div += -model - gainr;
+                }
+            } else if (model > 0) {
                 if (data > 0)
                     div += data * logf(model) - model - data * logf(data) + data;
                 else
@@ -419,7 +473,7 @@ void orc_mle_xynbsxsy(const float* data, float psf_sigma, int sz, int iterations
                       float* params, float* crlb, float* ll, int nfits) {
     for (int i = 0; i < nfits; i++) {
         float theta[NVXY], diag[NVXY], l;
-        mle_one_sxsy(data + (size_t)sz * sz * i, psf_sigma, sz, iterations, theta, diag, &l);
+        mle_one_sxsy(data + (size_t)sz * sz * i, NULL, psf_sigma, sz, iterations, theta, diag, &l);
         for (int k = 0; k < NVXY; k++) {
             params[(size_t)nfits * k + i] = theta[k];
             crlb[(size_t)nfits * k + i] = diag[k];
@@ -478,17 +532,21 @@ static void deriv_int_gauss_2d_z(int ii, int jj, const float* theta, const ZCal*
 }
 
 /* cuda_gaussMLEv2.cu:280-411: theta = {x, y, N, bg, z}. */
-static void mle_one_z(const float* s_data, const ZCal* c, int sz, int iterations, float* theta, float* diagout,
-                      float* llout) {
+static void mle_one_z(const float* s_data, const float* gr, float z0, const ZCal* c, int sz, int iterations,
+                      float* theta, float* diagout, float* llout) {
     float m[NVS * NVS], diag[NVS], minv[NVS * NVS];
     float dudt[NVS], d2udt2[NVS], num[NVS], den[NVS];
     float nmax, model, cf, df, data, psfx, psfy, div;
+    /* sCMOS: the pixel's variance/gain^2 (gr != NULL).  Like the reference, the
+     * Fisher and log-likelihood loop below reuses the value of the last pixel of
+     * the last Newton iteration (cuda_gaussMLEv2.cu:620-621 set it only there). */
+    float gainr = 0.0f;
     memset(m, 0, sizeof(m));
     memset(minv, 0, sizeof(minv));
     orc_center_of_mass_2d(sz, s_data, &theta[0], &theta[1]);
     orc_gauss_fmaxmin_2d(sz, c->sx0, s_data, &nmax, &theta[3]);
     theta[2] = fmaxf(0.0f, (nmax - theta[3]) * 2 * kPi * c->sx0 * c->sy0 * sqrtf(2.0f));
-    theta[4] = 0;
+    theta[4] = z0; /* 0 (cuda_gaussMLEv2.cu:332), or z_initial of the sCMOS kernel (:896) */
     for (int kk = 0; kk < iterations; kk++) {
         memset(num, 0, sizeof(num));
         memset(den, 0, sizeof(den));
@@ -503,8 +561,15 @@ static void mle_one_z(const float* s_data, const ZCal* c, int sz, int iterations
                 d2udt2[3] = 0.0f;
                 cf = 0.0f;
                 df = 0.0f;
-                if (model > 10e-3f) cf = data / model - 1;
-                if (model > 10e-3f) df = data / (model * model);
+                if (gr) {
+                    /* cuda_gaussMLEv2.cu:620-632 */
+                    gainr = gr[sz * jj + ii];
+                    if (model > 10e-3f) cf = (data - model) / (model + gainr);
+                    if (model > 10e-3f) df = (data + gainr) / ((model + gainr) * (model + gainr));
+                } else {
+                    if (model > 10e-3f) cf = data / model - 1;
+                    if (model > 10e-3f) df = data / (model * model);
+                }
                 cf = fminf(cf, 10e4f);
                 df = fminf(df, 10e4f);
                 for (int ll = 0; ll < NVS; ll++) {
@@ -531,10 +596,17 @@ static void mle_one_z(const float* s_data, const ZCal* c, int sz, int iterations
             dudt[3] = 1.0f;
             for (int k = 0; k < NVS; k++)
                 for (int l = k; l < NVS; l++) {
-                    m[k * NVS + l] += dudt[l] * dudt[k] / model;
+                    m[k * NVS + l] += dudt[l] * dudt[k] / (gr ? model + gainr : model);
                     m[l * NVS + k] = m[k * NVS + l];
                 }
-            if (model > 0) {
+            if (gr) { /* cuda_gaussMLEv2.cu:676-679 */
+                if ((model + gainr) > 0) {
+                    if ((data + gainr) > 0)
+                        div += (data + gainr) * logf(model + gainr) - model - (data + gainr) * logf(data + gainr) + data;
+                    else
+                        div += -model - gainr;
+                }
+            } else if (model > 0) {
                 if (data > 0)
                     div += data * logf(model) - model - data * logf(data) + data;
                 else
@@ -551,8 +623,45 @@ void orc_mle_xynbz(const float* data, float sx0, float ax, float ay, float bx, f
     const ZCal c = {sx0, ax, ay, bx, by, gamma, d, sy0};
     for (int i = 0; i < nfits; i++) {
         float theta[NVS], diag[NVS], l;
-        mle_one_z(data + (size_t)sz * sz * i, &c, sz, iterations, theta, diag, &l);
+        mle_one_z(data + (size_t)sz * sz * i, NULL, 0.0f, &c, sz, iterations, theta, diag, &l);
         for (int k = 0; k < NVS; k++) {
+            params[(size_t)nfits * k + i] = theta[k];
+            crlb[(size_t)nfits * k + i] = diag[k];
+        }
+        ll[i] = l;
+    }
+}
+
+/* The sCMOS kernels (cuda_gaussMLEv2.cu:559-1124): fit type 0 _SCMOSXYNB_, 1
+ * _SCMOSXYNBS_, 2 _SCMOSXYNBSXSY_, 3 _SCMOSXYNBZ_ (zcal = {sx0, ax, ay, bx, by,
+ * gamma, d, sy0}, z0 one starting z per fit).  Pixel (ii, jj) of fit i reads the
+ * gain-ratio map at (coords[2i+1] + jj) * mapsz + coords[2i] + ii, formed in
+ * float as the reference forms GRind. */
+void orc_mle_scmos(int fit_type, const float* data, const float* coords, const float* gain, int mapsz,
+                   const float* z0, float psf_sigma, const float* zcal, int sz, int iterations, float* params,
+                   float* crlb, float* ll, int nfits) {
+    const int np = fit_type == 2 ? 6 : fit_type == 0 ? 4 : 5;
+    float gr[64 * 64];
+    ZCal c = {psf_sigma, 0, 0, 0, 0, 0, 1, psf_sigma};
+    if (fit_type == 3) {
+        c.sx0 = zcal[0]; c.ax = zcal[1]; c.ay = zcal[2]; c.bx = zcal[3]; c.by = zcal[4];
+        c.gamma = zcal[5]; c.d = zcal[6]; c.sy0 = zcal[7];
+    }
+    for (int i = 0; i < nfits; i++) {
+        float theta[6], diag[6], l;
+        for (int ii = 0; ii < sz; ii++)
+            for (int jj = 0; jj < sz; jj++) {
+                const int idx = (int)((coords[2 * i + 1] + jj) * mapsz + coords[2 * i] + ii);
+                gr[sz * jj + ii] = gain[idx];
+            }
+        const float* d = data + (size_t)sz * sz * i;
+        switch (fit_type) {
+        case 0: mle_one(d, gr, psf_sigma, sz, iterations, theta, diag, &l); break;
+        case 1: mle_one_s(d, gr, psf_sigma, sz, iterations, theta, diag, &l); break;
+        case 2: mle_one_sxsy(d, gr, psf_sigma, sz, iterations, theta, diag, &l); break;
+        default: mle_one_z(d, gr, z0[i], &c, sz, iterations, theta, diag, &l); break;
+        }
+        for (int k = 0; k < np; k++) {
             params[(size_t)nfits * k + i] = theta[k];
             crlb[(size_t)nfits * k + i] = diag[k];
         }

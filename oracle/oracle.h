@@ -43,6 +43,10 @@ void orc_mle_xynbsxsy(const float* data, float psf_sigma, int sz, int iterations
  * sx0 sqrt(alpha(z - gamma; ax, bx, d)) along ii and sy0 sqrt(alpha(z + gamma; ay, by, d)) along jj. */
 void orc_mle_xynbz(const float* data, float sx0, float ax, float ay, float bx, float by, float gamma, float d,
                    float sy0, int sz, int iterations, float* params, float* crlb, float* ll, int nfits);
+/* The sCMOS kernels (cuda_gaussMLEv2.cu:559-1124): fit_type 0 XYNB, 1 XYNBS, 2 XYNBSXSY, 3 XYNBZ. */
+void orc_mle_scmos(int fit_type, const float* data, const float* coords, const float* gain, int mapsz,
+                   const float* z0, float psf_sigma, const float* zcal, int sz, int iterations, float* params,
+                   float* crlb, float* ll, int nfits);
 /* Helpers exposed for unit tests. */
 float orc_int_gauss_1d(int ii, float x, float sigma);
 void  orc_center_of_mass_2d(int sz, const float* data, float* x, float* y);

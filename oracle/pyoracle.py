@@ -77,6 +77,8 @@ def _declare(L):
     L.orc_mle_xynbs.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
     L.orc_mle_xynbsxsy.argtypes = [c_void_p, c_float, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
     L.orc_mle_xynbz.argtypes = [c_void_p] + [c_float] * 8 + [c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]
+    L.orc_mle_scmos.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_float, c_void_p, c_int, c_int,
+                                c_void_p, c_void_p, c_void_p, c_int]
     L.orc_int_gauss_1d.argtypes = [c_int, c_float, c_float]
     L.orc_int_gauss_1d.restype = c_float
     L.orc_center_of_mass_2d.argtypes = [c_int, c_void_p, POINTER(c_float), POINTER(c_float)]
@@ -166,6 +168,27 @@ def mle_xynbz(boxes: np.ndarray, cal, iterations: int = 20):
     lib().orc_mle_xynbz(boxes.ctypes.data, *[float(v) for v in cal], sz, iterations, p.ctypes.data, c.ctypes.data,
                         ll.ctypes.data, n)
     return p.reshape(5, n), c.reshape(5, n), ll
+
+
+def mle_scmos(fit_type: int, boxes: np.ndarray, coords: np.ndarray, gain: np.ndarray, sigma: float,
+              iterations: int = 20, zcal=None, z0=None):
+    """The sCMOS kernels restated (fit_type 0 XYNB, 1 XYNBS, 2 XYNBSXSY, 3 XYNBZ).
+    coords (N, 2) box corners (x along ii, y along jj) in the square gain map
+    gain (mapsz, mapsz) read as gain[(y + jj) * mapsz + x + ii]."""
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32)
+    n, sz = boxes.shape[0], boxes.shape[1]
+    npar = 6 if fit_type == 2 else 4 if fit_type == 0 else 5
+    coords = np.ascontiguousarray(coords, dtype=np.float32)
+    gain = np.ascontiguousarray(gain, dtype=np.float32)
+    zc = np.ascontiguousarray(zcal if zcal is not None else np.zeros(8), np.float32)
+    zz = np.ascontiguousarray(z0 if z0 is not None else np.zeros(n), np.float32)
+    p = np.zeros(npar * n, np.float32)
+    c = np.zeros(npar * n, np.float32)
+    ll = np.zeros(n, np.float32)
+    lib().orc_mle_scmos(int(fit_type), boxes.ctypes.data, coords.ctypes.data, gain.ctypes.data, gain.shape[0],
+                        zz.ctypes.data, float(sigma), zc.ctypes.data, sz, iterations, p.ctypes.data, c.ctypes.data,
+                        ll.ctypes.data, n)
+    return p.reshape(npar, n), c.reshape(npar, n), ll
 
 
 def gauss_coeffs(sigma: float):

@@ -141,6 +141,24 @@ def test_gauss_mle_fit_gateway_sigma_matches_device_entry():
                                          ptr(hl), None) != 0  # 'Z' without its calibration
 
 
+def test_gauss_mle_scmos_gateway_matches_device_entry():
+    from bamf_amd.gauss_mle import fit_boxes
+
+    rng = np.random.default_rng(19)
+    n, sz, mapsz = 1000, 7, 40
+    boxes = np.ascontiguousarray(synth.make_mle_boxes(n, sz, seed=19)[0], np.float32)
+    gain = rng.uniform(0.5, 30, (mapsz, mapsz)).astype(np.float32)
+    coords = rng.integers(0, mapsz - sz, (n, 2)).astype(np.float32)
+    p, c, ll = fit_boxes(boxes, 1.3, 20, "Sigma", None, coords, gain)
+    hp, hc, hl = np.zeros(5 * n, np.float32), np.zeros(5 * n, np.float32), np.zeros(n, np.float32)
+    rc = lib().bamf_gauss_mle_scmos_fit_host(ptr(boxes), sz, n, ptr(coords), ptr(gain), mapsz, ctypes.c_float(1.3),
+                                             20, 1, None, None, ptr(hp), ptr(hc), ptr(hl), None)
+    assert rc == 0, lib().bamf_last_error()
+    assert np.array_equal(hp.reshape(5, n).view(np.uint32), p.view(np.uint32))
+    assert np.array_equal(hc.reshape(5, n).view(np.uint32), c.view(np.uint32))
+    assert np.array_equal(hl.view(np.uint32), ll.view(np.uint32))
+
+
 def test_find_roi_gateway_matches_device_entry():
     from bamf_amd.find_roi import default_minval, find_roi
 

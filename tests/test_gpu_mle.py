@@ -161,3 +161,42 @@ def test_mle_z_matches_oracle(oracle, sz, n, seed):
     assert np.quantile(rel(lg, lo, 1.0), 0.999) < 1e-3
     res, _ = gauss_mle(boxes, "Z", "CCD", ZSIG, ZSTRUCT)
     assert np.array_equal(res["Z"], pg[4]) and res["Z_SE"].shape == (n,)
+
+
+def scmos_setup(n, sz, seed, mapsz=48):
+    rng = np.random.default_rng(seed)
+    gain = rng.uniform(0.5, 40.0, size=(mapsz, mapsz)).astype(np.float32)
+    coords = np.stack([rng.integers(0, mapsz - sz, n), rng.integers(0, mapsz - sz, n)], 1).astype(np.float32)
+    boxes, _ = synth.make_mle_boxes(n, sz, seed=seed, noise=False)
+    for i in range(n):
+        x0, y0 = int(coords[i, 0]), int(coords[i, 1])
+        g = gain[y0:y0 + sz, x0:x0 + sz]
+        boxes[i] = (rng.poisson(boxes[i] + g) - g).astype(np.float32)
+    return boxes, coords, gain
+
+
+@pytest.mark.parametrize("fit_type,ft", [("Basic", 0), ("Sigma", 1), ("SigmaYX", 2), ("Z", 3)])
+@pytest.mark.parametrize("sz,n,seed", [(7, 1500, 44), (11, 150, 45)])
+def test_mle_scmos_matches_oracle(oracle, fit_type, ft, sz, n, seed):
+    """The sCMOS kernels (kernel_MLEFit_SCMOS*) against the oracle, every fit type,
+    a specialised and the generic box size."""
+    boxes, coords, gain = scmos_setup(n, sz, seed)
+    z0 = np.linspace(-0.1, 0.1, n).astype(np.float32)
+    sig = ZSIG if fit_type == "Z" else 1.3
+    pg, cg, lg = fit_boxes(boxes, sig, 20, fit_type, ZSTRUCT if fit_type == "Z" else None, coords, gain, z0)
+    cal = (ZSIG[0], ZSTRUCT["Ax"], ZSTRUCT["Ay"], ZSTRUCT["Bx"], ZSTRUCT["By"], ZSTRUCT["Gamma"], ZSTRUCT["D"],
+           ZSIG[1])
+    po, co, lo = oracle.mle_scmos(ft, boxes, coords, gain, ZSIG[0] if fit_type == "Z" else 1.3, 20,
+                                  zcal=cal if fit_type == "Z" else None, z0=z0)
+    print(f"scmos {fit_type} sz={sz} n={n}", flush=True)
+    for k in range(pg.shape[0]):
+        r = rel(pg[k], po[k], 1.0)
+        assert np.quantile(r, 0.999) < 1e-4 and r.max() < 1e-3, (k, np.quantile(r, 0.999), r.max())
+    # near-singular Fisher matrices (an emitter leaving the box, clamped widths),
+    # inverted without pivoting as the reference does, turn last-ulp differences
+    # into inf on one side and a huge finite value on the other: allow a few
+    fo, fg = np.isfinite(co).all(0), np.isfinite(cg).all(0)
+    assert (fo != fg).mean() <= 2e-3 and fo.mean() > 0.99
+    fin = fo & fg
+    assert np.quantile(rel(cg[:, fin], co[:, fin], 1e-6), 0.999) < 1e-3
+    assert np.quantile(rel(lg, lo, 1.0), 0.999) < 1e-3
