@@ -72,6 +72,11 @@ def _declare(L: ctypes.CDLL) -> None:
                                 c_int64, c_void_p, c_void_p]
     L.bamf_find_roi_crop.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int64, c_int,
                                      c_void_p, c_void_p]
+    L.bamf_fit_keep.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_float, c_float, c_float, c_void_p,
+                                c_void_p]
+    L.bamf_kde_pdf.argtypes = [c_void_p, c_int64, c_double, c_double, c_double, c_int, c_void_p, c_void_p]
+    L.bamf_find_priors_host.argtypes = [c_void_p, c_int, c_int, c_int] + [c_float] * 7 + [c_int] + \
+        [c_void_p] * 3 + [c_int64, c_void_p, c_void_p]
     L.bamf_subregion_count.argtypes = [c_int, c_int, c_int, c_int, c_int]
     L.bamf_subregion_count.restype = c_int64
     L.bamf_subregion_pixels.argtypes = [c_int, c_int, c_int, c_int, c_int]

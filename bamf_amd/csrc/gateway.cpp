@@ -3,7 +3,9 @@
 //   bamf_crjmcmc        cRJMCMC (source/c/cRJMCMC.cpp:52-305), one ROI per call;
 //   bamf_gauss_mle_host gaussMLE(Data, 'Basic', 'CCD', Sigma) (@RJ/gaussMLE.m:64-211);
 //   bamf_gauss_mle_fit_host the same for FitType 'Basic', 'Sigma', 'SigmaYX' or 'Z', raw outputs;
-//   bamf_find_roi_host  findROI(Image, Sigma1, Sigma2, Boxsize, Minval) (@RJ/findROI.m:1-163).
+//   bamf_find_roi_host  findROI(Image, Sigma1, Sigma2, Boxsize, Minval) (@RJ/findROI.m:1-163);
+//   bamf_find_priors_host findPriors (@RJ/findPriors.m:35-80) on the three above and the
+//                       device threshold and kernel density of priors.hip.
 // The MATLAB mex functions under csrc/mex only map mxArrays onto these.
 #include <algorithm>
 #include <chrono>
@@ -361,6 +363,239 @@ int bamf_find_roi_host(const float* image, int ysize, int xsize, int nframes, fl
         return BAMF_E_HIP;
     }
     *count = total;
+    return BAMF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// psi(x) and psi'(x) for x > 0: the recurrences psi(x) = psi(x+1) - 1/x and
+// psi'(x) = psi'(x+1) + 1/x^2 up to x >= 8, then the asymptotic (Bernoulli)
+// series.
+double digamma(double x) {
+    double r = 0.0;
+    while (x < 8.0) {
+        r -= 1.0 / x;
+        x += 1.0;
+    }
+    const double f = 1.0 / (x * x);
+    return r + std::log(x) - 0.5 / x -
+           f * (1.0 / 12 - f * (1.0 / 120 - f * (1.0 / 252 - f * (1.0 / 240 - f * (1.0 / 132)))));
+}
+double trigamma(double x) {
+    double r = 0.0;
+    while (x < 8.0) {
+        r += 1.0 / (x * x);
+        x += 1.0;
+    }
+    const double f = 1.0 / (x * x);
+    return r + 1.0 / x + f / 2.0 +
+           f / x * (1.0 / 6 - f * (1.0 / 30 - f * (1.0 / 42 - f * (1.0 / 30 - f * 5.0 / 66))));
+}
+
+// Maximum-likelihood Gamma(a, b) (shape, scale) of positive samples, as
+// fitdist(x, 'Gamma') estimates it: log a - psi(a) = log(mean) - mean(log),
+// b = mean / a; Newton from the closed-form starting point of Minka (2002).
+void gamma_mle(const std::vector<double>& x, double* a_out, double* b_out) {
+    double m = 0.0, ml = 0.0;
+    for (double v : x) {
+        m += v;
+        ml += std::log(v);
+    }
+    m /= (double)x.size();
+    ml /= (double)x.size();
+    const double s = std::log(m) - ml;
+    double a = (3.0 - s + std::sqrt((s - 3.0) * (s - 3.0) + 24.0 * s)) / (12.0 * s);
+    for (int it = 0; it < 100; it++) {
+        const double f = std::log(a) - digamma(a) - s, fp = 1.0 / a - trigamma(a);
+        double an = a - f / fp;
+        if (!(an > 0.0)) an = a / 2.0;
+        const bool done = std::fabs(an - a) <= 1e-14 * a;
+        a = an;
+        if (done) break;
+    }
+    *a_out = a;
+    *b_out = m / a;
+}
+
+double gampdf(double x, double a, double b) {
+    if (x < 0.0) return 0.0;
+    if (x == 0.0) return a < 1.0 ? INFINITY : a == 1.0 ? 1.0 / b : 0.0;
+    return std::exp((a - 1.0) * std::log(x) - x / b - std::lgamma(a) - a * std::log(b));
+}
+
+// quantile(x, p) of MATLAB for a vector: sorted samples at the positions
+// (i - 0.5)/n, linear interpolation between them, clamped to the extremes.
+double matlab_quantile(std::vector<float> v, double p) {
+    std::sort(v.begin(), v.end());
+    const double n = (double)v.size();
+    const double pos = n * p + 0.5;  // 1-based
+    if (pos <= 1.0) return v.front();
+    if (pos >= n) return v.back();
+    const int64_t f = (int64_t)std::floor(pos);
+    return (double)v[f - 1] + (pos - (double)f) * ((double)v[f] - (double)v[f - 1]);
+}
+
+// The prior grid 0:dp:top (MATLAB colon: floor(top/dp) + 1 points, with the
+// colon's tolerance of a few ulps) and the kernel density of x on it.
+int kde_on_grid(const std::vector<float>& x, double dp, std::vector<double>* pdf) {
+    float top = 0.f;
+    for (float v : x) top = std::max(top, v);
+    const double end = (double)top + 1000.0 * dp;  // findPriors.m:47
+    const int64_t ng = (int64_t)std::floor(end / dp * (1.0 + 4e-16)) + 1;
+    BAMF_CHECK_ARG(ng > 0 && ng < (1 << 26), "prior grid of %lld points", (long long)ng);
+    DevBufs b;
+    float* d_x = b.upload(x.data(), x.size());
+    double* d_pdf = b.alloc<double>((size_t)ng);
+    if (b.err != hipSuccess) {
+        set_error("findPriors gateway: %s", hipGetErrorString(b.err));
+        return BAMF_E_HIP;
+    }
+    int rc = bamf_kde_pdf(d_x, (int64_t)x.size(), 75.0, 0.0, dp, (int)ng, d_pdf, nullptr);
+    if (rc) return rc;
+    pdf->resize((size_t)ng);
+    b.err = hipMemcpy(pdf->data(), d_pdf, (size_t)ng * 8, hipMemcpyDeviceToHost);
+    if (b.err != hipSuccess) {
+        set_error("findPriors gateway: %s", hipGetErrorString(b.err));
+        return BAMF_E_HIP;
+    }
+    return BAMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bamf_find_priors_host(const float* frames, int ysize, int xsize, int nframes, float psf_sigma,
+                          float mean_photons, float max_xy_se, float min_pvalue, float min_photons, float p_dp,
+                          float bg_prior_range, int bg_prior, double* signal_pdf, double* bg_pdf, double* offset_pdf,
+                          int64_t capacity, int64_t* len, double info[8]) {
+    BAMF_CHECK_ARG(frames && len && signal_pdf && bg_pdf && offset_pdf, "null argument");
+    BAMF_CHECK_ARG(psf_sigma > 0.f && p_dp > 0.f && capacity >= 0, "bad argument");
+    // findPriors.m:35-40: Boxsize 8, Sigma2 = 2 Sigma1, Minval from MeanPhotons
+    const int box = 8;
+    const double minval = (double)mean_photons / (4.0 * 3.141592653589793 * (double)psf_sigma * psf_sigma) / 4.0;
+    int64_t cap = std::max<int64_t>(1024, (int64_t)ysize * xsize * nframes / 64), count = 0;
+    std::vector<int32_t> xc, yc, fr;
+    std::vector<float> stack;
+    for (int pass = 0; pass < 2; pass++) {
+        xc.assign((size_t)cap, 0);
+        yc.assign((size_t)cap, 0);
+        fr.assign((size_t)cap, 0);
+        stack.assign((size_t)cap * box * box, 0.f);
+        const int rc = bamf_find_roi_host(frames, ysize, xsize, nframes, psf_sigma, 2.0f * psf_sigma, box,
+                                          (float)minval, cap, xc.data(), yc.data(), fr.data(), stack.data(),
+                                          nullptr, &count);
+        if (rc) return rc;
+        if (count <= cap) break;
+        cap = count;
+    }
+    const int64_t nf = count;
+    BAMF_CHECK_ARG(nf > 0, "findPriors: no ROI found");
+    // findPriors.m:40: gaussMLE(ROIStack, 'Basic', 'CCD', Sigma1)
+    std::vector<float> prm(4 * (size_t)nf), crl(4 * (size_t)nf), ll((size_t)nf);
+    int rc = bamf_gauss_mle_fit_host(stack.data(), box, nf, psf_sigma, 20, BAMF_FIT_BASIC, nullptr, prm.data(),
+                                     crl.data(), ll.data(), nullptr);
+    if (rc) return rc;
+    // findPriors.m:41: threhsSMA on the device
+    std::vector<uint8_t> keep((size_t)nf);
+    {
+        DevBufs b;
+        const float* d_p = b.upload(prm.data(), prm.size());
+        const float* d_c = b.upload(crl.data(), crl.size());
+        const float* d_l = b.upload(ll.data(), ll.size());
+        uint8_t* d_k = b.alloc<uint8_t>((size_t)nf);
+        if (b.err == hipSuccess)
+            rc = bamf_fit_keep(d_p, d_c, d_l, nf, box, max_xy_se, min_pvalue, min_photons, d_k, nullptr);
+        if (rc) return rc;
+        if (b.err == hipSuccess) b.err = hipMemcpy(keep.data(), d_k, (size_t)nf, hipMemcpyDeviceToHost);
+        if (b.err != hipSuccess) {
+            set_error("findPriors gateway: %s", hipGetErrorString(b.err));
+            return BAMF_E_HIP;
+        }
+    }
+    // isreal(SMD.X_SE) and isreal(SMD.Y_SE) test whole arrays: one negative
+    // CRLB (a complex square root) rejects every fit (threhsSMA.m:9-12)
+    for (int64_t i = 0; i < nf; i++)
+        if (crl[i] < 0.f || crl[(size_t)nf + i] < 0.f) {
+            std::fill(keep.begin(), keep.end(), 0);
+            break;
+        }
+    std::vector<float> photon, bgv;
+    for (int64_t i = 0; i < nf; i++)
+        if (keep[i]) {
+            photon.push_back(prm[2 * (size_t)nf + i]);
+            bgv.push_back(prm[3 * (size_t)nf + i]);
+        }
+    BAMF_CHECK_ARG(photon.size() >= 2, "findPriors: %zu fits pass the threshold", photon.size());
+    const size_t nkept = photon.size();
+    // findPriors.m:43-47: trim at the 0.98 quantile, kernel density on the grid
+    if (photon.size() > 200) {
+        const float id = (float)matlab_quantile(photon, 0.98);
+        photon.erase(std::remove_if(photon.begin(), photon.end(), [&](float v) { return v > id; }), photon.end());
+    }
+    std::vector<double> pdf;
+    if ((rc = kde_on_grid(photon, p_dp, &pdf))) return rc;
+    // findPriors.m:50-59: a second mode above the highest one cuts the tail
+    int refit = 0;
+    {
+        const size_t n = pdf.size();
+        std::vector<int> sg(n);
+        for (size_t i = 0; i < n; i++) {
+            const double gr = n == 1 ? 0.0
+                              : i == 0 ? pdf[1] - pdf[0]
+                              : i == n - 1 ? pdf[n - 1] - pdf[n - 2]
+                                           : (pdf[i + 1] - pdf[i - 1]) / 2.0;
+            sg[i] = (gr > 0) - (gr < 0);
+        }
+        std::vector<int64_t> ind;  // 1-based, where sign(gradient) changes
+        for (size_t i = 1; i < n; i++)
+            if (sg[i] != sg[i - 1]) ind.push_back((int64_t)i + 1);
+        if (!ind.empty()) {
+            double mx = -INFINITY;
+            for (int64_t k : ind) mx = std::max(mx, pdf[k - 1]);
+            int64_t imax = -1;  // first index of the whole density holding that value
+            for (size_t i = 0; i < n && imax < 0; i++)
+                if (pdf[i] == mx) imax = (int64_t)i + 1;
+            std::vector<int64_t> rem;
+            for (int64_t k : ind)
+                if (k > imax) rem.push_back(k);
+            if (rem.size() >= 2) {
+                const double cut = (double)rem[0] * p_dp;
+                photon.erase(std::remove_if(photon.begin(), photon.end(), [&](float v) { return (double)v > cut; }),
+                             photon.end());
+                if ((rc = kde_on_grid(photon, p_dp, &pdf))) return rc;
+                refit = 1;
+            }
+        }
+    }
+    // findPriors.m:66-74: Gamma fit of the kept backgrounds; exponential background prior
+    std::vector<double> bgd(bgv.begin(), bgv.end());
+    double ga, gb;
+    gamma_mle(bgd, &ga, &gb);
+    double mean_ph = 0.0;
+    for (float v : photon) mean_ph += (double)v;
+    mean_ph /= (double)photon.size();
+    const double mean_bg = mean_ph / (double)bg_prior_range;
+    const int64_t ng = (int64_t)pdf.size();
+    *len = ng;
+    for (int64_t i = 0; i < std::min(ng, capacity); i++) {
+        const double x = (double)i * p_dp;
+        signal_pdf[i] = pdf[(size_t)i];
+        offset_pdf[i] = gampdf(x, ga, gb);
+        bg_pdf[i] = bg_prior ? std::exp(-x / mean_bg) / mean_bg : 0.0;
+    }
+    if (info) {
+        info[0] = (double)nf;
+        info[1] = (double)nkept;
+        info[2] = (double)photon.size();
+        info[3] = ga;
+        info[4] = gb;
+        info[5] = refit;
+        info[6] = mean_ph;
+        info[7] = minval;
+    }
     return BAMF_OK;
 }
 

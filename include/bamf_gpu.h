@@ -116,6 +116,36 @@ int bamf_find_roi_crop(const float* d_stack, int xsize, int ysize,
                        float* d_roi_stack, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Prior construction (@RJ/findPriors.m:35-80).
+ * bamf_fit_keep: @RJ/threhsSMA.m for fit type 'Basic' on gaussMLE outputs
+ *   (params/crlb parameter-major [4][nfits]: Y, X, Photons, BG; SE = sqrt(CRLB)):
+ *   keep[i] = X_SE, Y_SE < max_xy_se, p-value 1 - gammainc(-LL, (box^2 - 4)/2)
+ *   > min_pvalue, Photons > min_photons, 0 < X, Y < box, BG > 0.  (The
+ *   whole-array isreal() tests of threhsSMA.m, which reject every fit when one
+ *   X or Y CRLB is negative, are the caller's: bamf_find_priors_host applies them.)
+ * bamf_kde_pdf: pdf(fitdist(x, 'Kernel', 'BandWidth', h), x0 + dx*(0:ngrid-1)),
+ *   the normal kernel density in double.
+ * bamf_find_priors_host: the whole of findPriors.m:35-80 on host frames in
+ *   MATLAB layout (Data(y, x, f)): findROI (Boxsize 8, sigma and 2 sigma,
+ *   Minval = MeanPhotons/(4 pi sigma^2)/4), gaussMLE 'Basic', threhsSMA, the
+ *   0.98-quantile trim (above 200 fits), the kernel density on 0:p_dp:max+1000
+ *   p_dp, the second-mode cut and refit, the Gamma fit of the backgrounds, and
+ *   P_EmPhotons (signal_pdf), P_Offset (offset_pdf = gampdf) and P_BgEmPhotons
+ *   (bg_pdf = exppdf with mean mean(Photons)/bg_prior_range, or zeros when
+ *   bg_prior is 0).  *len receives the grid length (entries beyond capacity are
+ *   not written); info = {ROIs, fits kept, photons used, Gamma a, Gamma b, refit,
+ *   mean photons, Minval}.
+ * ------------------------------------------------------------------------ */
+int bamf_fit_keep(const float* d_params, const float* d_crlb, const float* d_ll, int64_t nfits, int box,
+                  float max_xy_se, float min_pvalue, float min_photons, uint8_t* d_keep, void* stream);
+int bamf_kde_pdf(const float* d_samples, int64_t n, double bandwidth, double x0, double dx, int ngrid,
+                 double* d_pdf, void* stream);
+int bamf_find_priors_host(const float* frames, int ysize, int xsize, int nframes, float psf_sigma,
+                          float mean_photons, float max_xy_se, float min_pvalue, float min_photons, float p_dp,
+                          float bg_prior_range, int bg_prior, double* signal_pdf, double* bg_pdf, double* offset_pdf,
+                          int64_t capacity, int64_t* len, double info[8]);
+
+/* ------------------------------------------------------------------------
  * Frame tiling and the stitched posterior image.
  * bamf_make_subregions replaces @RJ/makeSubRegions.m:42-102 (called at
  * RJ.m:192,227,249): frames in MATLAB layout (Data(y, x, f) at
