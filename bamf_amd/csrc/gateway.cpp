@@ -48,19 +48,24 @@ extern "C" {
 int bamf_crjmcmc(const bamf_mcmc_params* mcmc, const float* data, int xsize, int ysize, int n0, const float* i0,
                  const float* x0, const float* y0, float bg0, const float* off_pdf, int off_len,
                  const float* sig_pdf, int sig_len, const float* back_pdf, int back_len, const float* scmos,
-                 int opt_model, uint64_t seed, float* pgrid, float* pback, bamf_chain_rec* acc_rec,
+                 const float* sampled_psf, const int32_t* psf_size, int opt_model, uint64_t seed, float* pgrid, float* pback, bamf_chain_rec* acc_rec,
                  bamf_chain_rec* prop_rec, float* acc_em, float* prop_em, int32_t rec_cap, int32_t em_cap,
                  int64_t counts[3], double stat[16]) {
     BAMF_CHECK_ARG(mcmc && data && off_pdf && sig_pdf && back_pdf, "null argument");
     BAMF_CHECK_ARG(xsize >= 1 && ysize >= 1 && xsize <= 32 && ysize <= 32, "ROI sides must be in 1..32");
     BAMF_CHECK_ARG(n0 >= 0 && (n0 == 0 || (i0 && x0 && y0)), "E_active needs I, X and Y for N > 0");
-    BAMF_CHECK_ARG(opt_model == 0, "OptModel = 1 (the sampled PSF) is not supported");
+    BAMF_CHECK_ARG(opt_model == 0 || opt_model == 1, "OptModel must be 0 (erf PSF) or 1 (sampled PSF)");
+    BAMF_CHECK_ARG(opt_model == 0 || (sampled_psf && psf_size), "OptModel = 1 needs SampledPSF and PSFsize");
     BAMF_CHECK_ARG(!acc_rec == !prop_rec && !acc_rec == !acc_em && !acc_rec == !prop_em,
                    "chain outputs need all four arrays");
     BAMF_CHECK_ARG(!acc_rec || (counts && rec_cap >= 0 && em_cap >= 0), "chain outputs need counts and capacities");
     bamf_rjmcmc_plan* plan = nullptr;
     int rc = bamf_rjmcmc_plan_create(mcmc, off_pdf, off_len, sig_pdf, sig_len, back_pdf, back_len, &plan);
     if (rc != BAMF_OK) return rc;
+    if (opt_model == 1 && (rc = bamf_rjmcmc_set_sampled_psf(plan, sampled_psf, psf_size)) != BAMF_OK) {
+        bamf_rjmcmc_plan_destroy(plan);
+        return rc;
+    }
     const int P = xsize * ysize, Z = mcmc->grid_zoom;
     const int64_t grid = (int64_t)P * Z * Z;
     DevBufs b;

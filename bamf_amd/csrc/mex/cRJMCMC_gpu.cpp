@@ -91,6 +91,22 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     const int xsize = (int)mxGetScalar(prhs[3]), ysize = (int)mxGetScalar(prhs[4]);
     const int n0 = (int)mxGetScalar(mxGetField(E, 0, "N"));
     const uint64_t seed = nrhs > 11 ? (uint64_t)mxGetScalar(prhs[11]) : 0;
+    // SampledPSF (prhs[9]) with its size from MCMC.PSFsize (cRJMCMC.cpp:35), as
+    // rjmcmc.m:91 sets it; the array's own dimensions when the field is absent
+    const int opt_model = (int)mxGetScalar(prhs[10]);
+    int32_t psf_size[4] = {1, 1, 1, 1};
+    const mxArray* ps = mxGetField(M, 0, "PSFsize");
+    if (ps && mxIsInt32(ps) && mxGetNumberOfElements(ps) >= 4) {
+        for (int k = 0; k < 4; k++) psf_size[k] = (int32_t)((const int32_t*)mxGetData(ps))[k];
+    } else {
+        const mwSize nd = mxGetNumberOfDimensions(prhs[9]);
+        const mwSize* d = mxGetDimensions(prhs[9]);
+        for (mwSize k = 0; k < nd && k < 4; k++) psf_size[k] = (int32_t)d[k];
+    }
+    if (opt_model == 1 && (!mxIsSingle(prhs[9]) ||
+                           (int64_t)mxGetNumberOfElements(prhs[9]) !=
+                               (int64_t)psf_size[0] * psf_size[1] * psf_size[2] * psf_size[3]))
+        mexErrMsgIdAndTxt("bamf:args", "SampledPSF must be single with PSFsize elements");
     const int64_t total = (int64_t)p.n_burnin + p.n_trials;
     const int32_t rec_cap = (int32_t)(total + 1);
     const int32_t em_cap = (int32_t)(total * 16 + 1024);
@@ -106,7 +122,8 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
         (const float*)mxGetData(mxGetField(E, 0, "X")), (const float*)mxGetData(mxGetField(E, 0, "Y")),
         (float)mxGetScalar(mxGetField(E, 0, "BG")), (const float*)mxGetData(prhs[5]), longest_side(prhs[5]),
         (const float*)mxGetData(prhs[6]), longest_side(prhs[6]), (const float*)mxGetData(prhs[7]),
-        longest_side(prhs[7]), (const float*)mxGetData(prhs[8]), (int)mxGetScalar(prhs[10]), seed,
+        longest_side(prhs[7]), (const float*)mxGetData(prhs[8]), (const float*)mxGetData(prhs[9]), psf_size,
+        opt_model, seed,
         (float*)mxGetData(plhs[0]), (float*)mxGetData(plhs[1]), acc.data(), prop.data(), acc_em.data(),
         prop_em.data(), rec_cap, em_cap, counts, stat);
     if (rc != BAMF_OK) mexErrMsgIdAndTxt("bamf:run", "%s", bamf_last_error());

@@ -188,11 +188,14 @@ enum { EV_NONE = 0, EV_MOVE, EV_SPLIT, EV_MERGE, EV_BIRTH, EV_DEATH, EV_GSPLIT, 
 // when P_Burnin entries 5 or 6 are nonzero).  kSEG: the launch runs a segment
 // of the chain (state loaded from and saved to HBM, seed read from device
 // memory when given).  kLRN: the learned proposals of BAMF_RJ_LEARNED.
-constexpr int kGJ = 1, kSEG = 2, kLRN = 4;
+// kNUM: the sampled-PSF model (NumModel) in place of the erf stencils.
+constexpr int kGJ = 1, kSEG = 2, kLRN = 4, kNUM = 8;
 
 template <int PPL, int F>
 struct Chain {
     static constexpr bool GJ = (F & kGJ) != 0, SEG = (F & kSEG) != 0, LRN = (F & kLRN) != 0;
+    static constexpr bool NUM = (F & kNUM) != 0;
+    static_assert(!(NUM && LRN), "the learned proposals use the erf stencil");
     const ChainArgs* a;  // the kernel argument block (in the kernarg segment)
     WaveLds<PPL>& L;
     const int lane;
@@ -525,6 +528,58 @@ struct Chain {
         wsync();
         return ysub < Y ? vy : 0.0f;
     }
+    // ---------------- sampled PSF (NumModel, LibRJMCMC.cpp:67-169) ----------------
+    // Table entry (y, x, phase_y, phase_x) at y + P0 x + P01 phase_y + P012
+    // phase_x; OS = PSFsize(4) phases per pixel.  Per emitter (wave-uniform,
+    // in double as the reference forms them): the bases of the four
+    // interpolation corners (phase and next phase along each axis, the phase
+    // after the last being phase 0 one pixel on) and the two ratios; pixel
+    // (x, y) then reads corner k at nb[k] + y + P0 x, a linear index outside
+    // the table reading the table's last entry (LibRJMCMC.cpp:131-158).  No
+    // contraction, so each term rounds as the reference's does.
+    int nb[4];
+    float nxr, nyr;
+    __device__ __forceinline__ void num_setup(float xe, float ye) {
+#pragma clang fp contract(off)
+        const int p0 = a->psf_dims[0], p01 = p0 * a->psf_dims[1], p012 = p01 * a->psf_dims[2];
+        const double os = (double)a->psf_dims[3];
+        const int st0 = p0 / 2;
+        const double xi = (double)xe, yi = (double)ye, fx = floor(xi), fy = floor(yi);
+        const int stx = (int)ceil(st0 - xi), imx = (int)floor(os * (xi - fx));
+        const int sty = (int)ceil(st0 - yi), imy = (int)floor(os * (yi - fy));
+        const bool wx = imx == os - 1, wy = imy == os - 1;
+        const int stx2 = wx ? stx - 1 : stx, imx2 = wx ? 0 : imx + 1;
+        const int sty2 = wy ? sty - 1 : sty, imy2 = wy ? 0 : imy + 1;
+        nb[0] = sty + stx * p0 + imy * p01 + imx * p012;
+        nb[1] = sty + stx2 * p0 + imy * p01 + imx2 * p012;
+        nb[2] = sty2 + stx * p0 + imy2 * p01 + imx * p012;
+        nb[3] = sty2 + stx2 * p0 + imy2 * p01 + imx2 * p012;
+        nyr = (float)(1 - ((yi - fy) - (double)imy / os) * os);
+        nxr = (float)(1 - ((xi - fx) - (double)imx / os) * os);
+    }
+    // This lane's pixel of slot s in the unit image of the last num_setup
+    // emitter (padding lanes read clamped, in-table entries).
+    __device__ __forceinline__ float num_at(int s) const {
+#pragma clang fp contract(off)
+        const int n = a->psf_numel;
+        const int q = ysub + px(s) * a->psf_dims[0];
+        int i[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int v = nb[k] + q;
+            i[k] = (unsigned)v >= (unsigned)n ? n - 1 : v;
+        }
+        const float* T = a->psf;
+        const float A = nxr * T[i[0]] + (1.0f - nxr) * T[i[1]];
+        const float B = nxr * T[i[2]] + (1.0f - nxr) * T[i[3]];
+        return nyr * A + (1.0f - nyr) * B;
+    }
+    __device__ __forceinline__ static float num_acc(float m, float I, float v, bool neg) {
+#pragma clang fp contract(off)
+        const float pr = I * v;
+        return neg ? m - pr : m + pr;
+    }
+
     // ---------------- conditional intensity fit (learned mode) ----------------
     // R = M minus emitter (I, x, y) into mt[], leaving that emitter's unit
     // stencil in place for cond_fit.
@@ -590,20 +645,29 @@ struct Chain {
                 m[s] = ok ? cBG + cABG * (float)px(s) + cBBG * (float)ysub + sc : 1.0f;
             }
         const Em& c = cur();
-        aflops += (uint64_t)(4 * P) + (uint64_t)cN * (uint64_t)(2 * P + 8 * (X + Y));
+        aflops += (uint64_t)(4 * P) + (uint64_t)cN * (uint64_t)(NUM ? 10 * P : 2 * P + 8 * (X + Y));
         for (int n = 0; n < cN; n++) {
-            const float vy = stencil(c.I[n], c.X[n], c.Y[n]);
+            if constexpr (NUM) {
+                num_setup(c.X[n], c.Y[n]);
+                const float I = c.I[n];
 #pragma unroll
-            for (int s = 0; s < PPL; s++)
-                if (s < ppl) m[s] = m[s] + L.sten[px(s)] * vy;
-            wsync();
+                for (int s = 0; s < PPL; s++)
+                    if (s < ppl && has(s)) m[s] = num_acc(m[s], I, num_at(s), false);
+            } else {
+                const float vy = stencil(c.I[n], c.X[n], c.Y[n]);
+#pragma unroll
+                for (int s = 0; s < PPL; s++)
+                    if (s < ppl) m[s] = m[s] + L.sten[px(s)] * vy;
+                wsync();
+            }
         }
         nacc = 0;
     }
     // M' = M + background change + the stencil list; returns the sum over
     // pixels of M - M' + D log(M'/M) (likelihood_ratio, LibRJMCMC.cpp:1677-1678).
     __device__ __forceinline__ float proposal_llr_sum() {
-        aflops += (uint64_t)(6 * P) + (uint64_t)ndelta * (uint64_t)(2 * P + 8 * (X + Y)) + (bgdelta ? 5 * P : 0);
+        aflops += (uint64_t)(6 * P) + (uint64_t)ndelta * (uint64_t)(NUM ? 10 * P : 2 * P + 8 * (X + Y)) +
+                  (bgdelta ? 5 * P : 0);
 #pragma unroll
         for (int s = 0; s < PPL; s++) mt[s] = m[s];
         lo_s = ppl;
@@ -622,6 +686,16 @@ struct Chain {
             const Em& E = ((code >> 8) & 1) ? tst() : cur();
             const int k = code & 255;
             const bool neg = (code >> 9) & 1;
+            if constexpr (NUM) {  // every pixel reads the table
+                num_setup(E.X[k], E.Y[k]);
+                const float I = E.I[k];
+                lo_s = 0;
+                hi_s = ppl - 1;
+#pragma unroll
+                for (int s = 0; s < PPL; s++)
+                    if (s < ppl && has(s)) mt[s] = num_acc(mt[s], I, num_at(s), neg);
+                continue;
+            }
             const float vy = stencil(E.I[k], E.X[k], E.Y[k]);
             // slots holding columns sxa..sxb
             const int s0 = sxa >> (6 - lgY), s1 = sxb >> (6 - lgY);
@@ -1830,11 +1904,14 @@ using ChainKernelFn = void (*)(ChainArgs);
 
 // The instantiations per class: feature sets 0 (plain), kGJ, kSEG,
 // kGJ | kSEG, kLRN and all three (learned mode with group jumps or
-// segments); the plain kernel also in each register budget of kTuneMinBlocks.
+// segments), and one sampled-PSF kernel (kNUM with kGJ | kSEG, whose code
+// for those features is inert in runs without them); the plain kernel also
+// in each register budget of kTuneMinBlocks.
 template <int C>
 ChainKernelFn chain_kernel(int f, int tune) {
     constexpr int P = kClassPPL[C], M = kClassMinBlocks[C];
     static_assert(kTuneMinBlocks[kTuneDefault] == M, "the default budget is the class's");
+    if (f & kNUM) return k_chain<P, M, kGJ | kSEG | kNUM>;
     if (f & kLRN) return f == kLRN ? k_chain<P, M, kLRN> : k_chain<P, M, kGJ | kSEG | kLRN>;
     if (f == kGJ) return k_chain<P, M, kGJ>;
     if (f == kSEG) return k_chain<P, M, kSEG>;

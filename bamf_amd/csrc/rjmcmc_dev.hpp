@@ -125,6 +125,11 @@ struct ChainArgs {
     int32_t rec_cap, em_cap;
     uint64_t* alg_flops;
     int32_t recompute_every;  // full model rebuild after this many accepted jumps
+    // sampled-PSF model (OptModel = 1; kernels with the kNUM feature): the
+    // PSFsize(1) x .. x PSFsize(4) table in column-major order and its size
+    const float* psf;
+    int32_t psf_dims[4];
+    int32_t psf_numel;
     // segments (seg_begin = 0 and seg_end >= total: the whole chain in one launch)
     int32_t seg_begin, seg_end;
     SegState* seg_state;       // one per ROI; read when seg_begin > 0, written when seg_end < total

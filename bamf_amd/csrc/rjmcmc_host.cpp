@@ -115,7 +115,11 @@ struct bamf_rjmcmc_plan {
     float l_alpha = 1.0f, l_halfwidth = 0.5f, l_beta = 0.0f;
     int device = 0;
     int cu = 0;
-    int blocks_per_cu[kClasses][8] = {};  // [class][features] at the default register budget
+    int blocks_per_cu[kClasses][16] = {};  // [class][features] at the default register budget
+    // sampled PSF (OptModel = 1), plan-owned; null: the erf model
+    float* d_psf = nullptr;
+    int32_t psf_dims[4] = {0, 0, 0, 0};
+    int32_t psf_numel = 0;
     int blocks_per_cu_tune[kClasses][kTuneChoices] = {};  // one-launch plain kernels per register budget
     // autotuned (or set) launch configuration of the one-launch plain kernels
     int tune[kClasses] = {kTuneDefault, kTuneDefault};
@@ -149,6 +153,7 @@ void plan_release(bamf_rjmcmc_plan* pl) {
     if (pl->cap) (void)hipStreamDestroy(pl->cap);
     if (pl->d_seg) (void)hipFree(pl->d_seg);
     if (pl->d_tune_status) (void)hipFree(pl->d_tune_status);
+    if (pl->d_psf) (void)hipFree(pl->d_psf);
     delete pl;
 }
 void drop_graph(bamf_rjmcmc_plan* pl) {
@@ -173,7 +178,8 @@ int enqueue_run(bamf_rjmcmc_plan* plan, const ChainArgs& a, int nseg, bool segme
     // phases (LibRJMCMC.cpp:705-710, 737-742): reachable only when its
     // entries 5 or 6 are nonzero
     const bool gj = a.p.p_burnin[5] != 0.0f || a.p.p_burnin[6] != 0.0f;
-    const int features = (gj ? 1 : 0) | (segmented ? 2 : 0) | (a.mode == BAMF_RJ_LEARNED ? 4 : 0);
+    const int features =
+        (gj ? 1 : 0) | (segmented ? 2 : 0) | (a.mode == BAMF_RJ_LEARNED ? 4 : 0) | (a.psf ? 8 : 0);
     if (concurrent) {
         BAMF_HIP_TRY(hipEventRecord(plan->ev_fork, s));
         BAMF_HIP_TRY(hipStreamWaitEvent(plan->side, plan->ev_fork, 0));
@@ -297,7 +303,7 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, pl->device) == hipSuccess) pl->cu = prop.multiProcessorCount;
     for (int c = 0; c < kClasses; c++)
-        for (int f = 0; f < 8; f++) {
+        for (int f = 0; f < 16; f++) {
             int bpc = 0;
             if (chain_occupancy_blocks(c, f, kTuneDefault, &bpc) == BAMF_OK) pl->blocks_per_cu[c][f] = bpc;
         }
@@ -352,6 +358,36 @@ int bamf_rjmcmc_set_intensity_proposal(bamf_rjmcmc_plan* plan, int kind, float i
     return BAMF_OK;
 }
 
+int bamf_rjmcmc_set_sampled_psf(bamf_rjmcmc_plan* plan, const float* h_psf, const int32_t psf_size[4]) {
+    BAMF_CHECK_ARG(plan, "null argument");
+    int64_t numel = 0;
+    if (h_psf) {
+        BAMF_CHECK_ARG(psf_size, "the sampled PSF needs its size");
+        numel = 1;
+        for (int k = 0; k < 4; k++) {
+            BAMF_CHECK_ARG(psf_size[k] >= 1, "PSFsize entries must be >= 1");
+            numel *= psf_size[k];
+            BAMF_CHECK_ARG(numel <= 0x7fffffff, "sampled PSF too large");
+        }
+    }
+    drop_graph(plan);
+    if (plan->d_psf) BAMF_HIP_TRY(hipFree(plan->d_psf));  // waits for runs that read it
+    plan->d_psf = nullptr;
+    plan->psf_numel = 0;
+    if (!h_psf) return BAMF_OK;
+    BAMF_HIP_TRY(hipMalloc(&plan->d_psf, sizeof(float) * (size_t)numel));
+    hipError_t e = hipMemcpy(plan->d_psf, h_psf, sizeof(float) * (size_t)numel, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(plan->d_psf);
+        plan->d_psf = nullptr;
+        set_error("sampled PSF upload failed: %s", hipGetErrorString(e));
+        return BAMF_E_HIP;
+    }
+    std::memcpy(plan->psf_dims, psf_size, sizeof(plan->psf_dims));
+    plan->psf_numel = (int32_t)numel;
+    return BAMF_OK;
+}
+
 int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed) {
     BAMF_CHECK_ARG(plan && claimed, "null argument");
     uint32_t q[kCountsAt + 2];
@@ -371,6 +407,7 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     if (mode == BAMF_RJ_LEARNED) {
         BAMF_CHECK_ARG(plan->d_q && plan->d_mu, "learned mode needs bamf_rjmcmc_set_proposal first");
         BAMF_CHECK_ARG(plan->params.is_backg == 0, "learned mode supports IsBackg = 0 only");
+        BAMF_CHECK_ARG(!plan->d_psf, "learned mode supports the erf PSF model only");
     }
     BAMF_CHECK_ARG(batch->n_roi >= 0, "n_roi must be >= 0");
     if (batch->n_roi == 0) return BAMF_OK;
@@ -459,6 +496,11 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
         a.c_mean = (float)plan->sig_mean;
         a.c_sd = (float)plan->sig_sd;
         a.l_single_from = plan->l_single_from;
+    }
+    if (plan->d_psf) {
+        a.psf = plan->d_psf;
+        std::memcpy(a.psf_dims, plan->psf_dims, sizeof(a.psf_dims));
+        a.psf_numel = plan->psf_numel;
     }
     a.recompute_every = 128;
     hipStream_t s = as_stream(stream);

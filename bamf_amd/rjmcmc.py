@@ -54,7 +54,9 @@ def make_params(psf_sigma=1.3, n_burnin=3000, n_trials=3000, p_burnin=(0.4, 0.15
 class Plan:
     """bamf_rjmcmc_plan: parameters plus the three prior tables on the device."""
 
-    def __init__(self, params: McmcParams, off_pdf, sig_pdf, back_pdf):
+    def __init__(self, params: McmcParams, off_pdf, sig_pdf, back_pdf, sampled_psf=None):
+        """sampled_psf: a 4-D SampledPSF array (y, x, phase_y, phase_x), the chains then
+        using NumModel (OptModel = 1, LibRJMCMC.cpp:62-170) for the model image."""
         self.params = params
         self._tables = [np.ascontiguousarray(t, np.float32) for t in (off_pdf, sig_pdf, back_pdf)]
         h = ctypes.c_void_p()
@@ -63,6 +65,21 @@ class Plan:
                                             b.ctypes.data, len(b), ctypes.byref(h)), "bamf_rjmcmc_plan_create")
         self.handle = h
         self._proposal = None
+        if sampled_psf is not None:
+            self.set_sampled_psf(sampled_psf)
+
+    def set_sampled_psf(self, psf) -> None:
+        """OptModel = 1 with SampledPSF = psf (4-D, MATLAB index order); None: the erf model."""
+        if psf is None:
+            check(lib().bamf_rjmcmc_set_sampled_psf(self.handle, None, None), "bamf_rjmcmc_set_sampled_psf")
+            return
+        a = np.asarray(psf, np.float32)
+        if a.ndim != 4:
+            raise BamfError("the sampled PSF is a 4-D array (y, x, phase_y, phase_x)")
+        flat = np.ascontiguousarray(a.ravel(order="F"))
+        dims = np.array(a.shape, np.int32)
+        check(lib().bamf_rjmcmc_set_sampled_psf(self.handle, flat.ctypes.data, dims.ctypes.data),
+              "bamf_rjmcmc_set_sampled_psf")
 
     def set_proposal(self, d_q, d_mu, alpha: float = 0.2, halfwidth: float = 0.35, beta: float = 0.5) -> None:
         """Learned proposal maps (device buffers in the batch's pixel layout)."""

@@ -66,9 +66,11 @@ def declare(L: ctypes.CDLL) -> None:
     L.bamf_rjmcmc_run.argtypes = [c_void_p, POINTER(RoiBatch), c_uint64, c_int, POINTER(Replay), POINTER(RjOut),
                                   c_void_p]
     L.bamf_rjmcmc_last_claimed.argtypes = [c_void_p, POINTER(c_int64)]
+    L.bamf_rjmcmc_set_sampled_psf.argtypes = [c_void_p, c_void_p, c_void_p]
     L.bamf_math_erf.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
     L.bamf_crjmcmc.argtypes = [POINTER(McmcParams), c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
-                               c_float, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_uint64,
+                               c_float, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p,
+                               c_void_p, c_int, c_uint64,
                                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
                                c_void_p, c_void_p]
     L.bamf_gauss_mle_host.argtypes = [c_void_p, c_int, c_int64, c_float, c_int] + [c_void_p] * 10

@@ -49,6 +49,28 @@ def roi_expected(I, X, Y, bg, xsize: int, ysize: int, sigma: float) -> np.ndarra
     return m.ravel()  # x-major, y fastest
 
 
+def sampled_gauss_psf(size: int, oversampling: int, sigma_y: float, sigma_x: float | None = None,
+                      size_x: int | None = None) -> np.ndarray:
+    """A SampledPSF table (y, x, phase_y, phase_x) of the pixel-integrated
+    Gaussian, in the layout NumModel reads (LibRJMCMC.cpp:86-169): entry
+    (d0, d1, iy, ix) is the unit-intensity value of the pixel whose lower
+    edges lie at d0 - ST0 - iy / OS and d1 - ST0 - ix / OS from the emitter,
+    ST0 = size // 2 on both axes (the reference takes it from PSFsize(1))."""
+    sigma_x = sigma_y if sigma_x is None else sigma_x
+    size_x = size if size_x is None else size_x
+    st0 = size // 2
+    ph = np.arange(oversampling) / oversampling
+
+    def axis(n, sig):
+        e = np.arange(n)[:, None] - st0 - ph[None, :]  # (n, OS)
+        c = 1.0 / (sig * math.sqrt(2.0))
+        return 0.5 * (erf((e + 1.0) * c) - erf(e * c))
+
+    gy, gx = axis(size, sigma_y), axis(size_x, sigma_x)
+    t = gy[:, None, :, None] * gx[None, :, None, :]
+    return np.asfortranarray(t.astype(np.float32))
+
+
 def make_rois(n_roi: int, seed: int, sides=(16,), density=None, mean_n=3.0, n_cap=None, photons=(5.0, 200.0),
               bg=(10.0, 10.0), sigma=1.3, pixel_um=0.1):
     """Synthetic ROIs.  Returns a dict of per-ROI arrays and the flat data pool.

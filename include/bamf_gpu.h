@@ -307,7 +307,8 @@ typedef struct bamf_rjmcmc_out {
     uint64_t* d_alg_flops;   /* [n_roi]: algorithmic floating-point operations of the chain: per jump
                                 6P (likelihood ratio) + 2P + 8(X+Y) per changed emitter + 5P for a
                                 background move + 4P for a birth pixel draw; 4P + N(2P + 8(X+Y)) per
-                                model rebuild (P = X*Y pixels) */
+                                model rebuild (P = X*Y pixels); with a sampled PSF 10P in place of
+                                2P + 8(X+Y) */
 } bamf_rjmcmc_out;
 
 #define BAMF_RJ_PLAIN 0   /* counter-based Philox draws keyed by (seed, key[r], step) */
@@ -386,6 +387,15 @@ int bamf_rjmcmc_set_proposal(bamf_rjmcmc_plan* plan, const float* d_q, const flo
  * single-emitter moves (the reference's 0.25, LibRJMCMC.cpp:882; up to 0.5,
  * taken from the group random walk, jump type 3). */
 int bamf_rjmcmc_set_intensity_proposal(bamf_rjmcmc_plan* plan, int kind, float inflation, float single_share);
+/* Sampled-PSF model for later runs of this plan: SampledPSF with OptModel = 1
+ * (cRJMCMC.cpp:73-74; likelihood_ratio LibRJMCMC.cpp:1650-1686 and
+ * find_Birth :1693-1714 then read NumModel, :62-170, instead of Model).
+ * h_psf is the PSFsize(1) x PSFsize(2) x PSFsize(3) x PSFsize(4) array in
+ * MATLAB's column-major order (entry (y, x, phase_y, phase_x), PSFsize(4) =
+ * the oversampling); psf_size = MCMC.PSFsize (cRJMCMC.cpp:35).  The table is
+ * copied to plan-owned device memory (synchronous).  h_psf = NULL restores the
+ * erf model (OptModel = 0).  Not with BAMF_RJ_LEARNED runs. */
+int bamf_rjmcmc_set_sampled_psf(bamf_rjmcmc_plan* plan, const float* h_psf, const int32_t psf_size[4]);
 /* The work-queue counters are reset by bamf_rjmcmc_run itself; this returns
  * how many chains the last run on the plan claimed (synchronous). */
 int bamf_rjmcmc_last_claimed(bamf_rjmcmc_plan* plan, int64_t* claimed);
@@ -469,7 +479,9 @@ int bamf_chain_summary(const bamf_chain_rec* d_acc_rec, const float* d_acc_em, c
  *   n0, i0, x0, y0, bg0 E_active (N, I, X, Y, BG; BG = 0 becomes 1, :132-133);
  *   off/sig/back_pdf    OffSetPDF, SignalPDF, BackPDF with their lengths;
  *   scmos               sCMOScorr (same layout as data) or NULL for zeros;
- *   opt_model           must be 0 (the analytical PSF);
+ *   sampled_psf,        SampledPSF and MCMC.PSFsize (4 entries), read when
+ *   psf_size            opt_model = 1 (see bamf_rjmcmc_set_sampled_psf), else ignored;
+ *   opt_model           0 the erf PSF model, 1 the sampled PSF (NumModel);
  *   seed                the Philox stream replacing the process-global rand();
  *   pgrid, pback        (Ysize*Zoom) x (Xsize*Zoom) floats, column-major, or NULL;
  *   acc_rec/prop_rec,   Chain and ProposedChain records and their emitter
@@ -481,7 +493,7 @@ int bamf_chain_summary(const bamf_chain_rec* d_acc_rec, const float* d_acc_em, c
 int bamf_crjmcmc(const bamf_mcmc_params* mcmc, const float* data, int xsize, int ysize, int n0, const float* i0,
                  const float* x0, const float* y0, float bg0, const float* off_pdf, int off_len,
                  const float* sig_pdf, int sig_len, const float* back_pdf, int back_len, const float* scmos,
-                 int opt_model, uint64_t seed, float* pgrid, float* pback, bamf_chain_rec* acc_rec,
+                 const float* sampled_psf, const int32_t* psf_size, int opt_model, uint64_t seed, float* pgrid, float* pback, bamf_chain_rec* acc_rec,
                  bamf_chain_rec* prop_rec, float* acc_em, float* prop_em, int32_t rec_cap, int32_t em_cap,
                  int64_t counts[3], double stat[16]);
 

@@ -55,6 +55,7 @@ typedef struct {
 #define ORC_E_UNSUPPORTED (-3)
 #define ORC_E_NMAX (-4)
 #define ORC_E_TAPE (-5)
+#define ORC_E_ARG (-6)
 
 int orc_rjmcmc(const float* data, int xsize, int ysize, const float* scmos,
                const orc_mcmc_params* p,
@@ -62,6 +63,15 @@ int orc_rjmcmc(const float* data, int xsize, int ysize, const float* scmos,
                const float* back_pdf, int back_len,
                int n0, const float* i0, const float* x0, const float* y0, float bg0,
                orc_src* src, orc_out* out);
+/* The same chain with likelihood_ratio's model choice: opt_model 1 evaluates
+ * NumModel on the sampled PSF psf of size psf_dims[4] (LibRJMCMC.cpp:62-170). */
+int orc_rjmcmc_psf(const float* data, int xsize, int ysize, const float* scmos,
+                   const orc_mcmc_params* p,
+                   const float* off_pdf, int off_len, const float* sig_pdf, int sig_len,
+                   const float* back_pdf, int back_len,
+                   int n0, const float* i0, const float* x0, const float* y0, float bg0,
+                   const float* psf, const int32_t* psf_dims, int opt_model,
+                   orc_src* src, orc_out* out);
 
 /* Pieces exposed for unit tests. */
 float orc_urand_from_raw(uint32_t r);
@@ -71,6 +81,8 @@ int   orc_stat_tables(const float* pdf, int len, float dx, float* xarr, float* c
 float orc_stat_genrand(const float* uniq_x, const float* uniq_cdf, int nuniq, float u);
 void  orc_model(int n, const float* I, const float* X, const float* Y, float bg, float abg, float bbg,
                 int xsize, int ysize, float sigma, const float* scmos, float* m);
+void  orc_num_model(int n, const float* I, const float* X, const float* Y, float bg, float abg, float bbg,
+                    int xsize, int ysize, const float* psf, const int32_t* dims, const float* scmos, float* m);
 void  orc_gsplit_map(int np, const float* I, const float* X, const float* Y, const int* idx, float u1, float u2,
                      float u3, float* I1, float* X1, float* Y1, float* In, float* Xn, float* Yn, float* I0);
 void  orc_gmerge_map(int np, const float* I, const float* X, const float* Y, int jj, const int* idx, float* I1,

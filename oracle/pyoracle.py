@@ -105,6 +105,12 @@ def _declare(L):
     L.orc_stat_genrand.restype = c_float
     L.orc_model.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float, c_int, c_int, c_float,
                             c_void_p, c_void_p]
+    L.orc_num_model.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float, c_int, c_int,
+                                c_void_p, c_void_p, c_void_p, c_void_p]
+    L.orc_rjmcmc_psf.argtypes = [c_void_p, c_int, c_int, c_void_p, POINTER(McmcParams),
+                                 c_void_p, c_int, c_void_p, c_int, c_void_p, c_int,
+                                 c_int, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_int,
+                                 POINTER(Src), POINTER(Out)]
     L.orc_gsplit_map.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float,
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.orc_gmerge_map.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -259,11 +265,33 @@ def stat_tables(pdf: np.ndarray, dx: float):
     return x, c, ux[:nu], uc[:nu]
 
 
+def _psf_flat(psf):
+    if psf is None:
+        return None, None
+    a = np.asarray(psf, np.float32)
+    if a.ndim != 4:
+        raise ValueError("the sampled PSF is a 4-D array")
+    return np.ascontiguousarray(a.ravel(order="F")), np.array(a.shape, np.int32)
+
+
+def num_model(I, X, Y, bg, abg, bbg, xsize, ysize, psf, scmos=None):
+    """NumModel (LibRJMCMC.cpp:67-169) restated: the model in the chain's pixel order (x * ysize + y)."""
+    I, X, Y = (np.ascontiguousarray(v, np.float32) for v in (I, X, Y))
+    flat, dims = _psf_flat(psf)
+    sc = np.zeros(xsize * ysize, np.float32) if scmos is None else np.ascontiguousarray(scmos, np.float32)
+    m = np.zeros(xsize * ysize, np.float32)
+    lib().orc_num_model(len(I), I.ctypes.data, X.ctypes.data, Y.ctypes.data, float(bg), float(abg), float(bbg),
+                        xsize, ysize, flat.ctypes.data, dims.ctypes.data, sc.ctypes.data, m.ctypes.data)
+    return m
+
+
 def rjmcmc(data: np.ndarray, xsize: int, ysize: int, params: McmcParams, off_pdf, sig_pdf, back_pdf,
            bg0: float, *, src_kind=SRC_PHILOX, seed=0, roi=0, tape=None, record=False, scmos=None,
-           n0=0, i0=None, x0=None, y0=None, grids=False, chain=False, fin_cap=1024, rec_cap=None, em_cap=None):
+           n0=0, i0=None, x0=None, y0=None, grids=False, chain=False, fin_cap=1024, rec_cap=None, em_cap=None,
+           psf=None):
     """One chain, restated.  Returns a dict with steps (STEP_DTYPE), counters, final state,
-    optional replay tape, grids and Chain / Chain_Test records."""
+    optional replay tape, grids and Chain / Chain_Test records.  psf: a 4-D sampled PSF
+    (SampledPSF with OptModel = 1), indexed [y, x, phase_y, phase_x]."""
     L = lib()
     data = np.ascontiguousarray(data, np.float32).ravel()
     if scmos is None:
@@ -320,10 +348,13 @@ def rjmcmc(data: np.ndarray, xsize: int, ysize: int, params: McmcParams, off_pdf
     i0 = np.ascontiguousarray(i0, np.float32)
     x0 = np.ascontiguousarray(x0, np.float32)
     y0 = np.ascontiguousarray(y0, np.float32)
-    st = L.orc_rjmcmc(data.ctypes.data, xsize, ysize, scmos.ctypes.data, ctypes.byref(params),
-                      off_pdf.ctypes.data, len(off_pdf), sig_pdf.ctypes.data, len(sig_pdf),
-                      back_pdf.ctypes.data, len(back_pdf), n0, i0.ctypes.data, x0.ctypes.data, y0.ctypes.data,
-                      float(bg0), ctypes.byref(src), ctypes.byref(out))
+    flat, dims = _psf_flat(psf)
+    st = L.orc_rjmcmc_psf(data.ctypes.data, xsize, ysize, scmos.ctypes.data, ctypes.byref(params),
+                          off_pdf.ctypes.data, len(off_pdf), sig_pdf.ctypes.data, len(sig_pdf),
+                          back_pdf.ctypes.data, len(back_pdf), n0, i0.ctypes.data, x0.ctypes.data, y0.ctypes.data,
+                          float(bg0), None if flat is None else flat.ctypes.data,
+                          None if dims is None else dims.ctypes.data, 0 if psf is None else 1,
+                          ctypes.byref(src), ctypes.byref(out))
     r = {"status": st, "steps": steps, "counters": np.array(out.counters[:], np.int32), "n": out.n,
          "bg": out.bg, "abg": out.abg, "bbg": out.bbg,
          "I": fin_i[:out.n].copy(), "X": fin_x[:out.n].copy(), "Y": fin_y[:out.n].copy(), "sig": fin_s[:out.n].copy(),

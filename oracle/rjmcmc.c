@@ -15,6 +15,8 @@
  *    float, while distance() rounds the squared sum to float before sqrtf;
  *  - pi is the double literal 3.141592 (LibRJMCMCreturn.hpp:2);
  *  - no FMA contraction.
+ * OptModel 1 (orc_rjmcmc_psf): NumModel (LibRJMCMC.cpp:62-170) in place of
+ * Model, pinned by closed forms of the interpolation (tests/test_oracle_numpsf.py).
  * gSplit/gMerge (LibRJMCMC.cpp:1389-1584): their proposal maps are pinned
  * (numerical Jacobian, split-then-merge round trip); the assembly of their
  * posterior ratios is parity unpinned, like split/merge's.
@@ -48,6 +50,7 @@ typedef struct {
     stat_t off, sig, back;
     em_t cur, T;
     float *M, *Mt;
+    const float* psf; int32_t pdim[4]; int opt_model; /* SampledPSF, PSFsize, OptModel (cRJMCMC.cpp:35,73-74) */
     orc_src* src; orc_out* out; orc_step* st;
     int32_t draws;
     int err;
@@ -189,11 +192,55 @@ void orc_model(int n, const float* I, const float* X, const float* Y, float bg, 
     free(nx);
     free(ny);
 }
+/* ---------------- sampled-PSF model (NumModel, LibRJMCMC.cpp:67-169) ----------------
+ * psf is the PSFsize(1) x PSFsize(2) x PSFsize(3) x PSFsize(4) array in
+ * column-major order: entry (y, x, phase_y, phase_x) at y + P0 x + P01 phase_y
+ * + P012 phase_x, the oversampling OS = PSFsize(4).  Each emitter is read off
+ * the table at its sub-pixel phase and the next one (the phase after the last
+ * being phase 0 of the next pixel), interpolated bilinearly with the ratios
+ * formed in double; a linear index outside the table reads its last entry
+ * (LibRJMCMC.cpp:131-158, where the reference also prints a message). */
+static int num_clamp(int i, int n) { return (i >= n || i < 0) ? n - 1 : i; }
+void orc_num_model(int n, const float* I, const float* X, const float* Y, float bg, float abg, float bbg,
+                   int xsize, int ysize, const float* psf, const int32_t* dims, const float* scmos, float* m) {
+    const double os = (double)dims[3];
+    const int p0 = dims[0], p01 = dims[0] * dims[1], p012 = p01 * dims[2], p0123 = p012 * dims[3];
+    for (int xx = 0; xx < xsize; xx++)
+        for (int yy = 0; yy < ysize; yy++)
+            m[yy + xx * ysize] = bg + abg * xx + bbg * yy + scmos[yy + xx * ysize];
+    for (int nn = 0; nn < n; nn++) {
+        const double xi = (double)X[nn], yi = (double)Y[nn];
+        const int st0 = dims[0] / 2;
+        const int stx = (int)ceil(st0 - xi), imx = (int)floor(os * (xi - floor(xi)));
+        const int sty = (int)ceil(st0 - yi), imy = (int)floor(os * (yi - floor(yi)));
+        int stx2, imx2, sty2, imy2;
+        if (imx == os - 1) { stx2 = stx - 1; imx2 = 0; } else { stx2 = stx; imx2 = imx + 1; }
+        if (imy == os - 1) { sty2 = sty - 1; imy2 = 0; } else { sty2 = sty; imy2 = imy + 1; }
+        const int o1 = imy * p01 + imx * p012, o2 = imy * p01 + imx2 * p012;
+        const int o3 = imy2 * p01 + imx * p012, o4 = imy2 * p01 + imx2 * p012;
+        const float yr = (float)(1 - ((yi - floor(yi)) - (double)imy / os) * os);
+        const float xr = (float)(1 - ((xi - floor(xi)) - (double)imx / os) * os);
+        for (int xx = 0; xx < xsize; xx++)
+            for (int yy = 0; yy < ysize; yy++) {
+                const int i1 = num_clamp((sty + yy) + (stx + xx) * p0 + o1, p0123);
+                const int i2 = num_clamp((sty + yy) + (stx2 + xx) * p0 + o2, p0123);
+                const int i3 = num_clamp((sty2 + yy) + (stx + xx) * p0 + o3, p0123);
+                const int i4 = num_clamp((sty2 + yy) + (stx2 + xx) * p0 + o4, p0123);
+                const float A = xr * psf[i1] + (1.0f - xr) * psf[i2];
+                const float B = xr * psf[i3] + (1.0f - xr) * psf[i4];
+                m[yy + xx * ysize] = m[yy + xx * ysize] + I[nn] * (yr * A + (1.0f - yr) * B);
+            }
+    }
+}
+/* likelihood_ratio's choice of model (LibRJMCMC.cpp:1653-1671, 1695-1702) */
 static void model_of(ch_t* c, const em_t* e, float* m) {
-    orc_model(e->n, e->I, e->X, e->Y, e->bg, e->abg, e->bbg, c->X, c->Y, c->p.psf_sigma, c->scmos, m);
+    if (c->opt_model == 1)
+        orc_num_model(e->n, e->I, e->X, e->Y, e->bg, e->abg, e->bbg, c->X, c->Y, c->psf, c->pdim, c->scmos, m);
+    else
+        orc_model(e->n, e->I, e->X, e->Y, e->bg, e->abg, e->bbg, c->X, c->Y, c->p.psf_sigma, c->scmos, m);
 }
 
-/* LibRJMCMC.cpp:1650-1686 (OptModel 0).  Returns exp(sum); *sum_out = the sum. */
+/* LibRJMCMC.cpp:1650-1686.  Returns exp(sum); *sum_out = the sum. */
 static float likelihood_ratio(ch_t* c, const em_t* t, float* sum_out) {
     model_of(c, t, c->Mt);
     int N = c->X * c->Y;
@@ -854,7 +901,24 @@ int orc_rjmcmc(const float* data, int xsize, int ysize, const float* scmos,
                const float* back_pdf, int back_len,
                int n0, const float* i0, const float* x0, const float* y0, float bg0,
                orc_src* src, orc_out* out) {
+    return orc_rjmcmc_psf(data, xsize, ysize, scmos, p, off_pdf, off_len, sig_pdf, sig_len, back_pdf, back_len, n0,
+                          i0, x0, y0, bg0, NULL, NULL, 0, src, out);
+}
+
+int orc_rjmcmc_psf(const float* data, int xsize, int ysize, const float* scmos,
+                   const orc_mcmc_params* p,
+                   const float* off_pdf, int off_len, const float* sig_pdf, int sig_len,
+                   const float* back_pdf, int back_len,
+                   int n0, const float* i0, const float* x0, const float* y0, float bg0,
+                   const float* psf, const int32_t* psf_dims, int opt_model,
+                   orc_src* src, orc_out* out) {
+    if (opt_model == 1 && !(psf && psf_dims && psf_dims[0] > 0 && psf_dims[1] > 0 && psf_dims[2] > 0 &&
+                            psf_dims[3] > 0))
+        return ORC_E_ARG;
     ch_t* c = (ch_t*)calloc(1, sizeof(ch_t));
+    c->opt_model = opt_model == 1 ? 1 : 0;
+    c->psf = psf;
+    if (c->opt_model) memcpy(c->pdim, psf_dims, sizeof(c->pdim));
     orc_step dummy;
     int P = xsize * ysize;
     c->data = data; c->scmos = scmos; c->X = xsize; c->Y = ysize; c->p = *p;

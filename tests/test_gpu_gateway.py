@@ -24,7 +24,7 @@ def ptr(a):
     return None if a is None else a.ctypes.data_as(P)
 
 
-def crjmcmc(p, data, xs, ys, bg0, seed, n0=0, i0=None, x0=None, y0=None, rec_cap=0, em_cap=0):
+def crjmcmc(p, data, xs, ys, bg0, seed, n0=0, i0=None, x0=None, y0=None, rec_cap=0, em_cap=0, psf=None):
     z = p.grid_zoom
     pg = np.zeros(xs * ys * z * z, np.float32)
     pb = np.zeros_like(pg)
@@ -35,9 +35,13 @@ def crjmcmc(p, data, xs, ys, bg0, seed, n0=0, i0=None, x0=None, y0=None, rec_cap
     counts = np.zeros(3, np.int64)
     stat = np.zeros(16, np.float64)
     rec = rec_cap > 0
+    psf_flat = psf_size = None
+    if psf is not None:
+        psf_flat = np.ascontiguousarray(np.asarray(psf, np.float32).ravel(order="F"))
+        psf_size = np.array(psf.shape, np.int32)
     rc = lib().bamf_crjmcmc(ctypes.byref(p), ptr(data), xs, ys, n0, ptr(i0), ptr(x0), ptr(y0), ctypes.c_float(bg0),
                             ptr(OFF_PDF), len(OFF_PDF), ptr(SIG_PDF), len(SIG_PDF), ptr(BACK_PDF), len(BACK_PDF), None,
-                            0, seed, ptr(pg), ptr(pb), ptr(acc) if rec else None, ptr(prop) if rec else None,
+                            ptr(psf_flat), ptr(psf_size), 0 if psf is None else 1, seed, ptr(pg), ptr(pb), ptr(acc) if rec else None, ptr(prop) if rec else None,
                             ptr(ae) if rec else None, ptr(pe) if rec else None, rec_cap, em_cap, ptr(counts),
                             ptr(stat))
     return rc, pg, pb, acc, prop, ae, pe, counts, stat
