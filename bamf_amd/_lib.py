@@ -75,6 +75,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bamf_fit_keep.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_float, c_float, c_float, c_void_p,
                                 c_void_p]
     L.bamf_kde_pdf.argtypes = [c_void_p, c_int64, c_double, c_double, c_double, c_int, c_void_p, c_void_p]
+    L.bamf_psf_shift_average.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p]
+    L.bamf_psf_workspace_bytes.argtypes = [c_int64]
+    L.bamf_psf_workspace_bytes.restype = c_int64
+    L.bamf_find_psf_host.argtypes = [c_void_p, c_int, c_int64, c_float, c_float, c_float, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]
     L.bamf_find_priors_host.argtypes = [c_void_p, c_int, c_int, c_int] + [c_float] * 7 + [c_int] + \
         [c_void_p] * 3 + [c_int64, c_void_p, c_void_p]
     L.bamf_subregion_count.argtypes = [c_int, c_int, c_int, c_int, c_int]

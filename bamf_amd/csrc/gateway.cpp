@@ -5,7 +5,9 @@
 //   bamf_gauss_mle_fit_host the same for FitType 'Basic', 'Sigma', 'SigmaYX' or 'Z', raw outputs;
 //   bamf_find_roi_host  findROI(Image, Sigma1, Sigma2, Boxsize, Minval) (@RJ/findROI.m:1-163);
 //   bamf_find_priors_host findPriors (@RJ/findPriors.m:35-80) on the three above and the
-//                       device threshold and kernel density of priors.hip.
+//                       device threshold and kernel density of priors.hip;
+//   bamf_find_psf_host  findPSF (@RJ/findPSF.m:30-119): the 'Sigma' fits and the
+//                       shift-average of psf.hip, then the table split on the host.
 // The MATLAB mex functions under csrc/mex only map mxArrays onto these.
 #include <algorithm>
 #include <chrono>
@@ -601,6 +603,94 @@ int bamf_find_priors_host(const float* frames, int ysize, int xsize, int nframes
         info[6] = mean_ph;
         info[7] = minval;
     }
+    return BAMF_OK;
+}
+
+int bamf_find_psf_host(const float* roi_stack, int box, int64_t n, float gain, float offset, float psf_sigma,
+                       float* sampled_psf, int32_t psf_size[4], float* fit_x, float* fit_y) {
+    constexpr int B = 8, M = B + 24, SUB = 64, OS = 4, PADR = 112;
+    BAMF_CHECK_ARG(box == B, "findPSF's fixed 64 x 64 x 4 x 4 split needs BoxSize 8 (got %d)", box);
+    BAMF_CHECK_ARG(roi_stack && sampled_psf && psf_size && n >= 1 && n <= 0x7fffffff, "need ROIs and outputs");
+    BAMF_CHECK_ARG(gain != 0.0f && psf_sigma > 0.0f, "gain must be nonzero and PSFSigma > 0");
+    const size_t np = (size_t)n * B * B;
+    std::vector<float> r(np);
+    for (size_t i = 0; i < np; i++) r[i] = (roi_stack[i] - offset) / gain;  // findPSF.m:30
+    DevBufs b;
+    float* d_r = b.upload(r.data(), np);
+    float* d_par = b.alloc<float>(5 * (size_t)n);
+    float* d_crlb = b.alloc<float>(5 * (size_t)n);
+    float* d_ll = b.alloc<float>((size_t)n);
+    float* d_xy = b.alloc<float>(2 * (size_t)n);
+    void* d_ws = b.alloc<char>((size_t)bamf_psf_workspace_bytes(n));
+    float* d_img = b.alloc<float>(M * M);
+    if (b.err != hipSuccess) {
+        set_error("findPSF device setup failed: %s", hipGetErrorString(b.err));
+        return BAMF_E_HIP;
+    }
+    // findPSF.m:31: the 'Sigma' fits give the positions to align
+    int rc = bamf_gauss_mle_xynbs(d_r, psf_sigma, B, 20, d_par, d_crlb, d_ll, (int)n, nullptr);
+    if (rc != BAMF_OK) return rc;
+    std::vector<float> par(2 * (size_t)n), xy(2 * (size_t)n);
+    BAMF_HIP_TRY(hipMemcpy(par.data(), d_par, 2 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    // positions in 1-based pixel-centre coordinates (the fit's, in which pixel k
+    // spans [k, k + 1), plus one half): the convention under which this table
+    // reproduces the PSF through NumModel
+    for (int64_t i = 0; i < n; i++) {
+        xy[i] = par[n + i] + 0.5f;  // X (second parameter, gaussMLE.m:184)
+        xy[n + i] = par[i] + 0.5f;  // Y
+        if (fit_x) fit_x[i] = xy[i];
+        if (fit_y) fit_y[i] = xy[n + i];
+    }
+    BAMF_HIP_TRY(hipMemcpy(d_xy, xy.data(), 2 * (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    rc = bamf_psf_shift_average(d_r, d_xy, d_xy + n, n, B, d_ws, d_img, nullptr);
+    if (rc != BAMF_OK) return rc;
+    std::vector<float> img(M * M);
+    BAMF_HIP_TRY(hipMemcpy(img.data(), d_img, sizeof(float) * M * M, hipMemcpyDeviceToHost));
+    // findPSF.m:58-68: minus the mean of the four edge means, negatives zeroed, peak 1
+    double e[4] = {0, 0, 0, 0};
+    for (int k = 0; k < M; k++) {
+        e[0] += img[0 + M * k];        // first row
+        e[1] += img[(M - 1) + M * k];  // last row
+        e[2] += img[k + M * (M - 1)];  // last column
+        e[3] += img[k];                // first column
+    }
+    const float off = (float)((e[0] + e[1] + e[2] + e[3]) / (4.0 * M));
+    float peak = 0.0f;
+    for (float& v : img) {
+        v = v - off;
+        if (v < 0.0f) v = 0.0f;
+        peak = std::max(peak, v);
+    }
+    for (float& v : img) v = v / peak;
+    // findPSF.m:73-117: padded by 112 per side (256 x 256); phase (jj, ii) takes
+    // rows l = (4 - jj) % 4 + 4a and columns k = (4 - ii) % 4 + 4b, the
+    // middle third (21..43) moved by one for nonzero phases, then sum 1
+    auto padded = [&](int l, int k) {
+        const int y = l - PADR, x = k - PADR;
+        return (y >= 0 && y < M && x >= 0 && x < M) ? img[y + M * x] : 0.0f;
+    };
+    std::vector<float> t(SUB * SUB);
+    const int lo = 21, hi = 43;  // round(64/3), round(128/3) as 0-based targets of the move
+    for (int ii = 0; ii < OS; ii++)
+        for (int jj = 0; jj < OS; jj++) {
+            const int l0 = (OS - jj) % OS, k0 = (OS - ii) % OS;
+            for (int bcol = 0; bcol < SUB; bcol++)
+                for (int a = 0; a < SUB; a++) t[a + SUB * bcol] = padded(l0 + OS * a, k0 + OS * bcol);
+            if (jj > 0)
+                for (int a = hi; a >= lo; a--)
+                    for (int bcol = 0; bcol < SUB; bcol++) t[a + SUB * bcol] = t[a - 1 + SUB * bcol];
+            if (ii > 0)
+                for (int bcol = hi; bcol >= lo; bcol--)
+                    for (int a = 0; a < SUB; a++) t[a + SUB * bcol] = t[a + SUB * (bcol - 1)];
+            double sum = 0.0;
+            for (float v : t) sum += v;
+            float* dst = sampled_psf + (size_t)SUB * SUB * (jj + OS * ii);
+            for (int i = 0; i < SUB * SUB; i++) dst[i] = (float)(t[i] / sum);
+        }
+    psf_size[0] = SUB;
+    psf_size[1] = SUB;
+    psf_size[2] = OS;
+    psf_size[3] = OS;
     return BAMF_OK;
 }
 

@@ -575,6 +575,34 @@ int bamf_find_roi_host(const float* image, int ysize, int xsize, int nframes, fl
  * (a self-check of the device math; no counterpart in the reference). */
 int bamf_math_erf(const float* d_x, float* d_y, int64_t n, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Sampled PSF from isolated emitters (@RJ/findPSF.m:30-119).
+ *
+ * bamf_psf_shift_average replaces findPSF.m:38-56: for the first
+ * min(n, 1000) ROIs (BoxSize x BoxSize, BoxSize = 8, MATLAB layout, already
+ * (ROIStack - Offset) / Gain) the centred spectrum fftshift(fft2(fftshift(R /
+ * 10))) with the phase ramp that moves the emitter at (d_x, d_y) (1-based
+ * pixel-centre coordinates) to the common position, summed in ROI order, and
+ * |fftshift(ifft2(.))| of that sum zero-padded by 12 per side into d_image
+ * ((BoxSize + 24)^2 floats, column-major).  Workspace:
+ * bamf_psf_workspace_bytes(n). */
+int bamf_psf_shift_average(const float* d_rois, const float* d_x, const float* d_y, int64_t n, int box,
+                           void* d_workspace, float* d_image, void* stream);
+int64_t bamf_psf_workspace_bytes(int64_t n);
+/* SampledPSFm = findPSF(ROIStack, Gain, Offset, PSFSigma, BoxSize) on host
+ * arrays: the 'Sigma' fits of findPSF.m:31 (this library's gaussMLE 'Sigma',
+ * 20 iterations, in place of the un-vendored SMA_Core.gaussMLE; positions
+ * converted to 1-based pixel centres, the fit's + 0.5), bamf_psf_shift_average,
+ * then findPSF.m:58-117
+ * (edge-mean offset, peak 1, padded to 256 x 256, split into the 4 x 4 phase
+ * images of 64 x 64, the middle third moved by one for nonzero phases, each
+ * summing to 1).  sampled_psf: 64 x 64 x 4 x 4 floats (SampledPSF(y, x,
+ * phase_y, phase_x), MATLAB order); psf_size = {64, 64, 4, 4}; fit_x / fit_y
+ * (n floats each, or NULL) the positions used.  BoxSize must be 8, the size
+ * for which the reference's fixed split is consistent.  Synchronous. */
+int bamf_find_psf_host(const float* roi_stack, int box, int64_t n, float gain, float offset, float psf_sigma,
+                       float* sampled_psf, int32_t psf_size[4], float* fit_x, float* fit_y);
+
 #ifdef __cplusplus
 }
 #endif
