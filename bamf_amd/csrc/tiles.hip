@@ -49,7 +49,7 @@ __host__ __device__ inline int64_t tile_prefix(int i, int box, int ov) {
 // share a column range, contiguous in the compact batch, tile iy's W x H(iy)
 // block after tile iy-1's): metadata by the first Ysub threads, then the
 // pixels, one stacked row per work-item, several loads in flight per thread.
-constexpr int kTileThreads = 256, kTileUnroll = 4;
+constexpr int kTileThreads = 256, kTileUnroll = 8;
 __global__ __launch_bounds__(kTileThreads) void k_tiles(const float* __restrict__ data, TileGeo g,
                                                         float* __restrict__ pixels, int64_t* __restrict__ pix_offset,
                                                         int32_t* __restrict__ xsize, int32_t* __restrict__ ysize,
@@ -84,13 +84,16 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const float* __restrict_
     }
     const float* src = data + ((int64_t)f * g.xpix + (sx - 1)) * g.ypix;  // column sx of frame f, row 1
     if (pixels) {
-        // Stacked row R of the column (the tiles' rows one after another, R <
-        // sum of the heights): its tile iy has a closed form (tiles 2 .. Ysub-1
-        // all have height hm), so each work-item resolves its row once and then
-        // walks the w columns: frame reads coalesced across rows, compact
-        // writes contiguous within each tile's column.
+        // Work items (stacked row R, chunk of kTileUnroll columns): R runs
+        // over the tiles' rows one after another (tiles 2 .. Ysub-1 all have
+        // height hm, so R's tile has a closed form); consecutive work-items
+        // take consecutive rows of one chunk, so the frame loads are coalesced
+        // across rows and each work-item keeps kTileUnroll loads in flight;
+        // the compact writes are contiguous within each tile's column.
         const int h1 = tile_size(1, g.ysub, g.ypix, g.box, g.ov), hm = g.box + 2 * g.ov;
-        for (int R = threadIdx.x; R < (int)g.sh; R += kTileThreads) {
+        const int sh = (int)g.sh, nch = (w + kTileUnroll - 1) / kTileUnroll;
+        for (int item = threadIdx.x; item < sh * nch; item += kTileThreads) {
+            const int c = item / sh, R = item - c * sh;
             int iy, y;
             if (R < h1 || g.ysub == 1) {
                 iy = 1; y = R;
@@ -99,17 +102,18 @@ __global__ __launch_bounds__(kTileThreads) void k_tiles(const float* __restrict_
                 y = R - h1 - (iy - 2) * hm;
             }
             const int h = tile_size(iy, g.ysub, g.ypix, g.box, g.ov);
-            const float* sp = src + tile_start(iy, g.box, g.ov) - 1 + y;
-            float* dp = pixels + col0 + (int64_t)w * tile_prefix(iy, g.box, g.ov) + y;
-            int x = 0;
-            for (; x + kTileUnroll <= w; x += kTileUnroll) {
+            const int x0 = c * kTileUnroll, nx = min(kTileUnroll, w - x0);
+            const float* sp = src + (int64_t)x0 * g.ypix + tile_start(iy, g.box, g.ov) - 1 + y;
+            float* dp = pixels + col0 + (int64_t)w * tile_prefix(iy, g.box, g.ov) + (int64_t)x0 * h + y;
+            if (nx == kTileUnroll) {
                 float v[kTileUnroll];
 #pragma unroll
-                for (int u = 0; u < kTileUnroll; u++) v[u] = sp[(int64_t)(x + u) * g.ypix];
+                for (int u = 0; u < kTileUnroll; u++) v[u] = sp[(int64_t)u * g.ypix];
 #pragma unroll
-                for (int u = 0; u < kTileUnroll; u++) dp[(int64_t)(x + u) * h] = v[u];
+                for (int u = 0; u < kTileUnroll; u++) dp[(int64_t)u * h] = v[u];
+            } else {
+                for (int u = 0; u < nx; u++) dp[(int64_t)u * h] = sp[(int64_t)u * g.ypix];
             }
-            for (; x < w; x++) dp[(int64_t)x * h] = sp[(int64_t)x * g.ypix];
         }
     }
     if (roi_data) {  // ROI_Data(1:ROIey, 1:ROIex, Ind), zero elsewhere (makeSubRegions.m:45,95)
@@ -179,6 +183,7 @@ int bamf_make_subregions(const float* d_data, int ypix, int xpix, int nframes, i
     if (n == 0) return BAMF_OK;
     BAMF_CHECK_ARG(d_data || !(d_pixels || d_roi_data), "null frames");
     BAMF_CHECK_ARG((int64_t)g.xsub * nframes <= 0x7fffffff, "too many tile columns");
+    BAMF_CHECK_ARG((int64_t)g.sh * (box + 2 * overlap) <= 0x7fffffff, "frames too tall for the tile copy");
     hipLaunchKernelGGL(k_tiles, dim3((unsigned)((int64_t)g.xsub * nframes)), dim3(kTileThreads), 0,
                        as_stream(stream), d_data, g, d_pixels,
                        d_pix_offset, d_xsize, d_ysize, d_frame, d_roi_ind, d_roi_data, zoom, d_grid_offset,
