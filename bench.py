@@ -79,7 +79,7 @@ _CPU = {}
 def _cpu_worker(idx):
     from oracle import pyoracle as po
 
-    rois, p, off, sig, back, bg0 = _CPU["args"]
+    rois, p, off, sig, back, bg0, psf = _CPU["args"]
     op = po.McmcParams.from_buffer_copy(p)
     out = []
     for r in idx:
@@ -87,14 +87,14 @@ def _cpu_worker(idx):
         s = int(rois["side"][r])
         t0 = time.perf_counter()
         res = po.rjmcmc(rois["data"][o[r]:o[r + 1]], s, s, op, off, sig, back, bg0=bg0, seed=SEED, roi=int(r),
-                        record=False)
+                        record=False, psf=psf)
         dt = time.perf_counter() - t0
         st = res["steps"][p.n_burnin:]
         out.append((int(r), st["n"].copy(), st["photons"].copy(), dt))
     return out
 
 
-def cpu_baseline(rois, p, off, sig, back, bg0, sample, cores, thin):
+def cpu_baseline(rois, p, off, sig, back, bg0, sample, cores, thin, psf=None):
     import multiprocessing as mp
 
     from bamf_amd.ess import fixed_ess_rate, roi_posteriors
@@ -103,7 +103,7 @@ def cpu_baseline(rois, p, off, sig, back, bg0, sample, cores, thin):
     po.build()
     rng = np.random.default_rng(SEED)
     idx = np.sort(rng.choice(len(rois["side"]), size=sample, replace=False))
-    _CPU["args"] = (rois, p, off, sig, back, bg0)
+    _CPU["args"] = (rois, p, off, sig, back, bg0, psf)
     chunks = [idx[i::cores * 4] for i in range(cores * 4)]
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
@@ -425,6 +425,9 @@ def main():
     ap.add_argument("--proposal", default="both", choices=["plain", "learned", "both"],
                     help="chain proposal(s) timed: the value is the plain chain's, learned is reported beside it")
     ap.add_argument("--train-steps", type=int, default=3000, help="Adam steps of 256 ROIs for the learned proposal")
+    ap.add_argument("--psf", default="erf", choices=["erf", "sampled"],
+                    help="chain PSF model: the erf stencil (OptModel 0) or a sampled 72 x 72 x 4 x 4 table of the "
+                         "same Gaussian (OptModel 1, NumModel); sampled runs time the plain chain only")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (the HBM counter calibration run)")
     args = ap.parse_args()
@@ -439,11 +442,17 @@ def main():
         return front_end_line(args)
     rois, nb, nt, rho, sig, off, back, bg0 = workload(args.workload)
     p = params(nb, nt, rho)
+    psf = None
+    if args.psf == "sampled":
+        from bamf_amd import synth
+
+        psf = synth.sampled_gauss_psf(72, 4, p.psf_sigma)
+        args.proposal = "plain"  # the learned proposals use the erf stencil
     cpu = None
     if not args.no_cpu:
         cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
         print(f"[bench] cpu baseline: {args.cpu_sample} ROIs on {cores} cores", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(rois, p, off, sig, back, bg0, args.cpu_sample, cores, args.thin)
+        cpu = cpu_baseline(rois, p, off, sig, back, bg0, args.cpu_sample, cores, args.thin, psf)
         print(f"[bench] cpu baseline done: {cpu['chains_per_s']:.1f} chains/s", file=sys.stderr, flush=True)
 
     from bamf_amd import _hip, proposal
@@ -451,7 +460,7 @@ def main():
     from bamf_amd.rjmcmc import LEARNED, PLAIN, Plan, RoiSet, alloc_outputs, fetch, lpt_order, run_batch
 
     n = len(rois["side"])
-    plan = Plan(p, off, sig, back)
+    plan = Plan(p, off, sig, back, sampled_psf=psf)
     order = lpt_order(rois["side"], rois["side"], rois["n_true"])
     rs = RoiSet.from_synth(rois, bg0=np.full(n, bg0, np.float32), order=order)
     trace_len = nt // args.thin
@@ -499,7 +508,7 @@ def main():
     ms, res, ess, ok, rmse, value = timed(PLAIN)
     flops = float(res.alg_flops.astype(np.float64).sum())
     achieved = flops / (ms / 1e3) / 1e12
-    traffic = read_traffic(ROOT / "profiles" / f"rjmcmc_chain_{args.workload}_pmc.txt")
+    traffic = read_traffic(ROOT / "profiles" / f"rjmcmc_chain_{args.workload}_pmc.txt") if psf is None else None
     plain = {"value": value, "ms_per_step": ms, "strict_posteriors_per_s": float(ok.sum() / (ms / 1e3)),
              "ess_sum_rate": float(ess.sum() / 200.0 / (ms / 1e3)),
              "mean_ess": float(ess.mean()), "median_ess": float(np.median(ess)), "rmse_nm_final_state": rmse,
@@ -512,6 +521,8 @@ def main():
         "config": {"workload": args.workload, "n_roi": n, "roi_sides": sorted(set(int(s) for s in rois["side"])),
                    "jumps_per_chain": nb + nt, "burnin": nb, "ess_target": 200, "proposal": "plain",
                    "parallelism": "1 GPU, one wavefront per chain, persistent work queues per ROI size class",
+                   "psf_model": "erf (OptModel 0)" if psf is None else
+                   "sampled (OptModel 1), 72 x 72 x 4 x 4 table of the same Gaussian",
                    "launch_tuning": tuning},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
