@@ -113,3 +113,39 @@ def test_sampled_psf_argument_checks():
         plan.set_sampled_psf(np.zeros((4, 4, 2), np.float32))
     rc, *_ = crjmcmc(p, np.zeros(64, np.float32), 8, 8, 1.0, 1, psf=np.zeros((0, 4, 2, 2), np.float32))
     assert rc != 0 and b"PSFsize" in lib().bamf_last_error()
+
+
+def test_sampled_psf_segments_and_graph_equal_one_launch():
+    """The sampled-PSF kernel carries the segment feature: a chain split into
+    launches (its model image kept in HBM between them), and the sequence
+    replayed from a HIP graph with new seeds, equal one-launch runs bit for bit."""
+    from bamf_amd import _hip
+    from bamf_amd.rjmcmc import alloc_outputs, fetch
+    from test_gpu_segments import KW, assert_same, rois_uneven
+
+    psf = tables()["big"]
+    rois = rois_uneven(96, seed=62)
+    n = len(rois["side"])
+    p = params(nb=500, nt=500, rho=0.0275)
+    rs = RoiSet.from_synth(rois, bg0=np.full(n, 20.0, np.float32))
+    ref = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF, sampled_psf=psf)
+    one = run_batch(ref, rs, seed=9, **KW)
+    assert np.all(one.status == 0)
+    seg_plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF, sampled_psf=psf)
+    seg_plan.set_segments(4)
+    print("sampled PSF: 4 segments", flush=True)
+    assert_same(one, run_batch(seg_plan, rs, seed=9, **KW), n, 100)  # 500 trials thinned by 5
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF, sampled_psf=psf)
+    plan.set_segments(4, graph=True)
+    total = p.n_burnin + p.n_trials
+    outs = alloc_outputs(n, total, KW["hist_len"], KW["final_cap"], KW["trace_len"], True, True, rs, p.grid_zoom,
+                         KW["chain_caps"])
+    for it, seed in enumerate((9, 10, 9)):
+        for k in ("pgrid", "pback"):
+            outs[k].upload(np.zeros(outs[k].nbytes // 4, np.float32))
+        print(f"sampled PSF graph replay {it}, seed {seed}", flush=True)
+        run_batch(plan, rs, seed=seed, outs=outs, trace_thin=KW["trace_thin"], sync=False)
+        _hip.synchronize()
+        got = fetch(outs, n, total, KW["hist_len"], KW["final_cap"], KW["trace_len"])
+        want = one if seed == 9 else run_batch(ref, rs, seed=seed, **KW)
+        assert_same(want, got, n, 100)
