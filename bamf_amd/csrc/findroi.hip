@@ -187,8 +187,10 @@ __global__ __launch_bounds__(64) void k_cols_iir(float* a, int xsize, int ysize,
 // Row recursion for both sigmas of the DoG pipeline (in must not alias the
 // outputs).  The forward pass keeps only the recursion state: the three last
 // forward outputs of each chunk, which are the next chunk's starting state, are
-// parked at their own positions in out0/out1 (the backward pass overwrites them
-// after use).  The backward pass walks the chunks from the end, re-reads a
+// parked in the checkpoint array ck, laid out [chunk][value][row] so that a
+// wave's 64 rows store and load each value as one 256-byte run (parking them
+// inside the outputs would make partial-line stores).  The backward pass walks
+// the chunks from the end, re-reads a
 // chunk, recomputes its forward outputs in registers from the parked state (the
 // same operations on the same inputs, hence the same values) and runs the
 // backward recursion on them.  HBM traffic: the stack read twice and each
@@ -199,14 +201,14 @@ __global__ __launch_bounds__(64) void k_cols_iir(float* a, int xsize, int ysize,
 // Rows row_begin + blockIdx.x * kRowsPerBlock ... of nrows.
 template <bool FULL>
 __global__ __launch_bounds__(kRowsPerBlock) void k_rows_dog(const float* __restrict__ in, float* __restrict__ out0,
-                                                           float* __restrict__ out1, int xsize, int64_t nrows,
-                                                           int64_t row_begin, Coeffs c0, Coeffs c1) {
+                                                           float* __restrict__ out1, float* __restrict__ ck,
+                                                           int xsize, int64_t nrows, int64_t row_begin, Coeffs c0,
+                                                           Coeffs c1) {
     __shared__ float tA[kRowsPerBlock][kChunk + 1];
     const int t = threadIdx.x;
     const int64_t row0 = row_begin + (int64_t)blockIdx.x * kRowsPerBlock;
     const int nr = FULL ? kRowsPerBlock : (int)min<int64_t>(kRowsPerBlock, nrows - row0);
     const int nchunk = (xsize + kChunk - 1) / kChunk;
-    const int64_t rbase = (row0 + t) * xsize;  // this work-item's row (used when t < nr)
     // Chunk transfers.  FULL: element i of work-item t is the float4 at row
     // kRowStep4 * i + t / 8, columns 4 (t % 8) ..+3.  Otherwise element i is row
     // kRowStep * i + t / kChunk, column t % kChunk, guarded.  Both address by a
@@ -322,11 +324,10 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_rows_dog(const float* __restr
             s1 = a1; s2 = a2; s3 = a3; u1 = b1; u2 = b2; u3 = b3;
             if (FULL || nc == kChunk) forward(std::true_type{}, nc);
             else forward(std::false_type{}, nc);
-            if (ch + 2 < nchunk) {  // park the state of chunk ch + 1 (a full chunk)
-                float* r0 = out0 + rbase + x0 + kChunk - 3;
-                float* r1 = out1 + rbase + x0 + kChunk - 3;
-                r0[0] = a3; r0[1] = a2; r0[2] = a1;
-                r1[0] = b3; r1[1] = b2; r1[2] = b1;
+            if (ch + 2 < nchunk) {  // park the state of chunk ch + 1: [chunk][value][row], coalesced
+                float* q = ck + (int64_t)(ch + 1) * 6 * nrows + (row0 + t);
+                q[0] = a3; q[nrows] = a2; q[2 * nrows] = a1;
+                q[3 * nrows] = b3; q[4 * nrows] = b2; q[5 * nrows] = b1;
             }
         }
         __syncthreads();
@@ -347,10 +348,9 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_rows_dog(const float* __restr
         if (ch > 0) {
             fetch_any(x0 - kChunk, kChunk);
             if (t < nr && ch > 1) {
-                const float* r0 = out0 + rbase + x0 - kChunk - 3;
-                const float* r1 = out1 + rbase + x0 - kChunk - 3;
-                s3 = r0[0]; s2 = r0[1]; s1 = r0[2];
-                u3 = r1[0]; u2 = r1[1]; u1 = r1[2];
+                const float* q = ck + (int64_t)(ch - 1) * 6 * nrows + (row0 + t);
+                s3 = q[0]; s2 = q[nrows]; s1 = q[2 * nrows];
+                u3 = q[3 * nrows]; u2 = q[4 * nrows]; u1 = q[5 * nrows];
             }
         }
         if (t < nr) {
@@ -858,9 +858,13 @@ static int64_t nwords_of(int64_t n) { return (n + 31) / 32; }
 static int64_t nblocks_of(int64_t nwords) { return (nwords + kWordsPerBlock - 1) / kWordsPerBlock; }
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// Column-pass checkpoints: 6 floats per column per kColChunk rows.
+// Recursion checkpoints, 6 floats each: the column pass's per column per
+// kColChunk rows, the row pass's per row per kChunk columns (the passes run
+// one after the other and share the region).
 static size_t col_ck_bytes(int xsize, int ysize, int nframes) {
-    return (size_t)xsize * nframes * ceil_div(ysize, kColChunk) * 6 * sizeof(float);
+    const size_t cols = (size_t)xsize * nframes * ceil_div(ysize, kColChunk) * 6 * sizeof(float);
+    const size_t rows = (size_t)ysize * nframes * ceil_div(xsize, kChunk) * 6 * sizeof(float);
+    return cols > rows ? cols : rows;
 }
 
 size_t bamf_find_roi_workspace_bytes(int xsize, int ysize, int nframes) {
@@ -903,13 +907,14 @@ int bamf_find_roi(const float* d_stack, int xsize, int ysize, int nframes, float
     // whole blocks of a chunk-multiple row length unguarded, the rest guarded
     const int64_t nfull = xsize % kChunk == 0 ? nrows / kRowsPerBlock : 0;
     if (nfull > 0) {
-        hipLaunchKernelGGL(k_rows_dog<true>, dim3((unsigned)nfull), dim3(kRowsPerBlock), 0, s, d_stack, A, Bf, xsize,
-                           nrows, (int64_t)0, c1, c2);
+        hipLaunchKernelGGL(k_rows_dog<true>, dim3((unsigned)nfull), dim3(kRowsPerBlock), 0, s, d_stack, A, Bf, colck,
+                           xsize, nrows, (int64_t)0, c1, c2);
         BAMF_LAUNCH_CHECK("k_rows_dog");
     }
     if (nrows > nfull * kRowsPerBlock) {
         hipLaunchKernelGGL(k_rows_dog<false>, dim3((unsigned)ceil_div(nrows - nfull * kRowsPerBlock, kRowsPerBlock)),
-                           dim3(kRowsPerBlock), 0, s, d_stack, A, Bf, xsize, nrows, nfull * kRowsPerBlock, c1, c2);
+                           dim3(kRowsPerBlock), 0, s, d_stack, A, Bf, colck, xsize, nrows, nfull * kRowsPerBlock, c1,
+                           c2);
         BAMF_LAUNCH_CHECK("k_rows_dog");
     }
     hipLaunchKernelGGL(k_cols_dog, dim3((unsigned)ceil_div(xsize, kColThreads), (unsigned)nframes),
