@@ -216,9 +216,11 @@ def _cpu_findroi(fr):
 def _cpu_mle(rng_):
     from oracle import pyoracle as po
 
-    boxes, sigma, ft = _FE["args"]
+    boxes, sigma, ft, sc = _FE["args"]
     b = boxes[rng_[0]:rng_[1]]
-    if ft == "Basic":
+    if sc is not None:  # sCMOS: box corners and the variance / gain^2 map
+        po.mle_scmos(("Basic", "Sigma", "SigmaYX").index(ft), b, sc[0][rng_[0]:rng_[1]], sc[1], sigma, 20)
+    elif ft == "Basic":
         po.mle_xynb(b, sigma, 20)
     elif ft == "Sigma":
         po.mle_xynbs(b, sigma, 20)
@@ -279,6 +281,15 @@ def bench_gauss_mle(args):
 
     n, sz, sigma = 100_000, 7, 1.3
     boxes, _ = synth.make_mle_boxes(n, sz, seed=SEED + 4, sigma=sigma)
+    ft = args.fit_type
+    sc_host = None
+    if args.camera == "SCMOS":
+        if ft == "Z":
+            raise SystemExit("--camera SCMOS times the 'Basic', 'Sigma' and 'SigmaYX' fits")
+        mapsz = 256
+        rng = np.random.default_rng(SEED + 9)
+        sc_host = (rng.integers(0, mapsz - sz, (n, 2)).astype(np.float32),
+                   rng.uniform(0.5, 3.0, (mapsz, mapsz)).astype(np.float32))
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle as po
@@ -286,7 +297,7 @@ def bench_gauss_mle(args):
         po.build()
         cores = args.cpu_cores or min(os.cpu_count() or 1, 16)
         sample = args.cpu_sample_fits
-        _FE["args"] = (boxes[:sample], sigma, args.fit_type)
+        _FE["args"] = (boxes[:sample], sigma, args.fit_type, sc_host)
         step = (sample + cores - 1) // cores
         dt = _cpu_pool(_cpu_mle, [(i, min(sample, i + step)) for i in range(0, sample, step)], cores)
         print(f"[bench] cpu baseline: {sample} fits in {dt:.1f} s", file=sys.stderr, flush=True)
@@ -296,17 +307,20 @@ def bench_gauss_mle(args):
     d = _hip.DeviceBuffer.from_array(boxes)
     dp, dc, dl = _hip.DeviceBuffer(24 * n), _hip.DeviceBuffer(24 * n), _hip.DeviceBuffer(4 * n)
     s = _hip.Stream()
-    ft = args.fit_type
     sg = (sigma, sigma) if ft == "Z" else sigma
     zs = _ZCAL if ft == "Z" else None
-    ms = _time_gpu(lambda: gauss_mle_device(d, sg, sz, 20, n, dp, dc, dl, stream=s, fit_type=ft, zstruct=zs), s,
-                   args.warmup, args.steps)
-    basic = ft == "Basic"
+    scm = None
+    if sc_host is not None:
+        scm = (_hip.DeviceBuffer.from_array(sc_host[0]), _hip.DeviceBuffer.from_array(sc_host[1]), 256, None)
+    ms = _time_gpu(lambda: gauss_mle_device(d, sg, sz, 20, n, dp, dc, dl, stream=s, fit_type=ft, zstruct=zs,
+                                            scmos=scm), s, args.warmup, args.steps)
+    basic = ft == "Basic" and scm is None
     flops = MLE_FLOP_PER_FIT * n if basic else None  # the algorithmic count is derived for the XYNB fit
     achieved = flops / (ms / 1e3) / 1e12 if basic else None
-    return {"metric": f"gaussMLE {ft} fits/s (7 x 7 boxes, 20 iterations)", "value": n / (ms / 1e3), "unit": "fits/s",
-            "ms_per_step": ms, "dtype": "f32", "higher_is_better": True,
-            "config": {"workload": "gauss_mle", "fit_type": ft, "fits": n, "box": sz, "iterations": 20},
+    return {"metric": f"gaussMLE {ft} {args.camera} fits/s (7 x 7 boxes, 20 iterations)", "value": n / (ms / 1e3),
+            "unit": "fits/s", "ms_per_step": ms, "dtype": "f32", "higher_is_better": True,
+            "config": {"workload": "gauss_mle", "fit_type": ft, "camera": args.camera, "fits": n, "box": sz,
+                       "iterations": 20},
             "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / PEAK_FP32_TFLOPS if basic else None, "kernel": "k_mle_xynb",
                          "traffic": read_traffic(ROOT / "profiles" / "mle_xynb_gauss_mle_100k_7x7_pmc.txt")
@@ -445,6 +459,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=3000, help="Adam steps of 256 ROIs for the learned proposal")
     ap.add_argument("--fit-type", default="Basic", choices=["Basic", "Sigma", "SigmaYX", "Z"],
                     help="gaussMLE fit type of --workload gauss_mle")
+    ap.add_argument("--camera", default="CCD", choices=["CCD", "SCMOS"],
+                    help="gaussMLE camera of --workload gauss_mle (SCMOS: a 256 x 256 variance/gain^2 map)")
     ap.add_argument("--psf", default="erf", choices=["erf", "sampled"],
                     help="chain PSF model: the erf stencil (OptModel 0) or a sampled 72 x 72 x 4 x 4 table of the "
                          "same Gaussian (OptModel 1, NumModel); sampled runs time the plain chain only")
