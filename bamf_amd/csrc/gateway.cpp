@@ -17,6 +17,10 @@
 
 #include "common.hpp"
 
+namespace {
+constexpr int64_t kFindRoiMaxFramesPerCall = 65535;  // bamf_find_roi's limit (one grid dimension per frame)
+}
+
 using bamf::set_error;
 
 namespace {
@@ -183,6 +187,15 @@ int mle_fit_host(const float* data, int sz, int64_t nfits, const float* coords, 
     BAMF_CHECK_ARG(fit_type != BAMF_FIT_Z || zcal, "fit type Z needs its calibration");
     const bool sc = coords != nullptr;
     BAMF_CHECK_ARG(!sc || (gain && mapsz >= sz && (fit_type != BAMF_FIT_Z || z0)), "sCMOS fits need the gain map");
+    if (sc) {
+        // the kernels read gain[(cy + j) * mapsz + cx + i] for 0 <= i, j < sz: every box must lie on the map
+        for (int64_t i = 0; i < nfits; i++) {
+            const float cx = coords[2 * i], cy = coords[2 * i + 1];
+            BAMF_CHECK_ARG(cx >= 0 && cy >= 0 && cx + sz <= mapsz && cy + sz <= mapsz,
+                           "sCMOS fit %lld: box corner (%g, %g) of a %d x %d box is not on the %d x %d gain map",
+                           (long long)i, cx, cy, sz, sz, mapsz, mapsz);
+        }
+    }
     const int np = fit_type == BAMF_FIT_SIGMAYX ? 6 : fit_type == BAMF_FIT_BASIC ? 4 : 5;
     if (nfits == 0) {
         if (fit_seconds) *fit_seconds = 0;
@@ -316,6 +329,7 @@ int bamf_find_roi_host(const float* image, int ysize, int xsize, int nframes, fl
     const size_t per_frame = (size_t)frame * 4 * 4 + (size_t)frame / 8 + 64;
     int64_t fpc = (int64_t)((free_b / 2) / per_frame);
     fpc = std::min<int64_t>(fpc, ((int64_t)1 << 31) / frame);
+    fpc = std::min<int64_t>(fpc, kFindRoiMaxFramesPerCall);  // bamf_find_roi's per-call frame limit
     fpc = std::max<int64_t>(1, std::min<int64_t>(fpc, nframes));
     const size_t ws_bytes = bamf_find_roi_workspace_bytes(xsize, ysize, (int)fpc);
     const int64_t npix_max = frame * fpc;

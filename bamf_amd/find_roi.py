@@ -56,9 +56,15 @@ class FindRoiDevice:
         return np.unpackbits(w.view(np.uint8), bitorder="little")[:n]
 
 
+MAX_FRAMES_PER_CALL = 65535  # bamf_find_roi's per-call limit
+
+
 def find_roi(image: np.ndarray, sigma1: float = 1.3, sigma2: float = 2 * 1.3, boxsize: int = 7,
-             minval: float | None = None, capacity: int | None = None):
-    """image: MATLAB array (Ysize, Xsize[, Z]); any numpy order (it is read as MATLAB column-major)."""
+             minval: float | None = None, capacity: int | None = None, frames_per_call: int = MAX_FRAMES_PER_CALL):
+    """image: MATLAB array (Ysize, Xsize[, Z]); any numpy order (it is read as MATLAB column-major).
+    Frames go to the device in consecutive runs of at most frames_per_call
+    (findROI.m:98-123 chunks the stack the same way); the filters act frame by
+    frame, so the chunking changes no result and detections stay in find() order."""
     if minval is None:
         minval = default_minval()
     img = np.asarray(image, dtype=np.float32)
@@ -66,6 +72,27 @@ def find_roi(image: np.ndarray, sigma1: float = 1.3, sigma2: float = 2 * 1.3, bo
         img = img[:, :, None]
     ysize, xsize, nframes = img.shape
     flat = np.asarray(img, order="F").ravel(order="F")  # MATLAB memory order
+    fpc = max(1, min(int(frames_per_call), MAX_FRAMES_PER_CALL, max(nframes, 1)))
+    npf = xsize * ysize
+    smd_parts, stacks, res_parts = [], [], []
+    for f0 in range(0, max(nframes, 1), fpc):
+        nf = min(fpc, nframes - f0)
+        if nf <= 0:
+            break
+        smd, stack, res = _find_roi_call(flat[f0 * npf:(f0 + nf) * npf], ysize, xsize, nf, sigma1, sigma2, boxsize,
+                                         minval, capacity)
+        smd["FrameNum"] += f0
+        smd_parts.append(smd)
+        stacks.append(stack)
+        res_parts.append(res)
+    if not smd_parts:
+        return ({"XBoxCorner": np.zeros(0), "YBoxCorner": np.zeros(0), "FrameNum": np.zeros(0)},
+                np.zeros((boxsize, boxsize, 0), np.float32), np.zeros((ysize, xsize, 0)))
+    smd = {k: np.concatenate([p[k] for p in smd_parts]) for k in ("XBoxCorner", "YBoxCorner", "FrameNum")}
+    return smd, np.concatenate(stacks, axis=2), np.concatenate(res_parts, axis=2)
+
+
+def _find_roi_call(flat, ysize, xsize, nframes, sigma1, sigma2, boxsize, minval, capacity):
     if capacity is None:
         capacity = max(1024, flat.size // 16)
     d_stack = _hip.DeviceBuffer.from_array(flat)
@@ -75,7 +102,7 @@ def find_roi(image: np.ndarray, sigma1: float = 1.3, sigma2: float = 2 * 1.3, bo
     _hip.synchronize()
     n, lin, row, col, frame = dev.results()
     if n > capacity:
-        return find_roi(image, sigma1, sigma2, boxsize, minval, capacity=n)
+        return _find_roi_call(flat, ysize, xsize, nframes, sigma1, sigma2, boxsize, minval, n)
     res = dev.res_bits().astype(np.float64).reshape((ysize, xsize, nframes), order="F")
     stack = np.zeros((boxsize, boxsize, n), dtype=np.float32)
     if n:

@@ -533,7 +533,12 @@ int bamf_gauss_mle_fit_host(const float* data, int sz, int64_t nfits, float sigm
  * one starting z per fit.  The model adds the ratio g to model and data: cf =
  * (data - model)/(model + g), df = (data + g)/(model + g)^2; like the reference,
  * the CRLB and log-likelihood use the ratio of the box's last pixel (ii = jj =
- * sz-1) for every pixel (the reference forms the ratio only in its Newton loop). */
+ * sz-1) for every pixel (the reference forms the ratio only in its Newton loop).
+ * Contract (the kernels do not check it; like the reference, they read the map
+ * at the caller's corners): every corner satisfies 0 <= x, 0 <= y,
+ * x + sz <= mapsz and y + sz <= mapsz, so each box lies on the map.
+ * bamf_gauss_mle_scmos_fit_host checks this on the host and refuses the call
+ * (BAMF_E_INVALID_ARG) otherwise. */
 int bamf_gauss_mle_scmos_xynb(const float* d_data, const float* d_coords, const float* d_gain_ratio,
                               float psf_sigma, int sz, int mapsz, int iterations, float* d_params,
                               float* d_crlb, float* d_ll, int nfits, void* stream);

@@ -34,3 +34,24 @@ def test_gauss_coeffs_host_matches_oracle(oracle):
         out = (ctypes.c_float * 5)()
         L.bamf_gauss_coeffs(s, out)
         assert list(out) == oracle.gauss_coeffs(s)
+
+
+def test_scmos_gateway_refuses_box_off_the_map():
+    """bamf_gauss_mle_scmos_fit_host checks every box corner against the gain map
+    on the host before any device work (the kernels read the map at the caller's
+    corners)."""
+    import numpy as np
+
+    L = _lib.lib()
+    sz, mapsz, n = 7, 16, 3
+    data = np.zeros((n, sz, sz), np.float32)
+    gain = np.ones((mapsz, mapsz), np.float32)
+    out = [np.zeros(4 * n, np.float32), np.zeros(4 * n, np.float32), np.zeros(n, np.float32)]
+    for bad in ([10.0, 0.0], [0.0, 9.5], [-1.0, 2.0], [3.0, -0.5]):
+        coords = np.array([[0, 0], [9, 9], bad], np.float32)
+        rc = L.bamf_gauss_mle_scmos_fit_host(data.ctypes.data, sz, n, coords.ctypes.data, gain.ctypes.data, mapsz,
+                                             1.3, 20, 0, None, None, out[0].ctypes.data, out[1].ctypes.data,
+                                             out[2].ctypes.data, None)
+        assert rc == -1, bad
+        assert b"not on the" in L.bamf_last_error()
+

@@ -79,6 +79,8 @@ def test_find_roi_pipeline_bit_exact(oracle, nframes, size, box, seed):
     (1, 64, 96, 17),    # wide window: the reference's two max passes
     (1, 400, 50, 4),    # column checkpoints in 64-column work-groups
     (1, 1100, 40, 2),   # more column chunks than the checkpointed column pass holds
+    (1, 40, 96, 5),     # whole row blocks unguarded, then a guarded partial block (row_begin > 0)
+    (3, 50, 64, 7),     # the same over several frames
 ])
 def test_find_roi_pipeline_shapes(oracle, nframes, ysize, xsize, box):
     side = max(ysize, xsize)
@@ -121,3 +123,43 @@ def test_find_roi_matlab_mirror(oracle):
     for b in range(len(lin_o)):
         r, c, f = row_o[b] - 1, col_o[b] - 1, fr_o[b] - 1
         assert np.array_equal(stack[:, :, b], img[r:r + 7, c:c + 7, f])
+
+
+def test_find_roi_mirror_frame_chunks(oracle):
+    """The MATLAB mirror in runs of 3 frames equals one call over all 7 frames."""
+    frames = synth.make_frames(7, 48, seed=21)
+    img = np.asfortranarray(np.transpose(frames, (2, 1, 0)))
+    one = find_roi(img, 1.3, 2.6, 7)
+    runs = find_roi(img, 1.3, 2.6, 7, frames_per_call=3)
+    for k in ("XBoxCorner", "YBoxCorner", "FrameNum"):
+        assert np.array_equal(one[0][k], runs[0][k])
+    assert np.array_equal(one[1], runs[1]) and np.array_equal(one[2], runs[2])
+    _, lin_o, row_o, col_o, fr_o = oracle.find_roi(frames.ravel(), 48, 48, 7, 1.3, 2.6, 7, default_minval())
+    assert np.array_equal(runs[0]["FrameNum"], fr_o) and np.array_equal(runs[0]["YBoxCorner"], row_o)
+
+
+def test_find_roi_more_frames_than_one_call(oracle):
+    """65,600 frames of 16 x 16: more than one bamf_find_roi call holds, through
+    the mirror and the host gateway (findROI.m handles any frame count)."""
+    nf, side = 65600, 16
+    rng = np.random.default_rng(31)
+    frames = rng.poisson(10.0, (nf, side, side)).astype(np.float32)
+    hot = rng.integers(0, nf, 4000)
+    frames[hot, rng.integers(3, 13, 4000), rng.integers(3, 13, 4000)] += 400.0
+    mv = default_minval()
+    print(f"frames={nf} side={side}", flush=True)
+    _, lin_o, row_o, col_o, fr_o = oracle.find_roi(frames.ravel(), side, side, nf, 1.3, 2.6, 7, mv)
+    assert len(fr_o) > 0 and fr_o.max() > 65535
+    img = np.transpose(frames, (2, 1, 0))  # MATLAB (Y, X, Z) view of the same memory
+    smd, _, _ = find_roi(img, 1.3, 2.6, 7, mv)
+    assert np.array_equal(smd["FrameNum"], fr_o) and np.array_equal(smd["YBoxCorner"], row_o)
+    assert np.array_equal(smd["XBoxCorner"], col_o)
+    cap = len(fr_o) + 10
+    xc, yc, fn = (np.zeros(cap, np.int32) for _ in range(3))
+    count = np.zeros(1, np.int64)
+    check(lib().bamf_find_roi_host(np.ascontiguousarray(frames).ctypes.data, side, side, nf, 1.3, 2.6, 7, mv, cap,
+                                   xc.ctypes.data, yc.ctypes.data, fn.ctypes.data, None, None, count.ctypes.data),
+          "bamf_find_roi_host")
+    n = int(count[0])
+    assert n == len(fr_o)
+    assert np.array_equal(fn[:n], fr_o) and np.array_equal(yc[:n], row_o) and np.array_equal(xc[:n], col_o)
