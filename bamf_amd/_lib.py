@@ -70,6 +70,9 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bamf_find_roi.argtypes = [c_void_p, c_int, c_int, c_int, c_float, c_float, c_int, c_float,
                                 c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_int64, c_void_p, c_void_p]
+    L.bamf_find_roi_noalloc.argtypes = [c_void_p, c_int, c_int, c_int, c_float, c_float, c_int, c_float,
+                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p,
+                                        c_void_p]
     L.bamf_find_roi_crop.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int64, c_int,
                                      c_void_p, c_void_p]
     L.bamf_fit_keep.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_float, c_float, c_float, c_void_p,

@@ -26,6 +26,7 @@
 //                              (per-block popcount, scan, ordered emit) plus the
 //                              box corners of findROI.m:130-148.
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 #include "common.hpp"
@@ -874,18 +875,31 @@ size_t bamf_find_roi_workspace_bytes(int xsize, int ysize, int nframes) {
            align256(col_ck_bytes(xsize, ysize, nframes));
 }
 
-int bamf_find_roi(const float* d_stack, int xsize, int ysize, int nframes, float sigma1, float sigma2,
-                  int boxsize, float minval, void* d_workspace, size_t workspace_bytes, uint32_t* d_res_bits,
-                  int64_t* d_lin_idx, int32_t* d_start_row, int32_t* d_start_col, int32_t* d_frame,
-                  int64_t capacity, int64_t* d_count, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// Argument checks shared by both entry points (nothing is launched on failure).
+int check_find_roi_args(const float* d_stack, int xsize, int ysize, int nframes, int boxsize, int64_t capacity,
+                        const int64_t* d_count) {
     BAMF_CHECK_ARG(xsize > 0 && ysize > 0 && nframes >= 0, "bad sizes");
     BAMF_CHECK_ARG(boxsize >= 0, "boxsize must be >= 0");
     BAMF_CHECK_ARG(capacity >= 0, "capacity must be >= 0");
     BAMF_CHECK_ARG(d_count, "d_count is required");
     BAMF_CHECK_ARG(nframes <= 65535, "nframes must be <= 65535 per call");
-    const size_t need = bamf_find_roi_workspace_bytes(xsize, ysize, nframes);
-    BAMF_CHECK_ARG(d_workspace && workspace_bytes >= need, "workspace too small (%zu < %zu)", workspace_bytes, need);
     BAMF_CHECK_ARG(d_stack, "null stack");
+    return BAMF_OK;
+}
+
+// The pipeline of bamf_find_roi on a workspace of at least
+// bamf_find_roi_workspace_bytes(xsize, ysize, nframes) bytes.  Every
+// workspace byte that is read is written earlier in the same call (the two
+// filtered stacks and the checkpoints by the row and column passes, the bitmap
+// by k_fill_u32, the block counts by k_bits_count), so the workspace's prior
+// contents never matter.
+int find_roi_on(const float* d_stack, int xsize, int ysize, int nframes, float sigma1, float sigma2, int boxsize,
+                float minval, void* d_workspace, uint32_t* d_res_bits, int64_t* d_lin_idx, int32_t* d_start_row,
+                int32_t* d_start_col, int32_t* d_frame, int64_t capacity, int64_t* d_count, void* stream) {
     hipStream_t s = as_stream(stream);
     const int64_t n = (int64_t)xsize * ysize * nframes;
     const int64_t nw = nwords_of(n), nb = nblocks_of(nw);
@@ -943,6 +957,67 @@ int bamf_find_roi(const float* d_stack, int xsize, int ysize, int nframes, float
                        ysize, boxsize, capacity, d_lin_idx, d_start_row, d_start_col, d_frame);
     BAMF_LAUNCH_CHECK("k_bits_emit");
     return BAMF_OK;
+}
+
+// Workspace of bamf_find_roi_noalloc: one copy per device, held from the
+// library's first use on that device until the process ends.  Host code never
+// passes this name to a kernel: on the host it is only a handle for
+// hipGetSymbolAddress, which gives each device's own address of its copy.
+constexpr size_t kNoAllocScratchBytes = (size_t)BAMF_FIND_ROI_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current device's scratch copy, looked up once
+// per device.
+int scratch_on_current_device(void** ptr) {
+    int dev = 0;
+    BAMF_HIP_TRY(hipGetDevice(&dev));
+    BAMF_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d this library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        BAMF_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        BAMF_CHECK_ARG(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return BAMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bamf_find_roi(const float* d_stack, int xsize, int ysize, int nframes, float sigma1, float sigma2,
+                  int boxsize, float minval, void* d_workspace, size_t workspace_bytes, uint32_t* d_res_bits,
+                  int64_t* d_lin_idx, int32_t* d_start_row, int32_t* d_start_col, int32_t* d_frame,
+                  int64_t capacity, int64_t* d_count, void* stream) {
+    int rc = check_find_roi_args(d_stack, xsize, ysize, nframes, boxsize, capacity, d_count);
+    if (rc) return rc;
+    const size_t need = bamf_find_roi_workspace_bytes(xsize, ysize, nframes);
+    BAMF_CHECK_ARG(d_workspace && workspace_bytes >= need, "workspace too small (%zu < %zu)", workspace_bytes, need);
+    return find_roi_on(d_stack, xsize, ysize, nframes, sigma1, sigma2, boxsize, minval, d_workspace, d_res_bits,
+                       d_lin_idx, d_start_row, d_start_col, d_frame, capacity, d_count, stream);
+}
+
+int bamf_find_roi_noalloc(const float* d_stack, int xsize, int ysize, int nframes, float sigma1, float sigma2,
+                          int boxsize, float minval, uint32_t* d_res_bits, int64_t* d_lin_idx,
+                          int32_t* d_start_row, int32_t* d_start_col, int32_t* d_frame, int64_t capacity,
+                          int64_t* d_count, void* stream) {
+    int rc = check_find_roi_args(d_stack, xsize, ysize, nframes, boxsize, capacity, d_count);
+    if (rc) return rc;
+    const size_t need = bamf_find_roi_workspace_bytes(xsize, ysize, nframes);
+    BAMF_CHECK_ARG(need <= kNoAllocScratchBytes,
+                   "stack too large for bamf_find_roi_noalloc: it needs %zu workspace bytes, the scratch holds %zu "
+                   "(use bamf_find_roi with a caller workspace, or fewer frames per call)",
+                   need, kNoAllocScratchBytes);
+    void* ws = nullptr;
+    if ((rc = scratch_on_current_device(&ws))) return rc;
+    return find_roi_on(d_stack, xsize, ysize, nframes, sigma1, sigma2, boxsize, minval, ws, d_res_bits, d_lin_idx,
+                       d_start_row, d_start_col, d_frame, capacity, d_count, stream);
 }
 
 int bamf_find_roi_crop(const float* d_stack, int xsize, int ysize, const int32_t* d_start_row,

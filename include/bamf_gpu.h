@@ -108,6 +108,22 @@ int bamf_find_roi(const float* d_stack, int xsize, int ysize, int nframes,
                   void* d_workspace, size_t workspace_bytes, uint32_t* d_res_bits,
                   int64_t* d_lin_idx, int32_t* d_start_row, int32_t* d_start_col,
                   int32_t* d_frame, int64_t capacity, int64_t* d_count, void* stream);
+/* bamf_find_roi without a workspace argument.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The workspace is a library-owned scratch array of
+ * BAMF_FIND_ROI_NOALLOC_SCRATCH_BYTES per device, held from the first call on
+ * that device until the process ends; the call allocates nothing.  Stacks
+ * whose bamf_find_roi_workspace_bytes() exceeds it are refused with
+ * BAMF_E_INVALID_ARG (64 MiB: up to 115 frames of 256 x 256 per call).  Every
+ * call on one device shares that device's scratch: calls that overlap on one
+ * GPU (other streams, other threads) race and must be serialised by the
+ * caller.  Arguments and results are those of bamf_find_roi. */
+#define BAMF_FIND_ROI_NOALLOC_SCRATCH_BYTES (64u << 20)
+int bamf_find_roi_noalloc(const float* d_stack, int xsize, int ysize, int nframes,
+                          float sigma1, float sigma2, int boxsize, float minval, uint32_t* d_res_bits,
+                          int64_t* d_lin_idx, int32_t* d_start_row, int32_t* d_start_col,
+                          int32_t* d_frame, int64_t capacity, int64_t* d_count, void* stream);
 /* ROIStack crop (findROI.m:157-160): box b is boxsize x boxsize, column-major
  * (row r, col c at r + boxsize*c), cut from frame d_frame[b] at the 1-based corner. */
 int bamf_find_roi_crop(const float* d_stack, int xsize, int ysize,

@@ -55,3 +55,9 @@ def test_scmos_gateway_refuses_box_off_the_map():
         assert rc == -1, bad
         assert b"not on the" in L.bamf_last_error()
 
+
+def test_find_roi_noalloc_refuses_before_device_work():
+    L = _lib.lib()
+    assert L.bamf_find_roi_workspace_bytes(256, 256, 115) <= 64 << 20 < L.bamf_find_roi_workspace_bytes(256, 256, 116)
+    rc = L.bamf_find_roi_noalloc(None, 8, 8, 1, 1.3, 2.6, 7, 1.0, None, None, None, None, None, 0, None, None)
+    assert rc == -1
