@@ -87,4 +87,21 @@ void  orc_gsplit_map(int np, const float* I, const float* X, const float* Y, con
                      float u3, float* I1, float* X1, float* Y1, float* In, float* Xn, float* Yn, float* I0);
 void  orc_gmerge_map(int np, const float* I, const float* X, const float* Y, int jj, const int* idx, float* I1,
                      float* X1, float* Y1, float* U1, float* U2, float* U3, float* I0);
+void  orc_split_map(float I, float mux, float muy, float u1, float u2, float u3, float* out6);
+void  orc_merge_map(float I1, float X1, float Y1, float I2, float X2, float Y2, float* out6);
+/* Planted mistakes, for the oracle's pinning tests only (see rjmcmc.c). */
+enum {
+    ORC_SLIP_NONE = 0,
+    ORC_SLIP_SPLIT_JACOB_POW1 = 1,   /* split: I / (1 - u1) in place of I / (1 - u1)^2 */
+    ORC_SLIP_SPLIT_P_INVERTED = 2,   /* split: P[2] / P[1] */
+    ORC_SLIP_SPLIT_PRM_N = 3,        /* split: MeanN / NEm in place of MeanN / (NEm + 1) */
+    ORC_SLIP_SPLIT_NO_AREA = 4,      /* split: the 1 / Area position prior dropped */
+    ORC_SLIP_MERGE_JACOB_POW3 = 5,   /* merge: Ijp / (1 - u1)^3 */
+    ORC_SLIP_MERGE_PRM_N1 = 6,       /* merge: MeanN / (NEm + 1) in place of MeanN / NEm */
+    ORC_SLIP_GSPLIT_JACOB_NP = 7,    /* gSplit: (1 - u3)^Np in place of ^(Np + 1) */
+    ORC_SLIP_GSPLIT_US_NO_NP = 8,    /* gSplit: the 1 / Np density of u3 dropped */
+    ORC_SLIP_GMERGE_P_INVERTED = 9,  /* gMerge: P[6] / P[5] */
+    ORC_SLIP_BACKFORE_SWAPPED = 10   /* backFore: the two pdfs swapped */
+};
+void  orc_set_slip(int slip);
 #endif

@@ -115,6 +115,12 @@ def _declare(L):
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.orc_gmerge_map.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p, c_void_p]
+    L.orc_split_map.argtypes = [c_float] * 6 + [c_void_p]
+    L.orc_split_map.restype = None
+    L.orc_merge_map.argtypes = [c_float] * 6 + [c_void_p]
+    L.orc_merge_map.restype = None
+    L.orc_set_slip.argtypes = [c_int]
+    L.orc_set_slip.restype = None
 
 
 # ---------------------------------------------------------------------------
@@ -231,6 +237,41 @@ def model(I, X, Y, bg, abg, bbg, xsize, ysize, sigma, scmos=None):
     lib().orc_model(len(I), I.ctypes.data, X.ctypes.data, Y.ctypes.data, bg, abg, bbg, xsize, ysize, sigma,
                     np.ascontiguousarray(scmos, np.float32).ctypes.data, m.ctypes.data)
     return m
+
+
+def split_map(I, mux, muy, u1, u2, u3):
+    """split's map: (I1, X1, Y1, I2, X2, Y2) as float32."""
+    out = np.zeros(6, np.float32)
+    lib().orc_split_map(I, mux, muy, u1, u2, u3, out.ctypes.data)
+    return out
+
+
+def merge_map(I1, X1, Y1, I2, X2, Y2):
+    """merge's map: (I, mux, muy, u1, u2, u3) as float32."""
+    out = np.zeros(6, np.float32)
+    lib().orc_merge_map(I1, X1, Y1, I2, X2, Y2, out.ctypes.data)
+    return out
+
+
+SLIPS = {"split_jacob_pow1": 1, "split_p_inverted": 2, "split_prm_n": 3, "split_no_area": 4,
+         "merge_jacob_pow3": 5, "merge_prm_n1": 6, "gsplit_jacob_np": 7, "gsplit_us_no_np": 8,
+         "gmerge_p_inverted": 9, "backfore_swapped": 10}
+
+
+class planted_slip:
+    """Context manager that plants one named mistake in the oracle's ratios
+    (for the oracle's own pinning tests: the check must catch it)."""
+
+    def __init__(self, name):
+        self.code = SLIPS[name]
+
+    def __enter__(self):
+        lib().orc_set_slip(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_slip(0)
+        return False
 
 
 def gsplit_map(I, X, Y, idx, u1, u2, u3):

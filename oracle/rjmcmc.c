@@ -17,9 +17,13 @@
  *  - no FMA contraction.
  * OptModel 1 (orc_rjmcmc_psf): NumModel (LibRJMCMC.cpp:62-170) in place of
  * Model, pinned by closed forms of the interpolation (tests/test_oracle_numpsf.py).
- * gSplit/gMerge (LibRJMCMC.cpp:1389-1584): their proposal maps are pinned
- * (numerical Jacobian, split-then-merge round trip); the assembly of their
- * posterior ratios is parity unpinned, like split/merge's.
+ * split/merge, gSplit/gMerge and backFore (LibRJMCMC.cpp:972-1255,
+ * 1389-1627) are pinned by tests/test_oracle_split_merge.py and
+ * tests/test_oracle_gsplit.py: numerical Jacobians of the proposal maps,
+ * split-then-merge round trips whose log ratios cancel (the group pair to the
+ * reference's (MeanN/(N+1))^2, its gMerge prior on N being inverted), split's
+ * and merge's ratios assembled from independent pieces, the backFore label
+ * law, and planted mistakes (orc_set_slip) that each of these checks catches.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -381,6 +385,38 @@ static void accept_state(ch_t* c) {
 }
 static double lg(float v) { return log((double)v); }
 
+/* Planted mistakes for the oracle's own pinning tests (0 = none): each one
+ * changes one factor of one ratio, so a test can show that its check catches
+ * that mistake.  Never set outside those tests. */
+static int g_slip = 0;
+void orc_set_slip(int slip) { g_slip = slip; }
+
+/* split's proposal map (LibRJMCMC.cpp:1006-1021): emitter (I, mux, muy) with
+ * (u1, u2, u3) -> out = {I1, X1, Y1} (the split emitter in place) and
+ * {I2, X2, Y2} (the added one). */
+void orc_split_map(float I, float mux, float muy, float u1, float u2, float u3, float* out) {
+    out[0] = u1 * I;
+    out[1] = mux + u2;
+    out[2] = muy + u3;
+    out[3] = (1 - u1) * I;
+    out[4] = mux - u1 * u2 / (1 - u1);
+    out[5] = muy - u1 * u3 / (1 - u1);
+}
+
+/* merge's map (LibRJMCMC.cpp:1141-1150): emitter jj (I1, X1, Y1) absorbs
+ * ii (I2, X2, Y2) -> out = {I, mux, muy, u1, u2, u3}. */
+void orc_merge_map(float I1, float X1, float Y1, float I2, float X2, float Y2, float* out) {
+    float Ijp = I1 + I2;
+    float muxjp = (I1 * X1 + I2 * X2) / Ijp;
+    float muyjp = (I1 * Y1 + I2 * Y2) / Ijp;
+    out[0] = Ijp;
+    out[1] = muxjp;
+    out[2] = muyjp;
+    out[3] = I1 / Ijp;
+    out[4] = X1 - muxjp;
+    out[5] = Y1 - muyjp;
+}
+
 /* ---------------- jumps ---------------- */
 /* LibRJMCMC.cpp:752-970 */
 static void do_move(ch_t* c) {
@@ -527,10 +563,12 @@ static void do_split(ch_t* c) {
     float u1 = urand(c);
     float u2 = p->split_std * randn(c);
     float u3 = p->split_std * randn(c);
-    T->I[jj] = u1 * Ijp;
-    T->X[jj] = mux + u2;
-    T->Y[jj] = muy + u3;
-    if (!em_add(c, T, (1 - u1) * Ijp, mux - u1 * u2 / (1 - u1), muy - u1 * u3 / (1 - u1), BF)) return;
+    float sm[6];
+    orc_split_map(Ijp, mux, muy, u1, u2, u3, sm);
+    T->I[jj] = sm[0];
+    T->X[jj] = sm[1];
+    T->Y[jj] = sm[2];
+    if (!em_add(c, T, sm[3], sm[4], sm[5], BF)) return;
     if (!em_test(T, (int)p->bnd_out, c->X, c->Y)) {
         c->st->flags |= 8;
         record(c, T, LLR, PR, JumpType, Accept);
@@ -553,6 +591,10 @@ static void do_split(ch_t* c) {
     double om = (double)(1 - u1);
     Jacob = (float)((double)Ijp / (om * om));
     float Pr12 = c->P[1] / c->P[2];
+    if (g_slip == ORC_SLIP_SPLIT_JACOB_POW1) Jacob = (float)((double)Ijp / om);
+    if (g_slip == ORC_SLIP_SPLIT_P_INVERTED) Pr12 = c->P[2] / c->P[1];
+    if (g_slip == ORC_SLIP_SPLIT_PRM_N) PrM = c->meanN / NEm;
+    if (g_slip == ORC_SLIP_SPLIT_NO_AREA) PrX = 1;
     PR = LLR * PrI * PrX * Us * Jacob * Pr12 * PrM;
     c->st->log_pr = (double)sum + lg(PrI) + lg(PrX) + lg(Us) + lg(Jacob) + lg(Pr12) + lg(PrM);
     Accept = (urand(c) < PR);
@@ -581,12 +623,9 @@ static void do_merge(ch_t* c) {
     int ii = em_closest(c, E, jj, 3 * p->psf_sigma);
     if (E->sig[jj] == 1) cnt[C_NFG + J_MERGE]++; else cnt[C_NBG + J_MERGE]++;
     if (em_distance(E, ii, jj) > 3 * p->psf_sigma) { update_grids(c); return; }
-    float Ijp = E->I[jj] + E->I[ii];
-    float muxjp = (E->I[jj] * E->X[jj] + E->I[ii] * E->X[ii]) / Ijp;
-    float muyjp = (E->I[jj] * E->Y[jj] + E->I[ii] * E->Y[ii]) / Ijp;
-    float u1 = E->I[jj] / Ijp;
-    float u2 = E->X[jj] - muxjp;
-    float u3 = E->Y[jj] - muyjp;
+    float mm[6];
+    orc_merge_map(E->I[jj], E->X[jj], E->Y[jj], E->I[ii], E->X[ii], E->Y[ii], mm);
+    float Ijp = mm[0], muxjp = mm[1], muyjp = mm[2], u1 = mm[3], u2 = mm[4], u3 = mm[5];
     T->I[jj] = Ijp;
     T->X[jj] = muxjp;
     T->Y[jj] = muyjp;
@@ -617,6 +656,8 @@ static void do_merge(ch_t* c) {
     double om = (double)(1 - u1);
     Jacob = (float)((double)Ijp / (om * om));
     PrM = c->meanN / NEm;
+    if (g_slip == ORC_SLIP_MERGE_JACOB_POW3) Jacob = (float)((double)Ijp / (om * om * om));
+    if (g_slip == ORC_SLIP_MERGE_PRM_N1) PrM = c->meanN / (NEm + 1);
     PR = LLR / (PrI * PrX * Jacob * Us * PrM * c->P[1] / c->P[2]);
     c->st->log_pr = (double)sum - (lg(PrI) + lg(PrX) + lg(Jacob) + lg(Us) + lg(PrM) + lg(c->P[1]) - lg(c->P[2]));
     Accept = (urand(c) < PR);
@@ -729,6 +770,7 @@ static void do_backfore(ch_t* c) {
     int nn = imin((int)floorf((float)E->n * Ua), E->n - 1);
     float BackP = findpdf(&c->back, E->I[nn]);
     float SignalP = findpdf(&c->sig, E->I[nn]);
+    if (g_slip == ORC_SLIP_BACKFORE_SWAPPED) { float t = BackP; BackP = SignalP; SignalP = t; }
     if (E->sig[nn] == 1) {
         if (SignalP != 0) { if (BackP / SignalP > urand(c)) E->sig[nn] = 0; }
         else E->sig[nn] = 0;
@@ -827,6 +869,8 @@ static void do_gsplit(ch_t* c) {
     PrX = 1 / Area;
     Jacob = I0 / powf(1 - u3, (float)(np + 1));
     float P56 = c->P[5] / c->P[6];
+    if (g_slip == ORC_SLIP_GSPLIT_JACOB_NP) Jacob = I0 / powf(1 - u3, (float)np);
+    if (g_slip == ORC_SLIP_GSPLIT_US_NO_NP) Us = 1 / (PDFu1 * PDFu2);
     PR = LLR * PrI * PrX * Us * Jacob * PrM * P56;
     c->st->log_pr = (double)sum + lg(PrI) + lg(PrX) + lg(Us) + lg(Jacob) + lg(PrM) + lg(P56);
     Accept = (urand(c) < PR);
@@ -872,6 +916,7 @@ static void do_gmerge(ch_t* c) {
     PrX = 1 / Area;
     Jacob = I0 / powf(1 - U3, (float)(np + 1));
     float P56 = c->P[5] / c->P[6];
+    if (g_slip == ORC_SLIP_GMERGE_P_INVERTED) P56 = c->P[6] / c->P[5];
     PR = LLR / (PrI * PrX * Us * Jacob * PrM * P56);
     c->st->log_pr = (double)sum - (lg(PrI) + lg(PrX) + lg(Us) + lg(Jacob) + lg(PrM) + lg(P56));
     if (PDFu1 == 0 || PDFu2 == 0) {

@@ -11,7 +11,8 @@
     also in the trials phase), the draw counts, the accept rule, N and the
     conserved total intensity of accepted group jumps, and the counters that
     these jumps leave alone.
-The assembly of the posterior ratios from these pieces is parity unpinned.
+The assembly of the posterior ratios from these pieces is pinned by the
+gSplit-then-gMerge ratio product in tests/test_oracle_split_merge.py.
 """
 import numpy as np
 import pytest
