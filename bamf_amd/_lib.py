@@ -91,6 +91,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.bamf_subregion_pixels.restype = c_int64
     L.bamf_make_subregions.argtypes = [c_void_p, c_int, c_int, c_int, c_int, c_int] + [c_void_p] * 7 + \
         [c_int, c_void_p, c_void_p, c_void_p]
+    L.bamf_synth_cells.argtypes = [c_int, c_int]
+    L.bamf_synth_cells.restype = c_int64
+    L.bamf_synth_frames.argtypes = [c_int, c_int, c_int] + [c_float] * 5 + [c_uint64, c_int, c_int] + \
+        [c_void_p] * 4
+    L.bamf_synth_roi_batch.argtypes = [c_int, c_int] + [c_float] * 7 + [c_uint64, c_int] + [c_void_p] * 6
     if hasattr(L, "bamf_rjmcmc_plan_create"):
         from . import rjmcmc_abi
 

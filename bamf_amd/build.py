@@ -18,7 +18,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libbamf_gpu.so"
-HIP_SOURCES = ["runtime.cpp", "mle.hip", "findroi.hip", "rjmcmc.hip", "rjmcmc_host.cpp", "net.hip", "gateway.cpp", "findmap.hip", "tiles.hip", "priors.hip", "psf.hip"]
+HIP_SOURCES = ["runtime.cpp", "mle.hip", "findroi.hip", "rjmcmc.hip", "rjmcmc_host.cpp", "net.hip", "gateway.cpp", "findmap.hip", "tiles.hip", "priors.hip", "psf.hip", "synth.hip"]
 ARCH = os.environ.get("BAMF_OFFLOAD_ARCH", "gfx950")
 
 

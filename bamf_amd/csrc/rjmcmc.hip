@@ -33,6 +33,7 @@
 
 #include "common.hpp"
 #include "erf_fast.hpp"
+#include "philox.hpp"
 #include "rjmcmc_dev.hpp"
 
 #pragma clang fp contract(off)
@@ -104,26 +105,6 @@ __device__ __forceinline__ float wave_scan(float v) {
 __device__ __forceinline__ int kth_bit(uint64_t m, int k) {
     for (int i = 0; i < k; i++) m &= m - 1;
     return __builtin_ctzll(m);
-}
-
-__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
-                                       uint32_t k1, uint32_t* o) {
-#pragma unroll
-    for (int r = 0; r < 10; r++) {
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
-        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-        c0 = n0;
-        c1 = lo1;
-        c2 = n2;
-        c3 = lo0;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    o[0] = c0;
-    o[1] = c1;
-    o[2] = c2;
-    o[3] = c3;
 }
 
 // LibRJMCMCreturn.hpp:28
@@ -230,6 +211,7 @@ struct Chain {
     double logpost;
     int ndelta;
     bool bgdelta;
+    float cbg_pdf, tbg_pdf;  // offset-prior pdf at the current / proposed BG
     int sxa, sxb;      // x range of the last stencil's nonzero columns (sxa > sxb: none)
     int lo_s, hi_s;    // pixel slots the proposal changes
     bool e_ok;         // the current set passes Emitters::test
@@ -296,9 +278,16 @@ struct Chain {
             w = (c == 0 ? o[0] : c == 1 ? o[1] : c == 2 ? o[2] : o[3]) >> 1;
         }
         wu = (float)w / 2147483648.0f;
-        float t = wu;
+        // lane l: ((u_l + u_{l+1}) + u_{l+2}) + ... + u_{l+11}, the reference's
+        // left-to-right order; each term arrives by a one-lane wavefront shift
+        // (DPP wave_shl:1, lane l reads lane l + 1).  Lanes above 52 sum
+        // shifted-in zeros and are never read (randn() takes positions <= 52).
+        float t = wu, v = wu;
 #pragma unroll
-        for (int i = 1; i < 12; i++) t = t + __shfl_down(wu, i);
+        for (int i = 1; i < 12; i++) {
+            v = dpp_f<0x130, 0xf>(v);
+            t = t + v;
+        }
         wr = t;
     }
     // Re-centres the window when fewer than n positions remain in it.
@@ -1013,10 +1002,18 @@ struct Chain {
     }
     // LibRJMCMC.cpp:918-936
     __device__ __forceinline__ float fin_move(float sum, float LLR) {
-        const float BgPDF = findpdf(a->off, a->p.dx, cBG);
-        const float BgPDFtest = findpdf(a->off, a->p.dx, tBG);
-        const float PrBg = (BgPDFtest / BgPDF) * expf(-(tBBG * tBBG - cBBG * cBBG) / 2) *
-                           expf(-(tABG * tABG - cABG * cABG) / 2);
+        const float BgPDF = cbg_pdf;  // findpdf(OffsetStat, BG) of the current state
+        float PrBg;
+        if (bgdelta) {
+            const float BgPDFtest = findpdf(a->off, a->p.dx, tBG);
+            tbg_pdf = BgPDFtest;
+            PrBg = (BgPDFtest / BgPDF) * expf(-(tBBG * tBBG - cBBG * cBBG) / 2) *
+                   expf(-(tABG * tABG - cABG * cABG) / 2);
+        } else {
+            // BG, ABG, BBG unchanged: both exponentials are expf(-0) = 1, so the
+            // factor is BgPDF / BgPDF (1, or NaN for a zero pdf), bit for bit
+            PrBg = BgPDF / BgPDF;
+        }
         lp_prod = (double)f_pri * (double)PrBg;
         lp_neg = false;
         return LLR * f_pri * PrBg;
@@ -1357,7 +1354,8 @@ struct Chain {
         const float ur = urand();
         float pixprob = 1.0f;
         int pix;
-        const bool learned = LRN && a->mode == BAMF_RJ_LEARNED;
+        // learned births need the network's map; without it births are the reference's
+        const bool learned = LRN && a->mode == BAMF_RJ_LEARNED && a->prop_map != nullptr;
         if constexpr (LRN) {
             if (learned) pix = map_pixel(ur, pixprob);
             else pix = birth_pixel(ur);
@@ -1532,6 +1530,7 @@ struct Chain {
                     }
                     ci ^= 1;
                     cN = tN;
+                    if (bgdelta) cbg_pdf = tbg_pdf;  // only a background move changes BG
                     cBG = tBG;
                     cABG = tABG;
                     cBBG = tBBG;
@@ -1601,6 +1600,7 @@ struct Chain {
         e_ok = g.e_ok != 0;
         nacc = g.nacc;
         cBG = g.bg;
+        cbg_pdf = findpdf(a->off, a->p.dx, cBG);
         cABG = g.abg;
         cBBG = g.bbg;
         Em& c = L.e[0];
@@ -1685,6 +1685,7 @@ struct Chain {
             e_ok = false;  // initial emitters are checked in full by the first test
             const float bg0 = a->bg0 ? a->bg0[roi] : 0.0f;
             cBG = bg0 == 0 ? 1.0f : bg0;
+            cbg_pdf = findpdf(a->off, a->p.dx, cBG);
             cABG = 0;
             cBBG = 0;
             wsync();

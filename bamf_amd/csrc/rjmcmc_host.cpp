@@ -112,6 +112,7 @@ struct bamf_rjmcmc_plan {
     float l_single_from = 0.75f;  // learned mode's share of single-emitter moves among moves: 1 - this
     const float* d_q = nullptr;   // learned proposal maps (caller-owned)
     const float* d_mu = nullptr;
+    bool proposal_set = false;    // bamf_rjmcmc_set_proposal was called (maps may be NULL)
     float l_alpha = 1.0f, l_halfwidth = 0.5f, l_beta = 0.0f;
     int device = 0;
     int cu = 0;
@@ -333,13 +334,15 @@ int bamf_math_erf(const float* d_x, float* d_y, int64_t n, void* stream) {
 
 int bamf_rjmcmc_set_proposal(bamf_rjmcmc_plan* plan, const float* d_q, const float* d_mu, float alpha,
                              float halfwidth, float beta) {
-    BAMF_CHECK_ARG(plan && d_q && d_mu, "null argument");
+    BAMF_CHECK_ARG(plan, "null argument");
+    BAMF_CHECK_ARG((d_q == nullptr) == (d_mu == nullptr), "give both maps or neither");
     BAMF_CHECK_ARG(alpha >= 0.0f && alpha <= 1.0f, "alpha must be in [0, 1]");
     BAMF_CHECK_ARG(halfwidth > 0.0f, "halfwidth must be > 0");
     BAMF_CHECK_ARG(beta >= 0.0f && beta <= 1.0f, "beta must be in [0, 1]");
     BAMF_CHECK_ARG(plan->sig_integral > 0.0, "the signal pdf table integrates to 0");
     plan->d_q = d_q;
     plan->d_mu = d_mu;
+    plan->proposal_set = true;
     plan->l_alpha = alpha;
     plan->l_halfwidth = halfwidth;
     plan->l_beta = beta;
@@ -351,6 +354,7 @@ int bamf_rjmcmc_set_intensity_proposal(bamf_rjmcmc_plan* plan, int kind, float i
     BAMF_CHECK_ARG(kind == 0 || kind == 1, "kind must be 0 (network window) or 1 (conditional fit)");
     BAMF_CHECK_ARG(inflation > 0.0f, "inflation must be > 0");
     BAMF_CHECK_ARG(single_share >= 0.25f && single_share <= 0.5f, "single_share must be in [0.25, 0.5]");
+    BAMF_CHECK_ARG(kind == 1 || plan->d_mu, "kind 0 draws from the network's window: set the maps first");
     plan->l_ikind = kind;
     plan->c_infl = inflation;
     plan->l_single_from = 1.0f - single_share;
@@ -405,7 +409,7 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     BAMF_CHECK_ARG(mode == BAMF_RJ_PLAIN || mode == BAMF_RJ_REPLAY || mode == BAMF_RJ_LEARNED, "unknown mode %d",
                    mode);
     if (mode == BAMF_RJ_LEARNED) {
-        BAMF_CHECK_ARG(plan->d_q && plan->d_mu, "learned mode needs bamf_rjmcmc_set_proposal first");
+        BAMF_CHECK_ARG(plan->proposal_set, "learned mode needs bamf_rjmcmc_set_proposal first");
         BAMF_CHECK_ARG(plan->params.is_backg == 0, "learned mode supports IsBackg = 0 only");
         BAMF_CHECK_ARG(!plan->d_psf, "learned mode supports the erf PSF model only");
     }
@@ -485,6 +489,8 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     a.rec_cap = out->rec_cap;
     a.em_cap = out->em_cap;
     if (mode == BAMF_RJ_LEARNED) {
+        BAMF_CHECK_ARG(plan->d_mu || plan->l_ikind == 1 || plan->l_beta == 0.0f,
+                       "the network-window intensity proposal needs the maps (bamf_rjmcmc_set_proposal)");
         a.prop_map = plan->d_q;
         a.prop_mu = plan->d_mu;
         a.l_alpha = plan->l_alpha;

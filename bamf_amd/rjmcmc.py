@@ -83,7 +83,8 @@ class Plan:
 
     def set_proposal(self, d_q, d_mu, alpha: float = 0.2, halfwidth: float = 0.35, beta: float = 0.5) -> None:
         """Learned proposal maps (device buffers in the batch's pixel layout)."""
-        check(lib().bamf_rjmcmc_set_proposal(self.handle, d_q.vp, d_mu.vp, ctypes.c_float(alpha),
+        check(lib().bamf_rjmcmc_set_proposal(self.handle, d_q.vp if d_q is not None else None,
+                                             d_mu.vp if d_mu is not None else None, ctypes.c_float(alpha),
                                              ctypes.c_float(halfwidth), ctypes.c_float(beta)),
               "bamf_rjmcmc_set_proposal")
         self._proposal = (d_q, d_mu)  # keep the buffers alive while the plan uses them

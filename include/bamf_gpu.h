@@ -624,6 +624,42 @@ int64_t bamf_psf_workspace_bytes(int64_t n);
 int bamf_find_psf_host(const float* roi_stack, int box, int64_t n, float gain, float offset, float psf_sigma,
                        float* sampled_psf, int32_t psf_size[4], float* fit_x, float* fit_y);
 
+
+/* ------------------------------------------------------------------------
+ * Synthetic data from the forward model on the device (no counterpart in the
+ * reference, whose examples load recorded data): the frames and ROIs the
+ * benchmarks and the proposal network's training use.  Pixel (y, x) covers
+ * [x, x+1) x [y, y+1) and has the expected count
+ *   bg + sum_e I_e dPhi_x(e) dPhi_y(e)  (the pixel-integrated Gaussian of
+ * LibRJMCMC.cpp:15-59), observed as a Poisson count (noise = 1) or as the
+ * expected count itself (noise = 0).  Emitters form a Poisson point process
+ * of `density` per pixel^2 with Gamma(gamma_k, gamma_theta) intensities.
+ * Deterministic for a given seed.
+ *
+ * bamf_synth_frames: nframes frames of ysize rows of xsize contiguous pixels
+ *   (the FindROI kernels' layout).  The process is drawn per 16 x 16 cell
+ *   (bamf_synth_cells(ysize, xsize) cells per frame, row-major over cells):
+ *   d_truth[((f * cells + c) * cap + k) * 3 + {0, 1, 2}] = x, y, I of emitter k
+ *   of cell c (frame coordinates), d_count[f * cells + c] = its emitters (a
+ *   Poisson count cut at cap, 1..128).  PSF tails beyond 16 pixels are
+ *   dropped.
+ * bamf_synth_roi_batch: n ROIs of side x side (1..32) in the ROI layout
+ *   [roi][x][y] (data[y + x * side]); per ROI a density ~ U[density_lo,
+ *   density_hi], N ~ Poisson(density side^2) (at most 128), uniform positions,
+ *   a background ~ U[bg_lo, bg_hi].  d_occ / d_logi (may be NULL): 1 at pixel
+ *   (floor(x + 1/2), floor(y + 1/2)) of each emitter and the log of the
+ *   intensity summed there (the proposal network's training targets);
+ *   d_nem: N per ROI; d_truth (may be NULL): per ROI 1 + 3 * 128 floats, the
+ *   background, then x, y, I per emitter.
+ * ------------------------------------------------------------------------ */
+int64_t bamf_synth_cells(int ysize, int xsize);
+int bamf_synth_frames(int nframes, int ysize, int xsize, float density, float gamma_k, float gamma_theta, float bg,
+                      float sigma, uint64_t seed, int cap, int noise, float* d_frames, float* d_truth,
+                      int32_t* d_count, void* stream);
+int bamf_synth_roi_batch(int n, int side, float density_lo, float density_hi, float bg_lo, float bg_hi,
+                         float gamma_k, float gamma_theta, float sigma, uint64_t seed, int noise, float* d_img,
+                         float* d_occ, float* d_logi, int32_t* d_nem, float* d_truth, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
