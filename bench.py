@@ -8,15 +8,19 @@ post-burn-in) with P_Burnin = [.4 .15 .15 .15 .15 0 0 0] and
 P_Trials = [.6 .1 .1 .1 .1 0 0 0].
 
 A step is one batched run of every chain of the workload (the persistent
-launches of both ROI size classes).  An ROI posterior is an ROI's
-post-burn-in chain at ESS = 200, the ESS being the smaller of the
-emitter-count and total-photon traces' ESS (Geyer).  value = ROI posteriors
-per second at that fixed ESS: each ROI's share of the timed step (inputs
-resident in HBM) is rescaled to the length its chain needs to reach ESS 200
-(burn-in kept, the rest scaled by 200 / ESS), and value = ROIs / total
-(bamf_amd.ess.fixed_ess_rate).  The strict count (ROIs reaching ESS 200
-within the fixed chain length) and the summed-ESS rate are reported too, as
-is the learned-proposal chain on the same ROIs.  The CPU baseline is the oracle chain (a C restatement of the
+launches of both ROI size classes).  An ROI posterior is an ROI whose
+post-burn-in chain reaches ESS >= 200, the ESS being the smaller of the
+emitter-count and total-photon traces' ESS (Geyer's initial monotone
+sequence estimator).  value = such ROIs per second: the number of ROIs whose
+chain reaches ESS 200 within the configured length, over the median step
+time (HIP events around each step on the launch stream, inputs resident in
+HBM).  Reported beside it: the fixed-ESS rate extrapolated to every ROI
+(each ROI's share of the step rescaled to the length its chain would need,
+bamf_amd.ess.fixed_ess_rate), the summed-ESS rate, the learned-proposal
+chain on the same ROIs, and the posterior agreement of both chains with the
+long oracle reference chains of tests/golden/posterior_batched_uneven.npz
+(emitter-count KS p and localization RMSE against the oracle's).  The CPU
+baseline is the oracle chain (a C restatement of the
 reference's RJMCMC class) on a sample of the same ROIs over all host cores,
 timed before the GPU is touched.
 """
@@ -118,7 +122,8 @@ def cpu_baseline(rois, p, off, sig, back, bg0, sample, cores, thin, psf=None):
     # each ROI's share of the wall time: its own chain time over the worker count
     share = np.array([r[3] for r in res]) * (dt / sum(r[3] for r in res))
     burn = p.n_burnin / (p.n_burnin + p.n_trials)
-    return {"value": fixed_ess_rate(ess, share, burn), "unit": "ROI posteriors/s", "cores": cores, "kind": "port",
+    return {"value": float(ok.sum() / dt), "unit": "ROI posteriors/s", "cores": cores, "kind": "port",
+            "extrapolated_fixed_ess_rate": fixed_ess_rate(ess, share, burn),
             "sample": f"{sample} of the workload's ROIs (seeded random subset), full {p.n_burnin}+{p.n_trials}-jump "
                       f"chains of the oracle C chain, {cores} forked worker processes, {dt:.1f} s wall",
             "chains_per_s": float(sample / dt), "jumps_per_s": float(jumps / dt),
@@ -127,6 +132,44 @@ def cpu_baseline(rois, p, off, sig, back, bg0, sample, cores, thin, psf=None):
 
 
 # ---------------------------------------------------------------- accuracy
+POSTERIOR_REF = ROOT / "tests" / "golden" / "posterior_batched_uneven.npz"
+
+
+def posterior_agreement(mode, prm=None, k=1024, cap=96):
+    """The chain's posterior against the long oracle reference chains: k
+    independent chains per reference ROI of the reference's length, final
+    states as samples; emitter-count KS p-values and the localization RMSE
+    against the oracle's (bamf_amd.posterior)."""
+    from bamf_amd import _hip, posterior, proposal
+    from bamf_amd.rjmcmc import LEARNED, run_batch
+
+    if not POSTERIOR_REF.exists():
+        return None
+    ref = posterior.load_reference(POSTERIOR_REF)
+    plan = posterior.reference_plan(ref)
+    rois = posterior.replicated_batch(ref, k)
+    keep = None
+    if mode == LEARNED:
+        mm = proposal.MapMaker(prm)
+        dq, dmu = _hip.DeviceBuffer(rois.total_pixels * 4), _hip.DeviceBuffer(rois.total_pixels * 4)
+        mm.run(rois, dq, dmu, 0.05)
+        _hip.synchronize()
+        mm.close()
+        plan.set_proposal(dq, dmu, alpha=0.2, halfwidth=0.35, beta=1.0)
+        plan.set_intensity_proposal("fit", 1.3, 0.5)
+        keep = (dq, dmu)
+    res = run_batch(plan, rois, seed=SEED + 77, mode=mode, final_cap=cap, hist_len=128)
+    assert np.all(res.status == 0)
+    ns, pos = posterior.final_samples(res.final, len(ref["side"]), k, cap)
+    a = posterior.agreement(ref, ns, pos)
+    del keep
+    return {"reference": f"tests/golden/posterior_batched_uneven.npz (10 ROIs of this workload, "
+                         f"{int(ref['chains'])} oracle chains of {int(ref['n_burnin']) + int(ref['n_trials'])} jumps "
+                         f"per ROI)", "chains_per_roi": k, "ks_p_min": min(a["ks_p"]),
+            "ks_p_pooled": a["ks_p_pooled"], "rmse_nm": a["rmse_nm"], "oracle_rmse_nm": a["oracle_rmse_nm"],
+            "rmse_vs_oracle": a["rmse_ratio"] - 1.0}
+
+
 def localization_rmse_nm(final, truth, cap, pixel_nm=100.0):
     """RMSE of final-state emitters matched one-to-one to the truth (greedy
     nearest pairs within 1 px), in nm."""
@@ -321,7 +364,7 @@ def bench_gauss_mle(args):
             "unit": "fits/s", "ms_per_step": ms, "dtype": "f32", "higher_is_better": True,
             "config": {"workload": "gauss_mle", "fit_type": ft, "camera": args.camera, "fits": n, "box": sz,
                        "iterations": 20},
-            "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / PEAK_FP32_TFLOPS if basic else None, "kernel": "k_mle_xynb",
                          "traffic": read_traffic(ROOT / "profiles" / "mle_xynb_gauss_mle_100k_7x7_pmc.txt")
                          if basic else None, "alg_flop_per_launch": flops},
@@ -453,6 +496,8 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch configuration")
+    ap.add_argument("--no-posterior", action="store_true",
+                    help="skip the posterior comparison with the oracle reference chains")
     ap.add_argument("--thin", type=int, default=5)
     ap.add_argument("--proposal", default="both", choices=["plain", "learned", "both"],
                     help="chain proposal(s) timed: the value is the plain chain's, learned is reported beside it")
@@ -521,31 +566,36 @@ def main():
                       outs=outs, hist_len=129, final_cap=cap, trace_len=trace_len)
         stream.synchronize()
         _hip.synchronize()
-        ev0.record(stream)
+        evs = [_hip.Event() for _ in range(args.steps + 1)]
+        evs[0].record(stream)
         for k in range(args.steps):
             if pre:
                 pre()
             run_batch(plan, rs, seed=SEED + k, mode=mode, trace_thin=args.thin, stream=stream, sync=False,
                       outs=outs, hist_len=129, final_cap=cap, trace_len=trace_len)
-        ev1.record(stream)
-        ev1.synchronize()
-        ms = ev0.elapsed_ms(ev1) / args.steps
+            evs[k + 1].record(stream)
+        evs[-1].synchronize()
+        step_ms = np.array([evs[k].elapsed_ms(evs[k + 1]) for k in range(args.steps)])
+        ms = float(np.median(step_ms))
         res = fetch(outs, n, nb + nt, 129, cap, trace_len)
         assert np.all(res.status == 0), np.unique(res.status, return_counts=True)
         ess, ok = roi_posteriors(res.trace_n, res.trace_photons)
         rmse = localization_rmse_nm(res.final, rois["truth"], cap)
-        print(f"[bench] {('plain', 'replay', 'learned')[mode]}: {ms:.1f} ms/step, mean ESS {ess.mean():.2f}, "
-              f"RMSE {rmse:.2f} nm", file=sys.stderr, flush=True)
+        print(f"[bench] {('plain', 'replay', 'learned')[mode]}: {ms:.1f} ms/step (median of {args.steps}; "
+              f"mean {step_ms.mean():.1f}), mean ESS {ess.mean():.2f}, RMSE {rmse:.2f} nm", file=sys.stderr,
+              flush=True)
         # each ROI's share of the launch: its algorithmic work's share
         cost = res.alg_flops.astype(np.float64)
         share = (ms / 1e3) * cost / cost.sum()
-        return ms, res, ess, ok, rmse, fixed_ess_rate(ess, share, nb / (nb + nt))
+        return ms, res, ess, ok, rmse, fixed_ess_rate(ess, share, nb / (nb + nt)), float(step_ms.mean())
 
-    ms, res, ess, ok, rmse, value = timed(PLAIN)
+    ms, res, ess, ok, rmse, extrap, ms_mean = timed(PLAIN)
+    value = float(ok.sum() / (ms / 1e3))
     flops = float(res.alg_flops.astype(np.float64).sum())
     achieved = flops / (ms / 1e3) / 1e12
     traffic = read_traffic(ROOT / "profiles" / f"rjmcmc_chain_{args.workload}_pmc.txt") if psf is None else None
-    plain = {"value": value, "ms_per_step": ms, "strict_posteriors_per_s": float(ok.sum() / (ms / 1e3)),
+    plain = {"value": value, "ms_per_step": ms, "ms_per_step_mean": ms_mean, "strict_posteriors_per_s": value,
+             "extrapolated_fixed_ess_rate": extrap, "roi_posteriors": int(ok.sum()),
              "ess_sum_rate": float(ess.sum() / 200.0 / (ms / 1e3)),
              "mean_ess": float(ess.mean()), "median_ess": float(np.median(ess)), "rmse_nm_final_state": rmse,
              "chains_per_s": n / (ms / 1e3), "jumps_per_s": n * (nb + nt) / (ms / 1e3)}
@@ -560,14 +610,18 @@ def main():
                    "psf_model": "erf (OptModel 0)" if psf is None else
                    "sampled (OptModel 1), 72 x 72 x 4 x 4 table of the same Gaussian",
                    "launch_tuning": tuning},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": "k_chain", "alg_flop_per_launch": flops},
+                     "kernel": "k_chain (both size classes, one run)", "alg_flop_per_launch": flops},
         "cpu_baseline": cpu,
         "quality": plain,
     }
     if cpu is not None:
         line["gpu_over_cpu"] = value / cpu["value"] if cpu["value"] > 0 else None
+    if args.workload == "batched_uneven" and psf is None and not args.no_posterior:
+        line["quality"]["posterior_vs_oracle"] = posterior_agreement(PLAIN)
+        print(f"[bench] plain posterior vs oracle: {line['quality']['posterior_vs_oracle']}", file=sys.stderr,
+              flush=True)
     if args.proposal in ("learned", "both"):
         t0 = time.perf_counter()
         prm, hist = proposal.train(args.train_steps, batch=256, lr=3e-3, seed=SEED % 1000)
@@ -586,17 +640,25 @@ def main():
         em1.record(stream)
         em1.synchronize()
         map_ms = em0.elapsed_ms(em1)
-        lms, lres, less, lok, lrmse, lvalue = timed(LEARNED, pre=lambda: mm.run(rs, dq, dmu, stream=stream))
+        lms, lres, less, lok, lrmse, lextrap, lms_mean = timed(LEARNED,
+                                                               pre=lambda: mm.run(rs, dq, dmu, stream=stream))
+        lvalue = float(lok.sum() / (lms / 1e3))
         net_tflops = proposal.NET_FLOP_PER_CANVAS * n / (map_ms / 1e3) / 1e12
         line["learned"] = {
-            "value": lvalue, "ms_per_step": lms, "maps_ms": map_ms, "train_s": train_s,
-            "train_rois": args.train_steps * 256, "strict_posteriors_per_s": float(lok.sum() / (lms / 1e3)),
+            "value": lvalue, "ms_per_step": lms, "ms_per_step_mean": lms_mean, "maps_ms": map_ms, "train_s": train_s,
+            "train_rois": args.train_steps * 256, "strict_posteriors_per_s": lvalue,
+            "extrapolated_fixed_ess_rate": lextrap, "roi_posteriors": int(lok.sum()),
             "ess_sum_rate": float(less.sum() / 200.0 / (lms / 1e3)),
             "mean_ess": float(less.mean()), "median_ess": float(np.median(less)), "rmse_nm_final_state": lrmse,
             "rmse_vs_plain": lrmse / rmse - 1.0 if rmse else None,
             "network_roofline": {"bound": "mfma", "achieved": net_tflops, "peak": 2500.0, "unit": "TFLOP/s",
                                  "frac": net_tflops / 2500.0, "kernel": "proposal maps (bf16 MFMA conv)"}}
+        if args.workload == "batched_uneven" and not args.no_posterior:
+            line["learned"]["posterior_vs_oracle"] = posterior_agreement(LEARNED, prm)
+            print(f"[bench] learned posterior vs oracle: {line['learned']['posterior_vs_oracle']}", file=sys.stderr,
+                  flush=True)
         line["learned_over_plain"] = lvalue / value if value > 0 else None
+        line["learned_over_plain_extrapolated"] = lextrap / extrap if extrap > 0 else None
         mm.close()
     print(json.dumps(line), flush=True)
     return 0

@@ -49,6 +49,15 @@ typedef struct {
     int32_t* prop_head; float* prop_val; float* prop_em;
     int64_t n_rec, n_acc_em, n_prop_em;
     int32_t status;
+    /* Thinned state snapshots (test infrastructure: reference posterior
+     * samples).  After trials step t (0-based) with t + 1 > snap_skip and
+     * (t + 1 - snap_skip) % snap_thin == 0, the current state is appended:
+     * snap_n[k] = N, and N entries {X, Y, I} at snap_em (3 floats each).
+     * snap_thin = 0: none. */
+    int32_t snap_thin, snap_skip;
+    int64_t snap_cap, snap_em_cap;
+    int32_t* snap_n; float* snap_em;
+    int64_t n_snap, n_snap_em;
 } orc_out;
 
 #define ORC_OK 0

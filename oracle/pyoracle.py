@@ -63,7 +63,10 @@ class Out(Structure):
                 ("rec_cap", c_int64), ("em_cap", c_int64),
                 ("acc_head", POINTER(c_int32)), ("acc_val", POINTER(c_float)), ("acc_em", POINTER(c_float)),
                 ("prop_head", POINTER(c_int32)), ("prop_val", POINTER(c_float)), ("prop_em", POINTER(c_float)),
-                ("n_rec", c_int64), ("n_acc_em", c_int64), ("n_prop_em", c_int64), ("status", c_int32)]
+                ("n_rec", c_int64), ("n_acc_em", c_int64), ("n_prop_em", c_int64), ("status", c_int32),
+                ("snap_thin", c_int32), ("snap_skip", c_int32), ("snap_cap", c_int64), ("snap_em_cap", c_int64),
+                ("snap_n", POINTER(c_int32)), ("snap_em", POINTER(c_float)), ("n_snap", c_int64),
+                ("n_snap_em", c_int64)]
 
 
 SRC_PHILOX, SRC_TAPE, SRC_LIBC = 0, 1, 2
@@ -329,7 +332,7 @@ def num_model(I, X, Y, bg, abg, bbg, xsize, ysize, psf, scmos=None):
 def rjmcmc(data: np.ndarray, xsize: int, ysize: int, params: McmcParams, off_pdf, sig_pdf, back_pdf,
            bg0: float, *, src_kind=SRC_PHILOX, seed=0, roi=0, tape=None, record=False, scmos=None,
            n0=0, i0=None, x0=None, y0=None, grids=False, chain=False, fin_cap=1024, rec_cap=None, em_cap=None,
-           psf=None):
+           psf=None, snap_thin=0, snap_skip=0, snap_em_cap=None):
     """One chain, restated.  Returns a dict with steps (STEP_DTYPE), counters, final state,
     optional replay tape, grids and Chain / Chain_Test records.  psf: a 4-D sampled PSF
     (SampledPSF with OptModel = 1), indexed [y, x, phase_y, phase_x]."""
@@ -384,6 +387,14 @@ def rjmcmc(data: np.ndarray, xsize: int, ysize: int, params: McmcParams, off_pdf
         out.acc_val, out.acc_em = _f32p(heads[1]), _f32p(heads[2])
         out.prop_head = heads[3].ctypes.data_as(POINTER(c_int32))
         out.prop_val, out.prop_em = _f32p(heads[4]), _f32p(heads[5])
+    snaps = None
+    if snap_thin > 0:
+        ns = max((params.n_trials - snap_skip) // snap_thin, 0) + 1
+        ec = snap_em_cap or ns * 128
+        snaps = (np.zeros(ns, np.int32), np.zeros(ec * 3, np.float32))
+        out.snap_thin, out.snap_skip, out.snap_cap, out.snap_em_cap = snap_thin, snap_skip, ns, ec
+        out.snap_n = snaps[0].ctypes.data_as(POINTER(c_int32))
+        out.snap_em = _f32p(snaps[1])
     if i0 is None:
         i0 = x0 = y0 = np.zeros(1, np.float32)
     i0 = np.ascontiguousarray(i0, np.float32)
@@ -404,6 +415,12 @@ def rjmcmc(data: np.ndarray, xsize: int, ysize: int, params: McmcParams, off_pdf
         r["tape"] = rec[:src.consumed].copy()
     if grids:
         r["pgrid"], r["pback"] = pg, pb
+    if snaps is not None:
+        k, ke = int(out.n_snap), int(out.n_snap_em)
+        if ke > len(snaps[1]) // 3:
+            raise RuntimeError("snapshot emitter capacity exceeded")
+        r["snap_n"] = snaps[0][:k].copy()
+        r["snap_em"] = snaps[1][:ke * 3].reshape(ke, 3).copy()
     if chain:
         nr = out.n_rec
         r["chain"] = {"acc_head": heads[0][:nr * 4].reshape(nr, 4), "acc_val": heads[1][:nr * 5].reshape(nr, 5),

@@ -983,6 +983,7 @@ int orc_rjmcmc_psf(const float* data, int xsize, int ysize, const float* scmos,
     memset(out->counters, 0, sizeof(out->counters));
     for (int k = 0; k < 5; k++) { out->counters[C_N + k] = 1; out->counters[C_NFG + k] = 1; out->counters[C_NBG + k] = 1; }
     out->n_rec = out->n_acc_em = out->n_prop_em = 0;
+    out->n_snap = out->n_snap_em = 0;
     /* initial state (cRJMCMC.cpp:120-133) */
     c->cur.n = n0 > ORC_NMAX ? ORC_NMAX : n0;
     for (int k = 0; k < c->cur.n; k++) { c->cur.I[k] = i0[k]; c->cur.X[k] = x0[k]; c->cur.Y[k] = y0[k]; c->cur.sig[k] = 1; }
@@ -1014,6 +1015,19 @@ int orc_rjmcmc_psf(const float* data, int xsize, int ysize, const float* scmos,
         c->st->photons = ph;
         c->st->draws = c->draws;
         if (!(c->st->flags & 2)) c->st->n_prop = c->cur.n;
+        if (!burn && out->snap_thin > 0) {
+            int64_t t1 = (int64_t)(s - p->n_burnin) + 1 - out->snap_skip;
+            if (t1 > 0 && t1 % out->snap_thin == 0 && out->n_snap < out->snap_cap) {
+                out->snap_n[out->n_snap++] = c->cur.n;
+                for (int k = 0; k < c->cur.n; k++) {
+                    if (out->n_snap_em < out->snap_em_cap) {
+                        float* q = out->snap_em + 3 * out->n_snap_em;
+                        q[0] = c->cur.X[k]; q[1] = c->cur.Y[k]; q[2] = c->cur.I[k];
+                    }
+                    out->n_snap_em++;
+                }
+            }
+        }
     }
     out->n = c->cur.n; out->bg = c->cur.bg; out->abg = c->cur.abg; out->bbg = c->cur.bbg;
     for (int k = 0; k < c->cur.n && k < out->fin_cap; k++) {
