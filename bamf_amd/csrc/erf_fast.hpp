@@ -2,12 +2,19 @@
 //
 // Library erff selects a different approximation per argument range; when the
 // 64 lanes of a wavefront hold arguments spanning those ranges (a PSF stencil
-// does, by construction), every branch executes.  This erf has one code path:
-// the Chebyshev fit of erfc published in W. H. Press et al., "Numerical
-// Recipes in C", 2nd ed. (1992), section 6.2 (function erfcc), whose
-// fractional error is below 1.2e-7 for all x; erf(x) = 1 - erfc(x).
-// Evaluated in single precision with hardware reciprocal and exp, the result
-// is within 1e-6 absolute of erf (tests/test_gpu_rjmcmc.py checks it).
+// does, by construction), every branch executes.  This erf evaluates two
+// forms on every lane and selects one:
+//   |x| < 1/2   the Maclaurin series erf(x) = 2/sqrt(pi) sum_n (-1)^n
+//               x^(2n+1) / (n! (2n+1)) through x^13 (truncation below 1e-9
+//               relative), so small arguments keep full relative precision;
+//   otherwise   1 - erfc(|x|) with the sign of x, erfc from the Chebyshev fit
+//               published in W. H. Press et al., "Numerical Recipes in C",
+//               2nd ed. (1992), section 6.2 (function erfcc, fractional error
+//               below 1.2e-7), its reciprocal refined by one Newton step and
+//               its exponential from the accurate expf; there erfc <= 0.48,
+//               so the absolute error stays below 1e-7.
+// Evaluated in single precision; within 1.5e-7 absolute of erf everywhere
+// (tests/test_gpu_rjmcmc.py checks it against float64 erf).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -15,7 +22,9 @@ namespace bamf {
 
 __device__ __forceinline__ float erfc_nr(float x) {
     const float z = fabsf(x);
-    const float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * z);
+    const float d = 1.0f + 0.5f * z;
+    float t = __builtin_amdgcn_rcpf(d);
+    t = fmaf(fmaf(-t, d, 1.0f), t, t);  // one Newton step: the reciprocal to within an ulp
     float p = 0.17087277f;
     p = fmaf(p, t, -0.82215223f);
     p = fmaf(p, t, 1.48851587f);
@@ -26,10 +35,24 @@ __device__ __forceinline__ float erfc_nr(float x) {
     p = fmaf(p, t, 0.37409196f);
     p = fmaf(p, t, 1.00002368f);
     p = fmaf(p, t, -1.26551223f);
-    const float r = t * __expf(fmaf(-z, z, p));
+    const float r = t * expf(fmaf(-z, z, p));
     return x >= 0.0f ? r : 2.0f - r;
 }
 
-__device__ __forceinline__ float erf_nr(float x) { return 1.0f - erfc_nr(x); }
+__device__ __forceinline__ float erf_nr(float x) {
+    // series: 2/sqrt(pi) (x - x^3/3 + x^5/10 - x^7/42 + x^9/216 - x^11/1320 + x^13/9360)
+    const float x2 = x * x;
+    float s = 1.0f / 9360.0f;
+    s = fmaf(s, x2, -1.0f / 1320.0f);
+    s = fmaf(s, x2, 1.0f / 216.0f);
+    s = fmaf(s, x2, -1.0f / 42.0f);
+    s = fmaf(s, x2, 1.0f / 10.0f);
+    s = fmaf(s, x2, -1.0f / 3.0f);
+    s = fmaf(s, x2, 1.0f);
+    const float small = 1.1283791670955126f * x * s;
+    const float r = erfc_nr(fabsf(x));
+    const float large = x >= 0.0f ? 1.0f - r : r - 1.0f;
+    return fabsf(x) < 0.5f ? small : large;
+}
 
 }  // namespace bamf

@@ -19,7 +19,7 @@ def oracle_params(oracle, p):
 
 
 def run_oracle_chains(oracle, rois, p, seed, src=None, bg0=10.0, record=True, grids=False, back=None,
-                      sig=None, off=None, scmos=None, n0=None, psf=None):
+                      sig=None, off=None, scmos=None, n0=None, psf=None, chain=False):
     """One oracle chain per ROI.  Returns (results, tapes)."""
     src = oracle.SRC_PHILOX if src is None else src
     op = oracle_params(oracle, p)
@@ -35,7 +35,7 @@ def run_oracle_chains(oracle, rois, p, seed, src=None, bg0=10.0, record=True, gr
                             SIG_PDF if sig is None else sig, BACK_PDF if back is None else back,
                             bg0=float(bg0[r]) if np.ndim(bg0) else bg0, src_kind=src, seed=seed, roi=r,
                             record=record, grids=grids, scmos=None if scmos is None else scmos[o[r]:o[r + 1]],
-                            psf=psf, **kw)
+                            psf=psf, chain=chain, **kw)
         out.append(res)
         tapes.append(res.get("tape"))
     return out, tapes

@@ -5,16 +5,25 @@ recorded decisions and birth pixels.  The proposals are then computed from
 identical states, so every step must match the oracle exactly in what it
 proposes (kind, JumpType, draws, bounds-test outcome, N, BG, photons, final
 emitters, grids, counters).  The GPU's own accept decisions must equal the
-oracle's except at near-ties, and its double-precision log PR must agree with
-the oracle's within 1e-5 relative, with an absolute floor of 64 single-precision
-ulps of sum|M| over the ROI (the rounding the reference's own single-precision
-sum carries).
+oracle's except at near-ties.  Its log PR is held to the oracle's own
+single-precision error, per step: the likelihood-ratio sum
+S = sum(M - M' + D log(M'/M)) is evaluated in float64 from the step's two
+states (S64); the oracle's float32 sum S32 (its float32 models, terms and
+left-to-right sum) misses it by e_o = |S32 - S64|; the GPU's sum (S32 plus the
+difference of the two log PRs, whose other factors are the same floats) must
+miss it by at most 2 max(e_o, r), r the float32 rounding scale of the step's
+sum: 2^-24 times the quadrature sum of the roundings of every changed model
+value (M and M', each entering its term with weight 1 + D/M), every term
+and every partial sum.
 
 Free-running: both use the same Philox stream, so chains agree step for step
 until a near-tie decision differs.
 """
+import math
+
 import numpy as np
 import pytest
+from scipy import special
 
 from bamf_amd import synth
 from bamf_amd.rjmcmc import PLAIN, REPLAY, Plan, RoiSet, make_replay, run_batch
@@ -30,6 +39,97 @@ def accept_uniform(tape, steps):
     ok = steps["draws"] > 0
     u[ok] = tape[ends[ok] - 1].astype(np.float64) / 2147483648.0
     return u
+
+
+def _model64(I, X, Y, bg, abg, bbg, xs, ys, sigma):
+    """The pixel-integrated Gaussian model (LibRJMCMC.cpp:42-59) in float64, layout y + x*ys."""
+    c = 1.0 / (sigma * math.sqrt(2.0))
+    x = np.arange(xs, dtype=np.float64)
+    y = np.arange(ys, dtype=np.float64)
+    m = bg + abg * x[:, None] + bbg * y[None, :]
+    if len(I):
+        I, X, Y = (np.asarray(v, np.float64) for v in (I, X, Y))
+        gx = 0.5 * (special.erf((x[None, :] + 1 - X[:, None]) * c) - special.erf((x[None, :] - X[:, None]) * c))
+        gy = 0.5 * (special.erf((y[None, :] + 1 - Y[:, None]) * c) - special.erf((y[None, :] - Y[:, None]) * c))
+        m = m + np.einsum("e,ex,ey->xy", I, gx, gy)
+    return m.ravel()
+
+
+def _states(ch, which):
+    head, val, em = ch[which + "_head"], ch[which + "_val"], ch[which + "_em"]
+    out = []
+    for k in range(len(head)):
+        n, at = int(head[k, 0]), int(head[k, 3])
+        e = em[at:at + n]
+        out.append((e[:, 2], e[:, 0], e[:, 1], float(val[k, 2]), float(val[k, 3]), float(val[k, 4])))
+    return out
+
+
+def llr_sum_errors(oracle, ores_r, data, xs, ys, sigma, bg0, init=None):
+    """Per step with a likelihood evaluation: (step, S64, e_o = |S32 - S64|, rounding scale)."""
+    st, ch = ores_r["steps"], ores_r["chain"]
+    acc, prop = _states(ch, "acc"), _states(ch, "prop")
+    d64 = data.astype(np.float64)
+    P = xs * ys
+    before = init if init is not None else (np.zeros(0), np.zeros(0), np.zeros(0), bg0, 0.0, 0.0)
+    out, k = [], 0
+    for s in np.flatnonzero((st["flags"] & 2) != 0):
+        a, t = before, prop[k]
+        if not (st["flags"][s] & 8):
+            m32 = oracle.model(a[0], a[1], a[2], a[3], a[4], a[5], xs, ys, sigma)
+            mt32 = oracle.model(t[0], t[1], t[2], t[3], t[4], t[5], xs, ys, sigma)
+            # logf as the oracle's libm evaluates it: the float64 log rounded once to float32
+            # (numpy's own float32 log is off by a few ulps)
+            lq = np.log((mt32 / m32).astype(np.float64)).astype(np.float32)
+            terms32 = (m32 - mt32) + data * lq
+            s32 = float(np.add.accumulate(terms32.astype(np.float32), dtype=np.float32)[-1])
+            m64 = _model64(*a, xs, ys, sigma)
+            mt64 = _model64(*t, xs, ys, sigma)
+            terms64 = (m64 - mt64) + d64 * np.log(mt64 / m64)
+            s64 = float(terms64.sum())
+            # float32 rounding scale of this step's sum, independent roundings in quadrature:
+            # each changed model value (weight 1 + D/M in its term), each term, each partial
+            # sum.  The device forms M' incrementally (M plus or minus stencils), so M' is
+            # rounded at the larger of the two magnitudes: removing a bright emitter leaves
+            # the rounding of the brighter M in the dimmer M'.
+            ch = np.abs(mt64 - m64) > 2.0 ** -24 * m64
+            big = np.maximum(m64, mt64)[ch]
+            w = (m64 + d64)[ch] ** 2 + (big * (1.0 + d64[ch] / mt64[ch])) ** 2
+            scale = 2.0 ** -24 * math.sqrt(float(w.sum() + (terms64 ** 2).sum() + (np.cumsum(terms64) ** 2).sum()))
+            out.append((s, s64, s32, scale))
+        before = acc[k]
+        k += 1
+    return out
+
+
+def check_llr_against_float64(oracle, gres, ores, rois_data, offs, xsz, ysz, sigma, bg0, inits=None):
+    """GPU log-likelihood-ratio sums within twice the oracle's own float32 error of the float64 sum."""
+    ratios, excess, n_steps = [], [], 0
+    for r in range(len(ores)):
+        o, gs = ores[r]["steps"], gres.steps[r]
+        data = rois_data[offs[r]:offs[r + 1]]
+        init = None
+        if inits is not None:
+            I, X, Y = inits[r]
+            init = (I, X, Y, bg0, 0.0, 0.0)
+        for s, s64, s32, scale in llr_sum_errors(oracle, ores[r], data, int(xsz[r]), int(ysz[r]), sigma, bg0, init):
+            if not (np.isfinite(o["log_pr"][s]) and np.isfinite(gs["log_pr"][s])):
+                continue
+            sg = s32 + (float(gs["log_pr"][s]) - float(o["log_pr"][s]))
+            e_o, e_g = abs(s32 - s64), abs(sg - s64)
+            ratios.append(e_g / max(e_o, scale))
+            excess.append(((e_g - 2.0 * e_o) / scale, r, int(s), e_g, e_o, scale))
+            n_steps += 1
+    ratios = np.array(ratios)
+    q = np.quantile(ratios, [0.5, 0.99, 0.999, 1.0])
+    worst = max(excess)
+    print(f"float64 re-evaluation over {n_steps} steps: GPU error / max(oracle error, rounding scale) median "
+          f"{q[0]:.2f}, p99 {q[1]:.2f}, p99.9 {q[2]:.2f}, max {q[3]:.2f}; worst (e_g - 2 e_o) / scale "
+          f"{worst[0]:.2f} (roi, step, e_g, e_o, scale = {worst[1:]})", flush=True)
+    # every step: within twice the oracle's error plus 3 sigma of the step's own rounding
+    assert worst[0] <= 3.0, worst
+    # and 99.9 % of steps within twice the larger of the two
+    assert np.mean(ratios <= 2.0) >= 0.999, np.mean(ratios <= 2.0)
 
 
 def check_replay(rois, gres, ores, tapes, p, data_sum, stats=None):
@@ -50,12 +150,12 @@ def check_replay(rois, gres, ores, tapes, p, data_sum, stats=None):
         if births.any():
             assert np.mean(gs["pix"][births] == o["pix"][births]) >= 0.99
         lp_o, lp_g = o["log_pr"], gs["log_pr"]
-        # tolerance: 1e-5 relative, floor of 16 single-precision ulps of sum|M| ~ sum D
+        # near-tie band for the decision check: 1e-5 relative plus 16 single-precision
+        # ulps of sum D (the log PR itself is held to the float64 re-evaluation)
         tol = 1e-5 * np.abs(lp_o) + 16 * 2.0 ** -23 * max(float(data_sum(r)), 1.0)
         fin = np.isfinite(lp_o) & (o["pr"] > 1e-30) & (o["pr"] < 1e30) & np.isfinite(o["pr"])
         assert np.array_equal(np.isfinite(lp_g[fin]), np.ones(fin.sum(), bool)), r
         err = np.abs(lp_g[fin] - lp_o[fin])
-        assert np.all(err <= tol[fin]), (r, float(np.max(err - tol[fin])))
         rel_errs.append(err / np.maximum(np.abs(lp_o[fin]), 1e-300))
         # own decision vs the oracle's: differences only at near-ties
         dec = (gs["flags"] & 1) != (o["flags"] & 1)
@@ -82,13 +182,14 @@ def test_replay_config1(oracle):
     """Config-1 distribution (16x16, N ~ Poisson(3) capped at 10, bg 10), 3000 + 3000 jumps."""
     rois = synth.make_rois(48, seed=101, sides=(16,), mean_n=3.0, n_cap=10)
     p = params()
-    ores, tapes = run_oracle_chains(oracle, rois, p, seed=7)
+    ores, tapes = run_oracle_chains(oracle, rois, p, seed=7, chain=True)
     plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
     rs = RoiSet.from_synth(rois, bg0=np.full(48, 10.0, np.float32))
     rp = make_replay(tapes, np.stack([o["steps"] for o in ores]))
     print("replay config1: 48 ROIs", flush=True)
     g = run_batch(plan, rs, mode=REPLAY, replay=rp, record_steps=True, hist_len=129)
     check_replay(rois, g, ores, tapes, p, lambda r: rois["data"][rois["offset"][r]:rois["offset"][r + 1]].sum())
+    check_llr_against_float64(oracle, g, ores, rois["data"], rois["offset"], rois["side"], rois["side"], 1.3, 10.0)
 
 
 def test_replay_uneven_sizes_and_background(oracle):
@@ -114,7 +215,7 @@ def test_replay_uneven_sizes_and_background(oracle):
     back = synth.gamma_pdf_table(2, 40, 10.0, 1001)
     p = params(nb=1500, nt=1500, is_backg=1, p_burnin=(0.3, 0.15, 0.15, 0.15, 0.15, 0, 0, 0.1),
                p_trials=(0.5, 0.1, 0.1, 0.1, 0.1, 0, 0, 0.1))
-    ores, tapes = run_oracle_chains(oracle, rois, p, seed=11, bg0=15.0, back=back, n0=inits, grids=True)
+    ores, tapes = run_oracle_chains(oracle, rois, p, seed=11, bg0=15.0, back=back, n0=inits, grids=True, chain=True)
     plan = Plan(p, OFF_PDF, SIG_PDF, back)
     eoff = np.zeros(n, np.int64)
     n0 = np.array([len(t[0]) for t in inits], np.int32)
@@ -126,6 +227,7 @@ def test_replay_uneven_sizes_and_background(oracle):
     print("replay uneven: 20 ROIs", flush=True)
     g = run_batch(plan, rs, mode=REPLAY, replay=rp, record_steps=True, grids=True, hist_len=129)
     check_replay(rois, g, ores, tapes, p, lambda r: data[offs[r]:offs[r + 1]].sum())
+    check_llr_against_float64(oracle, g, ores, data, offs, xs, ys, 1.3, 15.0, inits=inits)
     off = 0
     for r in range(n):
         sz = int(xs[r] * ys[r] * 400)
@@ -211,7 +313,11 @@ def test_branch_free_erf_accuracy():
     _hip.synchronize()
     y = dy.download(np.float32)
     err = np.abs(y.astype(np.float64) - erf(x.astype(np.float64)))
-    assert err.max() < 1e-6, err.max()
+    assert err.max() < 1.5e-7, err.max()
+    # relative precision where erf is small (the series' range)
+    small = np.abs(x) < 0.5
+    rel = err[small] / np.maximum(np.abs(erf(x[small].astype(np.float64))), 1e-30)
+    assert rel.max() < 5e-7, rel.max()
 
 
 def test_birth_death_stationary_law_gpu():
