@@ -109,13 +109,17 @@ class DeviceBuffer:
         check(_load().hipMemcpy(ctypes.c_void_p(self.ptr), a.ctypes.data_as(ctypes.c_void_p), a.nbytes, H2D),
               "hipMemcpy H2D")
 
-    def download(self, dtype, count: int | None = None) -> np.ndarray:
+    def download(self, dtype, count: int | None = None, offset: int = 0) -> np.ndarray:
+        """count elements of dtype starting at element `offset`."""
         dtype = np.dtype(dtype)
         if count is None:
-            count = self.nbytes // dtype.itemsize
+            count = self.nbytes // dtype.itemsize - offset
+        if (offset + count) * dtype.itemsize > self.nbytes:
+            raise ValueError("download beyond the buffer")
         out = np.empty(count, dtype=dtype)
         if out.nbytes:
-            check(_load().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr), out.nbytes, D2H),
+            check(_load().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.c_void_p(self.ptr + offset * dtype.itemsize), out.nbytes, D2H),
                   "hipMemcpy D2H")
         return out
 

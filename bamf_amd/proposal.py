@@ -136,20 +136,38 @@ class Network:
 
 
 def train(steps: int, batch: int = 256, lr: float = 2e-3, seed: int = 0, side: int = 16, params=None,
-          log_every: int = 0):
-    """Adam training on fresh synthetic batches; returns (params, loss history)."""
+          log_every: int = 0, device_data: bool = True, chain_labels: float = 0.0, chain_jumps: int = 2000):
+    """Adam training on fresh synthetic batches; returns (params, loss history).
+
+    device_data: the batches come from the device forward model
+    (bamf_synth_roi_batch: the batched workload's density, background and
+    intensity distributions) straight into the training buffers; otherwise
+    from training_batch on the host.  chain_labels: the share of steps whose
+    targets are a plain chain's posterior sample on the same ROIs (the final
+    state of a chain of chain_jumps jumps: occupancy and log-intensity of its
+    emitters) instead of the ground truth."""
     rng = np.random.default_rng(seed)
     net = Network(init_params(seed) if params is None else params, batch, side, side)
     dimg, docc, dli = (_hip.DeviceBuffer(batch * side * side * 4) for _ in range(3))
     dloss = _hip.DeviceBuffer(8)
     stream = _hip.Stream()
+    labeller = _ChainLabeller(batch, side, chain_jumps, seed) if chain_labels > 0 else None
     hist = []
     for it in range(steps):
-        img, occ, logi = training_batch(rng, batch, side)
-        stream.synchronize()  # the previous step has consumed the buffers
-        dimg.upload(img)
-        docc.upload(occ)
-        dli.upload(logi)
+        use_chain = labeller is not None and (it % max(1, int(round(1.0 / chain_labels)))) == 0
+        if device_data:
+            check(lib().bamf_synth_roi_batch(batch, side, 0.005, 0.05, 5.0, 30.0, 5.0, 200.0, 1.3,
+                                             int(seed) * 1000003 + it, 1, dimg.vp, docc.vp, dli.vp, None, None,
+                                             stream.vp), "bamf_synth_roi_batch")
+        else:
+            img, occ, logi = training_batch(rng, batch, side)
+            stream.synchronize()  # the previous step has consumed the buffers
+            dimg.upload(img)
+            docc.upload(occ)
+            dli.upload(logi)
+        if use_chain:
+            stream.synchronize()
+            labeller.label(dimg, docc, dli, it)
         rate = lr * (0.1 ** (it / max(steps, 1)))  # exponential decay to lr/10
         check(lib().bamf_net_train_step(net.ptr, dimg.vp, docc.vp, dli.vp, batch, ctypes.c_float(rate), dloss.vp,
                                         stream.vp), "bamf_net_train_step")
@@ -160,6 +178,56 @@ def train(steps: int, batch: int = 256, lr: float = 2e-3, seed: int = 0, side: i
     p = net.params()
     net.close()
     return p, hist
+
+
+class _ChainLabeller:
+    """Posterior-sample targets: plain chains (the batched workload's parameters)
+    on the device images; each chain's final state gives the occupancy and
+    log-intensity targets by the rule of training_batch."""
+
+    def __init__(self, batch: int, side: int, jumps: int, seed: int):
+        from . import synth
+        from .rjmcmc import Plan, make_params
+
+        self.batch, self.side, self.seed = batch, side, seed
+        p = make_params(n_burnin=jumps // 2, n_trials=jumps - jumps // 2, rho=2.75 * 0.01)
+        self.plan = Plan(p, synth.gamma_pdf_table(4, 5, 10.0, 1001), synth.gamma_pdf_table(5, 200, 10.0, 1001),
+                         np.zeros(1001, np.float32))
+        self.cap = 96
+
+    def label(self, dimg, docc, dli, it: int) -> None:
+        from .rjmcmc import PLAIN, RoiSet, run_batch
+
+        n, s = self.batch, self.side
+        offs = np.arange(n + 1, dtype=np.int64) * s * s
+        xs = np.full(n, s, np.int32)
+        rois = RoiSet(None, offs, xs, xs, bg0=np.full(n, 20.0, np.float32),
+                      device=(dimg, _hip.DeviceBuffer.from_array(offs), _hip.DeviceBuffer.from_array(xs),
+                              _hip.DeviceBuffer.from_array(xs)))
+        res = run_batch(self.plan, rois, seed=self.seed * 7919 + it, mode=PLAIN, final_cap=self.cap)
+        occ, logi = chain_targets(res.final, s, self.cap)
+        docc.upload(occ)
+        dli.upload(logi)
+
+
+def chain_targets(final: np.ndarray, side: int, cap: int):
+    """Occupancy and log-intensity targets of chains' final states (rows of
+    [N, bg, abg, bbg, I[cap], X[cap], Y[cap]]) by the rule of training_batch:
+    pixel (floor(X + 1/2), floor(Y + 1/2)) clipped to the ROI holds the summed
+    intensity of its emitters; occupied where that sum is positive."""
+    n = final.shape[0]
+    isum = np.zeros((n, side, side), np.float64)
+    for r in range(n):
+        k = int(final[r, 0])
+        I = final[r, 4:4 + k].astype(np.float64)
+        X = final[r, 4 + cap:4 + cap + k]
+        Y = final[r, 4 + 2 * cap:4 + 2 * cap + k]
+        px = np.clip(np.floor(X + 0.5).astype(np.int64), 0, side - 1)
+        py = np.clip(np.floor(Y + 0.5).astype(np.int64), 0, side - 1)
+        np.add.at(isum[r], (px, py), np.maximum(I, 0.0))
+    occ = (isum > 0).astype(np.float32)
+    logi = np.where(isum > 0, np.log(np.maximum(isum, 1e-30)), 0.0).astype(np.float32)
+    return occ, logi
 
 
 # Multiply-adds of one 32 x 32 canvas through the network (x 2 = FLOP):

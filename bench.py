@@ -467,6 +467,167 @@ def bench_tiles(args):
             "cpu_baseline": cpu}
 
 
+# ---------------------------------------------------------------- dense low-SNR, end to end
+def bench_dense_lowsnr(args):
+    """Dense low-SNR stress, end to end: args.dense_frames frames of 512 x 512
+    (default 5000) from the device forward model (8 emitters per um^2 at 100 nm
+    pixels = 0.08 per pixel^2, Gamma(4, 125) photons (mean 500), background 50,
+    sigma 1.3), FindROI with Boxsize 32 (Minval = mean photons / (4 pi sigma^2) / 4,
+    the rule of findPriors.m), the first args.dense_rois detections cropped to
+    32 x 32 ROIs, then plain and learned-proposal chains of 10,000 + 10,000
+    jumps on them.  Generator rule for the per-ROI emitter count: the density is
+    the configured one over the whole frame, and a ROI holds whatever emitters
+    its 32 x 32 window covers (about 82 on average at this density, more than
+    the 40 the configuration's text assumes; the measured counts are reported).
+    A step: FindROI over every frame, the crops, and the chains."""
+    from bamf_amd import _hip, proposal, synth
+    from bamf_amd._lib import check, lib
+    from bamf_amd.ess import fixed_ess_rate, roi_posteriors
+    from bamf_amd.find_roi import FindRoiDevice
+    from bamf_amd.rjmcmc import LEARNED, PLAIN, Plan, RoiSet, alloc_outputs, fetch, make_params, run_batch
+
+    nf, side, box = args.dense_frames, 512, 32
+    dens, gk, gt, bg, sigma = 0.08, 4.0, 125.0, 50.0, 1.3
+    minval = (gk * gt) / (4 * math.pi * sigma ** 2) / 4
+    ncell = int(lib().bamf_synth_cells(side, side))
+    ccap = 64
+    d_frames = _hip.DeviceBuffer(nf * side * side * 4)
+    d_truth = _hip.DeviceBuffer(nf * ncell * ccap * 12)
+    d_cnt = _hip.DeviceBuffer(nf * ncell * 4)
+    stream = _hip.Stream()
+    e0, e1 = _hip.Event(), _hip.Event()
+    e0.record(stream)
+    check(lib().bamf_synth_frames(nf, side, side, dens, gk, gt, bg, sigma, SEED + 5, ccap, 1, d_frames.vp,
+                                  d_truth.vp, d_cnt.vp, stream.vp), "bamf_synth_frames")
+    e1.record(stream)
+    e1.synchronize()
+    print(f"[bench] dense: {nf} frames of {side} x {side} generated in {e0.elapsed_ms(e1):.1f} ms", file=sys.stderr,
+          flush=True)
+    cap = nf * 1024
+    fr = FindRoiDevice(side, side, nf, capacity=cap)
+    n_chain = args.dense_rois
+    d_rois = _hip.DeviceBuffer(n_chain * box * box * 4)
+    offs = np.arange(n_chain + 1, dtype=np.int64) * box * box
+    xs = np.full(n_chain, box, np.int32)
+    rs = RoiSet(None, offs, xs, xs, bg0=np.full(n_chain, bg, np.float32),
+                device=(d_rois, _hip.DeviceBuffer.from_array(offs), _hip.DeviceBuffer.from_array(xs),
+                        _hip.DeviceBuffer.from_array(xs)))
+    nb = nt = 10000
+    p = make_params(n_burnin=nb, n_trials=nt, rho=dens, p_burnin=(0.4, 0.15, 0.15, 0.15, 0.15, 0, 0, 0),
+                    p_trials=(0.6, 0.1, 0.1, 0.1, 0.1, 0, 0, 0))
+    sig = synth.gamma_pdf_table(gk, gt, DX, 1001)
+    off = synth.gamma_pdf_table(25, 2, DX, 1001)  # offset prior: mean 50, sd 10 photons/px
+    back = np.zeros(1001, np.float32)
+    plan = Plan(p, off, sig, back)
+    trace_len, cap_em = nt // args.thin, 128
+    outs = alloc_outputs(n_chain, nb + nt, 129, cap_em, trace_len, False, False, rs, p.grid_zoom)
+
+    def front():
+        fr.run(d_frames, sigma, 2 * sigma, box, minval, stream=stream)
+        check(lib().bamf_find_roi_crop(d_frames.vp, side, side, fr.row.vp, fr.col.vp, fr.frame.vp, n_chain, box,
+                                       d_rois.vp, stream.vp), "bamf_find_roi_crop")
+
+    # detections and the truth of the chained ROIs (emitters inside each window)
+    front()
+    stream.synchronize()
+    n_det, _, row, col, frm = fr.results()
+    if n_det < n_chain:
+        raise SystemExit(f"dense: only {n_det} detections for {n_chain} ROIs")
+    row, col, frm = row[:n_chain] - 1, col[:n_chain] - 1, frm[:n_chain] - 1
+    fmax = int(frm.max()) + 1
+    tr = d_truth.download(np.float32, fmax * ncell * ccap * 3).reshape(fmax, ncell, ccap, 3)
+    tc = d_cnt.download(np.int32, fmax * ncell).reshape(fmax, ncell)
+    per_frame = [np.concatenate([tr[f, c, :tc[f, c]] for c in range(ncell)]) for f in range(fmax)]
+    truth, n_true = [], np.zeros(n_chain, np.int64)
+    for b in range(n_chain):
+        em = per_frame[frm[b]]
+        # a crop's row runs along the frame's contiguous axis (findROI.m passes Xsize = size(Image, 2)):
+        # chain y = frame x - row corner, chain x = frame y - column corner
+        m = (em[:, 0] >= row[b]) & (em[:, 0] < row[b] + box) & (em[:, 1] >= col[b]) & (em[:, 1] < col[b] + box)
+        truth.append((em[m, 2], em[m, 1] - col[b], em[m, 0] - row[b]))
+        n_true[b] = m.sum()
+    print(f"[bench] dense: {n_det} detections ({n_det / nf:.1f} per frame); {n_chain} ROIs chained, truth "
+          f"emitters per ROI mean {n_true.mean():.1f}, max {n_true.max()}", file=sys.stderr, flush=True)
+
+    def timed(mode, pre=None):
+        for w in range(args.warmup):
+            front()
+            if pre:
+                pre()
+            run_batch(plan, rs, seed=SEED + 2000 + w, mode=mode, trace_thin=args.thin, stream=stream, sync=False,
+                      outs=outs, hist_len=129, final_cap=cap_em, trace_len=trace_len)
+        stream.synchronize()
+        steps, parts = [], []
+        for k in range(args.steps):
+            ev = [_hip.Event() for _ in range(4)]
+            ev[0].record(stream)
+            front()
+            ev[1].record(stream)
+            if pre:
+                pre()
+            ev[2].record(stream)
+            run_batch(plan, rs, seed=SEED + 3000 + k, mode=mode, trace_thin=args.thin, stream=stream, sync=False,
+                      outs=outs, hist_len=129, final_cap=cap_em, trace_len=trace_len)
+            ev[3].record(stream)
+            ev[3].synchronize()
+            steps.append(ev[0].elapsed_ms(ev[3]))
+            parts.append((ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2]), ev[2].elapsed_ms(ev[3])))
+        ms = float(np.median(steps))
+        part = np.median(np.array(parts), axis=0)
+        res = fetch(outs, n_chain, nb + nt, 129, cap_em, trace_len)
+        st = np.unique(res.status, return_counts=True)
+        ess, ok = roi_posteriors(res.trace_n, res.trace_photons)
+        rmse = localization_rmse_nm(res.final, truth, cap_em)
+        cost = res.alg_flops.astype(np.float64)
+        share = (part[2] / 1e3) * cost / cost.sum()
+        extrap = fixed_ess_rate(ess, share, 0.5)
+        print(f"[bench] dense {('plain', 'replay', 'learned')[mode]}: {ms:.1f} ms/step (FindROI+crop {part[0]:.1f}, "
+              f"maps {part[1]:.1f}, chains {part[2]:.1f}), status {dict(zip(*[v.tolist() for v in st]))}, "
+              f"mean ESS {ess.mean():.2f}, RMSE {rmse:.2f} nm", file=sys.stderr, flush=True)
+        return {"value": float(ok.sum() / (ms / 1e3)), "ms_per_step": ms, "find_roi_crop_ms": float(part[0]),
+                "maps_ms": float(part[1]), "chain_ms": float(part[2]), "roi_posteriors": int(ok.sum()),
+                "extrapolated_fixed_ess_rate": extrap, "mean_ess": float(ess.mean()),
+                "median_ess": float(np.median(ess)), "rmse_nm_final_state": rmse,
+                "final_n_mean": float(res.n_final.mean()), "status_counts": {int(a): int(b) for a, b in zip(*st)},
+                "alg_flop_per_step": float(cost.sum())}
+
+    plain = timed(PLAIN)
+    line = {"metric": "ROI posteriors/sec at fixed ESS & RMSE: plain vs learned-proposal RJMCMC",
+            "value": plain["value"], "unit": "ROI posteriors/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": plain["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 (accept test f64)", "data": "synthetic (device forward-model frames)",
+            "config": {"workload": "dense_lowsnr", "frames": nf, "frame_side": side, "density_per_um2": 8.0,
+                       "photons": "Gamma(4, 125)", "bg": bg, "find_roi_box": box, "minval": minval,
+                       "detections": int(n_det), "chained_rois": n_chain, "roi_side": box,
+                       "truth_emitters_per_roi_mean": float(n_true.mean()), "truth_emitters_per_roi_max": int(n_true.max()),
+                       "jumps_per_chain": nb + nt, "burnin": nb, "ess_target": 200},
+            "roofline": {"bound": "valu", "achieved": plain["alg_flop_per_step"] / (plain["chain_ms"] / 1e3) / 1e12,
+                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "kernel": "k_chain (32 x 32 class)",
+                         "traffic": None},
+            "cpu_baseline": None, "quality": plain}
+    line["roofline"]["frac"] = line["roofline"]["achieved"] / PEAK_FP32_TFLOPS
+    if args.proposal in ("learned", "both"):
+        t0 = time.perf_counter()
+        prm, _ = proposal.train(args.train_steps, batch=256, lr=3e-3, seed=SEED % 1000,
+                                chain_labels=args.chain_labels)
+        train_s = time.perf_counter() - t0
+        mm = proposal.MapMaker(prm)
+        dq, dmu = _hip.DeviceBuffer(rs.total_pixels * 4), _hip.DeviceBuffer(rs.total_pixels * 4)
+        plan.set_proposal(dq, dmu, alpha=0.2, halfwidth=0.35, beta=1.0)
+        plan.set_intensity_proposal("fit", 1.3, 0.5)
+        learned = timed(LEARNED, pre=lambda: mm.run(rs, dq, dmu, stream=stream))
+        learned["train_s"] = train_s
+        learned["train_rois"] = args.train_steps * 256
+        learned["train_chain_label_share"] = args.chain_labels
+        line["learned"] = learned
+        line["learned_over_plain"] = learned["value"] / plain["value"] if plain["value"] > 0 else None
+        line["learned_over_plain_extrapolated"] = (learned["extrapolated_fixed_ess_rate"] /
+                                                   plain["extrapolated_fixed_ess_rate"])
+        mm.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def front_end_line(args):
     res = {"find_roi": bench_find_roi, "gauss_mle": bench_gauss_mle, "find_map": bench_find_map,
            "tiles": bench_tiles}[args.workload](args)
@@ -489,7 +650,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batched_uneven",
-                    choices=["batched_uneven", "plain_16", "find_roi", "gauss_mle", "find_map", "tiles"])
+                    choices=["batched_uneven", "plain_16", "find_roi", "gauss_mle", "find_map", "tiles",
+                             "dense_lowsnr"])
+    ap.add_argument("--dense-frames", type=int, default=5000, help="frames of --workload dense_lowsnr")
+    ap.add_argument("--dense-rois", type=int, default=20000, help="ROIs chained by --workload dense_lowsnr")
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--cpu-sample-frames", type=int, default=2000)
     ap.add_argument("--cpu-sample-fits", type=int, default=100000)
@@ -501,7 +665,10 @@ def main():
     ap.add_argument("--thin", type=int, default=5)
     ap.add_argument("--proposal", default="both", choices=["plain", "learned", "both"],
                     help="chain proposal(s) timed: the value is the plain chain's, learned is reported beside it")
-    ap.add_argument("--train-steps", type=int, default=3000, help="Adam steps of 256 ROIs for the learned proposal")
+    ap.add_argument("--train-steps", type=int, default=4000,
+                    help="Adam steps of 256 device-generated ROIs for the learned proposal (4000: 1,024,000 ROIs)")
+    ap.add_argument("--chain-labels", type=float, default=0.125,
+                    help="share of training steps labelled by plain-chain posterior samples instead of the truth")
     ap.add_argument("--fit-type", default="Basic", choices=["Basic", "Sigma", "SigmaYX", "Z"],
                     help="gaussMLE fit type of --workload gauss_mle")
     ap.add_argument("--camera", default="CCD", choices=["CCD", "SCMOS"],
@@ -519,6 +686,8 @@ def main():
 
     if args.copy_mib:
         return copy_calibration(args.copy_mib)
+    if args.workload == "dense_lowsnr":
+        return bench_dense_lowsnr(args)
     if args.workload in ("find_roi", "gauss_mle", "find_map", "tiles"):
         return front_end_line(args)
     rois, nb, nt, rho, sig, off, back, bg0 = workload(args.workload)
@@ -624,7 +793,8 @@ def main():
               flush=True)
     if args.proposal in ("learned", "both"):
         t0 = time.perf_counter()
-        prm, hist = proposal.train(args.train_steps, batch=256, lr=3e-3, seed=SEED % 1000)
+        prm, hist = proposal.train(args.train_steps, batch=256, lr=3e-3, seed=SEED % 1000,
+                                   chain_labels=args.chain_labels)
         train_s = time.perf_counter() - t0
         print(f"[bench] network trained: {args.train_steps} x 256 ROIs in {train_s:.1f} s", file=sys.stderr,
               flush=True)
@@ -646,7 +816,8 @@ def main():
         net_tflops = proposal.NET_FLOP_PER_CANVAS * n / (map_ms / 1e3) / 1e12
         line["learned"] = {
             "value": lvalue, "ms_per_step": lms, "ms_per_step_mean": lms_mean, "maps_ms": map_ms, "train_s": train_s,
-            "train_rois": args.train_steps * 256, "strict_posteriors_per_s": lvalue,
+            "train_rois": args.train_steps * 256, "train_chain_label_share": args.chain_labels,
+            "strict_posteriors_per_s": lvalue,
             "extrapolated_fixed_ess_rate": lextrap, "roi_posteriors": int(lok.sum()),
             "ess_sum_rate": float(less.sum() / 200.0 / (lms / 1e3)),
             "mean_ess": float(less.mean()), "median_ess": float(np.median(less)), "rmse_nm_final_state": lrmse,

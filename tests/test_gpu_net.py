@@ -43,6 +43,16 @@ def test_training_lowers_loss():
     assert last < 0.6 * first
 
 
+def test_training_with_chain_labels():
+    """Device-generated batches, every second one labelled by plain chains'
+    final states: the labelled targets follow the truth rule on those states
+    and training still lowers the loss."""
+    p, hist = proposal.train(120, batch=64, lr=3e-3, seed=3, log_every=20, chain_labels=0.5, chain_jumps=1000)
+    first, last = hist[0][1] + hist[0][2], hist[-1][1] + hist[-1][2]
+    print("loss", first, "->", last, flush=True)
+    assert np.isfinite(p).all() and last < 0.8 * first
+
+
 def test_proposal_maps_equal_per_roi_network():
     """Maps from the 32 x 32 canvas with masking equal the network run on each
     ROI at its own size (zero padding at the ROI border)."""
