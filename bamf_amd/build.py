@@ -80,7 +80,30 @@ def build(verbose: bool = False, jobs: int = 4) -> Path:
         sys.stderr.write(r.stdout.decode(errors="replace"))
         raise RuntimeError("bench harness build failed")
     os.replace(str(exe) + ".tmp", exe)
+    build_mex(run)
     return LIB
+
+
+MEX_DIR = PKG / "mex"
+MEX_WRAPPERS = ["cRJMCMC_gpu", "findROI_gpu", "gaussMLE_gpu", "findPriors_gpu", "findPSF_gpu"]
+
+
+def build_mex(run) -> None:
+    """The mex wrappers as host shared libraries (g++) against the project's mx
+    runtime (csrc/mex/mx), linked to libbamf_gpu.so: bamf_amd/mex/<name>.so,
+    loaded by bamf_amd/mexsim.py."""
+    MEX_DIR.mkdir(exist_ok=True)
+    gxx = shutil.which("g++") or "g++"
+    for name in MEX_WRAPPERS:
+        out = MEX_DIR / f"{name}.so"
+        cmd = [gxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{ROOT / 'include'}",
+               f"-I{CSRC / 'mex' / 'mx'}", str(CSRC / "mex" / f"{name}.cpp"), str(CSRC / "mex" / "mx" / "mx.cpp"),
+               f"-L{PKG}", "-lbamf_gpu", "-Wl,-rpath,$ORIGIN/..", "-o", str(out) + ".tmp"]
+        _, r = run(cmd)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout.decode(errors="replace"))
+            raise RuntimeError(f"mex wrapper build failed: {name}")
+        os.replace(str(out) + ".tmp", out)
 
 
 if __name__ == "__main__":

@@ -1,8 +1,9 @@
 // MATLAB mex entry with the argument list and outputs of cRJMCMC
 // (source/c/cRJMCMC.cpp:52-305), running the chain on the GPU through
-// bamf_crjmcmc.  Build inside MATLAB (MATLAB's mex.h is needed; this image
-// has no MATLAB, so this file is not compiled by bamf_amd/build.py):
+// bamf_crjmcmc.  Inside MATLAB:
 //   mex -R2018a cRJMCMC_gpu.cpp -I<repo>/include -L<repo>/bamf_amd -lbamf_gpu
+// bamf_amd/build.py also builds it with g++ against the project's mx runtime
+// (mx/mex.h, mx/mx.cpp) into bamf_amd/mex/, where bamf_amd/mexsim.py calls it.
 //
 // [PGrid, PBack, Chain, ProposedChain, Stat] =
 //     cRJMCMC_gpu(Data, E_active, MCMC, Xsize, Ysize, OffSetPDF, SignalPDF, BackPDF,
@@ -19,6 +20,14 @@ float fieldf(const mxArray* s, const char* f) {
     const mxArray* v = mxGetField(s, 0, f);
     if (!v) mexErrMsgIdAndTxt("bamf:field", "MCMC struct lacks field %s", f);
     return (float)mxGetScalar(v);
+}
+
+// a single-precision field of at least n elements (n = 0: may be empty)
+const float* single_field(const mxArray* s, const char* f, size_t n) {
+    const mxArray* v = mxGetField(s, 0, f);
+    if (!v || !mxIsSingle(v) || mxGetNumberOfElements(v) < n)
+        mexErrMsgIdAndTxt("bamf:field", "field %s must be single with at least %zu elements", f, n);
+    return (const float*)mxGetData(v);
 }
 
 int longest_side(const mxArray* a) { return (int)(mxGetM(a) > mxGetN(a) ? mxGetM(a) : mxGetN(a)); }
@@ -60,6 +69,9 @@ mxArray* chain_struct(const std::vector<bamf_chain_rec>& rec, const std::vector<
 
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     if (nrhs < 11) mexErrMsgIdAndTxt("bamf:args", "cRJMCMC_gpu needs the 11 arguments of cRJMCMC");
+    if (!mxIsStruct(prhs[2])) mexErrMsgIdAndTxt("bamf:args", "MCMC must be a struct");
+    for (int k : {0, 5, 6, 7, 8, 9})
+        if (!mxIsSingle(prhs[k])) mexErrMsgIdAndTxt("bamf:args", "argument %d must be single", k + 1);
     const mxArray* E = prhs[1];
     const mxArray* M = prhs[2];
     bamf_mcmc_params p = {};
@@ -70,8 +82,8 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     p.i_std_bg = fieldf(M, "I_stdBg");
     p.x_std_fg = fieldf(M, "X_stdFg");
     p.x_std_bg = fieldf(M, "X_stdBg");
-    const float* pb = (const float*)mxGetData(mxGetField(M, 0, "P_Burnin"));  // single[8], as make_Params reads
-    const float* pt = (const float*)mxGetData(mxGetField(M, 0, "P_Trials"));
+    const float* pb = single_field(M, "P_Burnin", 8);  // single[8], as make_Params reads
+    const float* pt = single_field(M, "P_Trials", 8);
     for (int i = 0; i < 8; i++) {
         p.p_burnin[i] = pb[i];
         p.p_trials[i] = pt[i];
@@ -89,7 +101,12 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     p.is_backg = (int32_t)fieldf(M, "IsBackg");
 
     const int xsize = (int)mxGetScalar(prhs[3]), ysize = (int)mxGetScalar(prhs[4]);
+    if (!mxIsStruct(E) || !mxGetField(E, 0, "N") || !mxGetField(E, 0, "BG"))
+        mexErrMsgIdAndTxt("bamf:field", "E_active needs fields N, I, X, Y and BG");
     const int n0 = (int)mxGetScalar(mxGetField(E, 0, "N"));
+    const float* ei = single_field(E, "I", (size_t)(n0 > 0 ? n0 : 0));
+    const float* ex = single_field(E, "X", (size_t)(n0 > 0 ? n0 : 0));
+    const float* ey = single_field(E, "Y", (size_t)(n0 > 0 ? n0 : 0));
     const uint64_t seed = nrhs > 11 ? (uint64_t)mxGetScalar(prhs[11]) : 0;
     // SampledPSF (prhs[9]) with its size from MCMC.PSFsize (cRJMCMC.cpp:35), as
     // rjmcmc.m:91 sets it; the array's own dimensions when the field is absent
@@ -107,6 +124,11 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
                            (int64_t)mxGetNumberOfElements(prhs[9]) !=
                                (int64_t)psf_size[0] * psf_size[1] * psf_size[2] * psf_size[3]))
         mexErrMsgIdAndTxt("bamf:args", "SampledPSF must be single with PSFsize elements");
+    const size_t npx = (size_t)(xsize > 0 ? xsize : 0) * (size_t)(ysize > 0 ? ysize : 0);
+    if (mxGetNumberOfElements(prhs[0]) != npx)
+        mexErrMsgIdAndTxt("bamf:args", "Data must hold Xsize * Ysize pixels");
+    if (mxGetNumberOfElements(prhs[8]) != 0 && mxGetNumberOfElements(prhs[8]) != npx)
+        mexErrMsgIdAndTxt("bamf:args", "sCMOScorr must be empty or hold Xsize * Ysize pixels");
     const int64_t total = (int64_t)p.n_burnin + p.n_trials;
     const int32_t rec_cap = (int32_t)(total + 1);
     const int32_t em_cap = (int32_t)(total * 16 + 1024);
@@ -118,9 +140,8 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     plhs[0] = mxCreateNumericArray(2, dims, mxSINGLE_CLASS, mxREAL);
     plhs[1] = mxCreateNumericArray(2, dims, mxSINGLE_CLASS, mxREAL);
     const int rc = bamf_crjmcmc(
-        &p, (const float*)mxGetData(prhs[0]), xsize, ysize, n0, (const float*)mxGetData(mxGetField(E, 0, "I")),
-        (const float*)mxGetData(mxGetField(E, 0, "X")), (const float*)mxGetData(mxGetField(E, 0, "Y")),
-        (float)mxGetScalar(mxGetField(E, 0, "BG")), (const float*)mxGetData(prhs[5]), longest_side(prhs[5]),
+        &p, (const float*)mxGetData(prhs[0]), xsize, ysize, n0, ei, ex, ey, (float)mxGetScalar(mxGetField(E, 0, "BG")),
+        (const float*)mxGetData(prhs[5]), longest_side(prhs[5]),
         (const float*)mxGetData(prhs[6]), longest_side(prhs[6]), (const float*)mxGetData(prhs[7]),
         longest_side(prhs[7]), (const float*)mxGetData(prhs[8]), (const float*)mxGetData(prhs[9]), psf_size,
         opt_model, seed,

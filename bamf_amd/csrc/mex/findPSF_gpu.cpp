@@ -1,7 +1,9 @@
 // MATLAB mex wrapper: SampledPSFm = findPSF_gpu(ROIStack, Gain, Offset, PSFSigma, BoxSize)
 // with the arguments and output of @RJ/findPSF.m:1 (a drop-in for RJ.findPSF at
 // @RJ/findPSF_SMA.m:70 and @RJ/RJ.m:328), over bamf_find_psf_host.
-// Built inside MATLAB: mex -R2018a findPSF_gpu.cpp -L<dir> -lbamf_gpu
+// Inside MATLAB: mex -R2018a findPSF_gpu.cpp -L<dir> -lbamf_gpu; bamf_amd/build.py
+// also builds it with g++ against the project's mx runtime (mx/mex.h, mx/mx.cpp)
+// into bamf_amd/mex/, where bamf_amd/mexsim.py calls it.
 #include <cstdint>
 
 #include "mex.h"

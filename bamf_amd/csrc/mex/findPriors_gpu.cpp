@@ -1,7 +1,8 @@
 // MATLAB mex entry for the prior construction of @RJ/findPriors.m:35-80 on
-// the GPU through bamf_find_priors_host.  Build inside MATLAB (mex.h is
-// needed; not compiled by bamf_amd/build.py):
+// the GPU through bamf_find_priors_host.  Inside MATLAB:
 //   mex -R2018a findPriors_gpu.cpp -I<repo>/include -L<repo>/bamf_amd -lbamf_gpu
+// bamf_amd/build.py also builds it with g++ against the project's mx runtime
+// (mx/mex.h, mx/mx.cpp) into bamf_amd/mex/, where bamf_amd/mexsim.py calls it.
 //
 // [P_EmPhotons, P_BgEmPhotons, P_Offset] = findPriors_gpu(Data, PSF_Sigma, SMAStruct, P_dP, BgPriorRange, Y_N_BgPrior)
 // (the RJStruct fields findPriors.m:111-113 sets; SMAStruct: MeanPhotons,

@@ -1,7 +1,9 @@
 // MATLAB mex entry with the argument list and outputs of findROI
 // (@RJ/findROI.m:1-163), detecting on the GPU through bamf_find_roi_host.
-// Build inside MATLAB (mex.h is needed; not compiled by bamf_amd/build.py):
+// Inside MATLAB:
 //   mex -R2018a findROI_gpu.cpp -I<repo>/include -L<repo>/bamf_amd -lbamf_gpu
+// bamf_amd/build.py also builds it with g++ against the project's mx runtime
+// (mx/mex.h, mx/mx.cpp) into bamf_amd/mex/, where bamf_amd/mexsim.py calls it.
 //
 // [SMD, ROIStack, Res] = findROI_gpu(Image, Sigma1, Sigma2, Boxsize, Minval)
 // with findROI.m:60-75's defaults for missing arguments.

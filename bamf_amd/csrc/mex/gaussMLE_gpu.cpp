@@ -1,8 +1,10 @@
 // MATLAB mex entry with the argument list and outputs of gaussMLE for the
 // fits of FitType 'Basic', 'Sigma', 'SigmaYX' and 'Z' with CameraType 'CCD' or 'SCMOS'
 // (@RJ/gaussMLE.m:64-211), fitting on the GPU through bamf_gauss_mle_fit_host.
-// Build inside MATLAB (mex.h is needed; not compiled by bamf_amd/build.py):
+// Inside MATLAB:
 //   mex -R2018a gaussMLE_gpu.cpp -I<repo>/include -L<repo>/bamf_amd -lbamf_gpu
+// bamf_amd/build.py also builds it with g++ against the project's mx runtime
+// (mx/mex.h, mx/mx.cpp) into bamf_amd/mex/, where bamf_amd/mexsim.py calls it.
 //
 // [Results, Statistics] = gaussMLE_gpu(Data, FitType, 'CCD', Sigma[, ZStruct])
 // [Results, Statistics] = gaussMLE_gpu(Data, FitType, 'SCMOS', Sigma, VarianceImage, BoxCorners[, ZStruct])

@@ -350,38 +350,113 @@ def make_replay(tapes: list[np.ndarray], ref_steps: np.ndarray) -> dict:
     return d
 
 
+STAT_NAMES = ["NTrials", "Accept_Jump", "Accept_Split", "Accept_Merge", "Accept_Birth", "Accept_Death",
+              "Accept_JumpFg", "Accept_SplitFg", "Accept_MergeFg", "Accept_BirthFg", "Accept_DeathFg",
+              "Accept_JumpBg", "Accept_SplitBg", "Accept_MergeBg", "Accept_BirthBg", "Accept_DeathBg"]
+CHAIN_FIELDS = ["N", "X", "Y", "Photons", "Signal", "LLR", "PR", "JumpType", "Accepted", "BG", "ABG", "BBG"]
+
+
+def _chain_entries(rec, meta, em, nrec):
+    """The Chain struct array of cRJMCMC.cpp:159-229: NLoop entries, the last
+    left empty; N is 0 for the empty state (one placeholder emitter of zero
+    photons); LLR, PR, JumpType and Accepted are the accepted chain's for both
+    outputs (cRJMCMC.cpp:265-272)."""
+    out = []
+    for k in range(nrec - 1):
+        h, m = rec[k], meta[k]
+        nn = max(int(h["n"]), 1)
+        e = em[int(h["em_index"]):int(h["em_index"]) + nn].astype(np.float64)
+        out.append({"N": 0.0 if e[0, 2] == 0 else float(h["n"]), "X": e[:, 0], "Y": e[:, 1], "Photons": e[:, 2],
+                    "Signal": e[:, 3], "LLR": float(m["llr"]), "PR": float(m["pr"]),
+                    "JumpType": float(m["jump_type"]), "Accepted": float(m["accepted"]), "BG": float(h["bg"]),
+                    "ABG": float(h["abg"]), "BBG": float(h["bbg"])})
+    if nrec > 0:
+        out.append({f: None for f in CHAIN_FIELDS})
+    return out
+
+
+def crjmcmc(params: McmcParams, data: np.ndarray, xsize: int, ysize: int, e_active: dict, off_pdf, sig_pdf,
+            back_pdf, scmos=None, sampled_psf=None, opt_model: int = 0, seed: int = 0):
+    """cRJMCMC (cRJMCMC.cpp:52-305) for one ROI through the host gateway
+    bamf_crjmcmc.  data: Ysize x Xsize (MATLAB orientation; any numpy order is
+    read column-major); e_active: N, I, X, Y, BG.  Returns (PGrid, PBack, Chain,
+    ProposedChain, Stat): PGrid and PBack (Ysize*Zoom, Xsize*Zoom) float32,
+    the chains as lists of NLoop dicts with cRJMCMC's fields, Stat a dict."""
+    d = np.asarray(data, np.float32)
+    if d.size != xsize * ysize:
+        raise BamfError("Data must hold Xsize * Ysize pixels")
+    flat = np.asarray(d, order="F").ravel(order="F")
+    n0 = int(e_active.get("N", 0))
+    f32 = lambda v: np.ascontiguousarray(np.asarray(v, np.float32).ravel())  # noqa: E731
+    i0, x0, y0 = (f32(e_active.get(k, [])) for k in ("I", "X", "Y"))
+    if min(i0.size, x0.size, y0.size) < n0:
+        raise BamfError("E_active.I, X and Y must hold N entries")
+    off, sig, back = f32(off_pdf), f32(sig_pdf), f32(back_pdf)
+    sc = None if scmos is None or np.size(scmos) == 0 else np.asarray(scmos, np.float32).ravel(order="F")
+    if sc is not None and sc.size != flat.size:
+        raise BamfError("sCMOScorr must be empty or hold Xsize * Ysize pixels")
+    psf_flat = psf_size = None
+    if sampled_psf is not None:
+        sp = np.asarray(sampled_psf, np.float32)
+        psf_size = np.array(list(sp.shape) + [1] * (4 - sp.ndim), np.int32)
+        psf_flat = np.ascontiguousarray(sp.ravel(order="F"))
+    z = int(params.grid_zoom)
+    pg = np.zeros(xsize * ysize * z * z, np.float32)
+    pb = np.zeros_like(pg)
+    total = int(params.n_burnin) + int(params.n_trials)
+    rec_cap, em_cap = total + 1, total * 16 + 1024
+    acc, prop = np.zeros(rec_cap, CHAIN_REC_DTYPE), np.zeros(rec_cap, CHAIN_REC_DTYPE)
+    ae, pe = np.zeros((em_cap, 4), np.float32), np.zeros((em_cap, 4), np.float32)
+    counts = np.zeros(3, np.int64)
+    stat = np.zeros(16, np.float64)
+    ptr = lambda a: None if a is None or a.size == 0 else a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    check(lib().bamf_crjmcmc(ctypes.byref(params), ptr(flat), int(xsize), int(ysize), n0, ptr(i0), ptr(x0), ptr(y0),
+                             ctypes.c_float(float(e_active.get("BG", 0.0))), ptr(off), off.size, ptr(sig), sig.size,
+                             ptr(back), back.size, ptr(sc), ptr(psf_flat), ptr(psf_size), int(opt_model), int(seed),
+                             ptr(pg), ptr(pb), ptr(acc), ptr(prop), ptr(ae), ptr(pe), rec_cap, em_cap, ptr(counts),
+                             ptr(stat)), "bamf_crjmcmc")
+    if counts[1] > em_cap or counts[2] > em_cap:
+        raise BamfError("chain record capacity exceeded")
+    nrec = int(counts[0])
+    shape = (ysize * z, xsize * z)
+    return (pg.reshape(shape, order="F"), pb.reshape(shape, order="F"), _chain_entries(acc, acc, ae, nrec),
+            _chain_entries(prop, acc, pe, nrec), dict(zip(STAT_NAMES, stat.tolist())))
+
+
 def rjmcmc(data: np.ndarray, rjstruct: dict, ysize: int, xsize: int, drift_y: float = 0.0, drift_x: float = 0.0,
-           scmos_var=None, seed: int = 0):
-    """@RJ/rjmcmc.m for one ROI: builds the MCMC struct and initial state as
-    rjmcmc.m:43-101 does and runs one chain.  Returns (Chain, Stat, PImage, PBackg)
-    where Chain holds the per-step N, BG, LLR, PR, JumpType and Accepted."""
+           scmos_var=None, sampled_psf_in=None, opt_model_in=None, seed: int = 0):
+    """@RJ/rjmcmc.m for one ROI: the MCMC struct, initial state, sCMOS and
+    SampledPSF defaults of rjmcmc.m:40-102, then cRJMCMC (crjmcmc).  Returns
+    (Chain, Stat, PImage, PBackg, ProposedChain); PImage and PBackg are
+    SRZoom times the size of Data with the ROI's block filled
+    (rjmcmc.m:95-96,104-105)."""
+    d0 = np.asarray(data, np.float32)
+    if d0.ndim > 2 and d0.shape[2] > 1:
+        raise BamfError("The input data to rjmcmc() must be a single ROI.")
+    d0 = d0.reshape(d0.shape[0], -1) if d0.ndim > 2 else np.atleast_2d(d0)
     r = rjstruct
-    p_off = np.asarray(r["P_Offset"], np.float32)
-    bg0 = np.float32(r["P_dP"] * (int(np.argmax(p_off == p_off.max())) + 1 - 1))
+    p_off = np.asarray(r["P_Offset"], np.float64).ravel()
+    bg0 = np.float32(r["P_dP"] * int(np.flatnonzero(p_off == p_off.max())[0]))  # rjmcmc.m:46
     params = make_params(psf_sigma=r["PSF_Sigma"], n_burnin=r["N_Burnin"], n_trials=r["N_Trials"],
                          p_burnin=r["P_Burnin"], p_trials=r["P_Trials"], rho=r["Rho"], x_std_fg=r["XstdFg"],
                          i_std_fg=r["IstdFg"], bg_std=r["BGstd"], dx=r["P_dP"], grid_zoom=r["SRZoom"],
                          icutoff=r.get("MinPhoton", 0.0), is_backg=r.get("IsBackg", 0), drift_x=drift_x,
                          drift_y=drift_y)
-    plan = Plan(params, p_off, r["P_EmPhotons"], r["P_BgEmPhotons"])
-    d = np.asarray(data, np.float32)[:ysize, :xsize]
-    flat = np.asarray(d, order="F").ravel(order="F")  # column-major, y fastest
-    sc = None
+    d = d0[:ysize, :xsize]
+    sc = None  # rjmcmc.m:79-81: a scalar or absent sCMOSVar is zeros (adds nothing to the model)
     if scmos_var is not None and np.ndim(scmos_var) == 2:
-        sc = np.asarray(scmos_var, np.float32)[:ysize, :xsize].ravel(order="F")
-    rois = RoiSet(flat, np.array([0, flat.size]), [xsize], [ysize], bg0=[bg0], scmos=sc)
-    res = run_batch(plan, rois, seed=seed, record_steps=True, grids=True, hist_len=129)
-    st = res.steps[0]
-    rec = st[(st["flags"] & 2) != 0]
-    chain = {"N": rec["n"], "BG": rec["bg"], "LLR": rec["llr"], "PR": rec["pr"], "JumpType": rec["jump_type"],
-             "Accepted": rec["flags"] & 1}
-    c = res.counters[0].astype(np.float64)
-    stat = {"NTrials": float(params.n_trials + params.n_burnin)}
-    for i, nm in enumerate(("Jump", "Split", "Merge", "Birth", "Death")):
-        stat[f"Accept_{nm}"] = c[5 + i] / c[i]
-        stat[f"Accept_{nm}Fg"] = c[15 + i] / c[10 + i]
-        stat[f"Accept_{nm}Bg"] = c[25 + i] / c[20 + i]
-    z = params.grid_zoom
-    pimg = res.pgrid.reshape(xsize * z, ysize * z).T.copy()
-    pbg = res.pback.reshape(xsize * z, ysize * z).T.copy()
-    return chain, stat, pimg, pbg
+        sc = np.asarray(scmos_var, np.float32)[:ysize, :xsize]
+    psf, opt = None, 0  # rjmcmc.m:82-85,98-101: no SampledPSF means OptModel 0
+    if sampled_psf_in is not None and np.ndim(sampled_psf_in) > 0 and np.size(sampled_psf_in) > 0 and \
+            opt_model_in is not None and np.size(opt_model_in) > 0:
+        opt = int(np.asarray(opt_model_in).ravel()[0])
+        psf = sampled_psf_in if opt == 1 else None
+    e_active = {"N": 0, "I": [], "X": [], "Y": [], "BG": bg0}
+    pimg, pbg, chain, proposed, stat = crjmcmc(params, d, xsize, ysize, e_active, r["P_Offset"], r["P_EmPhotons"],
+                                               r["P_BgEmPhotons"], sc, psf, opt, seed)
+    z = int(params.grid_zoom)
+    PImage = np.zeros((z * d0.shape[0], z * d0.shape[1]), np.float32)
+    PBackg = np.zeros_like(PImage)
+    PImage[:z * ysize, :z * xsize] = pimg
+    PBackg[:z * ysize, :z * xsize] = pbg
+    return chain, stat, PImage, PBackg, proposed
