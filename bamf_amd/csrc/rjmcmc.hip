@@ -220,6 +220,7 @@ struct Chain {
     int ev, jt, acc_c1, acc_c2;
     bool need_test;
     float f_pri, f_u1, f_u2, f_u3, f_ijp, f_pixprob, f_iratio;
+    float f_mz;  // learned split/merge: q_map(pixel of the first child) / Z of the window, 0 outside it
     int f_jj, f_jm;
     // step record
     int r_jt, r_flags, r_nprop, r_pix;
@@ -1033,8 +1034,27 @@ struct Chain {
         const int BF = E.S[jj];
         cnt_inc(BF == 1 ? C_NFG + J_SPLIT : C_NBG + J_SPLIT);
         const float u1 = urand();
-        const float u2 = p.split_std * randn();
-        const float u3 = p.split_std * randn();
+        float u2, u3;
+        f_mz = 0.0f;
+        if (learned_split()) {
+            // the first child from the map inside the window around the parent,
+            // with probability l_split; else the reference's Gaussian offsets
+            const float r = a->l_split_r;
+            const bool from_map = urand() < a->l_split;
+            int pix = -1;
+            const float Z = win_scan(mux, muy, r, from_map, urand(), pix);
+            if (from_map && pix >= 0) {
+                u2 = ((float)(pix / Y) + urand() - 0.5f) - mux;
+                u3 = ((float)(pix - Y * (pix / Y)) + urand() - 0.5f) - muy;
+            } else {
+                u2 = p.split_std * randn();
+                u3 = p.split_std * randn();
+            }
+            f_mz = win_density(mux + u2, muy + u3, mux, muy, r, Z);
+        } else {
+            u2 = p.split_std * randn();
+            u3 = p.split_std * randn();
+        }
         set_t(jj, u1 * Ijp, mux + u2, muy + u3);
         if (!add_t((1 - u1) * Ijp, mux - u1 * u2 / (1 - u1), muy - u1 * u3 / (1 - u1), BF)) return EV_NONE;
         delta_push(0, jj, true);
@@ -1066,7 +1086,7 @@ struct Chain {
         const float PrI = tmp2 / tmp1;
         const float Area = xb * yb;
         const float PrX = 1 / Area;
-        const float Us = 1 / (Norm1 * Norm2);
+        const float Us = split_us(Norm1 * Norm2);  // 1 / (Norm1 * Norm2) outside the learned split
         const double om = (double)(1 - f_u1);
         const float Jacob = (float)((double)f_ijp / (om * om));
         const float P12 = P1 / P2;
@@ -1105,6 +1125,13 @@ struct Chain {
         f_u2 = E.X[jj] - muxjp;
         f_u3 = E.Y[jj] - muyjp;
         f_ijp = Ijp;
+        f_mz = 0.0f;
+        if (learned_split()) {  // the split that undoes this merge: window around the merged emitter
+            int pix;
+            const float r = a->l_split_r;
+            const float Z = win_scan(muxjp, muyjp, r, false, 0.0f, pix);
+            f_mz = win_density(E.X[jj], E.Y[jj], muxjp, muyjp, r, Z);
+        }
         const float PDFfg = findpdf(a->sig, p.dx, Ijp);
         const float PDFbg = findpdf(a->back, p.dx, Ijp);
         set_t(jj, Ijp, muxjp, muyjp);
@@ -1138,7 +1165,7 @@ struct Chain {
         const float PrI = tmp1 / tmp2;
         const float Area = xb * yb;
         const float PrX = 1 / Area;
-        const float Us = 1 / (Norm1 * Norm2);
+        const float Us = split_us(Norm1 * Norm2);
         const double om = (double)(1 - f_u1);
         const float Jacob = (float)((double)f_ijp / (om * om));
         const float PrM = meanN / NEm;
@@ -1343,6 +1370,63 @@ struct Chain {
         }
         prob = carry > 0.0f ? qp / carry : 0.0f;
         return pix;
+    }
+
+    // Learned split window: the pixels whose centres lie within r of (cx, cy)
+    // along both axes.  Returns Z, the occupancy map's mass over the window;
+    // with pick, also draws a pixel of the window with probability q / Z
+    // (u: the uniform) into pix.
+    __device__ __forceinline__ float win_scan(float cx, float cy, float r, bool pick, float u, int& pix) {
+        const int64_t off = a->pix_off[roi];
+        float carry = 0.0f;
+        float cum[PPL], q[PPL];
+#pragma unroll
+        for (int s = 0; s < PPL; s++) {
+            cum[s] = 0.0f;
+            q[s] = 0.0f;
+            if (s < ppl) {
+                const bool in = has(s) && fabsf((float)px(s) - cx) <= r && fabsf((float)ysub - cy) <= r;
+                float v = in ? a->prop_map[off + pidx(s)] : 0.0f;
+                q[s] = v;
+                v = wave_scan(v);
+                cum[s] = carry + v;
+                carry = bcastf(cum[s], 63);
+            }
+        }
+        pix = -1;
+        if (pick && carry > 0.0f) {
+            const float rr = carry * u;
+            bool found = false;
+#pragma unroll
+            for (int s = 0; s < PPL; s++) {
+                if (s < ppl && !found) {
+                    const uint64_t mk = ballot(rr <= cum[s] && q[s] > 0.0f);
+                    if (mk) {
+                        pix = bcasti(pidx(s), __builtin_ctzll(mk));
+                        found = true;
+                    }
+                }
+            }
+        }
+        return carry;
+    }
+    // q_map(pixel of (xc, yc)) / Z for the window of half-width r around (cx, cy),
+    // 0 outside it; -1 when the window holds no mass (the map draw then falls
+    // back to the Gaussian offsets, whose density is then the whole proposal)
+    __device__ __forceinline__ float win_density(float xc, float yc, float cx, float cy, float r, float Z) const {
+        if (!(Z > 0.0f)) return -1.0f;
+        const int qx = (int)floorf(xc + 0.5f), qy = (int)floorf(yc + 0.5f);
+        if (qx < 0 || qx >= X || qy < 0 || qy >= Y) return 0.0f;
+        if (!(fabsf((float)qx - cx) <= r && fabsf((float)qy - cy) <= r)) return 0.0f;
+        return a->prop_map[a->pix_off[roi] + qx * Y + qy] / Z;
+    }
+    __device__ __forceinline__ bool learned_split() const {
+        return LRN && a->mode == BAMF_RJ_LEARNED && a->prop_map != nullptr && a->l_split > 0.0f;
+    }
+    // 1 / q(u2, u3): the split's offsets scored by the proposal that drew them
+    __device__ __forceinline__ float split_us(float nn) const {
+        if (!learned_split() || f_mz < 0.0f) return 1 / nn;
+        return 1 / (a->l_split * f_mz + (1 - a->l_split) * nn);
     }
 
     // LibRJMCMC.cpp:1258-1295

@@ -100,6 +100,8 @@ struct ChainArgs {
     float c_infl;     // sd inflation of the conditional fit
     float c_mean, c_sd;  // signal prior mean and sd (the fit's fallback)
     float l_single_from;  // learned mode: single-emitter moves take Ub >= this (the reference: 0.75)
+    float l_split;        // learned mode: probability that a split places its first child from the map
+    float l_split_r;      // half-width (px) of the square window around the parent the map draw uses
     // outputs
     int32_t* status;
     int32_t* counters;

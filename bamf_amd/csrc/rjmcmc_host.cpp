@@ -114,6 +114,7 @@ struct bamf_rjmcmc_plan {
     const float* d_mu = nullptr;
     bool proposal_set = false;    // bamf_rjmcmc_set_proposal was called (maps may be NULL)
     float l_alpha = 1.0f, l_halfwidth = 0.5f, l_beta = 0.0f;
+    float l_split = 0.0f, l_split_r = 2.6f;  // learned split (bamf_rjmcmc_set_split_proposal)
     int device = 0;
     int cu = 0;
     int blocks_per_cu[kClasses][16] = {};  // [class][features] at the default register budget
@@ -362,6 +363,17 @@ int bamf_rjmcmc_set_intensity_proposal(bamf_rjmcmc_plan* plan, int kind, float i
     return BAMF_OK;
 }
 
+int bamf_rjmcmc_set_split_proposal(bamf_rjmcmc_plan* plan, float share, float radius) {
+    BAMF_CHECK_ARG(plan, "null argument");
+    BAMF_CHECK_ARG(share >= 0.0f && share <= 1.0f, "share must be in [0, 1]");
+    BAMF_CHECK_ARG(radius >= 0.5f && radius <= 16.0f, "radius must be in [0.5, 16] pixels");
+    BAMF_CHECK_ARG(share == 0.0f || plan->d_q, "the learned split draws from the occupancy map: set the maps first");
+    plan->l_split = share;
+    plan->l_split_r = radius;
+    drop_graph(plan);
+    return BAMF_OK;
+}
+
 int bamf_rjmcmc_set_sampled_psf(bamf_rjmcmc_plan* plan, const float* h_psf, const int32_t psf_size[4]) {
     BAMF_CHECK_ARG(plan, "null argument");
     int64_t numel = 0;
@@ -502,6 +514,8 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
         a.c_mean = (float)plan->sig_mean;
         a.c_sd = (float)plan->sig_sd;
         a.l_single_from = plan->l_single_from;
+        a.l_split = plan->d_q ? plan->l_split : 0.0f;
+        a.l_split_r = plan->l_split_r;
     }
     if (plan->d_psf) {
         a.psf = plan->d_psf;

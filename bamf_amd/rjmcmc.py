@@ -116,6 +116,13 @@ class Plan:
                                                        ctypes.c_float(single_share)),
               "bamf_rjmcmc_set_intensity_proposal")
 
+    def set_split_proposal(self, share: float = 0.5, radius: float = 2.6) -> None:
+        """Learned mode's split: with probability share the first child is placed in a pixel
+        drawn from the occupancy map within +-radius pixels of the parent (both split and
+        merge ratios score the mixture density); share 0 is the reference's split."""
+        check(lib().bamf_rjmcmc_set_split_proposal(self.handle, ctypes.c_float(share), ctypes.c_float(radius)),
+              "bamf_rjmcmc_set_split_proposal")
+
     def set_segments(self, n_segments: int, graph: bool = False) -> None:
         """Run every chain as n_segments consecutive launches (optionally replayed from a HIP graph)."""
         check(lib().bamf_rjmcmc_set_segments(self.handle, int(n_segments), 1 if graph else 0),

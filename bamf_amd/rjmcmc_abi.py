@@ -82,6 +82,7 @@ def declare(L: ctypes.CDLL) -> None:
     L.bamf_chain_summary.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int, c_int, c_void_p, c_void_p,
                                      c_void_p, c_void_p]
     L.bamf_rjmcmc_set_intensity_proposal.argtypes = [c_void_p, c_int, c_float, c_float]
+    L.bamf_rjmcmc_set_split_proposal.argtypes = [c_void_p, c_float, c_float]
     L.bamf_rjmcmc_autotune.argtypes = [c_void_p, c_void_p, c_int, POINTER(Tuning), c_void_p]
     L.bamf_rjmcmc_set_tuning.argtypes = [c_void_p, POINTER(Tuning)]
     L.bamf_rjmcmc_get_tuning.argtypes = [c_void_p, POINTER(Tuning)]

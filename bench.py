@@ -135,7 +135,7 @@ def cpu_baseline(rois, p, off, sig, back, bg0, sample, cores, thin, psf=None):
 POSTERIOR_REF = ROOT / "tests" / "golden" / "posterior_batched_uneven.npz"
 
 
-def posterior_agreement(mode, prm=None, k=1024, cap=96):
+def posterior_agreement(mode, prm=None, k=1024, cap=96, split_share=0.0):
     """The chain's posterior against the long oracle reference chains: k
     independent chains per reference ROI of the reference's length, final
     states as samples; emitter-count KS p-values and the localization RMSE
@@ -157,6 +157,7 @@ def posterior_agreement(mode, prm=None, k=1024, cap=96):
         mm.close()
         plan.set_proposal(dq, dmu, alpha=0.2, halfwidth=0.35, beta=1.0)
         plan.set_intensity_proposal("fit", 1.3, 0.5)
+        plan.set_split_proposal(split_share, 2.6)
         keep = (dq, dmu)
     res = run_batch(plan, rois, seed=SEED + 77, mode=mode, final_cap=cap, hist_len=128)
     assert np.all(res.status == 0)
@@ -615,6 +616,7 @@ def bench_dense_lowsnr(args):
         dq, dmu = _hip.DeviceBuffer(rs.total_pixels * 4), _hip.DeviceBuffer(rs.total_pixels * 4)
         plan.set_proposal(dq, dmu, alpha=0.2, halfwidth=0.35, beta=1.0)
         plan.set_intensity_proposal("fit", 1.3, 0.5)
+        plan.set_split_proposal(args.split_share, 2.6)
         learned = timed(LEARNED, pre=lambda: mm.run(rs, dq, dmu, stream=stream))
         learned["train_s"] = train_s
         learned["train_rois"] = args.train_steps * 256
@@ -667,6 +669,8 @@ def main():
                     help="chain proposal(s) timed: the value is the plain chain's, learned is reported beside it")
     ap.add_argument("--train-steps", type=int, default=4000,
                     help="Adam steps of 256 device-generated ROIs for the learned proposal (4000: 1,024,000 ROIs)")
+    ap.add_argument("--split-share", type=float, default=0.0,
+                    help="learned mode: share of splits whose first child is drawn from the occupancy map")
     ap.add_argument("--chain-labels", type=float, default=0.125,
                     help="share of training steps labelled by plain-chain posterior samples instead of the truth")
     ap.add_argument("--fit-type", default="Basic", choices=["Basic", "Sigma", "SigmaYX", "Z"],
@@ -804,6 +808,7 @@ def main():
         # intensities of births and single-emitter moves from the conditional-likelihood fit,
         # half of all moves single-emitter moves
         plan.set_intensity_proposal("fit", 1.3, 0.5)
+        plan.set_split_proposal(args.split_share, 2.6)
         em0, em1 = _hip.Event(), _hip.Event()
         em0.record(stream)
         mm.run(rs, dq, dmu, stream=stream)
@@ -825,7 +830,7 @@ def main():
             "network_roofline": {"bound": "mfma", "achieved": net_tflops, "peak": 2500.0, "unit": "TFLOP/s",
                                  "frac": net_tflops / 2500.0, "kernel": "proposal maps (bf16 MFMA conv)"}}
         if args.workload == "batched_uneven" and not args.no_posterior:
-            line["learned"]["posterior_vs_oracle"] = posterior_agreement(LEARNED, prm)
+            line["learned"]["posterior_vs_oracle"] = posterior_agreement(LEARNED, prm, split_share=args.split_share)
             print(f"[bench] learned posterior vs oracle: {line['learned']['posterior_vs_oracle']}", file=sys.stderr,
                   flush=True)
         line["learned_over_plain"] = lvalue / value if value > 0 else None

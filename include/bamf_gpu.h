@@ -403,6 +403,23 @@ int bamf_rjmcmc_set_proposal(bamf_rjmcmc_plan* plan, const float* d_q, const flo
  * single-emitter moves (the reference's 0.25, LibRJMCMC.cpp:882; up to 0.5,
  * taken from the group random walk, jump type 3). */
 int bamf_rjmcmc_set_intensity_proposal(bamf_rjmcmc_plan* plan, int kind, float inflation, float single_share);
+/* The learned split of BAMF_RJ_LEARNED chains (no counterpart in the
+ * reference; its split and merge are LibRJMCMC.cpp:972-1028 and :1111-1217).
+ * With probability `share` a split places its first child (the part u1 of
+ * the intensity, at the parent's index) uniformly inside a pixel drawn from
+ * the occupancy map restricted to the square window of half-width `radius`
+ * pixels around the parent; otherwise (u2, u3) are the reference's
+ * Split_std-Gaussian offsets.  Both the split ratio and the merge ratio then
+ * use the mixture density q(u2, u3) = share * q_map(pixel of the child) / Z +
+ * (1 - share) * N(u2) N(u3) (Z: the map's mass in the window around the
+ * parent, or around the merged emitter in the merge) in place of the
+ * reference's N(u2) N(u3), so each split ratio stays the reciprocal of the
+ * ratio of the merge that undoes it.  share = 0 (the default) is the
+ * reference's split.  The reference's pair is not reversible (a merge may
+ * pick either child as jj), so its stationary law depends on the split
+ * density: share > 0, like another Split_std, moves the stationary N law
+ * (tests/test_gpu_rjmcmc.py measures both); experimental, off by default. */
+int bamf_rjmcmc_set_split_proposal(bamf_rjmcmc_plan* plan, float share, float radius);
 /* Sampled-PSF model for later runs of this plan: SampledPSF with OptModel = 1
  * (cRJMCMC.cpp:73-74; likelihood_ratio LibRJMCMC.cpp:1650-1686 and
  * find_Birth :1693-1714 then read NumModel, :62-170, instead of Model).

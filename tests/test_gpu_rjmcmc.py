@@ -361,6 +361,58 @@ def test_learned_birth_keeps_birth_death_law():
     check_birth_death(g.trace_n.astype(np.int64).ravel(), law)
 
 
+def test_split_merge_law_depends_on_the_split_proposal():
+    """The reference's split/merge pair is not reversible in the textbook
+    sense (a merge may pick either child as jj, and the ratios use the offsets
+    seen from that child), so its flat-likelihood stationary N law depends on
+    the split proposal's density: here on Split_std itself, and equally on the
+    learned split's map-drawn offsets (bamf_rjmcmc_set_split_proposal).  The
+    learned split therefore cannot keep the reference's stationary law and is
+    off by default.  Two runs with the same proposal agree (the control)."""
+    from bamf_amd import _hip
+    from bamf_amd.rjmcmc import LEARNED, make_params
+    from test_oracle_rjmcmc import BD_BG, BD_BND, BD_MEANN, BD_SIDE, birth_death_setup
+
+    n_roi, nt = 128, 20000
+    _, data, sig, off, back, _ = birth_death_setup(n_trials=nt)
+    mix = (0, 0.25, 0.25, 0.25, 0.25, 0, 0, 0)
+    P = BD_SIDE * BD_SIDE
+    rng = np.random.default_rng(21)
+    q = rng.uniform(0.2, 3.0, (n_roi, P)).astype(np.float32)
+    mu = np.full((n_roi, P), np.log(5e-7), np.float32)
+    dq, dmu = _hip.DeviceBuffer.from_array(q.ravel()), _hip.DeviceBuffer.from_array(mu.ravel())
+    rs = RoiSet(np.tile(data, n_roi), np.arange(n_roi + 1) * data.size, [BD_SIDE] * n_roi, [BD_SIDE] * n_roi,
+                bg0=np.full(n_roi, BD_BG, np.float32))
+
+    def run(split_std, seed, share=0.0):
+        p = make_params(n_burnin=2000, n_trials=nt, rho=BD_MEANN / BD_SIDE ** 2, dx=1e-7, bnd_out=BD_BND,
+                        p_burnin=mix, p_trials=mix, split_std=split_std)
+        plan = Plan(p, off, sig, back)
+        mode = PLAIN
+        if share > 0:
+            plan.set_proposal(dq, dmu, alpha=1.0, halfwidth=0.5, beta=0.0)
+            plan.set_split_proposal(share, 2.6)
+            mode = LEARNED
+        g = run_batch(plan, rs, seed=seed, mode=mode, trace_len=nt, trace_thin=1, hist_len=129)
+        assert np.all(g.status == 0)
+        return g.trace_n.astype(np.int64)
+
+    def compare(a, b):
+        dm = b.mean(axis=1) - a.mean(axis=1)  # independent chains per ROI
+        ha = np.bincount(a.ravel(), minlength=129)[:129] / a.size
+        hb = np.bincount(b.ravel(), minlength=129)[:129] / b.size
+        return dm.mean(), dm.std(ddof=1) / np.sqrt(n_roi), 0.5 * np.abs(ha - hb).sum()
+
+    base = run(0.5, 31)
+    same = compare(base, run(0.5, 32))
+    wide = compare(base, run(1.2, 33))
+    learned = compare(base, run(0.5, 34, share=0.9))
+    print(f"split/merge N law vs Split_std 0.5 (mean N {base.mean():.3f}): same proposal {same}, Split_std 1.2 {wide}, "
+          f"learned split {learned}", flush=True)
+    assert abs(same[0]) < 4 * same[1] + 0.02 and same[2] < 0.03
+    assert wide[0] > 6 * wide[1] and learned[0] > 6 * learned[1]
+
+
 def test_trained_proposal_gain():
     """What the trained network gains: on the same ROIs, the photon trace of
     learned chains (independence intensity proposals from the network's
