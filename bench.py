@@ -669,6 +669,11 @@ def main():
                     help="chain proposal(s) timed: the value is the plain chain's, learned is reported beside it")
     ap.add_argument("--train-steps", type=int, default=4000,
                     help="Adam steps of 256 device-generated ROIs for the learned proposal (4000: 1,024,000 ROIs)")
+    ap.add_argument("--segments", type=int, default=0,
+                    help="run each chain as this many launches (state in HBM between them)")
+    ap.add_argument("--graph", action="store_true", help="with --segments: replay the launches from a HIP graph")
+    ap.add_argument("--no-graph-line", action="store_true",
+                    help="skip timing the graph-replayed segmented run beside the one-launch run")
     ap.add_argument("--split-share", type=float, default=0.0,
                     help="learned mode: share of splits whose first child is drawn from the occupancy map")
     ap.add_argument("--chain-labels", type=float, default=0.125,
@@ -728,6 +733,12 @@ def main():
         tuning = {"chains_per_simd": best["chains_per_simd"], "concurrent": best["concurrent"], "autotuned": True,
                   "sample_ms": best["ms"], "tune_s": time.perf_counter() - t0}
         print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
+    if args.segments > 1:
+        # each chain as consecutive launches, its state in HBM between them, optionally
+        # the launch sequence captured once in a HIP graph and replayed with new seeds
+        plan.set_segments(args.segments, graph=args.graph)
+        tuning["segments"] = args.segments
+        tuning["graph"] = bool(args.graph)
 
     def timed(mode, pre=None, pre_events=None):
         """Warm-up, then args.steps timed steps (HIP events on the launch stream)."""
@@ -763,6 +774,15 @@ def main():
         return ms, res, ess, ok, rmse, fixed_ess_rate(ess, share, nb / (nb + nt)), float(step_ms.mean())
 
     ms, res, ess, ok, rmse, extrap, ms_mean = timed(PLAIN)
+    graph_line = None
+    if args.segments <= 1 and not args.no_graph_line and args.workload == "batched_uneven":
+        # the same chains as 4 launches replayed from a HIP graph (state in HBM between
+        # segments): equal results bit for bit, timed beside the one-launch run
+        plan.set_segments(4, graph=True)
+        gms, gres, *_ = timed(PLAIN)
+        same = bool(np.array_equal(gres.n_final, res.n_final) and np.array_equal(gres.counters, res.counters))
+        graph_line = {"segments": 4, "graph": True, "ms_per_step": gms, "same_chains": same}
+        plan.set_segments(1)
     value = float(ok.sum() / (ms / 1e3))
     flops = float(res.alg_flops.astype(np.float64).sum())
     achieved = flops / (ms / 1e3) / 1e12
@@ -782,7 +802,7 @@ def main():
                    "parallelism": "1 GPU, one wavefront per chain, persistent work queues per ROI size class",
                    "psf_model": "erf (OptModel 0)" if psf is None else
                    "sampled (OptModel 1), 72 x 72 x 4 x 4 table of the same Gaussian",
-                   "launch_tuning": tuning},
+                   "launch_tuning": tuning, "graph_segments": graph_line},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "k_chain (both size classes, one run)", "alg_flop_per_launch": flops},
