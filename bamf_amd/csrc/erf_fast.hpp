@@ -39,6 +39,28 @@ __device__ __forceinline__ float erfc_nr(float x) {
     return x >= 0.0f ? r : 2.0f - r;
 }
 
+// The same Chebyshev fit with the hardware reciprocal and exponential and no
+// series branch: within 1e-6 absolute of erf at about 60 % of erf_nr's
+// instructions.  The Newton fits use it: their parameters meet their bar
+// (1e-4 relative to the oracle's libm erff) with either, and the fit spends
+// a third of its instructions in erf.
+__device__ __forceinline__ float erf_nr_fast(float x) {
+    const float z = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * z);
+    float p = 0.17087277f;
+    p = fmaf(p, t, -0.82215223f);
+    p = fmaf(p, t, 1.48851587f);
+    p = fmaf(p, t, -1.13520398f);
+    p = fmaf(p, t, 0.27886807f);
+    p = fmaf(p, t, -0.18628806f);
+    p = fmaf(p, t, 0.09678418f);
+    p = fmaf(p, t, 0.37409196f);
+    p = fmaf(p, t, 1.00002368f);
+    p = fmaf(p, t, -1.26551223f);
+    const float r = t * __expf(fmaf(-z, z, p));
+    return 1.0f - (x >= 0.0f ? r : 2.0f - r);
+}
+
 __device__ __forceinline__ float erf_nr(float x) {
     // series: 2/sqrt(pi) (x - x^3/3 + x^5/10 - x^7/42 + x^9/216 - x^11/1320 + x^13/9360)
     const float x2 = x * x;

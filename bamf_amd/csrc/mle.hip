@@ -112,7 +112,7 @@ __device__ inline AxisTerms axis_terms(int i, float x, float sigma, float sqrt_n
                                        float c1, float c2) {
     AxisTerms t;
     // branch-free erf: the fits of a wavefront evaluate different ranges
-    t.psf = 0.5f * (erf_nr((i - x + 1.0f) * sqrt_norm) - erf_nr((i - x) * sqrt_norm));
+    t.psf = 0.5f * (erf_nr_fast((i - x + 1.0f) * sqrt_norm) - erf_nr_fast((i - x) * sqrt_norm));
     float ta = (i + 1.0f - x) / sigma;
     float tb = (i - x) / sigma;
     float a = __expf(-0.5f * (ta * ta));

@@ -106,11 +106,7 @@ def test_mle_sigmayx_matches_oracle(oracle, sz, n, seed):
     print(f"sigma-yx fit sz={sz} n={n} seed={seed}", flush=True)
     for k in range(6):
         r = rel(pg[k], po[k], 1.0)
-        # (x, y, I, bg) to the hot path's 1e-4; the two widths, whose Newton steps on
-        # elliptic boxes are the worst-conditioned direction of the fit (last-ulp erf
-        # differences between the device and libm grow there over 20 iterations), to 2e-4
-        bar = 1e-4 if k < 4 else 2e-4
-        assert np.quantile(r, 0.999) < bar and r.max() < 1e-3, (k, np.quantile(r, 0.999), r.max())
+        assert np.quantile(r, 0.999) < 1e-4 and r.max() < 1e-3, (k, np.quantile(r, 0.999), r.max())
     # a fit whose emitter leaves the box with both widths clamped has a singular
     # Fisher matrix: NaN in the reference's arithmetic, so non-finite on both sides
     fin = np.isfinite(co).all(0)
