@@ -43,6 +43,7 @@ namespace rj {
 namespace {
 
 constexpr int kMaxDelta = 2 * kNMax + 8;
+static_assert(kNMax <= 256, "neighbour indices are stored as bytes");
 constexpr double kPiD = 3.141592;  // LibRJMCMCreturn.hpp:2 (a double literal)
 
 enum { C_N = 0, C_A = 5, C_NFG = 10, C_AFG = 15, C_NBG = 20, C_ABG = 25 };
@@ -59,9 +60,9 @@ struct Em {
 template <int PPL>
 struct WaveLds {
     Em e[2];
-    int delta[kMaxDelta];
+    int16_t delta[kMaxDelta];  // stencil list codes: emitter | set << 8 | sign << 9
     float sten[64];
-    int nb[kNMax + 1];
+    uint8_t nb[kNMax + 1];     // neighbour indices (< kNMax)
     int hist[kNMax + 1];
     int cnt[32];
     float dpx[64 * PPL];  // the ROI's data in the pixel-slot layout: slot s at [64 s + lane]
@@ -236,9 +237,13 @@ struct Chain {
 
     __device__ __forceinline__ Em& cur() { return L.e[ci]; }
     __device__ __forceinline__ Em& tst() { return L.e[ci ^ 1]; }
-    __device__ __forceinline__ int px(int s) const { return s * cps + xsub; }
-    __device__ __forceinline__ bool has(int s) const { return ysub < Y && px(s) < X; }
-    __device__ __forceinline__ int pidx(int s) const { return px(s) * Y + ysub; }
+    __device__ __forceinline__ static int opq(int v) {
+        asm volatile("" : "+v"(v));
+        return v;
+    }
+    __device__ __forceinline__ int px(int s) const { return s * cps + opq(xsub); }
+    __device__ __forceinline__ bool has(int s) const { return opq(ysub) < Y && px(s) < X; }
+    __device__ __forceinline__ int pidx(int s) const { return px(s) * Y + opq(ysub); }
 
     // ---------------- draws ----------------
     // Window of draws wbase .. wbase + 63 of this step.  Draw j of step s is
@@ -389,7 +394,7 @@ struct Chain {
         wsync();
     }
     __device__ __forceinline__ void delta_push(int which, int k, bool neg) {
-        if (lane == 0) L.delta[ndelta] = k | (which << 8) | ((neg ? 1 : 0) << 9);
+        if (lane == 0) L.delta[ndelta] = (int16_t)(k | (which << 8) | ((neg ? 1 : 0) << 9));
         ndelta++;
     }
     // Emitters::test (LibRJMCMC.cpp:492-514) on the proposed set: only the
@@ -476,7 +481,7 @@ struct Chain {
                 const int idx = pick(m0, m1, k);
                 if (idx < 64) m0 &= ~(1ull << idx);
                 else m1 &= ~(1ull << (idx - 64));
-                if (lane == 0) L.nb[n] = idx;
+                if (lane == 0) L.nb[n] = (uint8_t)idx;
             }
         }
         wsync();
@@ -918,7 +923,7 @@ struct Chain {
         } else if (Ub >= 0.5 && Ub < (LRN && a->mode == BAMF_RJ_LEARNED ? a->l_single_from : 0.75f)) {
             jt = 3;
             int np = closest_multi(nn, 2 * p.psf_sigma);
-            if (lane == 0) L.nb[np] = nn;
+            if (lane == 0) L.nb[np] = (uint8_t)nn;
             np++;
             wsync();
             for (int ii = 0; ii < np; ii++) {
@@ -1139,7 +1144,7 @@ struct Chain {
         remove_t(ii);
         f_jj = jj;
         f_jm = ii > jj ? jj : jj - 1;  // the merged emitter's index in the proposed set
-        L.nb[0] = ii;                   // keep ii for the prior ratio (same value from every lane)
+        L.nb[0] = (uint8_t)ii;          // keep ii for the prior ratio (same value from every lane)
         delta_push(0, jj, true);
         delta_push(0, ii, true);
         delta_push(1, f_jm, false);
@@ -1187,7 +1192,7 @@ struct Chain {
         const float Ua = urand();
         const int jj = min((int)floorf((float)cN * Ua), cN - 1);
         int np = closest_multi(jj, 2 * p.psf_sigma);
-        if (lane == 0) L.nb[np] = jj;
+        if (lane == 0) L.nb[np] = (uint8_t)jj;
         np++;
         wsync();
         if (np == 1) return EV_NONE;

@@ -16,12 +16,13 @@ constexpr int kMaxSide = 32;
 // slots per lane (fewer registers, more resident chains for small ROIs).
 constexpr int kClasses = 2;
 constexpr int kClassPPL[kClasses] = {4, 16};
-constexpr int kClassMinBlocks[kClasses] = {3, 3};  // work-groups per CU the register budget targets
+constexpr int kClassMinBlocks[kClasses] = {4, 4};  // work-groups per CU the register budget targets
 // Register budgets the autotuner chooses from for the one-launch plain
 // kernels (work-groups of kWavesPerBlock waves per CU = chains per SIMD):
-// 256, 168 and 128 VGPRs.
+// 168, 128 and 96 VGPRs.  The large class fits 4 work-groups per CU in LDS
+// (39 KB each), not 5.
 constexpr int kTuneChoices = 3;
-constexpr int kTuneMinBlocks[kTuneChoices] = {2, 3, 4};
+constexpr int kTuneMinBlocks[kTuneChoices] = {3, 4, 5};
 constexpr int kTuneDefault = 1;
 
 // One prior table (Stat, LibRJMCMC.cpp:176-217) on the device.

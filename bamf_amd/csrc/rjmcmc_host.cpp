@@ -589,7 +589,7 @@ int bamf_rjmcmc_set_tuning(bamf_rjmcmc_plan* plan, const bamf_rjmcmc_tuning* t) 
         idx[c] = -1;
         for (int k = 0; k < kTuneChoices; k++)
             if (kTuneMinBlocks[k] * kWavesPerBlock / 4 == t->chains_per_simd[c]) idx[c] = k;
-        BAMF_CHECK_ARG(idx[c] >= 0, "chains_per_simd[%d] = %d is not one of the compiled budgets (2, 3, 4)", c,
+        BAMF_CHECK_ARG(idx[c] >= 0, "chains_per_simd[%d] = %d is not one of the compiled budgets (3, 4, 5)", c,
                        t->chains_per_simd[c]);
     }
     BAMF_CHECK_ARG(t->concurrent == 0 || t->concurrent == 1, "concurrent must be 0 or 1");

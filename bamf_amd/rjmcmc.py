@@ -96,7 +96,7 @@ class Plan:
                                          stream.vp if stream else None), "bamf_rjmcmc_autotune")
         return {"chains_per_simd": list(t.chains_per_simd), "concurrent": bool(t.concurrent), "ms": float(t.ms)}
 
-    def set_tuning(self, chains_per_simd=(3, 3), concurrent: bool = True) -> None:
+    def set_tuning(self, chains_per_simd=(4, 4), concurrent: bool = True) -> None:
         t = Tuning()
         t.chains_per_simd[0], t.chains_per_simd[1] = int(chains_per_simd[0]), int(chains_per_simd[1])
         t.concurrent = 1 if concurrent else 0

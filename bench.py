@@ -726,7 +726,7 @@ def main():
     cap = 128
     outs = alloc_outputs(n, nb + nt, 129, cap, trace_len, False, False, rs, p.grid_zoom)
     stream = _hip.Stream()
-    tuning = {"chains_per_simd": [3, 3], "concurrent": True, "autotuned": False}
+    tuning = {"chains_per_simd": [4, 4], "concurrent": True, "autotuned": False}
     if not args.no_autotune:
         t0 = time.perf_counter()
         best = plan.autotune(rs, sample_jumps=400, stream=stream)

@@ -359,10 +359,10 @@ int bamf_rjmcmc_set_segments(bamf_rjmcmc_plan* plan, int n_segments, int use_gra
 
 /* Launch configuration of the one-launch chain kernels (no counterpart in the
  * reference).  chains_per_simd[c]: resident chains per SIMD that size class
- * c's kernel is compiled for (2, 3 or 4: 256, 168 or 128 VGPRs, so more
+ * c's kernel is compiled for (3, 4 or 5: 168, 128 or 96 VGPRs, so more
  * registers and fewer spills against more latency hiding); concurrent: the
  * two classes run side by side (1) or one after the other (0).  Segmented
- * runs and runs with gSplit/gMerge use 3 chains per SIMD. */
+ * runs and runs with gSplit/gMerge use 4 chains per SIMD. */
 typedef struct bamf_rjmcmc_tuning {
     int32_t chains_per_simd[2];
     int32_t concurrent;

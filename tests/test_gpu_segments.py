@@ -111,17 +111,17 @@ def test_tuning_changes_no_result():
     rs = RoiSet.from_synth(rois, bg0=np.full(n, 20.0, np.float32))
     kw = dict(KW, trace_len=80)
     plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
-    assert plan.tuning() == {"chains_per_simd": [3, 3], "concurrent": True}
+    assert plan.tuning() == {"chains_per_simd": [4, 4], "concurrent": True}
     base = run_batch(plan, rs, seed=9, **kw)
     assert np.all(base.status == 0)
-    for cps, conc in (((2, 2), True), ((4, 4), True), ((2, 4), False), ((4, 2), True)):
+    for cps, conc in (((3, 3), True), ((5, 5), True), ((3, 5), False), ((5, 3), True)):
         plan.set_tuning(cps, conc)
         print(f"tuning {cps} concurrent={conc}", flush=True)
         assert_same(base, run_batch(plan, rs, seed=9, **kw), n, 80)
     best = plan.autotune(rs, sample_jumps=200)
     print("autotune:", best, flush=True)
-    assert best["ms"] > 0 and all(c in (2, 3, 4) for c in best["chains_per_simd"])
+    assert best["ms"] > 0 and all(c in (3, 4, 5) for c in best["chains_per_simd"])
     assert plan.tuning()["chains_per_simd"] == best["chains_per_simd"]
     assert_same(base, run_batch(plan, rs, seed=9, **kw), n, 80)
     with pytest.raises(Exception):
-        plan.set_tuning((5, 3), True)
+        plan.set_tuning((2, 4), True)
