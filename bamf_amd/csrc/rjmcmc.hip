@@ -505,6 +505,23 @@ struct Chain {
             const int src = by + (ysub < Y ? ysub : 0);
             ey0 = __shfl(e, src);
             ey1 = __shfl(e, src + 1);
+        } else if (15.0f * C >= 5.0f) {
+            // one erf per lane over a window of 32 edges around each coordinate
+            // (x edges on lanes 0..31, y edges on 32..63): an edge outside it
+            // lies 15 px or more from the emitter, |e| >= 15 C >= 5, where erf
+            // is exactly -1 or +1, so those edges are filled in without evaluation
+            const int wx = (int)floorf(xe) - 15, wy = (int)floorf(ye) - 15;
+            const bool isx = lane < 32;
+            const int k = isx ? wx + lane : wy + (lane - 32);
+            const float e = erf_nr(((float)k - (isx ? xe : ye)) * C);
+            auto edge = [&](int kk, int w, int base) {
+                const int d = kk - w;
+                const float v = __shfl(e, base + (d < 0 ? 0 : (d > 31 ? 31 : d)));
+                return d < 0 ? -1.0f : (d > 31 ? 1.0f : v);
+            };
+            vx = 0.5f * (edge(lane + 1, wx, 0) - edge(lane, wx, 0));
+            ey0 = edge(ysub, wy, 32);
+            ey1 = edge(ysub + 1, wy, 32);
         } else {
             const float ex = erf_nr((l - xe) * C), ey = erf_nr((l - ye) * C);
             vx = 0.5f * (__shfl_down(ex, 1) - ex);
@@ -851,6 +868,22 @@ struct Chain {
     __device__ __forceinline__ float pdf_of(const Em& e, int k) {
         return e.S[k] == 1 ? findpdf(a->sig, a->p.dx, e.I[k]) : findpdf(a->back, a->p.dx, e.I[k]);
     }
+    // Lookups of one table in a single pass over the lanes: lane l looks up
+    // the (l mod 2)-th (or (l mod 4)-th) intensity, read back by readlane;
+    // each value is the one findpdf returns for it alone.
+    __device__ __forceinline__ void findpdf2(const StatDev& s, float Ia, float Ib, float& ra, float& rb) const {
+        const float v = findpdf(s, a->p.dx, (lane & 1) ? Ib : Ia);
+        ra = bcastf(v, 0);
+        rb = bcastf(v, 1);
+    }
+    __device__ __forceinline__ void findpdf3(const StatDev& s, float Ia, float Ib, float Ic, float& ra, float& rb,
+                                             float& rc) const {
+        const int q = lane & 3;
+        const float v = findpdf(s, a->p.dx, q == 0 ? Ia : q == 1 ? Ib : Ic);
+        ra = bcastf(v, 0);
+        rb = bcastf(v, 1);
+        rc = bcastf(v, 2);
+    }
 
     // The network's log-intensity at the pixel a birth would propose (x, y)
     // from (pixel (floor(x + 1/2), floor(y + 1/2))); -inf outside the ROI.
@@ -898,8 +931,7 @@ struct Chain {
                 const float tx = E.X[k] + xs * randn();
                 const float ty = E.Y[k] + xs * randn();
                 const float ti = (I0 / (float)(np + 1)) * (0.15f * randn() + 1);
-                IPDFtest = findpdf(fg ? a->sig : a->back, p.dx, ti);
-                IPDF = findpdf(fg ? a->sig : a->back, p.dx, E.I[k]);
+                findpdf2(fg ? a->sig : a->back, ti, E.I[k], IPDFtest, IPDF);
                 set_t(k, ti, tx, ty);
                 PrI = PrI * IPDFtest / IPDF;
                 XI0 = XI0 + E.I[k] * E.X[k];
@@ -915,8 +947,7 @@ struct Chain {
             const float tyn = (YI0 + E.I[nn] * E.Y[nn] - YII0) / tin;
             set_t(nn, tin, txn, tyn);
             const bool fg = E.S[nn] == 1;
-            IPDFtest = findpdf(fg ? a->sig : a->back, p.dx, tin);
-            IPDF = findpdf(fg ? a->sig : a->back, p.dx, E.I[nn]);
+            findpdf2(fg ? a->sig : a->back, tin, E.I[nn], IPDFtest, IPDF);
             PrI = PrI * IPDFtest / IPDF;
             delta_push(0, nn, true);
             delta_push(1, nn, false);
@@ -935,8 +966,7 @@ struct Chain {
                 const float ti = T.I[k] + is * randn();
                 const float tx = T.X[k] + xs * randn();
                 const float ty = T.Y[k] + xs * randn();
-                IPDFtest = findpdf(fg ? a->sig : a->back, p.dx, ti);
-                IPDF = findpdf(fg ? a->sig : a->back, p.dx, E.I[k]);
+                findpdf2(fg ? a->sig : a->back, ti, E.I[k], IPDFtest, IPDF);
                 wsync();
                 set_t(k, ti, tx, ty);
                 wsync();
@@ -992,8 +1022,7 @@ struct Chain {
                 const float I0 = T.I[nn];
                 qratio = (I0 > 0.0f && fabsf(logf(I0) - mu1) <= a->l_halfwidth) ? ti / I0 : 0.0f;
             }
-            IPDFtest = findpdf(fg ? a->sig : a->back, p.dx, ti);
-            IPDF = findpdf(fg ? a->sig : a->back, p.dx, E.I[nn]);
+            findpdf2(fg ? a->sig : a->back, ti, E.I[nn], IPDFtest, IPDF);
             set_t(nn, ti, tx, ty);
             PrI = IPDFtest / IPDF * qratio;  // qratio = 1 outside the independence kernel
             delta_push(0, nn, true);
@@ -1080,10 +1109,12 @@ struct Chain {
         const bamf_mcmc_params& p = a->p;
         const Em& E = cur();
         const Em& T = tst();
-        float tmp1 = 1, tmp2 = 1;
-        tmp1 = tmp1 * pdf_of(E, f_jj);
-        tmp2 = tmp2 * pdf_of(T, f_jj);
-        tmp2 = tmp2 * pdf_of(T, tN - 1);
+        // both children keep the parent's Signal label: one table for all three
+        float tmp1 = 1, tmp2 = 1, pe, pt0, pt1;
+        findpdf3(E.S[f_jj] == 1 ? a->sig : a->back, E.I[f_jj], T.I[f_jj], T.I[tN - 1], pe, pt0, pt1);
+        tmp1 = tmp1 * pe;
+        tmp2 = tmp2 * pt0;
+        tmp2 = tmp2 * pt1;
         const float Norm1 = normpdf(f_u2, 0, p.split_std);
         const float Norm2 = normpdf(f_u3, 0, p.split_std);
         const float NEm = (float)nem_cur();
@@ -1160,10 +1191,17 @@ struct Chain {
         const Em& E = cur();
         const Em& T = tst();
         const int ii = L.nb[0];
-        float tmp1 = 1, tmp2 = 1;
-        tmp1 = tmp1 * pdf_of(E, f_jj);
-        tmp1 = tmp1 * pdf_of(E, ii);
-        tmp2 = tmp2 * pdf_of(T, f_jm);
+        float tmp1 = 1, tmp2 = 1, pj, pi, pm;
+        if (E.S[f_jj] == E.S[ii] && T.S[f_jm] == E.S[f_jj]) {
+            findpdf3(E.S[f_jj] == 1 ? a->sig : a->back, E.I[f_jj], E.I[ii], T.I[f_jm], pj, pi, pm);
+        } else {
+            pj = pdf_of(E, f_jj);
+            pi = pdf_of(E, ii);
+            pm = pdf_of(T, f_jm);
+        }
+        tmp1 = tmp1 * pj;
+        tmp1 = tmp1 * pi;
+        tmp2 = tmp2 * pm;
         const float Norm1 = normpdf(f_u2, 0, p.split_std);
         const float Norm2 = normpdf(f_u3, 0, p.split_std);
         const float NEm = (float)nem_cur();

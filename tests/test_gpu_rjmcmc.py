@@ -318,6 +318,18 @@ def test_branch_free_erf_accuracy():
     small = np.abs(x) < 0.5
     rel = err[small] / np.maximum(np.abs(erf(x[small].astype(np.float64))), 1e-30)
     assert rel.max() < 5e-7, rel.max()
+    # saturation: from |x| = 5 on the value is exactly -1 or +1 (the 32 x 32 stencil
+    # fills the edges 15 px or more from an emitter with these values unevaluated
+    # when 15 C >= 5); checked on every float32 in [5, 40]
+    xs = np.arange(np.float32(5.0).view(np.int32), np.float32(40.0).view(np.int32) + 1, dtype=np.int32)
+    xs = xs.view(np.float32)
+    xs = np.concatenate([xs, -xs])
+    ds = _hip.DeviceBuffer.from_array(xs)
+    dt = _hip.DeviceBuffer(xs.nbytes)
+    check(lib().bamf_math_erf(ds.vp, dt.vp, len(xs), None), "bamf_math_erf")
+    _hip.synchronize()
+    ys = dt.download(np.float32)
+    assert np.array_equal(ys, np.sign(xs)), xs[ys != np.sign(xs)][:5]
 
 
 def test_birth_death_stationary_law_gpu():
