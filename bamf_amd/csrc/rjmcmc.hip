@@ -177,6 +177,9 @@ template <int PPL, int F>
 struct Chain {
     static constexpr bool GJ = (F & kGJ) != 0, SEG = (F & kSEG) != 0, LRN = (F & kLRN) != 0;
     static constexpr bool NUM = (F & kNUM) != 0;
+    // prior lookups paired over the lanes in the plain kernel only: in the
+    // feature variants the divergent lookup costs more registers than it saves
+    static constexpr bool PAIR = F == 0;
     static_assert(!(NUM && LRN), "the learned proposals use the erf stencil");
     const ChainArgs* a;  // the kernel argument block (in the kernarg segment)
     WaveLds<PPL>& L;
@@ -872,17 +875,28 @@ struct Chain {
     // the (l mod 2)-th (or (l mod 4)-th) intensity, read back by readlane;
     // each value is the one findpdf returns for it alone.
     __device__ __forceinline__ void findpdf2(const StatDev& s, float Ia, float Ib, float& ra, float& rb) const {
-        const float v = findpdf(s, a->p.dx, (lane & 1) ? Ib : Ia);
-        ra = bcastf(v, 0);
-        rb = bcastf(v, 1);
+        if constexpr (PAIR) {
+            const float v = findpdf(s, a->p.dx, (lane & 1) ? Ib : Ia);
+            ra = bcastf(v, 0);
+            rb = bcastf(v, 1);
+        } else {
+            ra = findpdf(s, a->p.dx, Ia);
+            rb = findpdf(s, a->p.dx, Ib);
+        }
     }
     __device__ __forceinline__ void findpdf3(const StatDev& s, float Ia, float Ib, float Ic, float& ra, float& rb,
                                              float& rc) const {
-        const int q = lane & 3;
-        const float v = findpdf(s, a->p.dx, q == 0 ? Ia : q == 1 ? Ib : Ic);
-        ra = bcastf(v, 0);
-        rb = bcastf(v, 1);
-        rc = bcastf(v, 2);
+        if constexpr (PAIR) {
+            const int q = lane & 3;
+            const float v = findpdf(s, a->p.dx, q == 0 ? Ia : q == 1 ? Ib : Ic);
+            ra = bcastf(v, 0);
+            rb = bcastf(v, 1);
+            rc = bcastf(v, 2);
+        } else {
+            ra = findpdf(s, a->p.dx, Ia);
+            rb = findpdf(s, a->p.dx, Ib);
+            rc = findpdf(s, a->p.dx, Ic);
+        }
     }
 
     // The network's log-intensity at the pixel a birth would propose (x, y)
