@@ -177,9 +177,10 @@ template <int PPL, int F>
 struct Chain {
     static constexpr bool GJ = (F & kGJ) != 0, SEG = (F & kSEG) != 0, LRN = (F & kLRN) != 0;
     static constexpr bool NUM = (F & kNUM) != 0;
-    // prior lookups paired over the lanes in the plain kernel only: in the
-    // feature variants the divergent lookup costs more registers than it saves
-    static constexpr bool PAIR = F == 0;
+    // prior lookups paired over the lanes in the one-launch plain and learned
+    // kernels: in the segmented and group-jump variants the divergent lookup
+    // costs more in spills than it saves (measured)
+    static constexpr bool PAIR = F == 0 || F == kLRN;
     static_assert(!(NUM && LRN), "the learned proposals use the erf stencil");
     const ChainArgs* a;  // the kernel argument block (in the kernarg segment)
     WaveLds<PPL>& L;
