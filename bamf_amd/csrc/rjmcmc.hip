@@ -1368,17 +1368,23 @@ struct Chain {
     // sum in pixel order, r = total * U, first pixel with r <= cumulative.
     __device__ __forceinline__ int birth_pixel(float u) {
         aflops += (uint64_t)(4 * P);
-        float carry = 0.0f;
+        // every slot's scan on its own (no dependence between slots, so the
+        // scans overlap), then the slot totals carried in slot order: lane 63
+        // of carry + scan is carry + total, the same sum as the running form
         float cum[PPL];
 #pragma unroll
         for (int s = 0; s < PPL; s++) {
-            cum[s] = 0.0f;
+            const float r = L.dpx[64 * s + lane] - m[s];
+            const float v = (0 > r) ? 0.0f : r;
+            cum[s] = wave_scan(s < ppl ? v : 0.0f);
+        }
+        float carry = 0.0f;
+#pragma unroll
+        for (int s = 0; s < PPL; s++) {
             if (s < ppl) {
-                const float r = L.dpx[64 * s + lane] - m[s];
-                float v = (0 > r) ? 0.0f : r;
-                v = wave_scan(v);
-                cum[s] = carry + v;
-                carry = bcastf(cum[s], 63);
+                const float tot = bcastf(cum[s], 63);
+                cum[s] = carry + cum[s];
+                carry = carry + tot;
             }
         }
         const float rr = carry * u;
