@@ -317,14 +317,50 @@ def bench_find_roi(args):
 _ZCAL = {"Ax": 0.05, "Ay": -0.05, "Bx": 0.02, "By": 0.03, "Gamma": 0.35, "D": 0.6}  # astigmatism for 'Z'
 
 
+def front_end_crops(n: int, sz: int = 7):
+    """The first n FindROI boxes of the front-end stack: 2000 frames of 256 x 256
+    from the device forward model (emitters at 0.3 /um^2 with 100 nm pixels,
+    Gamma(5, 200) photons, background 10, sigma 1.3), FindROI with sigma 1.3 / 2.6,
+    Boxsize sz and the default minimum, cropped on the device in find() order.
+    Returns the device buffer of the boxes and a host copy."""
+    from bamf_amd import _hip
+    from bamf_amd._lib import check, lib
+    from bamf_amd.find_roi import FindRoiDevice, default_minval
+
+    nf, side = 2000, 256
+    ncell = lib().bamf_synth_cells(side, side)
+    d_frames = _hip.DeviceBuffer(nf * side * side * 4)
+    d_truth = _hip.DeviceBuffer(nf * ncell * 128 * 12)
+    d_cnt = _hip.DeviceBuffer(nf * ncell * 4)
+    check(lib().bamf_synth_frames(nf, side, side, 0.003, 5.0, 200.0, 10.0, 1.3, SEED + 4, 128, 1, d_frames.vp,
+                                  d_truth.vp, d_cnt.vp, None), "bamf_synth_frames")
+    dev = FindRoiDevice(side, side, nf, capacity=nf * side * side // 64)
+    dev.run(d_frames, 1.3, 2.6, sz, default_minval())
+    _hip.synchronize()
+    found = dev.results()[0]
+    if found < n:
+        raise SystemExit(f"the front-end stack yields {found} FindROI boxes, fewer than {n}")
+    d_boxes = _hip.DeviceBuffer(n * sz * sz * 4)
+    check(lib().bamf_find_roi_crop(d_frames.vp, side, side, dev.row.vp, dev.col.vp, dev.frame.vp, n, sz,
+                                   d_boxes.vp, None), "bamf_find_roi_crop")
+    _hip.synchronize()
+    return d_boxes, d_boxes.download(np.float32).reshape(n, sz, sz), int(found)
+
+
 def bench_gauss_mle(args):
-    """gaussMLE: 100,000 single-emitter 7 x 7 boxes, sigma 1.3, 20 iterations, fit
-    type args.fit_type ('Basic' is the hot path's XYNB kernel)."""
+    """gaussMLE: 100,000 7 x 7 boxes, sigma 1.3, 20 iterations, fit type
+    args.fit_type ('Basic' is the hot path's XYNB kernel).  The boxes are the
+    first 100,000 FindROI crops of the front-end stack (--mle-boxes crops), or
+    single-emitter boxes drawn on the host (--mle-boxes synth)."""
     from bamf_amd import _hip, synth
     from bamf_amd.gauss_mle import gauss_mle_device
 
     n, sz, sigma = 100_000, 7, 1.3
-    boxes, _ = synth.make_mle_boxes(n, sz, seed=SEED + 4, sigma=sigma)
+    if args.mle_boxes == "crops":
+        d, boxes, found = front_end_crops(n, sz)
+    else:
+        boxes, _ = synth.make_mle_boxes(n, sz, seed=SEED + 4, sigma=sigma)
+        d, found = None, None
     ft = args.fit_type
     sc_host = None
     if args.camera == "SCMOS":
@@ -348,7 +384,8 @@ def bench_gauss_mle(args):
         cpu = {"value": sample / dt, "unit": "fits/s", "cores": cores, "kind": "port",
                "sample": f"first {sample} boxes, oracle C {args.fit_type} fit restatement, {cores} forked workers, "
                          f"{dt:.1f} s wall"}
-    d = _hip.DeviceBuffer.from_array(boxes)
+    if d is None:
+        d = _hip.DeviceBuffer.from_array(boxes)
     dp, dc, dl = _hip.DeviceBuffer(24 * n), _hip.DeviceBuffer(24 * n), _hip.DeviceBuffer(4 * n)
     s = _hip.Stream()
     sg = (sigma, sigma) if ft == "Z" else sigma
@@ -364,7 +401,8 @@ def bench_gauss_mle(args):
     return {"metric": f"gaussMLE {ft} {args.camera} fits/s (7 x 7 boxes, 20 iterations)", "value": n / (ms / 1e3),
             "unit": "fits/s", "ms_per_step": ms, "dtype": "f32", "higher_is_better": True,
             "config": {"workload": "gauss_mle", "fit_type": ft, "camera": args.camera, "fits": n, "box": sz,
-                       "iterations": 20},
+                       "iterations": 20, "boxes": args.mle_boxes if found is None else
+                       f"the first {n} of {found} FindROI crops of the 2000 x 256 x 256 front-end stack"},
             "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                          "frac": achieved / PEAK_FP32_TFLOPS if basic else None, "kernel": "k_mle_xynb",
                          "traffic": read_traffic(ROOT / "profiles" / "mle_xynb_gauss_mle_100k_7x7_pmc.txt")
@@ -680,6 +718,9 @@ def main():
                     help="share of training steps labelled by plain-chain posterior samples instead of the truth")
     ap.add_argument("--fit-type", default="Basic", choices=["Basic", "Sigma", "SigmaYX", "Z"],
                     help="gaussMLE fit type of --workload gauss_mle")
+    ap.add_argument("--mle-boxes", default="crops", choices=["crops", "synth"],
+                    help="--workload gauss_mle: the first 100,000 FindROI crops of the front-end stack, or "
+                         "single-emitter boxes drawn on the host")
     ap.add_argument("--camera", default="CCD", choices=["CCD", "SCMOS"],
                     help="gaussMLE camera of --workload gauss_mle (SCMOS: a 256 x 256 variance/gain^2 map)")
     ap.add_argument("--psf", default="erf", choices=["erf", "sampled"],
