@@ -177,10 +177,10 @@ template <int PPL, int F>
 struct Chain {
     static constexpr bool GJ = (F & kGJ) != 0, SEG = (F & kSEG) != 0, LRN = (F & kLRN) != 0;
     static constexpr bool NUM = (F & kNUM) != 0;
-    // prior lookups paired over the lanes in the one-launch plain and learned
-    // kernels: in the segmented and group-jump variants the divergent lookup
-    // costs more in spills than it saves (measured)
-    static constexpr bool PAIR = F == 0 || F == kLRN;
+    // prior lookups paired over the lanes in the plain, learned and segmented
+    // kernels: in the group-jump and sampled-PSF variants the divergent lookup
+    // has not been measured
+    static constexpr bool PAIR = F == 0 || F == kLRN || F == kSEG;
     static_assert(!(NUM && LRN), "the learned proposals use the erf stencil");
     const ChainArgs* a;  // the kernel argument block (in the kernarg segment)
     WaveLds<PPL>& L;
@@ -1789,13 +1789,15 @@ struct Chain {
         bnd = (int)p.bnd_out;
         xb = X + p.bnd_out;
         yb = Y + p.bnd_out;
-        key = a->key ? a->key[roi] : (uint32_t)roi;
+        // (read into scalar registers: the Philox key schedule then runs on the
+        // scalar unit)
+        key = __builtin_amdgcn_readfirstlane(a->key ? a->key[roi] : (uint32_t)roi);
         const int nb = p.n_burnin, total = p.n_burnin + p.n_trials;
         int s_begin = 0, s_end = total;
         SegState* ss = nullptr;
         if constexpr (SEG) {
-            sd_lo = a->seed_dev ? a->seed_dev[0] : a->seed_lo;
-            sd_hi = a->seed_dev ? a->seed_dev[1] : a->seed_hi;
+            sd_lo = __builtin_amdgcn_readfirstlane(a->seed_dev ? a->seed_dev[0] : a->seed_lo);
+            sd_hi = __builtin_amdgcn_readfirstlane(a->seed_dev ? a->seed_dev[1] : a->seed_hi);
             s_begin = a->seg_begin;
             s_end = min(a->seg_end, total);
             ss = a->seg_state ? a->seg_state + roi : nullptr;
