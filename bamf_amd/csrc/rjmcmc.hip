@@ -1768,10 +1768,13 @@ struct Chain {
     }
 
     __device__ __forceinline__ void run(int roi_) {
-        roi = roi_;
+        // the ROI index and sizes in scalar registers (they arrive by vector
+        // loads): the slot bookkeeping derived from them then runs on the
+        // scalar unit
+        roi = __builtin_amdgcn_readfirstlane(roi_);
         const bamf_mcmc_params& p = a->p;
-        X = a->xs[roi];
-        Y = a->ys[roi];
+        X = __builtin_amdgcn_readfirstlane(a->xs[roi]);
+        Y = __builtin_amdgcn_readfirstlane(a->ys[roi]);
         if (X < 1 || Y < 1 || X > kMaxSide || Y > kMaxSide || slots_for(X, Y) > PPL) {
             if (a->status && lane == 0) a->status[roi] = ST_SIZE;
             return;
