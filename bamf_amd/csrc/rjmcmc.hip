@@ -67,6 +67,11 @@ struct WaveLds {
     int cnt[32];
     float dpx[64 * PPL];  // the ROI's data in the pixel-slot layout: slot s at [64 s + lane]
     int64_t rec[3];       // chain-record counts: records, accepted-state entries, proposed-state entries
+    // factors of the ratios that are fixed for a ROI or a phase, formed once
+    // with the reference's operations (so the same floats): 1 / Area, the
+    // phase's P[1] / P[2] and P[4] / P[3], and normpdf's 1 / (sqrt(2 pi) s)
+    // and 2 s s for s = Split_std
+    float kc[5];
 };
 
 __device__ __forceinline__ void wsync() {
@@ -872,6 +877,12 @@ struct Chain {
     __device__ __forceinline__ float pdf_of(const Em& e, int k) {
         return e.S[k] == 1 ? findpdf(a->sig, a->p.dx, e.I[k]) : findpdf(a->back, a->p.dx, e.I[k]);
     }
+    // normpdf(x, 0, Split_std) (LibRJMCMCreturn.hpp:19-21) with its two
+    // constants formed once: -(x - 0)(x - 0) is -(x x), bit for bit
+    __device__ __forceinline__ float normpdf_split(float x) const {
+        const float e = expf(-x * x / L.kc[4]);
+        return L.kc[3] * e;
+    }
     // Lookups of one table in a single pass over the lanes: lane l looks up
     // the (l mod 2)-th (or (l mod 4)-th) intensity, read back by readlane;
     // each value is the one findpdf returns for it alone.
@@ -1062,7 +1073,8 @@ struct Chain {
         } else {
             // BG, ABG, BBG unchanged: both exponentials are expf(-0) = 1, so the
             // factor is BgPDF / BgPDF (1, or NaN for a zero pdf), bit for bit
-            PrBg = BgPDF / BgPDF;
+            // (x / x: 1 for finite nonzero x, NaN for 0, infinities and NaN)
+            PrBg = (fabsf(BgPDF) > 0.0f && fabsf(BgPDF) < INFINITY) ? 1.0f : __builtin_nanf("");
         }
         lp_prod = (double)f_pri * (double)PrBg;
         lp_neg = false;
@@ -1130,17 +1142,17 @@ struct Chain {
         tmp1 = tmp1 * pe;
         tmp2 = tmp2 * pt0;
         tmp2 = tmp2 * pt1;
-        const float Norm1 = normpdf(f_u2, 0, p.split_std);
-        const float Norm2 = normpdf(f_u3, 0, p.split_std);
+        const float Norm1 = normpdf_split(f_u2);
+        const float Norm2 = normpdf_split(f_u3);
         const float NEm = (float)nem_cur();
         const float PrM = meanN / (NEm + 1);
         const float PrI = tmp2 / tmp1;
         const float Area = xb * yb;
-        const float PrX = 1 / Area;
+        const float PrX = L.kc[0];  // 1 / Area
         const float Us = split_us(Norm1 * Norm2);  // 1 / (Norm1 * Norm2) outside the learned split
         const double om = (double)(1 - f_u1);
         const float Jacob = (float)((double)f_ijp / (om * om));
-        const float P12 = P1 / P2;
+        const float P12 = L.kc[1];  // P1 / P2
         lp_prod = (double)PrI * (double)PrX * (double)Us * (double)Jacob * (double)P12 * (double)PrM;
         lp_neg = false;
         return LLR * PrI * PrX * Us * Jacob * P12 * PrM;
@@ -1217,12 +1229,12 @@ struct Chain {
         tmp1 = tmp1 * pj;
         tmp1 = tmp1 * pi;
         tmp2 = tmp2 * pm;
-        const float Norm1 = normpdf(f_u2, 0, p.split_std);
-        const float Norm2 = normpdf(f_u3, 0, p.split_std);
+        const float Norm1 = normpdf_split(f_u2);
+        const float Norm2 = normpdf_split(f_u3);
         const float NEm = (float)nem_cur();
         const float PrI = tmp1 / tmp2;
         const float Area = xb * yb;
-        const float PrX = 1 / Area;
+        const float PrX = L.kc[0];  // 1 / Area
         const float Us = split_us(Norm1 * Norm2);
         const double om = (double)(1 - f_u1);
         const float Jacob = (float)((double)f_ijp / (om * om));
@@ -1577,7 +1589,7 @@ struct Chain {
     __device__ __forceinline__ float fin_death(float sum, float LLR) {
         const float NEm = (float)nem_cur();
         const float PrM = NEm / meanN;
-        const float P43 = P4 / P3;
+        const float P43 = L.kc[2];  // P4 / P3
         lp_prod = (double)P43 * (double)PrM;
         lp_neg = false;
         return LLR * P43 * PrM;
@@ -1792,6 +1804,13 @@ struct Chain {
         bnd = (int)p.bnd_out;
         xb = X + p.bnd_out;
         yb = Y + p.bnd_out;
+        if (lane == 0) {
+            const float Area = xb * yb;
+            L.kc[0] = 1 / Area;
+            const float sa = (float)sqrt(2.0f * kPiD), ss = p.split_std;
+            L.kc[3] = 1.0f / (sa * ss);
+            L.kc[4] = 2 * ss * ss;
+        }
         // (read into scalar registers: the Philox key schedule then runs on the
         // scalar unit)
         key = __builtin_amdgcn_readfirstlane(a->key ? a->key[roi] : (uint32_t)roi);
@@ -1864,6 +1883,13 @@ struct Chain {
             } else {
                 P1 = a->pvt[1]; P2 = a->pvt[2]; P3 = a->pvt[3]; P4 = a->pvt[4];
                 P56 = a->pvt[5] / a->pvt[6];
+            }
+            if (s == s_begin || s == nb) {  // the phase's fixed ratios
+                if (lane == 0) {
+                    L.kc[1] = P1 / P2;
+                    L.kc[2] = P4 / P3;
+                }
+                wsync();
             }
             dj = 0;
             fill_window(0);
