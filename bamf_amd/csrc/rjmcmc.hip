@@ -246,6 +246,11 @@ struct Chain {
 
     __device__ __forceinline__ Em& cur() { return L.e[ci]; }
     __device__ __forceinline__ Em& tst() { return L.e[ci ^ 1]; }
+    // The lane coordinates pass through an empty asm at each use: without it
+    // the compiler hoists every per-slot value derived from them (the slot's
+    // column, its pixel address, its validity mask) out of the chain loop and
+    // keeps all of them live across it, which spills; re-deriving one where
+    // it is used costs an instruction or two.
     __device__ __forceinline__ static int opq(int v) {
         asm volatile("" : "+v"(v));
         return v;
