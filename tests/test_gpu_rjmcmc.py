@@ -236,6 +236,38 @@ def test_replay_uneven_sizes_and_background(oracle):
         off += sz
 
 
+@pytest.mark.parametrize("sigma", [1.3, 2.5])
+def test_replay_wide_rois_both_stencil_forms(oracle, sigma):
+    """ROIs with X + Y + 2 > 64 (32 x 32, 31 x 32, 32 x 31): their stencils do not
+    fit one edge per lane.  At sigma 1.3 (15 C >= 5) the device evaluates one erf
+    per lane over 32-edge windows and fills the edges beyond with the saturated
+    +-1; at sigma 2.5 (15 C < 5) it evaluates both axes' edges on every lane.
+    Both must replay the oracle's chains exactly."""
+    rng = np.random.default_rng(17)
+    shapes = [(32, 32), (32, 32), (31, 32), (32, 31), (32, 32), (31, 32)]
+    xs = np.array([s[0] for s in shapes])
+    ys = np.array([s[1] for s in shapes])
+    n = len(shapes)
+    offs = np.zeros(n + 1, np.int64)
+    offs[1:] = np.cumsum(xs * ys)
+    data = np.zeros(offs[-1], np.float32)
+    for r in range(n):
+        k = rng.poisson(0.03 * xs[r] * ys[r] / 4)
+        X, Y = rng.uniform(0, xs[r], k), rng.uniform(0, ys[r], k)
+        mu = synth.roi_expected(rng.gamma(5, 200, k), X, Y, 12.0, int(xs[r]), int(ys[r]), sigma)
+        data[offs[r]:offs[r + 1]] = rng.poisson(mu)
+    rois = {"data": data, "offset": offs, "xsize": xs, "ysize": ys}
+    p = params(nb=700, nt=700, psf_sigma=sigma)
+    ores, tapes = run_oracle_chains(oracle, rois, p, seed=23, bg0=12.0, chain=True)
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    rs = RoiSet(data, offs, xs, ys, bg0=np.full(n, 12.0, np.float32))
+    rp = make_replay(tapes, np.stack([o["steps"] for o in ores]))
+    print(f"replay wide ROIs, sigma {sigma}: {n} ROIs", flush=True)
+    g = run_batch(plan, rs, mode=REPLAY, replay=rp, record_steps=True, hist_len=129)
+    check_replay(rois, g, ores, tapes, p, lambda r: data[offs[r]:offs[r + 1]].sum())
+    check_llr_against_float64(oracle, g, ores, data, offs, xs, ys, sigma, 12.0)
+
+
 def test_replay_group_jumps(oracle):
     """gSplit / gMerge (P_Burnin entries 5 and 6, also reached in the trials
     phase through P_Burnin's cumulative vector) on clustered ROIs."""
