@@ -2018,6 +2018,34 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, MINB) void k_chain(ChainArgs) 
     }
 }
 
+// Both size classes in one launch: the wavefronts drain the large-ROI list
+// and then the small-ROI list through one work-queue counter, each ROI run by
+// the chain of its class in the same LDS block (the small class's layout is
+// smaller and starts at the same address; every chain's run begins and ends
+// with a wavefront fence).  A wavefront takes the next ROI as soon as its
+// chain ends, whatever the class, so no work-group holds its CU share while
+// waiting for its last chain, and small ROIs fill the slots the large ones
+// free.  The register budget is the large class's.
+template <int MINB, int F>
+__global__ __launch_bounds__(64 * kWavesPerBlock, MINB) void k_chain_all(ChainArgs) {
+    static_assert(sizeof(WaveLds<kClassPPL[0]>) <= sizeof(WaveLds<kClassPPL[1]>), "the small layout fits");
+    __shared__ WaveLds<kClassPPL[1]> lds[kWavesPerBlock];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const ChainArgs* ap = (const ChainArgs*)(__builtin_amdgcn_kernarg_segment_ptr());
+    Chain<kClassPPL[1], F> large(ap, lds[w], lane);
+    Chain<kClassPPL[0], F> small(ap, *reinterpret_cast<WaveLds<kClassPPL[0]>*>(&lds[w]), lane);
+    const int n1 = *ap->list_count, n = n1 + *ap->list2_count;
+    while (true) {
+        int qi = 0;
+        if (lane == 0) qi = (int)atomicAdd(ap->queue, 1u);
+        qi = bcasti(qi, 0);
+        if (qi >= n) break;
+        if (qi < n1) large.run(ap->list[qi]);
+        else small.run(ap->list2[qi - n1]);
+    }
+}
+
 // Stable split of the ROI order into the size classes (P <= 64 * kClassPPL[c]
 // for the first class c that fits); also resets the nq work-queue counters
 // (two per segment).  One work-group walks the order in tiles of 1024 with a
@@ -2104,8 +2132,21 @@ ChainKernelFn chain_kernel(int f, int tune) {
     return tune == 0 ? k_chain<P, kTuneMinBlocks[0], 0> : tune == 2 ? k_chain<P, kTuneMinBlocks[2], 0> : k_chain<P, M, 0>;
 }
 
+// One launch over both classes (class index kClasses): plain, segmented and
+// learned chains, at the large class's default register budget.
+ChainKernelFn chain_kernel_all(int f) {
+    constexpr int M = kClassMinBlocks[1];
+    return f == kSEG ? k_chain_all<M, kSEG> : f == kLRN ? k_chain_all<M, kLRN> : k_chain_all<M, 0>;
+}
+ChainKernelFn select_chain(int cls, int features, int tune) {
+    return cls == kClasses ? chain_kernel_all(features)
+         : cls == 0        ? chain_kernel<0>(features, tune)
+                           : chain_kernel<1>(features, tune);
+}
+
 int launch_chain(const ChainArgs& a, int cls, int features, int tune, int grid, hipStream_t s) {
-    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(features, tune) : chain_kernel<1>(features, tune);
+    BAMF_CHECK_ARG(cls != kClasses || one_launch_features(features), "no one-launch kernel for features %d", features);
+    const ChainKernelFn k = select_chain(cls, features, tune);
     hipLaunchKernelGGL(k, dim3(grid), dim3(64 * kWavesPerBlock), 0, s, a);
     BAMF_LAUNCH_CHECK("k_chain");
     return BAMF_OK;
@@ -2119,7 +2160,8 @@ int launch_erf(const float* x, float* y, int64_t n, hipStream_t s) {
 
 int chain_occupancy_blocks(int cls, int features, int tune, int* blocks_per_cu) {
     int n = 0;
-    const ChainKernelFn k = cls == 0 ? chain_kernel<0>(features, tune) : chain_kernel<1>(features, tune);
+    BAMF_CHECK_ARG(cls != kClasses || one_launch_features(features), "no one-launch kernel for features %d", features);
+    const ChainKernelFn k = select_chain(cls, features, tune);
     BAMF_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(k),
                                                                64 * kWavesPerBlock, 0));
     *blocks_per_cu = n;

@@ -24,6 +24,13 @@ constexpr int kClassMinBlocks[kClasses] = {4, 4};  // work-groups per CU the reg
 constexpr int kTuneChoices = 3;
 constexpr int kTuneMinBlocks[kTuneChoices] = {3, 4, 5};
 constexpr int kTuneDefault = 1;
+// Feature sets (bit 0 gSplit/gMerge, bit 1 segmented, bit 2 learned, bit 3
+// sampled PSF) with a one-launch kernel over both classes: plain, segmented
+// and learned.
+constexpr bool one_launch_features(int f) { return f == 0 || f == 2 || f == 4; }
+// How a run launches the classes (bamf_rjmcmc_tuning.concurrent): one after
+// the other, side by side on two streams, or one launch over both.
+constexpr int kLaunchSequential = 0, kLaunchSideBySide = 1, kLaunchOne = 2;
 
 // One prior table (Stat, LibRJMCMC.cpp:176-217) on the device.
 struct StatDev {
@@ -84,6 +91,8 @@ struct ChainArgs {
     const int32_t* order;       // caller's ROI order (read by the classifier only)
     const int32_t* list;        // this launch's class list of ROI indices
     const int32_t* list_count;  // its length (device)
+    const int32_t* list2;       // one launch over both classes: the small-ROI list, drained after `list`
+    const int32_t* list2_count;
     uint32_t* queue;            // work-queue counter (reset by the classifier)
     // replay
     const uint32_t* tape;

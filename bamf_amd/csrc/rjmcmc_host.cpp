@@ -118,6 +118,7 @@ struct bamf_rjmcmc_plan {
     int device = 0;
     int cu = 0;
     int blocks_per_cu[kClasses][16] = {};  // [class][features] at the default register budget
+    int blocks_per_cu_all[16] = {};        // [features]: the one launch over both classes
     // sampled PSF (OptModel = 1), plan-owned; null: the erf model
     float* d_psf = nullptr;
     int32_t psf_dims[4] = {0, 0, 0, 0};
@@ -125,7 +126,7 @@ struct bamf_rjmcmc_plan {
     int blocks_per_cu_tune[kClasses][kTuneChoices] = {};  // one-launch plain kernels per register budget
     // autotuned (or set) launch configuration of the one-launch plain kernels
     int tune[kClasses] = {kTuneDefault, kTuneDefault};
-    bool concurrent = true;
+    int launch = kLaunchOne;  // how the classes are launched (kLaunch*)
     int32_t* d_tune_status = nullptr;
     int64_t tune_cap = 0;
     // segmented runs and their graph
@@ -165,10 +166,11 @@ void drop_graph(bamf_rjmcmc_plan* pl) {
 }
 
 // The launch sequence of one run: the classifier (which also resets the work
-// queues), then for each size class its segments in order on its own stream,
-// the small-ROI class on the side stream beside the caller's.
+// queues), then either one launch per segment over both classes, or for each
+// size class its segments in order on its own stream (the small-ROI class on
+// the side stream beside the caller's when the classes run side by side).
 int enqueue_run(bamf_rjmcmc_plan* plan, const ChainArgs& a, int nseg, bool segmented, const int tune[kClasses],
-                bool concurrent, hipStream_t s) {
+                int launch, hipStream_t s) {
     const int total = a.p.n_burnin + a.p.n_trials;
     int32_t* counts = reinterpret_cast<int32_t*>(plan->d_queue + kCountsAt);
     int rc = launch_classify(a.n_roi, a.order, a.xs, a.ys, plan->d_lists, plan->d_lists + plan->list_cap, counts,
@@ -182,6 +184,26 @@ int enqueue_run(bamf_rjmcmc_plan* plan, const ChainArgs& a, int nseg, bool segme
     const bool gj = a.p.p_burnin[5] != 0.0f || a.p.p_burnin[6] != 0.0f;
     const int features =
         (gj ? 1 : 0) | (segmented ? 2 : 0) | (a.mode == BAMF_RJ_LEARNED ? 4 : 0) | (a.psf ? 8 : 0);
+    if (launch == kLaunchOne && one_launch_features(features)) {
+        const int bpc = plan->blocks_per_cu_all[features];
+        int grid = (bpc > 0 ? bpc : 1) * cu;
+        if (grid > need) grid = need;
+        for (int k = 0; k < nseg; k++) {
+            ChainArgs ac = a;
+            ac.list = plan->d_lists + plan->list_cap;  // the large ROIs first
+            ac.list_count = counts + 1;
+            ac.list2 = plan->d_lists;
+            ac.list2_count = counts;
+            ac.queue = plan->d_queue + 2 * k + 1;
+            ac.seg_begin = (int)((int64_t)total * k / nseg);
+            ac.seg_end = k == nseg - 1 ? total : (int)((int64_t)total * (k + 1) / nseg);
+            rc = launch_chain(ac, kClasses, features, kTuneDefault, grid, s);
+            if (rc != BAMF_OK) return rc;
+        }
+        return BAMF_OK;
+    }
+    // features without a one-launch kernel run the classes side by side
+    const bool concurrent = launch != kLaunchSequential;
     if (concurrent) {
         BAMF_HIP_TRY(hipEventRecord(plan->ev_fork, s));
         BAMF_HIP_TRY(hipStreamWaitEvent(plan->side, plan->ev_fork, 0));
@@ -314,6 +336,11 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
             int bpc = 0;
             if (chain_occupancy_blocks(c, 0, t, &bpc) == BAMF_OK) pl->blocks_per_cu_tune[c][t] = bpc;
         }
+    for (int f = 0; f < 16; f++) {
+        int bpc = 0;
+        if (one_launch_features(f) && chain_occupancy_blocks(kClasses, f, kTuneDefault, &bpc) == BAMF_OK)
+            pl->blocks_per_cu_all[f] = bpc;
+    }
     *plan = pl;
     return BAMF_OK;
 }
@@ -542,7 +569,7 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     }
     a.seg_state = nseg > 1 ? plan->d_seg : nullptr;
     plan->last_nseg = nseg;
-    if (!plan->use_graph) return enqueue_run(plan, a, nseg, nseg > 1, plan->tune, plan->concurrent, s);
+    if (!plan->use_graph) return enqueue_run(plan, a, nseg, nseg > 1, plan->tune, plan->launch, s);
     // graph replay: the launch sequence is captured once per argument set and
     // replayed; the seed travels through device memory, written ahead of it
     a.seed_lo = a.seed_hi = 0;
@@ -550,7 +577,7 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
     if (!plan->gexec || plan->graph_nseg != nseg || std::memcmp(&plan->graph_key, &a, sizeof(a)) != 0) {
         drop_graph(plan);
         BAMF_HIP_TRY(hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_run(plan, a, nseg, true, plan->tune, plan->concurrent, plan->cap);
+        const int rc = enqueue_run(plan, a, nseg, true, plan->tune, plan->launch, plan->cap);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(plan->cap, &g);
         if (rc != BAMF_OK) {
@@ -577,7 +604,7 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
 int bamf_rjmcmc_get_tuning(const bamf_rjmcmc_plan* plan, bamf_rjmcmc_tuning* t) {
     BAMF_CHECK_ARG(plan && t, "null argument");
     for (int c = 0; c < kClasses; c++) t->chains_per_simd[c] = kTuneMinBlocks[plan->tune[c]] * kWavesPerBlock / 4;
-    t->concurrent = plan->concurrent ? 1 : 0;
+    t->concurrent = plan->launch;
     t->ms = 0.0f;
     return BAMF_OK;
 }
@@ -592,9 +619,11 @@ int bamf_rjmcmc_set_tuning(bamf_rjmcmc_plan* plan, const bamf_rjmcmc_tuning* t) 
         BAMF_CHECK_ARG(idx[c] >= 0, "chains_per_simd[%d] = %d is not one of the compiled budgets (3, 4, 5)", c,
                        t->chains_per_simd[c]);
     }
-    BAMF_CHECK_ARG(t->concurrent == 0 || t->concurrent == 1, "concurrent must be 0 or 1");
+    BAMF_CHECK_ARG(t->concurrent >= kLaunchSequential && t->concurrent <= kLaunchOne, "concurrent must be 0, 1 or 2");
+    BAMF_CHECK_ARG(t->concurrent != kLaunchOne || (idx[0] == kTuneDefault && idx[1] == kTuneDefault),
+                   "the one launch over both classes runs 4 chains per SIMD");
     for (int c = 0; c < kClasses; c++) plan->tune[c] = idx[c];
-    plan->concurrent = t->concurrent != 0;
+    plan->launch = t->concurrent;
     drop_graph(plan);
     return BAMF_OK;
 }
@@ -615,7 +644,8 @@ int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, in
     // the sample runs use this plan with short chains, one launch, no graph
     const bamf_mcmc_params saved = plan->params;
     const int saved_tune[kClasses] = {plan->tune[0], plan->tune[1]};
-    const bool saved_conc = plan->concurrent, saved_graph = plan->use_graph;
+    const int saved_launch = plan->launch;
+    const bool saved_graph = plan->use_graph;
     const int saved_nseg = plan->n_segments;
     plan->params.n_burnin = sample_jumps / 2;
     plan->params.n_trials = sample_jumps - sample_jumps / 2;
@@ -628,7 +658,7 @@ int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, in
     int rc = BAMF_OK;
     float best_ms = -1.0f;
     int best_t[kClasses] = {kTuneDefault, kTuneDefault};
-    bool best_conc = true;
+    int best_launch = kLaunchOne;
     // group jumps and segments use the default budget: only the concurrency is tunable then
     const bool gj = saved.p_burnin[5] != 0.0f || saved.p_burnin[6] != 0.0f;
     const int nt = gj ? 1 : kTuneChoices;
@@ -636,12 +666,14 @@ int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, in
         set_error("autotune: hipEventCreate failed");
         rc = BAMF_E_HIP;
     }
-    for (int conc = 1; conc >= 0 && rc == BAMF_OK; conc--)
+    for (int lm = kLaunchOne; lm >= kLaunchSequential && rc == BAMF_OK; lm--)
         for (int t1 = 0; t1 < nt && rc == BAMF_OK; t1++)
             for (int t0 = 0; t0 < nt && rc == BAMF_OK; t0++) {
+                // the one launch over both classes has the default budget only
+                if (lm == kLaunchOne && (t0 != kTuneDefault || t1 != kTuneDefault) && !gj) continue;
                 plan->tune[0] = gj ? kTuneDefault : t0;
                 plan->tune[1] = gj ? kTuneDefault : t1;
-                plan->concurrent = conc == 1;
+                plan->launch = lm;
                 float ms = 0.0f;
                 for (int rep = 0; rep < 2 && rc == BAMF_OK; rep++) {  // the first run loads the code objects
                     if (hipEventRecord(e0, s) != hipSuccess) rc = BAMF_E_HIP;
@@ -654,7 +686,7 @@ int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, in
                     best_ms = ms;
                     best_t[0] = plan->tune[0];
                     best_t[1] = plan->tune[1];
-                    best_conc = plan->concurrent;
+                    best_launch = plan->launch;
                 }
             }
     if (e0) (void)hipEventDestroy(e0);
@@ -665,13 +697,13 @@ int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, in
     if (rc != BAMF_OK) {
         plan->tune[0] = saved_tune[0];
         plan->tune[1] = saved_tune[1];
-        plan->concurrent = saved_conc;
+        plan->launch = saved_launch;
         if (rc == BAMF_E_HIP) set_error("autotune: a sample run failed on the device");
         return rc;
     }
     plan->tune[0] = best_t[0];
     plan->tune[1] = best_t[1];
-    plan->concurrent = best_conc;
+    plan->launch = best_launch;
     drop_graph(plan);
     rc = bamf_rjmcmc_get_tuning(plan, best);
     best->ms = best_ms;

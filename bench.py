@@ -8,7 +8,7 @@ post-burn-in) with P_Burnin = [.4 .15 .15 .15 .15 0 0 0] and
 P_Trials = [.6 .1 .1 .1 .1 0 0 0].
 
 A step is one batched run of every chain of the workload (the persistent
-launches of both ROI size classes).  An ROI posterior is an ROI whose
+launch over both ROI size classes, or the launch setting the autotuner picks).  An ROI posterior is an ROI whose
 post-burn-in chain reaches ESS >= 200, the ESS being the smaller of the
 emitter-count and total-photon traces' ESS (Geyer's initial monotone
 sequence estimator).  value = such ROIs per second: the number of ROIs whose
@@ -767,13 +767,15 @@ def main():
     cap = 128
     outs = alloc_outputs(n, nb + nt, 129, cap, trace_len, False, False, rs, p.grid_zoom)
     stream = _hip.Stream()
-    tuning = {"chains_per_simd": [4, 4], "concurrent": True, "autotuned": False}
+    tuning = {"chains_per_simd": [4, 4], "concurrent": 2, "autotuned": False}
     if not args.no_autotune:
         t0 = time.perf_counter()
         best = plan.autotune(rs, sample_jumps=400, stream=stream)
         tuning = {"chains_per_simd": best["chains_per_simd"], "concurrent": best["concurrent"], "autotuned": True,
                   "sample_ms": best["ms"], "tune_s": time.perf_counter() - t0}
         print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
+    tuning["launch"] = {2: "one launch over both size classes", 1: "a launch per size class, side by side",
+                        0: "a launch per size class, one after the other"}[tuning["concurrent"]]
     if args.segments > 1:
         # each chain as consecutive launches, its state in HBM between them, optionally
         # the launch sequence captured once in a HIP graph and replayed with new seeds
@@ -840,7 +842,7 @@ def main():
         "dtype": "f32 (accept test f64)", "data": "synthetic (seeded forward-model ROIs)",
         "config": {"workload": args.workload, "n_roi": n, "roi_sides": sorted(set(int(s) for s in rois["side"])),
                    "jumps_per_chain": nb + nt, "burnin": nb, "ess_target": 200, "proposal": "plain",
-                   "parallelism": "1 GPU, one wavefront per chain, persistent work queues per ROI size class",
+                   "parallelism": "1 GPU, one wavefront per chain, persistent launch draining one work queue (large ROIs first)",
                    "psf_model": "erf (OptModel 0)" if psf is None else
                    "sampled (OptModel 1), 72 x 72 x 4 x 4 table of the same Gaussian",
                    "launch_tuning": tuning, "graph_segments": graph_line},

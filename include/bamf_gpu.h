@@ -340,10 +340,12 @@ int bamf_rjmcmc_plan_create(const bamf_mcmc_params* params, const float* h_off_p
                             bamf_rjmcmc_plan** plan);
 int bamf_rjmcmc_plan_destroy(bamf_rjmcmc_plan* plan);
 /* Runs every chain of the batch to completion: one wavefront per chain, ROIs
- * split into two size classes (up to 4 and up to 16 pixel slots per lane),
- * each drained by a persistent launch through its own work queue; the two
- * launches run concurrently (the small class on an internal stream that
- * `stream` joins), so the call is asynchronous on `stream` as a whole. */
+ * split into two size classes (up to 4 and up to 16 pixel slots per lane).
+ * By default one persistent launch drains both classes through one work
+ * queue, the large ROIs first; the other launch settings
+ * (bamf_rjmcmc_set_tuning) give each class its own launch and queue, the two
+ * side by side (the small class on an internal stream that `stream` joins)
+ * or one after the other.  The call is asynchronous on `stream` as a whole. */
 int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_t seed, int mode,
                     const bamf_replay* replay, const bamf_rjmcmc_out* out, void* stream);
 /* Segmented runs (no counterpart in the reference, whose chain is one loop,
@@ -357,12 +359,18 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
  * bounded time, so other work on the device can interleave. */
 int bamf_rjmcmc_set_segments(bamf_rjmcmc_plan* plan, int n_segments, int use_graph);
 
-/* Launch configuration of the one-launch chain kernels (no counterpart in the
- * reference).  chains_per_simd[c]: resident chains per SIMD that size class
- * c's kernel is compiled for (3, 4 or 5: 168, 128 or 96 VGPRs, so more
- * registers and fewer spills against more latency hiding); concurrent: the
- * two classes run side by side (1) or one after the other (0).  Segmented
- * runs and runs with gSplit/gMerge use 4 chains per SIMD. */
+/* Launch configuration of the chain kernels (no counterpart in the
+ * reference).  concurrent: 2 (the default) runs both classes in one launch
+ * over one work queue, at 4 chains per SIMD (plain, segmented and learned
+ * chains; runs with gSplit/gMerge or the sampled PSF then run the classes
+ * side by side); 1 gives each class its own launch, the two side by side;
+ * 0 one after the other.  chains_per_simd[c] (settings 0 and 1, plain
+ * one-launch-per-segment runs): resident chains per SIMD that size class c's
+ * kernel is compiled for (3, 4 or 5: 168, 128 or 96 VGPRs for the small
+ * class, so more registers and fewer spills against more latency hiding;
+ * the large class fits 4 work-groups per CU in LDS, so its 5 compiles
+ * without a register limit and runs at 2).  Segmented runs and runs with
+ * gSplit/gMerge use 4 chains per SIMD. */
 typedef struct bamf_rjmcmc_tuning {
     int32_t chains_per_simd[2];
     int32_t concurrent;

@@ -47,8 +47,10 @@ def assert_same(a, b, n, n_trace):
         assert ca["prop_em"][r, :min(e2, 12000)].tobytes() == cb["prop_em"][r, :min(e2, 12000)].tobytes(), r
 
 
-@pytest.mark.parametrize("nseg", [2, 7])
-def test_segments_equal_one_launch(nseg):
+@pytest.mark.parametrize("nseg,launch", [(2, 2), (7, 2), (3, 1)])
+def test_segments_equal_one_launch(nseg, launch):
+    """launch 2: each segment one launch over both size classes (the default);
+    1: a launch per class and segment, the classes side by side."""
     rois = rois_uneven()
     n = len(rois["side"])
     p = params(nb=600, nt=600, rho=0.0275)
@@ -56,8 +58,9 @@ def test_segments_equal_one_launch(nseg):
     one = run_batch(Plan(p, OFF_PDF, SIG_PDF, BACK_PDF), rs, seed=5, **KW)
     assert np.all(one.status == 0)
     plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    plan.set_tuning((4, 4), launch)
     plan.set_segments(nseg)
-    print(f"segments: {nseg}", flush=True)
+    print(f"segments: {nseg}, launch setting {launch}", flush=True)
     seg = run_batch(plan, rs, seed=5, **KW)
     assert_same(one, seg, n, 120)
 
@@ -103,25 +106,31 @@ def test_segments_in_replay_mode(oracle):
 
 
 def test_tuning_changes_no_result():
-    """Every register budget and the sequential class order give the same
-    chains, bit for bit; the autotuner returns one of the compiled settings."""
+    """Every launch setting (one launch over both size classes, a launch per
+    class side by side or one after the other) and every register budget give
+    the same chains, bit for bit; the autotuner returns one of the compiled
+    settings."""
     rois = rois_uneven(120, seed=64)
     n = len(rois["side"])
     p = params(nb=400, nt=400, rho=0.0275)
     rs = RoiSet.from_synth(rois, bg0=np.full(n, 20.0, np.float32))
     kw = dict(KW, trace_len=80)
     plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
-    assert plan.tuning() == {"chains_per_simd": [4, 4], "concurrent": True}
+    assert plan.tuning() == {"chains_per_simd": [4, 4], "concurrent": 2}
     base = run_batch(plan, rs, seed=9, **kw)
     assert np.all(base.status == 0)
-    for cps, conc in (((3, 3), True), ((5, 5), True), ((3, 5), False), ((5, 3), True)):
+    for cps, conc in (((4, 4), 1), ((4, 4), 0), ((3, 3), 1), ((5, 5), 1), ((3, 5), 0), ((5, 3), 1), ((4, 4), 2)):
         plan.set_tuning(cps, conc)
         print(f"tuning {cps} concurrent={conc}", flush=True)
         assert_same(base, run_batch(plan, rs, seed=9, **kw), n, 80)
     best = plan.autotune(rs, sample_jumps=200)
     print("autotune:", best, flush=True)
-    assert best["ms"] > 0 and all(c in (3, 4, 5) for c in best["chains_per_simd"])
+    assert best["ms"] > 0 and all(c in (3, 4, 5) for c in best["chains_per_simd"]) and best["concurrent"] in (0, 1, 2)
     assert plan.tuning()["chains_per_simd"] == best["chains_per_simd"]
     assert_same(base, run_batch(plan, rs, seed=9, **kw), n, 80)
     with pytest.raises(Exception):
-        plan.set_tuning((2, 4), True)
+        plan.set_tuning((2, 4), 1)
+    with pytest.raises(Exception):
+        plan.set_tuning((3, 3), 2)  # the one launch over both classes has one register budget
+    with pytest.raises(Exception):
+        plan.set_tuning((4, 4), 3)
