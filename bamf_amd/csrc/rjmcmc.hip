@@ -1684,9 +1684,10 @@ struct Chain {
                 }
                 const bool acc = decide(PR, sum);
                 applied = acc;
-                // kept for step records and the accepted-jump total
+                // kept for step records and the accepted-jump total (each
+                // formed only when its output is requested)
                 double logpr = __builtin_nan("");
-                if (acc || a->steps) logpr = logpr_of(sum);
+                if ((acc && a->logpost) || a->steps) logpr = logpr_of(sum);
                 r_logpr = logpr;
                 if (acc) {
                     if (!GJ || acc_c1 >= 0) {  // gSplit/gMerge keep no counters

@@ -145,7 +145,7 @@ class BatchResult:
     n_final: np.ndarray
     final: np.ndarray
     hist: np.ndarray
-    logpost: np.ndarray
+    logpost: np.ndarray | None
     trace_n: np.ndarray | None
     trace_photons: np.ndarray | None
     steps: np.ndarray | None
@@ -236,13 +236,19 @@ def run_batch(plan: Plan, rois: RoiSet, seed: int = 0, mode: int = PLAIN, replay
     return fetch(outs, n, total, hist_len, final_cap, trace_len)
 
 
-def alloc_outputs(n, total, hist_len, final_cap, trace_len, record_steps, grids, rois, zoom, chain_caps=None):
+def alloc_outputs(n, total, hist_len, final_cap, trace_len, record_steps, grids, rois, zoom, chain_caps=None,
+                  logpost=True):
+    """logpost=False leaves out the accepted-jump log-PR total (a diagnostic with no counterpart
+    in the reference), and with it the double-precision log of each accepted jump."""
     B = _hip.DeviceBuffer
     d = {"status": B(4 * n), "counters": B(4 * 30 * n), "final": B(4 * n * (4 + 4 * final_cap)),
-         "hist": B(4 * n * hist_len), "logpost": B(8 * n), "flops": B(8 * n)}
+         "hist": B(4 * n * hist_len), "flops": B(8 * n)}
     o = RjOut()
     o.d_status, o.d_counters, o.d_final, o.final_cap = d["status"].ptr, d["counters"].ptr, d["final"].ptr, final_cap
-    o.d_n_hist, o.hist_len, o.d_logpost = d["hist"].ptr, hist_len, d["logpost"].ptr
+    o.d_n_hist, o.hist_len = d["hist"].ptr, hist_len
+    if logpost:
+        d["logpost"] = B(8 * n)
+        o.d_logpost = d["logpost"].ptr
     o.d_alg_flops = d["flops"].ptr
     if trace_len > 0:
         d["trace_n"] = B(2 * n * trace_len)
@@ -291,7 +297,7 @@ def fetch(d, n, total, hist_len, final_cap, trace_len) -> BatchResult:
         counters=d["counters"].download(np.int32).reshape(n, 30),
         n_final=fin[:, 0].astype(np.int32), final=fin,
         hist=d["hist"].download(np.int32).reshape(n, hist_len),
-        logpost=d["logpost"].download(np.float64),
+        logpost=d["logpost"].download(np.float64) if "logpost" in d else None,
         trace_n=d["trace_n"].download(np.int16).reshape(n, trace_len) if "trace_n" in d else None,
         trace_photons=d["trace_ph"].download(np.float32).reshape(n, trace_len) if "trace_ph" in d else None,
         steps=d["steps"].download(STEP_DTYPE).reshape(n, total) if "steps" in d else None)

@@ -559,7 +559,7 @@ def bench_dense_lowsnr(args):
     back = np.zeros(1001, np.float32)
     plan = Plan(p, off, sig, back)
     trace_len, cap_em = nt // args.thin, 128
-    outs = alloc_outputs(n_chain, nb + nt, 129, cap_em, trace_len, False, False, rs, p.grid_zoom)
+    outs = alloc_outputs(n_chain, nb + nt, 129, cap_em, trace_len, False, False, rs, p.grid_zoom, logpost=False)
 
     def front():
         fr.run(d_frames, sigma, 2 * sigma, box, minval, stream=stream)
@@ -765,7 +765,7 @@ def main():
     rs = RoiSet.from_synth(rois, bg0=np.full(n, bg0, np.float32), order=order)
     trace_len = nt // args.thin
     cap = 128
-    outs = alloc_outputs(n, nb + nt, 129, cap, trace_len, False, False, rs, p.grid_zoom)
+    outs = alloc_outputs(n, nb + nt, 129, cap, trace_len, False, False, rs, p.grid_zoom, logpost=False)
     stream = _hip.Stream()
     tuning = {"chains_per_simd": [4, 4], "concurrent": 2, "autotuned": False}
     if not args.no_autotune:

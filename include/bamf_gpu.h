@@ -292,7 +292,9 @@ typedef struct bamf_rjmcmc_out {
     int32_t final_cap;
     int32_t* d_n_hist;       /* [n_roi][hist_len]: post-burn-in histogram of N (last bin collects N >= hist_len-1) */
     int32_t hist_len;
-    double* d_logpost;       /* [n_roi]: sum of the log PR of accepted jumps */
+    double* d_logpost;       /* [n_roi] or NULL: sum of the log PR of accepted jumps (no counterpart in the
+                                reference; its double-precision log per accepted jump is formed only when
+                                this output or the step records are requested) */
     int16_t* d_trace_n;      /* [n_roi][trace_len]: N at every trace_thin-th post-burn-in step */
     float* d_trace_photons;  /* [n_roi][trace_len]: total photons at the same steps */
     int32_t trace_len;

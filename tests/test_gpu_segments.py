@@ -65,6 +65,34 @@ def test_segments_equal_one_launch(nseg, launch):
     assert_same(one, seg, n, 120)
 
 
+def test_logpost_output_optional():
+    """Without the log-PR total (d_logpost NULL) the chains and every other
+    output are the same, bit for bit."""
+    rois = rois_uneven(64, seed=65)
+    n = len(rois["side"])
+    p = params(nb=400, nt=400, rho=0.0275)
+    rs = RoiSet.from_synth(rois, bg0=np.full(n, 20.0, np.float32))
+    plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
+    kw = dict(hist_len=129, final_cap=128, trace_len=80, trace_thin=5)
+    base = run_batch(plan, rs, seed=13, **kw)
+    assert np.all(base.status == 0) and base.logpost is not None
+    total = p.n_burnin + p.n_trials
+    outs = alloc_outputs(n, total, 129, 128, 80, False, False, rs, p.grid_zoom, logpost=False)
+    print("run without the log-PR total", flush=True)
+    run_batch(plan, rs, seed=13, outs=outs, trace_thin=5, sync=False)
+    _hip.synchronize()
+    got = fetch(outs, n, total, 129, 128, 80)
+    assert got.logpost is None
+    for f in ("status", "counters", "hist", "alg_flops", "n_final", "trace_n", "trace_photons"):
+        assert getattr(base, f).tobytes() == getattr(got, f).tobytes(), f
+    for r in range(n):
+        k = int(base.final[r, 0])
+        assert base.final[r, 0:4].tobytes() == got.final[r, 0:4].tobytes(), r  # N, BG, ABG, BBG
+        for j in range(4):
+            sl = slice(4 + j * 128, 4 + j * 128 + k)
+            assert base.final[r, sl].tobytes() == got.final[r, sl].tobytes(), (r, j)
+
+
 def test_graph_replays_with_new_seeds():
     rois = rois_uneven(96, seed=62)
     n = len(rois["side"])
