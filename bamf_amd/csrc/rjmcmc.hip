@@ -1416,39 +1416,40 @@ struct Chain {
         return pix;
     }
     // Learned proposal: pixel drawn from the per-ROI proposal map q; returns
-    // the pixel and its probability q[pix] / sum(q).
-    __device__ __forceinline__ int map_pixel(float u, float& prob) {
+    // the pixel and its probability q[pix] / sum(q).  The running sums of q
+    // in pixel order depend on the ROI only: map_sums forms them once when the
+    // chain starts the ROI (slot scans carried in slot order) into qcum, a
+    // pixel whose q is not positive holding -inf so that no draw selects it;
+    // q_total is the sum.
+    float q_total;
+    __device__ __forceinline__ float* qcum_of() const { return a->qcum + (int64_t)roi * (64 * kMaxPPL); }
+    __device__ __forceinline__ void map_sums() {
         const int64_t off = a->pix_off[roi];
+        float* qc = qcum_of();
         float carry = 0.0f;
-        float cum[PPL], q[PPL];
-#pragma unroll
-        for (int s = 0; s < PPL; s++) {
-            cum[s] = 0.0f;
-            q[s] = 0.0f;
-            if (s < ppl) {
-                float v = has(s) ? a->prop_map[off + pidx(s)] : 0.0f;
-                q[s] = v;
-                v = wave_scan(v);
-                cum[s] = carry + v;
-                carry = bcastf(cum[s], 63);
-            }
+        for (int s = 0; s < ppl; s++) {
+            const float v = has(s) ? a->prop_map[off + pidx(s)] : 0.0f;
+            const float cum = carry + wave_scan(v);
+            carry = bcastf(cum, 63);
+            qc[64 * s + lane] = (has(s) && v > 0.0f) ? cum : -INFINITY;
         }
+        q_total = carry;
+    }
+    __device__ __forceinline__ int map_pixel(float u, float& prob) {
+        const float* qc = qcum_of();
+        const float carry = q_total;
         const float rr = carry * u;
         int pix = P - 1;
         float qp = 0.0f;
         bool found = false;
-#pragma unroll
-        for (int s = 0; s < PPL; s++) {
-            if (s < ppl && !found) {
-                const uint64_t mk = ballot(has(s) && rr <= cum[s] && q[s] > 0.0f);
-                if (mk) {
-                    const int l = __builtin_ctzll(mk);
-                    pix = bcasti(pidx(s), l);
-                    qp = bcastf(q[s], l);
-                    found = true;
-                }
+        for (int s = 0; s < ppl && !found; s++) {
+            const uint64_t mk = ballot(has(s) && rr <= qc[64 * s + lane]);
+            if (mk) {
+                pix = bcasti(pidx(s), __builtin_ctzll(mk));
+                found = true;
             }
         }
+        if (found) qp = a->prop_map[a->pix_off[roi] + pix];
         prob = carry > 0.0f ? qp / carry : 0.0f;
         return pix;
     }
@@ -1879,6 +1880,9 @@ struct Chain {
             tlen = 0;
         }
         pblk = -1;
+        if constexpr (LRN) {
+            if (a->mode == BAMF_RJ_LEARNED && a->prop_map != nullptr) map_sums();
+        }
         const int64_t rstride = total;
         for (int s = s_begin; s < s_end && status == ST_OK; s++) {
             step = s;

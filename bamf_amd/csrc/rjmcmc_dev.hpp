@@ -104,6 +104,10 @@ struct ChainArgs {
     // the log-uniform half-width and 1 / (integral of the signal pdf table)
     const float* prop_map;
     const float* prop_mu;
+    // learned births: each ROI's running sums of prop_map in pixel order, in
+    // the lane-slot layout ([roi][64 * kMaxPPL], -inf where the map is not
+    // positive), written by the chain when it starts the ROI
+    float* qcum;
     float l_alpha, l_halfwidth, l_zinv;
     float l_beta;  // probability of the independence intensity proposal in a single-emitter move
     int32_t l_ikind;  // that proposal: 0 the network's log-uniform window, 1 the conditional-likelihood fit

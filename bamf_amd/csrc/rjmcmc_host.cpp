@@ -111,6 +111,8 @@ struct bamf_rjmcmc_plan {
     float c_infl = 1.2f;
     float l_single_from = 0.75f;  // learned mode's share of single-emitter moves among moves: 1 - this
     const float* d_q = nullptr;   // learned proposal maps (caller-owned)
+    float* d_qcum = nullptr;      // the maps' running sums per ROI (learned births), qcum_cap ROIs
+    int64_t qcum_cap = 0;
     const float* d_mu = nullptr;
     bool proposal_set = false;    // bamf_rjmcmc_set_proposal was called (maps may be NULL)
     float l_alpha = 1.0f, l_halfwidth = 0.5f, l_beta = 0.0f;
@@ -157,6 +159,7 @@ void plan_release(bamf_rjmcmc_plan* pl) {
     if (pl->d_seg) (void)hipFree(pl->d_seg);
     if (pl->d_tune_status) (void)hipFree(pl->d_tune_status);
     if (pl->d_psf) (void)hipFree(pl->d_psf);
+    if (pl->d_qcum) (void)hipFree(pl->d_qcum);
     delete pl;
 }
 void drop_graph(bamf_rjmcmc_plan* pl) {
@@ -543,6 +546,14 @@ int bamf_rjmcmc_run(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, uint64_
         a.l_single_from = plan->l_single_from;
         a.l_split = plan->d_q ? plan->l_split : 0.0f;
         a.l_split_r = plan->l_split_r;
+        if (plan->d_q && plan->qcum_cap < batch->n_roi) {
+            if (plan->d_qcum) BAMF_HIP_TRY(hipFree(plan->d_qcum));
+            plan->d_qcum = nullptr;
+            plan->qcum_cap = 0;
+            BAMF_HIP_TRY(hipMalloc(&plan->d_qcum, sizeof(float) * 64 * kMaxPPL * (size_t)batch->n_roi));
+            plan->qcum_cap = batch->n_roi;
+        }
+        a.qcum = plan->d_q ? plan->d_qcum : nullptr;
     }
     if (plan->d_psf) {
         a.psf = plan->d_psf;
