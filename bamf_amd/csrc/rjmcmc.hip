@@ -147,7 +147,9 @@ __device__ __forceinline__ float findpdf(const StatDev& s, float dx, float I) {
 }
 
 // LibRJMCMC.cpp:248-282 with the uniform already drawn; the first uc > u is
-// found by a forward scan from the guide entry of u.
+// found by a forward scan from the guide entry of u, 64 entries per step
+// (lane l tests entry hi + l; the first lane that stops the scan gives the
+// index).  Called by the whole wavefront with a wave-uniform u.
 __device__ __forceinline__ float genrand(const StatDev& s, float u) {
     if (u <= 0) return 0;
     if (u >= 1) return s.ux[s.nu - 1];
@@ -155,7 +157,16 @@ __device__ __forceinline__ float genrand(const StatDev& s, float u) {
     int g = (int)(u * (float)kGuide);
     g = g > kGuide - 1 ? kGuide - 1 : g;
     int hi = s.guide[g] + 1;  // uc[guide[g]] <= g / kGuide <= u
-    while (hi < s.nu - 1 && !(s.uc[hi] - u > 0)) hi++;
+    const int lane = (int)__lane_id();
+    while (true) {
+        const int k = hi + lane;
+        const uint64_t stop = __ballot(k >= s.nu - 1 || s.uc[k] - u > 0);
+        if (stop) {
+            hi += (int)__builtin_ctzll(stop);
+            break;
+        }
+        hi += 64;
+    }
     return (s.ux[hi - 1] * (s.uc[hi] - u) + s.ux[hi] * (u - s.uc[hi - 1])) / (s.uc[hi] - s.uc[hi - 1]);
 }
 
