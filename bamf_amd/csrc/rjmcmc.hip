@@ -2062,13 +2062,13 @@ __global__ __launch_bounds__(64 * kWavesPerBlock, MINB) void k_chain_all(ChainAr
     }
 }
 
-// Stable split of the ROI order into the size classes (P <= 64 * kClassPPL[c]
-// for the first class c that fits); also resets the nq work-queue counters
-// (two per segment).  One work-group walks the order in tiles of 1024 with a
-// work-group scan.
+// Stable split of the ROI order into the size classes: ROIs of at most
+// small_max pixel slots (<= kClassPPL[0]) to the small class, the rest to the
+// large one; also resets the nq work-queue counters (two per segment).  One
+// work-group walks the order in tiles of 1024 with a work-group scan.
 __global__ __launch_bounds__(1024) void k_classify(int n_roi, const int32_t* order, const int32_t* xs,
                                                    const int32_t* ys, int32_t* list0, int32_t* list1,
-                                                   int32_t* counts, uint32_t* queues, int nq) {
+                                                   int32_t* counts, uint32_t* queues, int nq, int small_max) {
     __shared__ int sc[2][1024];
     __shared__ int base[2];
     const int t = threadIdx.x;
@@ -2079,7 +2079,7 @@ __global__ __launch_bounds__(1024) void k_classify(int n_roi, const int32_t* ord
         int roi = -1, c = -1;
         if (i < n_roi) {
             roi = order ? order[i] : i;
-            c = slots_for(xs[roi], ys[roi]) <= kClassPPL[0] ? 0 : 1;
+            c = slots_for(xs[roi], ys[roi]) <= small_max ? 0 : 1;
         }
         sc[0][t] = c == 0;
         sc[1][t] = c == 1;
@@ -2116,9 +2116,10 @@ __global__ void k_erf(const float* x, float* y, int64_t n) {
 }  // namespace
 
 int launch_classify(int n_roi, const int32_t* order, const int32_t* xs, const int32_t* ys, int32_t* list0,
-                    int32_t* list1, int32_t* counts, uint32_t* queues, int nq, hipStream_t s) {
+                    int32_t* list1, int32_t* counts, uint32_t* queues, int nq, int small_max, hipStream_t s) {
+    BAMF_CHECK_ARG(small_max >= 0 && small_max <= kClassPPL[0], "small-class slot bound %d out of range", small_max);
     hipLaunchKernelGGL(k_classify, dim3(1), dim3(1024), 0, s, n_roi, order, xs, ys, list0, list1, counts, queues,
-                       nq);
+                       nq, small_max);
     BAMF_LAUNCH_CHECK("k_classify");
     return BAMF_OK;
 }

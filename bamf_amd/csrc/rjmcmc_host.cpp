@@ -10,7 +10,7 @@
 namespace bamf {
 namespace rj {
 int launch_classify(int n_roi, const int32_t* order, const int32_t* xs, const int32_t* ys, int32_t* list0,
-                    int32_t* list1, int32_t* counts, uint32_t* queues, int nq, hipStream_t s);
+                    int32_t* list1, int32_t* counts, uint32_t* queues, int nq, int small_max, hipStream_t s);
 int launch_set_seed(uint32_t* d, uint32_t lo, uint32_t hi, hipStream_t s);
 // features: bit 0 gSplit/gMerge, bit 1 segmented, bit 2 learned (the kernel variants of rjmcmc.hip)
 int launch_chain(const ChainArgs& a, int cls, int features, int tune, int grid, hipStream_t s);
@@ -129,6 +129,7 @@ struct bamf_rjmcmc_plan {
     // autotuned (or set) launch configuration of the one-launch plain kernels
     int tune[kClasses] = {kTuneDefault, kTuneDefault};
     int launch = kLaunchOne;  // how the classes are launched (kLaunch*)
+    int small_max = kClassPPL[0];  // ROIs of at most this many pixel slots run in the small class
     int32_t* d_tune_status = nullptr;
     int64_t tune_cap = 0;
     // segmented runs and their graph
@@ -177,7 +178,7 @@ int enqueue_run(bamf_rjmcmc_plan* plan, const ChainArgs& a, int nseg, bool segme
     const int total = a.p.n_burnin + a.p.n_trials;
     int32_t* counts = reinterpret_cast<int32_t*>(plan->d_queue + kCountsAt);
     int rc = launch_classify(a.n_roi, a.order, a.xs, a.ys, plan->d_lists, plan->d_lists + plan->list_cap, counts,
-                             plan->d_queue, 2 * nseg, s);
+                             plan->d_queue, 2 * nseg, plan->small_max, s);
     if (rc != BAMF_OK) return rc;
     const int cu = plan->cu > 0 ? plan->cu : 256;
     const int need = (int)ceil_div(a.n_roi, kWavesPerBlock);
@@ -617,6 +618,7 @@ int bamf_rjmcmc_get_tuning(const bamf_rjmcmc_plan* plan, bamf_rjmcmc_tuning* t) 
     for (int c = 0; c < kClasses; c++) t->chains_per_simd[c] = kTuneMinBlocks[plan->tune[c]] * kWavesPerBlock / 4;
     t->concurrent = plan->launch;
     t->ms = 0.0f;
+    t->small_max_slots = plan->small_max;
     return BAMF_OK;
 }
 
@@ -633,8 +635,11 @@ int bamf_rjmcmc_set_tuning(bamf_rjmcmc_plan* plan, const bamf_rjmcmc_tuning* t) 
     BAMF_CHECK_ARG(t->concurrent >= kLaunchSequential && t->concurrent <= kLaunchOne, "concurrent must be 0, 1 or 2");
     BAMF_CHECK_ARG(t->concurrent != kLaunchOne || (idx[0] == kTuneDefault && idx[1] == kTuneDefault),
                    "the one launch over both classes runs 4 chains per SIMD");
+    BAMF_CHECK_ARG(t->small_max_slots >= 0 && t->small_max_slots <= kClassPPL[0],
+                   "small_max_slots must be in 0..%d (0: the default)", kClassPPL[0]);
     for (int c = 0; c < kClasses; c++) plan->tune[c] = idx[c];
     plan->launch = t->concurrent;
+    plan->small_max = t->small_max_slots > 0 ? t->small_max_slots : kClassPPL[0];
     drop_graph(plan);
     return BAMF_OK;
 }
@@ -655,7 +660,7 @@ int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, in
     // the sample runs use this plan with short chains, one launch, no graph
     const bamf_mcmc_params saved = plan->params;
     const int saved_tune[kClasses] = {plan->tune[0], plan->tune[1]};
-    const int saved_launch = plan->launch;
+    const int saved_launch = plan->launch, saved_small = plan->small_max;
     const bool saved_graph = plan->use_graph;
     const int saved_nseg = plan->n_segments;
     plan->params.n_burnin = sample_jumps / 2;
@@ -669,7 +674,9 @@ int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, in
     int rc = BAMF_OK;
     float best_ms = -1.0f;
     int best_t[kClasses] = {kTuneDefault, kTuneDefault};
-    int best_launch = kLaunchOne;
+    int best_launch = kLaunchOne, best_small = kClassPPL[0];
+    // the ROI tiles: the small class takes ROIs of at most 4, 2 or 1 pixel slots
+    constexpr int kSmallMax[3] = {kClassPPL[0], 2, 1};
     // group jumps and segments use the default budget: only the concurrency is tunable then
     const bool gj = saved.p_burnin[5] != 0.0f || saved.p_burnin[6] != 0.0f;
     const int nt = gj ? 1 : kTuneChoices;
@@ -678,6 +685,7 @@ int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, in
         rc = BAMF_E_HIP;
     }
     for (int lm = kLaunchOne; lm >= kLaunchSequential && rc == BAMF_OK; lm--)
+        for (int sm = 0; sm < 3 && rc == BAMF_OK; sm++)
         for (int t1 = 0; t1 < nt && rc == BAMF_OK; t1++)
             for (int t0 = 0; t0 < nt && rc == BAMF_OK; t0++) {
                 // the one launch over both classes has the default budget only
@@ -685,6 +693,7 @@ int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, in
                 plan->tune[0] = gj ? kTuneDefault : t0;
                 plan->tune[1] = gj ? kTuneDefault : t1;
                 plan->launch = lm;
+                plan->small_max = kSmallMax[sm];
                 float ms = 0.0f;
                 for (int rep = 0; rep < 2 && rc == BAMF_OK; rep++) {  // the first run loads the code objects
                     if (hipEventRecord(e0, s) != hipSuccess) rc = BAMF_E_HIP;
@@ -698,6 +707,7 @@ int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, in
                     best_t[0] = plan->tune[0];
                     best_t[1] = plan->tune[1];
                     best_launch = plan->launch;
+                    best_small = plan->small_max;
                 }
             }
     if (e0) (void)hipEventDestroy(e0);
@@ -709,12 +719,14 @@ int bamf_rjmcmc_autotune(bamf_rjmcmc_plan* plan, const bamf_roi_batch* batch, in
         plan->tune[0] = saved_tune[0];
         plan->tune[1] = saved_tune[1];
         plan->launch = saved_launch;
+        plan->small_max = saved_small;
         if (rc == BAMF_E_HIP) set_error("autotune: a sample run failed on the device");
         return rc;
     }
     plan->tune[0] = best_t[0];
     plan->tune[1] = best_t[1];
     plan->launch = best_launch;
+    plan->small_max = best_small;
     drop_graph(plan);
     rc = bamf_rjmcmc_get_tuning(plan, best);
     best->ms = best_ms;

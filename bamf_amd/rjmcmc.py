@@ -94,20 +94,25 @@ class Plan:
         t = Tuning()
         check(lib().bamf_rjmcmc_autotune(self.handle, ctypes.byref(rois.batch), int(sample_jumps), ctypes.byref(t),
                                          stream.vp if stream else None), "bamf_rjmcmc_autotune")
-        return {"chains_per_simd": list(t.chains_per_simd), "concurrent": int(t.concurrent), "ms": float(t.ms)}
+        return {"chains_per_simd": list(t.chains_per_simd), "concurrent": int(t.concurrent), "ms": float(t.ms),
+                "small_max_slots": int(t.small_max_slots)}
 
-    def set_tuning(self, chains_per_simd=(4, 4), concurrent: int = 2) -> None:
+    def set_tuning(self, chains_per_simd=(4, 4), concurrent: int = 2, small_max_slots: int = 4) -> None:
         """concurrent: 2 both size classes in one launch (4 chains per SIMD), 1 a launch per
-        class side by side, 0 one after the other; chains_per_simd per class (3, 4 or 5)."""
+        class side by side, 0 one after the other; chains_per_simd per class (3, 4 or 5);
+        small_max_slots: ROIs of at most this many pixel slots per lane run in the 4-slot
+        class (1..4), the rest in the 16-slot class."""
         t = Tuning()
         t.chains_per_simd[0], t.chains_per_simd[1] = int(chains_per_simd[0]), int(chains_per_simd[1])
         t.concurrent = int(concurrent)
+        t.small_max_slots = int(small_max_slots)
         check(lib().bamf_rjmcmc_set_tuning(self.handle, ctypes.byref(t)), "bamf_rjmcmc_set_tuning")
 
     def tuning(self) -> dict:
         t = Tuning()
         check(lib().bamf_rjmcmc_get_tuning(self.handle, ctypes.byref(t)), "bamf_rjmcmc_get_tuning")
-        return {"chains_per_simd": list(t.chains_per_simd), "concurrent": int(t.concurrent)}
+        return {"chains_per_simd": list(t.chains_per_simd), "concurrent": int(t.concurrent),
+                "small_max_slots": int(t.small_max_slots)}
 
     def set_intensity_proposal(self, kind: str = "window", inflation: float = 1.2, single_share: float = 0.25) -> None:
         """Learned mode's independence intensity proposal: "window" (the network's log-uniform

@@ -9,7 +9,8 @@ import numpy as np
 
 class Tuning(Structure):
     """bamf_rjmcmc_tuning."""
-    _fields_ = [("chains_per_simd", c_int32 * 2), ("concurrent", c_int32), ("ms", c_float)]
+    _fields_ = [("chains_per_simd", c_int32 * 2), ("concurrent", c_int32), ("ms", c_float),
+                ("small_max_slots", c_int32)]
 
 
 class McmcParams(Structure):

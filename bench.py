@@ -767,11 +767,12 @@ def main():
     cap = 128
     outs = alloc_outputs(n, nb + nt, 129, cap, trace_len, False, False, rs, p.grid_zoom, logpost=False)
     stream = _hip.Stream()
-    tuning = {"chains_per_simd": [4, 4], "concurrent": 2, "autotuned": False}
+    tuning = {"chains_per_simd": [4, 4], "concurrent": 2, "small_max_slots": 4, "autotuned": False}
     if not args.no_autotune:
         t0 = time.perf_counter()
         best = plan.autotune(rs, sample_jumps=400, stream=stream)
-        tuning = {"chains_per_simd": best["chains_per_simd"], "concurrent": best["concurrent"], "autotuned": True,
+        tuning = {"chains_per_simd": best["chains_per_simd"], "concurrent": best["concurrent"],
+                  "small_max_slots": best["small_max_slots"], "autotuned": True,
                   "sample_ms": best["ms"], "tune_s": time.perf_counter() - t0}
         print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
     tuning["launch"] = {2: "one launch over both size classes", 1: "a launch per size class, side by side",

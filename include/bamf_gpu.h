@@ -372,11 +372,17 @@ int bamf_rjmcmc_set_segments(bamf_rjmcmc_plan* plan, int n_segments, int use_gra
  * class, so more registers and fewer spills against more latency hiding;
  * the large class fits 4 work-groups per CU in LDS, so its 5 compiles
  * without a register limit and runs at 2).  Segmented runs and runs with
- * gSplit/gMerge use 4 chains per SIMD. */
+ * gSplit/gMerge use 4 chains per SIMD.  small_max_slots: the ROI tile each
+ * ROI runs in, by its pixel slots per lane (64-lane slots of whole columns,
+ * column height padded to a power of two): ROIs of at most this many slots
+ * (1..4) run in the 4-slot class, the rest in the 16-slot class; 0 in a
+ * struct passed to bamf_rjmcmc_set_tuning means the default, 4.  The
+ * autotuner tries 4, 2 and 1. */
 typedef struct bamf_rjmcmc_tuning {
     int32_t chains_per_simd[2];
     int32_t concurrent;
     float ms;  /* autotune: the chosen setting's time on the sample run */
+    int32_t small_max_slots;
 } bamf_rjmcmc_tuning;
 /* Times every setting on the caller's sample batch (its ROIs with chains of
  * sample_jumps jumps, plain mode), keeps the fastest in the plan and returns

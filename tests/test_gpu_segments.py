@@ -135,25 +135,29 @@ def test_segments_in_replay_mode(oracle):
 
 def test_tuning_changes_no_result():
     """Every launch setting (one launch over both size classes, a launch per
-    class side by side or one after the other) and every register budget give
-    the same chains, bit for bit; the autotuner returns one of the compiled
-    settings."""
+    class side by side or one after the other), every register budget and
+    every ROI tile bound (the ROIs of at most that many pixel slots run in the
+    4-slot class) give the same chains, bit for bit; the autotuner returns one
+    of the compiled settings."""
     rois = rois_uneven(120, seed=64)
     n = len(rois["side"])
     p = params(nb=400, nt=400, rho=0.0275)
     rs = RoiSet.from_synth(rois, bg0=np.full(n, 20.0, np.float32))
     kw = dict(KW, trace_len=80)
     plan = Plan(p, OFF_PDF, SIG_PDF, BACK_PDF)
-    assert plan.tuning() == {"chains_per_simd": [4, 4], "concurrent": 2}
+    assert plan.tuning() == {"chains_per_simd": [4, 4], "concurrent": 2, "small_max_slots": 4}
     base = run_batch(plan, rs, seed=9, **kw)
     assert np.all(base.status == 0)
-    for cps, conc in (((4, 4), 1), ((4, 4), 0), ((3, 3), 1), ((5, 5), 1), ((3, 5), 0), ((5, 3), 1), ((4, 4), 2)):
-        plan.set_tuning(cps, conc)
-        print(f"tuning {cps} concurrent={conc}", flush=True)
+    for cps, conc, sm in (((4, 4), 1, 4), ((4, 4), 0, 4), ((3, 3), 1, 4), ((5, 5), 1, 4), ((3, 5), 0, 4),
+                          ((5, 3), 1, 4), ((4, 4), 2, 1), ((4, 4), 2, 2), ((3, 5), 1, 1), ((4, 4), 2, 4)):
+        plan.set_tuning(cps, conc, sm)
+        assert plan.tuning()["small_max_slots"] == sm
+        print(f"tuning {cps} concurrent={conc} small_max_slots={sm}", flush=True)
         assert_same(base, run_batch(plan, rs, seed=9, **kw), n, 80)
     best = plan.autotune(rs, sample_jumps=200)
     print("autotune:", best, flush=True)
     assert best["ms"] > 0 and all(c in (3, 4, 5) for c in best["chains_per_simd"]) and best["concurrent"] in (0, 1, 2)
+    assert best["small_max_slots"] in (1, 2, 4)
     assert plan.tuning()["chains_per_simd"] == best["chains_per_simd"]
     assert_same(base, run_batch(plan, rs, seed=9, **kw), n, 80)
     with pytest.raises(Exception):
@@ -162,3 +166,5 @@ def test_tuning_changes_no_result():
         plan.set_tuning((3, 3), 2)  # the one launch over both classes has one register budget
     with pytest.raises(Exception):
         plan.set_tuning((4, 4), 3)
+    with pytest.raises(Exception):
+        plan.set_tuning((4, 4), 2, 5)  # the small class holds at most 4 slots per lane
