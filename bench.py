@@ -687,8 +687,9 @@ def front_end_line(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    # the measurement procedure of the workloads: 5 warm-up runs, then the median of 20 timed runs
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="batched_uneven",
                     choices=["batched_uneven", "plain_16", "find_roi", "gauss_mle", "find_map", "tiles",
                              "dense_lowsnr"])
