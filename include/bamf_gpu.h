@@ -405,7 +405,9 @@ int bamf_rjmcmc_get_tuning(const bamf_rjmcmc_plan* plan, bamf_rjmcmc_tuning* t);
  * of the random walk (an independence Metropolis-Hastings kernel: the ratio
  * gains q(I | x') / q(I' | x) = I' / I, or 0 when I lies outside the window at
  * the new position); the position still walks.  The maps are caller-owned and
- * must stay valid while runs use them. */
+ * must stay valid while runs use them.  Learned runs with maps keep each
+ * ROI's running sums of q in a plan-owned buffer (4 KB per ROI, allocated by
+ * the first such run of a larger batch), formed when a chain starts its ROI. */
 int bamf_rjmcmc_set_proposal(bamf_rjmcmc_plan* plan, const float* d_q, const float* d_mu, float alpha,
                              float halfwidth, float beta);
 /* The independence intensity proposal of BAMF_RJ_LEARNED single-emitter
